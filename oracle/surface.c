@@ -1,0 +1,350 @@
+/*
+ * Oracle (test infrastructure only): nested region operator, block
+ * preconditioner, GMRES, right-hand side and reaction potentials, FP64.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static orc_region* region_new(void) { return (orc_region*)calloc(1, sizeof(orc_region)); }
+
+/* SurfaceOperator_initStartIndices (SurfaceOperator.c:39-47): pre-order. */
+static void init_start(orc_region* r, int* index) {
+   for (int i = 0; i < r->nchild; i++) {
+      orc_region* c = r->child[i];
+      c->startphi = *index;
+      c->startdphi = *index + c->mynpan;
+      *index += 2 * c->mynpan;
+      init_start(c, index);
+   }
+}
+
+/* One dielectric surface with its charges inside: salt
+ * (FFTSVDpbeAPI.c:1381-1554) or no-salt (FFTSVDpbeAPI.c:1557-1730)
+ * formulation with no cavities and no Stern layer. */
+orc_problem* orc_problem_single(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
+                                double eps_in, double eps_out, double kappa, int maxpanels, int gridpoints,
+                                double svdeps) {
+   orc_problem* pr = (orc_problem*)calloc(1, sizeof(orc_problem));
+   pr->ntotal = npan;
+   pr->panels = (orc_panel*)malloc(sizeof(orc_panel) * (size_t)npan);
+   orc_panels_from_verts(verts9, npan, pr->panels);
+   double* cent = (double*)malloc(sizeof(double) * 3 * (size_t)npan);
+   for (int i = 0; i < npan; i++) memcpy(&cent[3 * i], pr->panels[i].c, 3 * sizeof(double));
+
+   orc_region* root = region_new();
+   root->kernel = kappa > 0.0 ? ORC_YUKAWA : ORC_LAPLACE;
+   root->eps = eps_out;
+   root->kappa = kappa > 0.0 ? kappa : 0.0;
+   root->mynpan = 0;
+   root->tree = orc_tree_build(pr->panels, npan, cent, npan, root->kernel, root->kappa, maxpanels, gridpoints, svdeps);
+   orc_tree_fill(root->tree);
+   root->nchild = 1;
+   root->child = (orc_region**)calloc(1, sizeof(orc_region*));
+   root->resint = (double*)calloc(npan, sizeof(double));
+   root->resext = (double*)calloc(npan, sizeof(double));
+
+   orc_region* in = region_new();
+   in->kernel = ORC_LAPLACE;
+   in->eps = eps_in;
+   in->kappa = 0.0;
+   in->mynpan = npan;
+   in->parent = root;
+   in->tree = orc_tree_build(pr->panels, npan, cent, npan, ORC_LAPLACE, 0.0, maxpanels, gridpoints, svdeps);
+   orc_tree_fill(in->tree);
+   in->resint = (double*)calloc(npan, sizeof(double));
+   in->resext = (double*)calloc(npan, sizeof(double));
+   in->ncharge = ncharge;
+   in->qpos = (double*)malloc(sizeof(double) * 3 * (ncharge > 0 ? ncharge : 1));
+   in->q = (double*)malloc(sizeof(double) * (ncharge > 0 ? ncharge : 1));
+   memcpy(in->qpos, qpos, sizeof(double) * 3 * ncharge);
+   memcpy(in->q, q, sizeof(double) * ncharge);
+   root->child[0] = in;
+   pr->root = root;
+   int index = 0;
+   init_start(root, &index);
+   free(cent);
+   return pr;
+}
+
+static void region_free(orc_region* r) {
+   if (!r) return;
+   for (int i = 0; i < r->nchild; i++) region_free(r->child[i]);
+   free(r->child);
+   orc_tree_free(r->tree);
+   free(r->qpos); free(r->q); free(r->resint); free(r->resext);
+   free(r);
+}
+
+void orc_problem_free(orc_problem* pr) {
+   region_free(pr->root);
+   free(pr->panels);
+   free(pr->precond);
+   free(pr->precond_phi);
+   free(pr->precond_dphi);
+   free(pr);
+}
+
+orc_tree* orc_problem_tree(orc_problem* pr, int which) { return which == 0 ? pr->root->tree : pr->root->child[0]->tree; }
+
+/* SurfaceOperator_multiply (SurfaceOperator.c:114-158) with
+ * joinpotentials / splitpotentials (SurfaceOperator.c:51-87). */
+static void region_multiply(orc_region* r, const double* src) {
+   int n = r->tree->npan;
+   double* phi = (double*)calloc(n, sizeof(double));
+   double* dphi = (double*)calloc(n, sizeof(double));
+   double* dest = (double*)calloc(n, sizeof(double));
+   int cur = 0;
+   for (int i = 0; i < r->mynpan; i++) {
+      phi[cur + i] = src[r->startphi + i];
+      dphi[cur + i] = -1.0 * src[r->startdphi + i];
+   }
+   cur += r->mynpan;
+   for (int c = 0; c < r->nchild; c++) {
+      orc_region* ch = r->child[c];
+      for (int i = 0; i < ch->mynpan; i++) {
+         phi[cur + i] = -1.0 * src[ch->startphi + i];
+         dphi[cur + i] = (ch->eps / r->eps) * src[ch->startdphi + i];
+      }
+      cur += ch->mynpan;
+   }
+   orc_tree_apply_both(r->tree, dphi, phi, dest);
+   for (int i = r->mynpan; i < n; i++) dest[i] += -4.0 * M_PI * phi[i];
+   memcpy(r->resint, dest, sizeof(double) * r->mynpan);
+   cur = r->mynpan;
+   for (int c = 0; c < r->nchild; c++) {
+      memcpy(r->child[c]->resext, dest + cur, sizeof(double) * r->child[c]->mynpan);
+      cur += r->child[c]->mynpan;
+   }
+   free(phi); free(dphi); free(dest);
+   for (int c = 0; c < r->nchild; c++) region_multiply(r->child[c], src);
+}
+
+static void region_join(orc_region* r, double* v) {
+   memcpy(v + r->startphi, r->resint, sizeof(double) * r->mynpan);
+   memcpy(v + r->startdphi, r->resext, sizeof(double) * r->mynpan);
+   for (int c = 0; c < r->nchild; c++) region_join(r->child[c], v);
+}
+
+static void region_clear(orc_region* r) {
+   memset(r->resint, 0, sizeof(double) * r->mynpan);
+   memset(r->resext, 0, sizeof(double) * r->mynpan);
+   for (int c = 0; c < r->nchild; c++) region_clear(r->child[c]);
+}
+
+/* SurfaceOperator_topmultiply (SurfaceOperator.c:90-107). */
+void orc_so_matvec(orc_problem* pr, const double* x, double* y) {
+   region_clear(pr->root);
+   region_multiply(pr->root, x);
+   region_join(pr->root, y);
+}
+
+/* SurfaceOperator_makeRHS (SurfaceOperator.c:291-321); rhs is accumulated. */
+static void region_rhs(orc_region* r, double* rhs) {
+   for (int i = 0; i < r->ncharge; i++) {
+      double qi = r->q[i];
+      if (qi == 0.0) continue;
+      const double* x = &r->qpos[3 * i];
+      for (int j = 0; j < r->mynpan; j++) {
+         const double* c = r->tree->panels[j].c;
+         double d = sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) + (x[2] - c[2]) * (x[2] - c[2]));
+         rhs[r->startphi + j] += qi / d;
+      }
+      for (int k = 0; k < r->nchild; k++) {
+         orc_region* ch = r->child[k];
+         for (int j = 0; j < ch->mynpan; j++) {
+            const double* c = ch->tree->panels[j].c;
+            double d = sqrt((x[0] - c[0]) * (x[0] - c[0]) + (x[1] - c[1]) * (x[1] - c[1]) + (x[2] - c[2]) * (x[2] - c[2]));
+            rhs[ch->startdphi + j] += qi / d;
+         }
+      }
+   }
+   for (int k = 0; k < r->nchild; k++) region_rhs(r->child[k], rhs);
+}
+
+void orc_rhs(orc_problem* pr, double* rhs) {
+   memset(rhs, 0, sizeof(double) * 2 * pr->ntotal);
+   region_rhs(pr->root, rhs);
+}
+
+static void leaf_diag(orc_tree* t, double* ds, double* dd) {
+   for (int i = 0; i < t->ncubes; i++) {
+      orc_cube* c = &t->cubes[i];
+      if (!c->leaf) continue;
+      for (int k = 0; k < c->npan; k++) {
+         ds[c->pan[k]] = c->Ds[(size_t)k * c->dcols + k];
+         dd[c->pan[k]] = c->Dd[(size_t)k * c->dcols + k];
+      }
+   }
+}
+
+/* Per-panel 2x2 blocks [[Dd, -Ds], [Dd, eps_r*Ds]] from the region's own
+ * tree diagonal (SurfaceOperator_generatePreconditioner,
+ * SurfaceOperator.c:170-197; Cube_extractdiagonal, Cube.c:2349-2390).  The
+ * reference factors the assembled 2N x 2N matrix with SuperLU; the blocks
+ * are disjoint so the factorisation is the per-block inverse stored here. */
+static void region_precond(orc_problem* pr, orc_region* r) {
+   if (r->mynpan > 0) {
+      double rel = r->parent ? r->eps / r->parent->eps : 1.0;
+      int n = r->tree->npan;
+      double* ds = (double*)calloc(n, sizeof(double));
+      double* dd = (double*)calloc(n, sizeof(double));
+      leaf_diag(r->tree, ds, dd);
+      for (int i = 0; i < r->mynpan; i++) {
+         int b = pr->nblocks++;
+         double a11 = dd[i], a12 = -ds[i], a21 = dd[i], a22 = rel * ds[i];
+         double det = a11 * a22 - a12 * a21;
+         pr->precond[4 * b + 0] = a22 / det;
+         pr->precond[4 * b + 1] = -a12 / det;
+         pr->precond[4 * b + 2] = -a21 / det;
+         pr->precond[4 * b + 3] = a11 / det;
+         pr->precond_phi[b] = r->startphi + i;
+         pr->precond_dphi[b] = r->startdphi + i;
+      }
+      free(ds);
+      free(dd);
+   }
+   for (int c = 0; c < r->nchild; c++) region_precond(pr, r->child[c]);
+}
+
+void orc_precond_build(orc_problem* pr) {
+   free(pr->precond); free(pr->precond_phi); free(pr->precond_dphi);
+   pr->precond = (double*)calloc(4 * (size_t)pr->ntotal, sizeof(double));
+   pr->precond_phi = (int*)calloc(pr->ntotal, sizeof(int));
+   pr->precond_dphi = (int*)calloc(pr->ntotal, sizeof(int));
+   pr->nblocks = 0;
+   region_precond(pr, pr->root);
+}
+
+void orc_precond_solve(orc_problem* pr, const double* b, double* x) {
+   int n2 = 2 * pr->ntotal;
+   double* t = (double*)malloc(sizeof(double) * n2);
+   memcpy(t, b, sizeof(double) * n2);
+   for (int k = 0; k < pr->nblocks; k++) {
+      int i = pr->precond_phi[k], j = pr->precond_dphi[k];
+      const double* M = &pr->precond[4 * k];
+      x[i] = M[0] * t[i] + M[1] * t[j];
+      x[j] = M[2] * t[i] + M[3] * t[j];
+   }
+   free(t);
+}
+
+static double vdot(const double* a, const double* b, int n) {
+   double s = 0;
+   for (int i = 0; i < n; i++) s += a[i] * b[i];
+   return s;
+}
+
+static void givens(double a, double b, double* c, double* s) {
+   if (b == 0.0) {
+      *c = 1.0;
+      *s = 0.0;
+   } else {
+      double r1 = fabs(a), r2 = fabs(b);
+      double L = sqrt(r1 * r1 + r2 * r2);
+      *c = a / L;
+      *s = -b / L;
+   }
+}
+
+static void grot(double c, double s, double* a, double* b) {
+   double a1 = *a, b1 = *b;
+   *a = a1 * c - b1 * s;
+   *b = c * b1 + s * a1;
+}
+
+/* Left-preconditioned full GMRES with MGS and Givens rotations
+ * (GMRES_SurfaceOperator, GMRES.c:446-539; givens/givensrotate,
+ * GMRES.c:3-22).  Returns the iteration count (number of Arnoldi steps);
+ * resid_hist (maxit entries, optional) receives |g_{i+1}|/||P^-1 b||. */
+int orc_gmres(orc_problem* pr, const double* rhs, double* sol, double tol, int maxit, double* hist) {
+   int n = 2 * pr->ntotal;
+   double* r = (double*)calloc(n, sizeof(double));
+   orc_precond_solve(pr, rhs, r);
+   double normr = sqrt(vdot(r, r, n));
+   if (normr < tol) {
+      memset(sol, 0, sizeof(double) * n);
+      free(r);
+      return 0;
+   }
+   double* c = (double*)calloc(maxit + 1, sizeof(double));
+   double* s = (double*)calloc(maxit + 1, sizeof(double));
+   double* g = (double*)calloc(maxit + 1, sizeof(double));
+   double* y = (double*)calloc(maxit + 1, sizeof(double));
+   double* H = (double*)calloc((size_t)(maxit + 1) * (maxit + 1), sizeof(double));
+   double** bv = (double**)calloc(maxit + 2, sizeof(double*));
+   double* P = (double*)calloc(n, sizeof(double));
+   g[0] = sqrt(vdot(r, r, n));
+   bv[0] = (double*)malloc(sizeof(double) * n);
+   for (int k = 0; k < n; k++) bv[0][k] = r[k] * (1.0 / g[0]);
+   int i;
+   int W = maxit + 1;
+   for (i = 0; i < maxit; i++) {
+      orc_so_matvec(pr, bv[i], P);
+      orc_precond_solve(pr, P, P);
+      for (int j = 0; j <= i; j++) {
+         H[(size_t)i * W + j] = vdot(P, bv[j], n);
+         double h = H[(size_t)i * W + j];
+         for (int k = 0; k < n; k++) P[k] -= h * bv[j][k];
+      }
+      H[(size_t)i * W + i + 1] = sqrt(vdot(P, P, n));
+      bv[i + 1] = (double*)malloc(sizeof(double) * n);
+      double inv = 1.0 / H[(size_t)i * W + i + 1];
+      for (int k = 0; k < n; k++) bv[i + 1][k] = P[k] * inv;
+      for (int k = 0; k < i; k++) grot(c[k], s[k], &H[(size_t)i * W + k], &H[(size_t)i * W + k + 1]);
+      givens(H[(size_t)i * W + i], H[(size_t)i * W + i + 1], &c[i], &s[i]);
+      grot(c[i], s[i], &H[(size_t)i * W + i], &H[(size_t)i * W + i + 1]);
+      g[i + 1] = 0.0;
+      grot(c[i], s[i], &g[i], &g[i + 1]);
+      double res = fabs(g[i + 1]) / normr;
+      if (hist) hist[i] = res;
+      if (res < tol) break;
+   }
+   if (i == maxit) i = maxit - 1;
+   for (int k = 0; k <= i; k++) y[k] = g[k];
+   for (int k = i; k >= 0; k--) {
+      y[k] /= H[(size_t)k * W + k];
+      for (int j = k - 1; j >= 0; j--) y[j] -= H[(size_t)k * W + j] * y[k];
+   }
+   memset(sol, 0, sizeof(double) * n);
+   for (int j = 0; j <= i; j++)
+      for (int k = 0; k < n; k++) sol[k] += y[j] * bv[j][k];
+   for (int j = 0; j <= i + 1; j++) free(bv[j]);
+   free(bv); free(r); free(P); free(c); free(s); free(g); free(y); free(H);
+   return i + 1;
+}
+
+/* Direct reaction potentials at the charges (SurfaceOperator_calculateReactionPotentials
+ * direct branch, SurfaceOperator.c:364-396); phi is accumulated per charge. */
+static void region_react(orc_region* r, const double* sol, double* phi) {
+   if (r->ncharge > 0) {
+      for (int j = 0; j < r->ncharge; j++) {
+         const double* x = &r->qpos[3 * j];
+         for (int i = 0; i < r->mynpan; i++) {
+            double sl, dl;
+            orc_laplace(x, &r->tree->panels[i], &sl, &dl);
+            phi[j] += (sol[r->startdphi + i] * sl - sol[r->startphi + i] * dl);
+            const double* c = r->tree->panels[i].c;
+            if (x[0] == c[0] && x[1] == c[1] && x[2] == c[2]) phi[j] -= 4.0 * M_PI * sol[r->startphi + i];
+         }
+      }
+      for (int k = 0; k < r->nchild; k++) {
+         orc_region* ch = r->child[k];
+         for (int i = 0; i < ch->mynpan; i++)
+            for (int j = 0; j < r->ncharge; j++) {
+               double sl, dl;
+               orc_laplace(&r->qpos[3 * j], &ch->tree->panels[i], &sl, &dl);
+               phi[j] += (-ch->eps / r->eps * sol[ch->startdphi + i] * sl + sol[ch->startphi + i] * dl);
+            }
+      }
+   }
+   for (int k = 0; k < r->nchild; k++) region_react(r->child[k], sol, phi);
+}
+
+void orc_reaction_potentials(orc_problem* pr, const double* sol, double* phi) {
+   int nq = pr->root->child[0]->ncharge;
+   memset(phi, 0, sizeof(double) * nq);
+   region_react(pr->root, sol, phi);
+}
