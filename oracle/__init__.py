@@ -32,6 +32,7 @@ def _load():
         "orc_integrate_batch": (None, [C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
         "orc_panel_geom": (None, [_dp, _dp]),
         "orc_gl4": (None, [_dp, _dp]),
+        "orc_yukawa_rule_v9": (None, [_dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
         "orc_dft_r2c": (None, [C.c_int, _dp, _dp]),
         "orc_dft_c2r": (None, [C.c_int, _dp, _dp]),
         "orc_pinv": (None, [_dp, C.c_int, C.c_int, _dp]),
@@ -102,6 +103,16 @@ def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):
     d = np.zeros((len(p), len(v)))
     lib().orc_integrate_batch(kernel, kappa, _d(v), len(v), _d(p), len(p), _d(s), _d(d))
     return s, d
+
+
+def yukawa_rule(x, v9, kappa, nodes, weights):
+    """Yukawa (slp, dlp) by the reference's edge/radial scheme with a given angular rule."""
+    s = C.c_double()
+    d = C.c_double()
+    xq = f64(nodes)
+    wq = f64(weights)
+    lib().orc_yukawa_rule_v9(_d(f64(x)), _d(f64(v9)), kappa, len(xq), _d(xq), _d(wq), C.byref(s), C.byref(d))
+    return s.value, d.value
 
 
 def panel_geom(v9):
@@ -194,7 +205,8 @@ class Tree:
                             dcols=dcols))
         return out
 
-    def cube_mats(self, cube, gp3):
+    def cube_mats(self, cube):
+        gp3 = int(self.gpl()[cube["level"]]) ** 3
         drows, dcols = len(cube["pt"]), cube["dcols"]
         Ds = np.zeros(max(drows * dcols, 1))
         Dd = np.zeros(max(drows * dcols, 1))

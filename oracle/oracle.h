@@ -47,6 +47,10 @@ typedef struct {
 void orc_panel_init(orc_panel* p, const double* v0, const double* v1, const double* v2);
 void orc_laplace(const double* x, const orc_panel* p, double* slp, double* dlp);
 void orc_yukawa(const double* x, const orc_panel* p, double kappa, double* slp, double* dlp);
+void orc_yukawa_rule(const double* x, const orc_panel* p, double kappa, int nq, const double* gx, const double* gw,
+                     double* slp, double* dlp);
+void orc_yukawa_rule_v9(const double* x, const double* v9, double kappa, int nq, const double* gx, const double* gw,
+                        double* slp, double* dlp);
 void orc_integrate(int kernel, double kappa, const double* x, const orc_panel* p, double* slp, double* dlp);
 double orc_green(int kernel, double kappa, const double* dest, const double* src);
 void orc_gl4(double* x, double* w);

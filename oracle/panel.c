@@ -250,8 +250,8 @@ void orc_gl4(double* x, double* w) {
 /* Radial line integral from the foot of the perpendicular, with the
  * angular range split geometrically (calcp_LogIntDoLine,
  * FlatIntegration.c:473-635); HELMHOLTZ kernel only, both layers. */
-static void yuk_line(double z, double x, double y1, double y2, double kappa, const double* gx, const double* gw,
-                     double* sl, double* dl) {
+static void yuk_line(double z, double x, double y1, double y2, double kappa, int nq, const double* gx,
+                     const double* gw, double* sl, double* dl) {
    double az = z > 0.0 ? z : -z;
    if (x < 0) x = -x;
    double start = y1, end;
@@ -265,7 +265,7 @@ static void yuk_line(double z, double x, double y1, double y2, double kappa, con
    while (start < y2) {
       double ph1 = atan2(start, x), ph2 = atan2(end, x);
       double dph = ph2 - ph1;
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < nq; i++) {
          double th = ph1 + (ph2 - ph1) * gx[i];
          double q = x * (1.0 / cos(th));
          double R = sqrt(q * q + z * z);
@@ -290,7 +290,7 @@ static void yuk_line(double z, double x, double y1, double y2, double kappa, con
  * reference applies when the edge-origin plane is not already the xy plane
  * is a rigid motion of points that already lie in the panel plane; it does
  * not change the integral and is omitted. */
-static void yuk_edge(const double* a, const double* b, const double* pnt, double kappa, const double* gx,
+static void yuk_edge(const double* a, const double* b, const double* pnt, double kappa, int nq, const double* gx,
                      const double* gw, double* sl, double* dl) {
    double u[2] = {a[0] - pnt[0], a[1] - pnt[1]};
    double w[2] = {b[0] - pnt[0], b[1] - pnt[1]};
@@ -316,10 +316,10 @@ static void yuk_edge(const double* a, const double* b, const double* pnt, double
       y2 = t;
    }
    double st = (y1 >= 0) ? y1 : 0;
-   yuk_line(z, ux, st, y2, kappa, gx, gw, sl, dl);
+   yuk_line(z, ux, st, y2, kappa, nq, gx, gw, sl, dl);
    if (y1 < 0.0) {
       double s2, d2;
-      yuk_line(z, ux, 0.0, -y1, kappa, gx, gw, &s2, &d2);
+      yuk_line(z, ux, 0.0, -y1, kappa, nq, gx, gw, &s2, &d2);
       *sl += s2;
       *dl += d2;
    }
@@ -334,6 +334,13 @@ void orc_yukawa(const double* x, const orc_panel* p, double kappa, double* slp, 
    }
    double gx[4], gw[4];
    orc_gl4(gx, gw);
+   orc_yukawa_rule(x, p, kappa, 4, gx, gw, slp, dlp);
+}
+
+/* The same scheme with an arbitrary angular rule (nodes on [0,1], weights
+ * summing to one); the reference fixes the 4-point Gauss-Legendre rule. */
+void orc_yukawa_rule(const double* x, const orc_panel* p, double kappa, int nq, const double* gx, const double* gw,
+                     double* slp, double* dlp) {
    double r0[3], pnt[3];
    sub3(r0, x, p->c);
    pnt[0] = dot3(p->axn[0], r0);
@@ -351,7 +358,7 @@ void orc_yukawa(const double* x, const orc_panel* p, double kappa, double* slp, 
       } else
          sg = -1.0;
       double sl, dl;
-      yuk_edge(p->pvn[j], p->pvn[nx], pnt, kappa, gx, gw, &sl, &dl);
+      yuk_edge(p->pvn[j], p->pvn[nx], pnt, kappa, nq, gx, gw, &sl, &dl);
       S += sg * sl;
       D += sg * dl;
    }
@@ -461,3 +468,10 @@ int orc_read_vertface(const char* vertfile, const char* facefile, int flip, int 
 }
 
 void orc_free(void* p) { free(p); }
+
+void orc_yukawa_rule_v9(const double* x, const double* v9, double kappa, int nq, const double* gx, const double* gw,
+                        double* slp, double* dlp) {
+   orc_panel p;
+   orc_panel_init(&p, v9, v9 + 3, v9 + 6);
+   orc_yukawa_rule(x, &p, kappa, nq, gx, gw, slp, dlp);
+}
