@@ -1,0 +1,251 @@
+"""MI355X-native FFT-SVD Poisson-Boltzmann boundary-element solver.
+
+Python host mirror of the reference's solve path (FFTSVDpbeAPI.h) over the
+C-ABI library ``_lib/libfftsvd_hip.so`` (hand-written gfx950 HIP kernels).
+There is no CPU fallback: every call goes to the GPU through the library,
+and loading fails loudly if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfftsvd_hip.so")
+HEADERS = [os.path.join(os.path.dirname(HERE), "include", h) for h in ("FFTSVDpbeAPI.h", "fftsvd_hip.h")]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+F64, MIXED = 0, 1
+LAPLACE, YUKAWA = 0, 1
+
+_SIGS = {
+    "fftsvd_last_error": (C.c_char_p, []),
+    "fftsvd_device_info": (C.c_int, [C.c_char_p, C.c_int, _ip]),
+    "fftsvd_set_precision": (C.c_int, [C.c_int]),
+    "fftsvd_problem_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "fftsvd_problem_create_multi": (C.c_int, [C.c_int, _ip, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double,
+                                              C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                              C.POINTER(_vp)]),
+    "fftsvd_problem_destroy": (C.c_int, [_vp]),
+    "fftsvd_problem_size": (C.c_int, [_vp, _ip, _ip]),
+    "fftsvd_problem_matvec": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_problem_precond": (C.c_int, [_vp, _dp, _dp]),
+    "fftsvd_problem_rhs": (C.c_int, [_vp, _dp]),
+    "fftsvd_problem_solve": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _dp]),
+    "fftsvd_problem_potentials": (C.c_int, [_vp, _dp, _dp]),
+    "fftsvd_problem_stats": (C.c_int, [_vp, _dp, C.c_int]),
+    "fftsvd_problem_bench_matvec": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "fftsvd_tree_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
+    "fftsvd_tree_cube_info": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
+    "fftsvd_tree_cube_panels": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip]),
+    "fftsvd_tree_cube_mats": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "fftsvd_tree_apply": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "fftsvd_tree_apply_transpose": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "fftsvd_integrate": (C.c_int, [C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+}
+
+_lib = None
+
+
+class FFTSVDError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FFTSVDError("libfftsvd_hip.so not built (run python -m fftsvd_amd.build): %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FFTSVDError(lib().fftsvd_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    _check(lib().fftsvd_device_info(name, 256, C.byref(cu)))
+    return name.value.decode(), cu.value
+
+
+def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):
+    """Panel integrals (slp, dlp) on the GPU, arrays (npt, npan)."""
+    v = _f64(verts9).reshape(-1, 9)
+    p = _f64(pts).reshape(-1, 3)
+    s = np.zeros((len(p), len(v)))
+    d = np.zeros((len(p), len(v)))
+    _check(lib().fftsvd_integrate(kernel, kappa, _d(v), len(v), _d(p), len(p), _d(s), _d(d)))
+    return s, d
+
+
+class Tree:
+    """One region's compressed operator (view)."""
+
+    def __init__(self, prob, which):
+        self.p = prob
+        self.which = which
+        depth, nc, npan = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().fftsvd_tree_info(prob.h, which, C.byref(depth), C.byref(nc), C.byref(npan)))
+        self.depth, self.ncubes, self.n = depth.value, nc.value, npan.value
+
+    def cubes(self):
+        out = []
+        for c in range(self.ncubes):
+            info = np.zeros(13, dtype=np.int32)
+            _check(lib().fftsvd_tree_cube_info(self.p.h, self.which, c, _i(info)))
+            lvl, i, j, k, leaf, npan, npt, nloc, nint, rr, cr, parent, dcols = [int(v) for v in info]
+            pan = np.zeros(max(npan, 1), dtype=np.int32)
+            dc = np.zeros(max(dcols, 1), dtype=np.int32)
+            _check(lib().fftsvd_tree_cube_panels(self.p.h, self.which, c, _i(pan), _i(dc)))
+            out.append(dict(id=c, level=lvl, idx=(i, j, k), leaf=leaf, npan=npan, npt=npt, nloc=nloc, nint=nint,
+                            rowrank=rr, colrank=cr, parent=parent, pan=pan[:npan].copy(), dcols=dc[:dcols].copy()))
+        return out
+
+    def cube_mats(self, cube, gp3):
+        np_, r, rr = cube["npan"], cube["rowrank"], cube["colrank"]
+        m = len(cube["dcols"])
+        VT = np.zeros(max(r * np_, 1))
+        U = np.zeros(max(np_ * rr, 1))
+        PVs = np.zeros(max(gp3 * r, 1))
+        PVd = np.zeros(max(gp3 * r, 1))
+        UTI = np.zeros(max(gp3 * rr, 1))
+        Ds = np.zeros(max(np_ * m, 1))
+        Dd = np.zeros(max(np_ * m, 1))
+        _check(lib().fftsvd_tree_cube_mats(self.p.h, self.which, cube["id"], _d(VT), _d(U), _d(PVs), _d(PVd), _d(UTI),
+                                           _d(Ds), _d(Dd)))
+        return dict(VT=VT[:r * np_].reshape(r, np_), U=U[:np_ * rr].reshape(np_, rr),
+                    PVs=PVs[:gp3 * r].reshape(gp3, r), PVd=PVd[:gp3 * r].reshape(gp3, r),
+                    UTI=UTI[:gp3 * rr].reshape(rr, gp3), Ds=Ds[:np_ * m].reshape(np_, m),
+                    Dd=Dd[:np_ * m].reshape(np_, m))
+
+    def apply_both(self, xs, xd, precision=F64):
+        y = np.zeros(self.n)
+        _check(lib().fftsvd_tree_apply(self.p.h, self.which, precision, _d(_f64(xs)), _d(_f64(xd)), _d(y)))
+        return y
+
+    def apply_transpose(self, layer, x, precision=F64):
+        y = np.zeros(self.n)
+        _check(lib().fftsvd_tree_apply_transpose(self.p.h, self.which, precision, layer, _d(_f64(x)), _d(y)))
+        return y
+
+
+class Problem:
+    """Region operator of one dielectric surface (or several, with cavities)
+    and its charges, device resident.  Mirrors the reference flow
+    generateSurfaceOperator -> generateSurfaceOperatorPreconditioner ->
+    generateRHS -> solveSurfaceOperator -> collectPotentials."""
+
+    def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, kappa=0.0, maxpanels=32, gridpoints=4,
+                 svdeps=1e-5, want_f32=True, cavities=None):
+        v = _f64(verts9).reshape(-1, 9)
+        self.qpos = _f64(qpos).reshape(-1, 3)
+        self.q = _f64(q).reshape(-1)
+        self.eps_in = eps_in
+        h = _vp()
+        if cavities:
+            cv = [_f64(c).reshape(-1, 9) for c in cavities]
+            allv = np.ascontiguousarray(np.vstack([v] + cv))
+            nd = np.array([len(v)], dtype=np.int32)
+            nc = np.array([len(c) for c in cv], dtype=np.int32)
+            par = np.zeros(len(cv), dtype=np.int32)
+            _check(lib().fftsvd_problem_create_multi(1, _i(nd), len(cv), _i(nc), _i(par), _d(allv), _d(self.qpos),
+                                                     _d(self.q), len(self.q), eps_in, eps_out, kappa, maxpanels,
+                                                     gridpoints, svdeps, int(want_f32), C.byref(h)))
+            self.n = len(allv)
+        else:
+            _check(lib().fftsvd_problem_create(_d(v), len(v), _d(self.qpos), _d(self.q), len(self.q), eps_in, eps_out,
+                                               kappa, maxpanels, gridpoints, svdeps, int(want_f32), C.byref(h)))
+            self.n = len(v)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fftsvd_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tree(self, which):
+        return Tree(self, which)
+
+    def matvec(self, x, precision=F64):
+        y = np.zeros(2 * self.n)
+        _check(lib().fftsvd_problem_matvec(self.h, precision, _d(_f64(x)), _d(y)))
+        return y
+
+    def rhs(self):
+        r = np.zeros(2 * self.n)
+        _check(lib().fftsvd_problem_rhs(self.h, _d(r)))
+        return r
+
+    def precond_solve(self, b):
+        x = np.zeros(2 * self.n)
+        _check(lib().fftsvd_problem_precond(self.h, _d(_f64(b)), _d(x)))
+        return x
+
+    def solve(self, rhs, tol=1e-6, maxit=1000, precision=F64):
+        """Returns (solution, iterations, residual history, info dict)."""
+        sol = np.zeros(2 * self.n)
+        it = C.c_int()
+        hist = np.zeros(maxit)
+        info = np.zeros(6)
+        _check(lib().fftsvd_problem_solve(self.h, precision, _d(_f64(rhs)), _d(sol), tol, maxit, C.byref(it),
+                                          _d(hist), _d(info)))
+        keys = ("iterations", "refinements", "final_residual", "ms", "matvecs64", "matvecs32")
+        return sol, it.value, hist[:max(0, min(it.value, maxit))], dict(zip(keys, info.tolist()))
+
+    def gmres(self, rhs, tol=1e-6, maxit=1000, precision=F64):
+        sol, it, hist, _ = self.solve(rhs, tol, maxit, precision)
+        return sol, it, hist
+
+    def reaction_potentials(self, sol):
+        phi = np.zeros(max(len(self.q), 1))
+        _check(lib().fftsvd_problem_potentials(self.h, _d(_f64(sol)), _d(phi)))
+        return phi[:len(self.q)]
+
+    def energy(self, sol, conversion=332.112):
+        """Delta G = 1/2 sum q phi * conversion / eps_in (reference FFTSVDsalt.c:166-172)."""
+        return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in
+
+    def stats(self):
+        out = np.zeros(16)
+        _check(lib().fftsvd_problem_stats(self.h, _d(out), 16))
+        keys = ("panels", "regions", "setup_ms", "near_entries", "far_entries", "spectra_bins", "translate_macs",
+                "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32")
+        return dict(zip(keys, out.tolist()))
+
+    def bench_matvec(self, precision=F64, nrep=20):
+        tot = C.c_double()
+        parts = np.zeros(4)
+        _check(lib().fftsvd_problem_bench_matvec(self.h, precision, nrep, C.byref(tot), _d(parts)))
+        return tot.value / nrep, dict(zip(("near", "far_source", "translate", "far_target"), parts.tolist()))

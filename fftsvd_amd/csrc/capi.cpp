@@ -1,0 +1,783 @@
+// C-ABI of libfftsvd_hip.so: the reference's FFTSVDpbeAPI.h solve path and
+// the extended array-based entry points of fftsvd_hip.h.
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+
+#include "../../include/FFTSVDpbeAPI.h"
+#include "../../include/fftsvd_hip.h"
+#include "solver.h"
+
+namespace fsvd {
+void launch_integrate_all(int kernel, double kappa, const PanelGeom* panels, int npan, const double* pts, int npt,
+                          double* slp, double* dlp, hipStream_t s);
+}
+
+using namespace fsvd;
+
+struct fftsvd_problem {
+   std::unique_ptr<Problem> pr;
+};
+
+static thread_local std::string g_err;
+static int g_precision = 0;
+
+#define API_TRY try {
+#define API_CATCH                                 \
+   }                                              \
+   catch (const std::exception& e) {              \
+      g_err = e.what();                           \
+      return -1;                                  \
+   }                                              \
+   catch (...) {                                  \
+      g_err = "unknown error";                    \
+      return -1;                                  \
+   }                                              \
+   return 0;
+
+extern "C" {
+
+/* Parameters (reference FFTSVDpbeAPI.c:18-34) */
+real dielectricvertexdensity = 0.0;
+real saltvertexdensity = 0.0;
+real innerdielectric = 0.0;
+real outerdielectric = 0.0;
+real ionicstrength = 0.0;
+real ionexclusionradius = 0.0;
+real proberadius = 0.0;
+real errortolerance = 0.0;
+char energycalculation = '\0';
+unsigned int outputmodpdb = 0;
+unsigned int outputfrc = 0;
+unsigned int addcoulombic = 0;
+unsigned int accelerateM3 = 0;
+unsigned int usequalocation = 0;
+unsigned int usecurved = 0;
+real inftydielectric = 0.0;
+real lambda = 0.0;
+
+const char* fftsvd_last_error(void) { return g_err.c_str(); }
+
+int fftsvd_set_precision(int precision) {
+   if (precision != 0 && precision != 1) {
+      g_err = "precision must be 0 (FP64) or 1 (mixed)";
+      return -1;
+   }
+   g_precision = precision;
+   return 0;
+}
+
+int fftsvd_device_info(char* name, int len, int* num_cu) {
+   API_TRY
+   int dev = 0;
+   FSVD_CHECK(hipGetDevice(&dev));
+   hipDeviceProp_t p;
+   FSVD_CHECK(hipGetDeviceProperties(&p, dev));
+   if (name && len > 0) {
+      std::snprintf(name, len, "%s (%s)", p.name, p.gcnArchName);
+   }
+   if (num_cu) *num_cu = p.multiProcessorCount;
+   API_CATCH
+}
+
+static SurfaceSpec make_spec(int ndiel, const int* npan_diel, int ncav, const int* npan_cav, const int* cav_parent,
+                             const double* verts9, const double* qpos, const double* q, int ncharge, double eps_in,
+                             double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
+                             int want_f32) {
+   SurfaceSpec sp;
+   size_t off = 0;
+   for (int i = 0; i < ndiel; i++) {
+      sp.dielectric_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_diel[i]);
+      off += 9 * (size_t)npan_diel[i];
+   }
+   for (int j = 0; j < ncav; j++) {
+      sp.cavity_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_cav[j]);
+      off += 9 * (size_t)npan_cav[j];
+      sp.cavity_parent.push_back(cav_parent[j]);
+   }
+   sp.qpos.assign(qpos, qpos + 3 * (size_t)ncharge);
+   sp.q.assign(q, q + ncharge);
+   sp.eps_in = eps_in;
+   sp.eps_out = eps_out;
+   sp.kappa = kappa;
+   sp.maxpanels = maxpanels;
+   sp.gridpoints = gridpoints;
+   sp.svdeps = svdeps;
+   sp.want32 = want_f32 != 0;
+   return sp;
+}
+
+int fftsvd_problem_create_multi(int ndiel, const int* npan_diel, int ncav, const int* npan_cav, const int* cav_parent,
+                                const double* verts9, const double* qpos, const double* q, int ncharge, double eps_in,
+                                double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
+                                int want_f32, fftsvd_problem** out) {
+   API_TRY
+   if (gridpoints != 4) throw std::runtime_error("only 4 grid points per cube edge (49-point sphere) are supported");
+   if (ndiel < 1) throw std::runtime_error("need at least one dielectric surface");
+   auto sp = make_spec(ndiel, npan_diel, ncav, npan_cav, cav_parent, verts9, qpos, q, ncharge, eps_in, eps_out, kappa,
+                       maxpanels, gridpoints, svdeps, want_f32);
+   auto* h = new fftsvd_problem;
+   h->pr = build_problem(sp);
+   *out = h;
+   API_CATCH
+}
+
+int fftsvd_problem_create(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
+                          double eps_in, double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
+                          int want_f32, fftsvd_problem** out) {
+   return fftsvd_problem_create_multi(1, &npan, 0, nullptr, nullptr, verts9, qpos, q, ncharge, eps_in, eps_out, kappa,
+                                      maxpanels, gridpoints, svdeps, want_f32, out);
+}
+
+int fftsvd_problem_destroy(fftsvd_problem* p) {
+   API_TRY
+   delete p;
+   API_CATCH
+}
+
+int fftsvd_problem_size(fftsvd_problem* p, int* npanels, int* ncharges) {
+   API_TRY
+   if (npanels) *npanels = p->pr->ntotal;
+   if (ncharges) *ncharges = (int)p->pr->q.size();
+   API_CATCH
+}
+
+int fftsvd_problem_matvec(fftsvd_problem* p, int precision, const double* x, double* y) {
+   API_TRY
+   Problem& pr = *p->pr;
+   size_t n = 2 * (size_t)pr.ntotal;
+   DBuf<double> dx, dy;
+   dx.upload(x, n, pr.stream);
+   dy.alloc(n);
+   problem_matvec(pr, precision, dx.p, dy.p);
+   FSVD_CHECK(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_precond(fftsvd_problem* p, const double* b, double* x) {
+   API_TRY
+   Problem& pr = *p->pr;
+   size_t n = 2 * (size_t)pr.ntotal;
+   DBuf<double> db, dx;
+   db.upload(b, n, pr.stream);
+   dx.alloc(n);
+   problem_precond(pr, db.p, dx.p);
+   FSVD_CHECK(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_rhs(fftsvd_problem* p, double* rhs) {
+   API_TRY
+   Problem& pr = *p->pr;
+   size_t n = 2 * (size_t)pr.ntotal;
+   DBuf<double> d;
+   d.alloc(n);
+   problem_rhs(pr, d.p);
+   FSVD_CHECK(hipMemcpyAsync(rhs, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, double* sol, double tol, int maxit,
+                         int* iters, double* resid_hist, double* info) {
+   API_TRY
+   Problem& pr = *p->pr;
+   size_t n = 2 * (size_t)pr.ntotal;
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   DBuf<double> db, dx;
+   db.upload(rhs, n, pr.stream);
+   dx.alloc(n);
+   std::vector<double> hist;
+   SolveInfo si = problem_solve(pr, precision, db.p, dx.p, tol, maxit, &hist);
+   FSVD_CHECK(hipMemcpyAsync(sol, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   if (iters) *iters = si.iterations;
+   if (resid_hist)
+      for (size_t k = 0; k < hist.size() && (int)k < maxit; k++) resid_hist[k] = hist[k];
+   if (info) {
+      info[0] = si.iterations;
+      info[1] = si.refinements;
+      info[2] = si.final_residual;
+      info[3] = si.ms;
+      info[4] = si.matvecs64;
+      info[5] = si.matvecs32;
+   }
+   API_CATCH
+}
+
+int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi) {
+   API_TRY
+   problem_potentials(*p->pr, sol, phi);
+   API_CATCH
+}
+
+static TreeOp& region_op(Problem& pr, int which) {
+   if (which < 0 || which >= (int)pr.regions.size()) throw std::runtime_error("no such region");
+   Region& R = pr.regions[which];
+   return R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+}
+
+/* out: 0 panels, 1 regions, 2 setup ms, 3 near entries (all trees), 4 far
+ * entries, 5 spectra bins, 6 translate complex MACs, 7 max depth,
+ * 8 leaves, 9 unique trees, 10 algorithmic bytes per FP64 matvec,
+ * 11 algorithmic bytes per FP32 matvec, 12 near bytes per FP64 matvec,
+ * 13 near bytes per FP32 matvec */
+int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
+   API_TRY
+   Problem& pr = *p->pr;
+   double v[16] = {0};
+   v[0] = pr.ntotal;
+   v[1] = (double)pr.regions.size();
+   v[2] = pr.setup_ms;
+   double glue = 0;
+   for (Region& R : pr.regions) {
+      int n = (int)R.panel_ids.size();
+      glue += 8.0 * (2 * n /*gather*/ + 3 * n /*split*/);
+      if (R.shares_tree_with >= 0 || !R.op) {
+         if (R.shares_tree_with >= 0) {
+            TreeOp& op = *pr.regions[R.shares_tree_with].op;
+            v[3] += op.stats.near_entries;
+            v[4] += op.stats.far_entries;
+            v[5] += op.stats.spectra_bins;
+            v[6] += op.stats.translate_macs;
+         }
+         continue;
+      }
+      TreeOp& op = *R.op;
+      v[3] += op.stats.near_entries;
+      v[4] += op.stats.far_entries;
+      v[5] += op.stats.spectra_bins;
+      v[6] += op.stats.translate_macs;
+      v[7] = std::max<double>(v[7], op.stats.depth);
+      v[8] += op.stats.nleaf;
+      v[9] += 1;
+   }
+   // algorithmic bytes: stored operator entries once, spectra written and
+   // read once, vectors (2 gathers + 1 write per point per tree) and glue
+   double vec = 0;
+   for (Region& R : pr.regions) vec += 8.0 * 3 * R.panel_ids.size();
+   v[10] = 8.0 * (v[3] + v[4]) + 16.0 * 2 * v[5] + vec + glue;
+   v[11] = 4.0 * (v[3] + v[4]) + 8.0 * 2 * v[5] + vec + glue;
+   v[12] = 8.0 * v[3];
+   v[13] = 4.0 * v[3];
+   for (int i = 0; i < nout && i < 16; i++) out[i] = v[i];
+   API_CATCH
+}
+
+int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, double* ms_total, double* ms_parts) {
+   API_TRY
+   Problem& pr = *p->pr;
+   hipStream_t s = pr.stream;
+   size_t n = 2 * (size_t)pr.ntotal;
+   std::vector<double> hx(n);
+   std::mt19937_64 rng(12345);
+   std::uniform_real_distribution<double> U(-1.0, 1.0);
+   for (auto& v : hx) v = U(rng);
+   DBuf<double> dx, dy;
+   dx.upload(hx, s);
+   dy.alloc(n);
+   for (int w = 0; w < 2; w++) problem_matvec(pr, precision, dx.p, dy.p);
+   hipEvent_t e0, e1;
+   FSVD_CHECK(hipEventCreate(&e0));
+   FSVD_CHECK(hipEventCreate(&e1));
+   FSVD_CHECK(hipEventRecord(e0, s));
+   for (int r = 0; r < nrep; r++) problem_matvec(pr, precision, dx.p, dy.p);
+   FSVD_CHECK(hipEventRecord(e1, s));
+   FSVD_CHECK(hipEventSynchronize(e1));
+   float ms = 0;
+   FSVD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+   if (ms_total) *ms_total = ms;
+   if (ms_parts) {
+      // per class: sum over trees of one pass, averaged over nrep passes
+      for (int part = 0; part < 4; part++) {
+         FSVD_CHECK(hipEventRecord(e0, s));
+         for (int r = 0; r < nrep; r++)
+            for (Region& R : pr.regions) {
+               if ((int)R.panel_ids.size() == 0) continue;
+               TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+               tree_apply_part(op, precision, part, R.xs.p, R.xd.p, R.yt.p, s);
+            }
+         FSVD_CHECK(hipEventRecord(e1, s));
+         FSVD_CHECK(hipEventSynchronize(e1));
+         float m = 0;
+         FSVD_CHECK(hipEventElapsedTime(&m, e0, e1));
+         ms_parts[part] = m / nrep;
+      }
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   API_CATCH
+}
+
+int fftsvd_tree_info(fftsvd_problem* p, int which, int* depth, int* ncubes, int* npanels) {
+   API_TRY
+   TreeOp& op = region_op(*p->pr, which);
+   if (depth) *depth = op.tree.depth;
+   if (ncubes) *ncubes = (int)op.tree.cubes.size();
+   if (npanels) *npanels = op.tree.npan;
+   API_CATCH
+}
+
+int fftsvd_tree_cube_info(fftsvd_problem* p, int which, int cube, int* info) {
+   API_TRY
+   TreeOp& op = region_op(*p->pr, which);
+   const Cube& c = op.tree.cubes.at(cube);
+   int dcols = 0;
+   if (op.cube_leaf[cube] >= 0) dcols = op.h_leaves[op.cube_leaf[cube]].m;
+   int v[13] = {c.level, c.idx[0], c.idx[1], c.idx[2], c.leaf, c.np, c.nq, (int)c.loc.size(), (int)c.itr.size(),
+                c.rowrank, c.colrank, c.parent, dcols};
+   std::memcpy(info, v, sizeof(v));
+   API_CATCH
+}
+
+int fftsvd_tree_cube_panels(fftsvd_problem* p, int which, int cube, int* pan, int* dcols) {
+   API_TRY
+   TreeOp& op = region_op(*p->pr, which);
+   const Cube& c = op.tree.cubes.at(cube);
+   if (pan)
+      for (int k = 0; k < c.np; k++) pan[k] = op.tree.pan_perm[c.p0 + k];
+   if (dcols && op.cube_leaf[cube] >= 0) {
+      const LeafDesc& L = op.h_leaves[op.cube_leaf[cube]];
+      for (int k = 0; k < L.m; k++) dcols[k] = op.tree.pan_perm[op.h_dcol[L.col_off + k]];
+   }
+   API_CATCH
+}
+
+int fftsvd_tree_cube_mats(fftsvd_problem* p, int which, int cube, double* VT, double* U, double* PVs, double* PVd,
+                          double* UTI, double* Ds, double* Dd) {
+   API_TRY
+   Problem& pr = *p->pr;
+   TreeOp& op = region_op(pr, which);
+   const Cube& c = op.tree.cubes.at(cube);
+   hipStream_t s = pr.stream;
+   auto get = [&](const double* src, size_t n) {
+      std::vector<double> h(n);
+      if (n) FSVD_CHECK(hipMemcpyAsync(h.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      return h;
+   };
+   if (c.level >= 2) {
+      FarLevel& lv = op.levels[c.level - 2];
+      int gp3 = lv.gp3;
+      if (op.cube_src_slot[cube] >= 0) {
+         const SrcDesc& sd = lv.h_src[op.cube_src_slot[cube]];
+         auto vt = get(lv.VT64.p + sd.vt_off, (size_t)sd.r * sd.np);
+         auto pv = get(lv.PV64.p + sd.pv_off, 2 * (size_t)sd.r * gp3);
+         if (VT) std::memcpy(VT, vt.data(), sizeof(double) * vt.size());
+         for (int g = 0; g < gp3; g++)
+            for (int k = 0; k < sd.r; k++) {
+               if (PVs) PVs[(size_t)g * sd.r + k] = pv[(size_t)k * gp3 + g];
+               if (PVd) PVd[(size_t)g * sd.r + k] = pv[(size_t)sd.r * gp3 + (size_t)k * gp3 + g];
+            }
+      }
+      if (op.cube_tgt_slot[cube] >= 0) {
+         const TgtDesc& td = lv.h_tgt[op.cube_tgt_slot[cube]];
+         auto ut = get(lv.UT64.p + td.u_off, (size_t)td.r * td.nq);
+         auto uti = get(lv.UTI64.p + td.uti_off, (size_t)td.r * gp3);
+         if (U)
+            for (int i = 0; i < td.nq; i++)
+               for (int k = 0; k < td.r; k++) U[(size_t)i * td.r + k] = ut[(size_t)k * td.nq + i];
+         if (UTI) std::memcpy(UTI, uti.data(), sizeof(double) * uti.size());
+      }
+   }
+   if (op.cube_leaf[cube] >= 0) {
+      const LeafDesc& L = op.h_leaves[op.cube_leaf[cube]];
+      auto blk = get(op.D64.p + L.d_off, 2 * (size_t)L.p * L.m);
+      for (int r = 0; r < L.p; r++)
+         for (int k = 0; k < L.m; k++) {
+            if (Ds) Ds[(size_t)r * L.m + k] = blk[(size_t)k * 2 * L.p + r];
+            if (Dd) Dd[(size_t)r * L.m + k] = blk[(size_t)k * 2 * L.p + L.p + r];
+         }
+   }
+   API_CATCH
+}
+
+int fftsvd_tree_apply(fftsvd_problem* p, int which, int precision, const double* xs, const double* xd, double* y) {
+   API_TRY
+   Problem& pr = *p->pr;
+   TreeOp& op = region_op(pr, which);
+   int n = op.tree.npan;
+   std::vector<double> a(n), b(n), c(n);
+   for (int k = 0; k < n; k++) {
+      a[k] = xs[op.tree.pan_perm[k]];
+      b[k] = xd[op.tree.pan_perm[k]];
+   }
+   DBuf<double> da, db, dc;
+   da.upload(a, pr.stream);
+   db.upload(b, pr.stream);
+   dc.alloc(op.tree.npt);
+   tree_apply(op, precision, da.p, db.p, dc.p, pr.stream);
+   c.resize(op.tree.npt);
+   FSVD_CHECK(hipMemcpyAsync(c.data(), dc.p, sizeof(double) * c.size(), hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   for (int k = 0; k < op.tree.npt; k++) y[op.tree.pt_perm[k]] = c[k];
+   API_CATCH
+}
+
+int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int layer, const double* x, double* y) {
+   API_TRY
+   Problem& pr = *p->pr;
+   TreeOp& op = region_op(pr, which);
+   std::vector<double> a(op.tree.npt), c(op.tree.npan);
+   for (int k = 0; k < op.tree.npt; k++) a[k] = x[op.tree.pt_perm[k]];
+   DBuf<double> da, dc;
+   da.upload(a, pr.stream);
+   dc.alloc(op.tree.npan);
+   tree_apply_transpose(op, precision, layer, da.p, dc.p, pr.stream);
+   FSVD_CHECK(hipMemcpyAsync(c.data(), dc.p, sizeof(double) * c.size(), hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   for (int k = 0; k < op.tree.npan; k++) y[op.tree.pan_perm[k]] = c[k];
+   API_CATCH
+}
+
+int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
+                     double* slp, double* dlp) {
+   API_TRY
+   std::vector<PanelGeom> g(npan);
+   for (int i = 0; i < npan; i++) panel_geometry(verts9 + 9 * (size_t)i, verts9 + 9 * (size_t)i + 3, verts9 + 9 * (size_t)i + 6, &g[i]);
+   hipStream_t s;
+   FSVD_CHECK(hipStreamCreate(&s));
+   {
+      DBuf<PanelGeom> dg;
+      DBuf<double> dp, ds, dd;
+      dg.upload(g, s);
+      dp.upload(pts, 3 * (size_t)npt, s);
+      ds.alloc((size_t)npan * npt);
+      dd.alloc((size_t)npan * npt);
+      launch_integrate_all(kernel, kappa, dg.p, npan, dp.p, npt, ds.p, dd.p, s);
+      FSVD_CHECK(hipMemcpyAsync(slp, ds.p, sizeof(double) * npan * npt, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipMemcpyAsync(dlp, dd.p, sizeof(double) * npan * npt, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   (void)hipStreamDestroy(s);
+   API_CATCH
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference solve-path API (FFTSVDpbeAPI.h)                                  */
+
+void error(char* message, ...) {
+   va_list ap;
+   va_start(ap, message);
+   printf("ERROR: ");
+   vprintf(message, ap);
+   printf("\n");
+   va_end(ap);
+   exit(-1);
+}
+
+void warning(char* message, ...) {
+   va_list ap;
+   va_start(ap, message);
+   printf("WARNING: ");
+   vprintf(message, ap);
+   printf("\n");
+   va_end(ap);
+}
+
+static Panel make_panel(const double* a, const double* b, const double* c) {
+   PanelGeom g;
+   panel_geometry(a, b, c, &g);
+   Panel p = (Panel)calloc(1, sizeof(_Panel));
+   p->centroid = (Vector3D)calloc(1, sizeof(_Vector3D));
+   p->normal = (Vector3D)calloc(1, sizeof(_Vector3D));
+   p->centroid->x = g.c[0];
+   p->centroid->y = g.c[1];
+   p->centroid->z = g.c[2];
+   p->normal->x = g.ax[6];
+   p->normal->y = g.ax[7];
+   p->normal->z = g.ax[8];
+   p->area = g.area;
+   p->maxedgelength = g.maxd;
+   std::memcpy(p->vertex[0], a, 3 * sizeof(double));
+   std::memcpy(p->vertex[1], b, 3 * sizeof(double));
+   std::memcpy(p->vertex[2], c, 3 * sizeof(double));
+   return p;
+}
+
+// .vert/.face reader (reference getFlatPanels + VertFace_*, FFTSVDpbeAPI.c:676-712, VertFace.c:26-202)
+static void get_flat_panels(const char* base, Panel** panels, unsigned int* numpanels, int accessible, int cavity) {
+   char vn[1100], fn[1100];
+   std::snprintf(vn, sizeof(vn), "%s.vert", base);
+   std::snprintf(fn, sizeof(fn), "%s.face", base);
+   FILE* fv = fopen(vn, "r");
+   if (!fv) error((char*)"Could not open vert file %s: %s", vn, strerror(errno));
+   FILE* ff = fopen(fn, "r");
+   if (!ff) error((char*)"Could not open face file %s: %s", fn, strerror(errno));
+   char line[128];
+   std::vector<double> V;
+   while (fgets(line, 128, fv)) {
+      double x = 0, y = 0, z = 0;
+      sscanf(line, "%lf %lf %lf", &x, &y, &z);
+      V.push_back(x);
+      V.push_back(y);
+      V.push_back(z);
+   }
+   std::vector<unsigned> F;
+   const bool flip = !(cavity && !usequalocation);
+   while (fgets(line, 128, ff)) {
+      unsigned a = 0, b = 0, c = 0, g = 0;
+      sscanf(line, "%u %u %u %u", &a, &b, &c, &g);
+      F.push_back(a - 1);
+      F.push_back(flip ? c - 1 : b - 1);
+      F.push_back(flip ? b - 1 : c - 1);
+      F.push_back(g);
+   }
+   fclose(fv);
+   fclose(ff);
+   std::vector<Panel> out;
+   size_t nv = V.size() / 3;
+   for (size_t f = 0; f < F.size() / 4; f++) {
+      unsigned ia = F[4 * f], ib = F[4 * f + 1], ic = F[4 * f + 2];
+      if (ia >= nv || ib >= nv || ic >= nv) error((char*)"face %u references a missing vertex", (unsigned)f + 1);
+      const double *A = &V[3 * (size_t)ia], *B = &V[3 * (size_t)ib], *C = &V[3 * (size_t)ic];
+      auto dist = [](const double* u, const double* v) {
+         return std::sqrt((u[0] - v[0]) * (u[0] - v[0]) + (u[1] - v[1]) * (u[1] - v[1]) + (u[2] - v[2]) * (u[2] - v[2]));
+      };
+      double d1 = dist(A, B), d2 = dist(B, C), d3 = dist(C, A);
+      double sp = 0.5 * (d1 + d2 + d3);
+      double area = std::sqrt(sp * (sp - d1) * (sp - d2) * (sp - d3));
+      if (!(area > 0.0)) continue;
+      if (accessible && F[4 * f + 3] != 1) continue;
+      out.push_back(make_panel(A, B, C));
+   }
+   *numpanels = (unsigned)out.size();
+   *panels = (Panel*)calloc(out.size() + 1, sizeof(Panel));
+   for (size_t k = 0; k < out.size(); k++) (*panels)[k] = out[k];
+}
+
+static std::vector<std::string> split_names(const char* line) {
+   std::vector<std::string> v;
+   std::string cur;
+   for (const char* p = line; *p; p++) {
+      if (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') {
+         if (!cur.empty()) v.push_back(cur);
+         cur.clear();
+      } else {
+         cur.push_back(*p);
+      }
+   }
+   if (!cur.empty()) v.push_back(cur);
+   return v;
+}
+
+void readSRF(const char* filename, Panel*** saltpanels, unsigned int** numsaltpanels, unsigned int* numsalts,
+             Panel*** dielectricpanels, unsigned int** numdielectricpanels, unsigned int* numdielectrics,
+             unsigned int** dielectricparent, Panel*** dielectriccavitypanels,
+             unsigned int** numdielectriccavitypanels, unsigned int* numdielectriccavities,
+             unsigned int** dielectriccavityparent, Panel*** saltcavitypanels, unsigned int** numsaltcavitypanels,
+             unsigned int* numsaltcavities, unsigned int** saltcavityparent, unsigned int* numtotalpanels) {
+   FILE* f = fopen(filename, "r");
+   if (!f) error((char*)"Could not open surface file %s: %s", filename, strerror(errno));
+   std::string dir(filename);
+   size_t slash = dir.rfind('/');
+   dir = (slash == std::string::npos) ? std::string() : dir.substr(0, slash + 1);
+   char line[1024];
+   auto next_line = [&]() -> std::string {
+      if (!fgets(line, sizeof(line), f)) return std::string();
+      return std::string(line);
+   };
+   std::string st = next_line(), dt = next_line();
+   char salttype = st.empty() ? 'f' : st[0], dieltype = dt.empty() ? 'f' : dt[0];
+   if (salttype != 'f' || dieltype != 'f') error((char*)"curved panels are not supported (flat 'f' surfaces only)");
+   *numtotalpanels = 0;
+   auto load = [&](const std::string& names, Panel*** panels, unsigned int** counts, unsigned int* num, int accessible,
+                   int cavity) {
+      auto v = split_names(names.c_str());
+      *num = (unsigned)v.size();
+      if (v.empty()) return;
+      *panels = (Panel**)calloc(v.size(), sizeof(Panel*));
+      *counts = (unsigned int*)calloc(v.size(), sizeof(unsigned int));
+      for (size_t i = 0; i < v.size(); i++) {
+         std::string path = v[i][0] == '/' ? v[i] : dir + v[i];
+         get_flat_panels(path.c_str(), &(*panels)[i], &(*counts)[i], accessible, cavity);
+         *numtotalpanels += (*counts)[i];
+      }
+   };
+   load(next_line(), saltpanels, numsaltpanels, numsalts, 1, 0);
+   load(next_line(), dielectricpanels, numdielectricpanels, numdielectrics, 0, 0);
+   load(next_line(), dielectriccavitypanels, numdielectriccavitypanels, numdielectriccavities, 0, 1);
+   load(next_line(), saltcavitypanels, numsaltcavitypanels, numsaltcavities, 1, 1);
+   auto parents = [&](unsigned n, unsigned int** out, const unsigned int* counts, bool drop999) {
+      std::string l = next_line();
+      auto v = split_names(l.c_str());
+      if (n == 0) return;
+      *out = (unsigned int*)calloc(n, sizeof(unsigned int));
+      for (unsigned i = 0; i < n; i++) {
+         (*out)[i] = i < v.size() ? (unsigned)atoi(v[i].c_str()) : 0;
+         if (drop999 && (*out)[i] == 999) *numtotalpanels -= counts[i];
+      }
+   };
+   parents(*numdielectrics, dielectricparent, *numdielectricpanels, false);
+   parents(*numdielectriccavities, dielectriccavityparent, *numdielectriccavities ? *numdielectriccavitypanels : nullptr,
+           true);
+   parents(*numsaltcavities, saltcavityparent, *numsaltcavities ? *numsaltcavitypanels : nullptr, true);
+   fclose(f);
+}
+
+static SurfaceOperator new_so() { return (SurfaceOperator)calloc(1, sizeof(_SurfaceOperator)); }
+
+void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBentries, unsigned int numPDBentries,
+                             Panel** saltpanels, unsigned int* numsaltpanels, unsigned int numsalts,
+                             Panel** dielectricpanels, unsigned int* numdielectricpanels, unsigned int numdielectrics,
+                             unsigned int* dielectricparent, Panel** dielectriccavitypanels,
+                             unsigned int* numdielectriccavitypanels, unsigned int numdielectriccavities,
+                             unsigned int* dielectriccavityparent, Panel** saltcavitypanels,
+                             unsigned int* numsaltcavitypanels, unsigned int numsaltcavities,
+                             unsigned int* saltcavityparent, unsigned int numtotalpanels) {
+   (void)saltpanels;
+   (void)numsaltpanels;
+   (void)numsalts;
+   (void)dielectricparent;
+   (void)saltcavitypanels;
+   (void)numsaltcavitypanels;
+   (void)numsaltcavities;
+   (void)saltcavityparent;
+   const double kappa = std::sqrt(ionicstrength) / DEBYE_CONSTANT;
+   if (ionicstrength > 0.0 && ionexclusionradius > 1e-6)
+      error((char*)"the Stern-layer formulation is not supported; set ionexclusionradius to 0");
+   SurfaceSpec sp;
+   for (unsigned i = 0; i < numdielectrics; i++) {
+      std::vector<double> v;
+      for (unsigned k = 0; k < numdielectricpanels[i]; k++)
+         for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) v.push_back(dielectricpanels[i][k]->vertex[a][d]);
+      sp.dielectric_verts.push_back(v);
+   }
+   std::vector<unsigned> cav_used;
+   for (unsigned j = 0; j < numdielectriccavities; j++) {
+      if (dielectriccavityparent[j] >= numdielectrics) continue;  // 999: removed by the mesher
+      std::vector<double> v;
+      for (unsigned k = 0; k < numdielectriccavitypanels[j]; k++)
+         for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) v.push_back(dielectriccavitypanels[j][k]->vertex[a][d]);
+      sp.cavity_verts.push_back(v);
+      sp.cavity_parent.push_back((int)dielectriccavityparent[j]);
+      cav_used.push_back(j);
+   }
+   for (unsigned a = 0; a < numPDBentries; a++) {
+      sp.qpos.push_back(PDBentries[a].x);
+      sp.qpos.push_back(PDBentries[a].y);
+      sp.qpos.push_back(PDBentries[a].z);
+      sp.q.push_back(toupper(PDBentries[a].chain) != 'X' ? PDBentries[a].charge : 0.0);
+   }
+   sp.eps_in = innerdielectric;
+   sp.eps_out = outerdielectric;
+   sp.kappa = ionicstrength > 0.0 ? kappa : 0.0;
+   sp.maxpanels = MAX_PANELS_PER_FINEST_CUBE;
+   sp.gridpoints = GRID_SIZE;
+   sp.svdeps = SVD_ERROR;
+   sp.want32 = true;
+   std::unique_ptr<Problem> pr;
+   try {
+      pr = build_problem(sp);
+   } catch (const std::exception& e) {
+      error((char*)"%s", e.what());
+   }
+   if ((unsigned)2 * pr->ntotal != 2 * numtotalpanels)
+      error((char*)"SurfaceOperator indexing is WRONG! (%u != %u)", 2 * pr->ntotal, 2 * numtotalpanels);
+   // host mirror of the region tree for callers that walk it
+   auto* h = new fftsvd_problem;
+   h->pr = std::move(pr);
+   Problem& P = *h->pr;
+   std::vector<SurfaceOperator> so(P.regions.size());
+   for (size_t r = 0; r < P.regions.size(); r++) so[r] = new_so();
+   for (size_t r = 0; r < P.regions.size(); r++) {
+      Region& R = P.regions[r];
+      SurfaceOperator o = so[r];
+      o->kernel = R.kernel == YUKAWA ? HELMHOLTZ_KERNEL : POISSON_KERNEL;
+      o->epsilon = R.eps;
+      o->kappa = R.kappa;
+      o->mynumpanels = R.mynpan;
+      o->startphi = R.startphi;
+      o->startdphi = R.startdphi;
+      o->tree = h;
+      o->parent = R.parent >= 0 ? so[R.parent] : nullptr;
+      o->numchildren = (unsigned)R.child.size();
+      o->children = (SurfaceOperator*)calloc(R.child.size() + 1, sizeof(SurfaceOperator));
+      for (size_t c = 0; c < R.child.size(); c++) o->children[c] = so[R.child[c]];
+      o->resultInternal = (Vector)calloc(R.panel_ids.size() + 1, sizeof(real));
+      o->resultExternal = (Vector)calloc(R.panel_ids.size() + 1, sizeof(real));
+      if (R.has_charges) {
+         Charge ch = (Charge)calloc(1, sizeof(_Charge));
+         ch->numcharges = numPDBentries;
+         ch->globalindexstart = 0;
+         ch->points = (Vector3D*)calloc(numPDBentries + 1, sizeof(Vector3D));
+         ch->charges = (Vector)calloc(numPDBentries + 1, sizeof(real));
+         for (unsigned a = 0; a < numPDBentries; a++) {
+            ch->points[a] = (Vector3D)calloc(1, sizeof(_Vector3D));
+            ch->points[a]->x = sp.qpos[3 * a];
+            ch->points[a]->y = sp.qpos[3 * a + 1];
+            ch->points[a]->z = sp.qpos[3 * a + 2];
+            ch->charges[a] = sp.q[a];
+         }
+         o->charges = ch;
+      }
+   }
+   *pbesurfaceoperator = so[0];
+}
+
+static Problem& so_problem(SurfaceOperator so) {
+   if (!so || !so->tree) error((char*)"surface operator was not generated by this library");
+   return *((fftsvd_problem*)so->tree)->pr;
+}
+
+void generateSurfaceOperatorPreconditioner(Preconditioner* preconditioner, SurfaceOperator so,
+                                           unsigned int numtotalpanels) {
+   Problem& pr = so_problem(so);
+   Preconditioner P = (Preconditioner)calloc(1, sizeof(_Preconditioner));
+   P->rows = P->columns = 2 * numtotalpanels;
+   P->nnz = 4 * pr.nblocks;
+   P->device = so->tree;
+   *preconditioner = P;
+}
+
+void generateRHS(Vector* rhs, SurfaceOperator so, unsigned int numtotalpanels) {
+   Problem& pr = so_problem(so);
+   *rhs = (Vector)calloc(2 * (size_t)numtotalpanels, sizeof(real));
+   if (fftsvd_problem_rhs((fftsvd_problem*)so->tree, *rhs)) error((char*)"%s", g_err.c_str());
+   (void)pr;
+}
+
+void solveSurfaceOperator(Vector* solution, SurfaceOperator so, Preconditioner preconditioner, Vector rhs,
+                          unsigned int numtotalpanels) {
+   (void)preconditioner;
+   so_problem(so);
+   *solution = (Vector)calloc(2 * (size_t)numtotalpanels, sizeof(real));
+   if (fftsvd_problem_solve((fftsvd_problem*)so->tree, g_precision, rhs, *solution, errortolerance, 1000, nullptr,
+                            nullptr, nullptr))
+      error((char*)"%s", g_err.c_str());
+}
+
+void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries, PDBentry* PDBinputentries,
+                       unsigned int numPDBinputentries, SurfaceOperator so, Vector solution) {
+   Problem& pr = so_problem(so);
+   std::vector<double> phi(pr.q.size() + 1, 0.0);
+   problem_potentials(pr, solution, phi.data());
+   for (unsigned a = 0; a < numPDBcollectentries; a++)
+      PDBcollectentries[a].potential = KT_CONVERSION * (a < pr.q.size() ? phi[a] : 0.0) / innerdielectric;
+   if (addcoulombic) {
+      for (unsigned c = 0; c < numPDBcollectentries; c++)
+         for (unsigned a = 0; a < numPDBinputentries; a++)
+            if (toupper(PDBinputentries[a].chain) != 'X') {
+               double dx = PDBinputentries[a].x - PDBcollectentries[c].x;
+               double dy = PDBinputentries[a].y - PDBcollectentries[c].y;
+               double dz = PDBinputentries[a].z - PDBcollectentries[c].z;
+               double distance = std::sqrt(dx * dx + dy * dy + dz * dz);
+               if (distance > 0.0)
+                  PDBcollectentries[c].potential +=
+                      KT_CONVERSION * PDBinputentries[a].charge / (innerdielectric * distance);
+            }
+   }
+}
+
+}  // extern "C"

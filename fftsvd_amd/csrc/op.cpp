@@ -1,0 +1,720 @@
+// Construction and application of one compressed FFT-SVD operator on the GPU.
+//
+// Construction (reference Tree_fill, Tree.c:513-565) runs as batched
+// device work: near-field integral blocks, sampled interaction matrices,
+// pivoted-MGS bases, equivalent-density projections and the small GEMMs
+// that fold them into PV / UTI.  The host decides the tree, the sampling
+// pattern and the per-level pseudoinverses and translation spectra.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+
+#include "host_linalg.h"
+#include "op.h"
+
+namespace fsvd {
+
+void launch_batched_copy(int n, const void* descs, hipStream_t s);
+
+namespace {
+
+const double SPHERE49[49][3] = {
+#include "sphere49.inc"
+};
+constexpr int NQ = 49;
+
+struct CopyDesc {
+   const double* src;
+   double* dst;
+   long long n;
+};
+
+inline int offset_code(const int* a, const int* b) {
+   return (a[0] - b[0] + 3) * 49 + (a[1] - b[1] + 3) * 7 + (a[2] - b[2] + 3);
+}
+
+double now_ms() {
+   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Splits an IntBlock along its larger dimension so no workgroup gets more
+// than `cap` integrals.
+void push_blocks(std::vector<IntBlock>& out, IntBlock b, int cap = 8192) {
+   if ((long long)b.npt * b.npan <= cap || (b.npt <= 1 && b.npan <= 1)) {
+      if (b.npt > 0 && b.npan > 0) out.push_back(b);
+      return;
+   }
+   if (b.npt >= b.npan) {
+      int step = std::max(1, cap / std::max(1, b.npan));
+      for (int i0 = 0; i0 < b.npt; i0 += step) {
+         IntBlock c = b;
+         c.npt = std::min(step, b.npt - i0);
+         c.pt_off = b.pt_off + i0;
+         c.out_off = b.out_off + (long long)i0 * b.ld_pt;
+         out.push_back(c);
+      }
+   } else {
+      int step = std::max(1, cap / std::max(1, b.npt));
+      for (int j0 = 0; j0 < b.npan; j0 += step) {
+         IntBlock c = b;
+         c.npan = std::min(step, b.npan - j0);
+         c.pan_off = b.pan_off + j0;
+         c.out_off = b.out_off + (long long)j0 * b.ld_pan;
+         out.push_back(c);
+      }
+   }
+}
+
+void run_integrals(const std::vector<IntBlock>& blocks, const std::vector<double>& bpts, const std::vector<int>& bpan,
+                   const PanelGeom* d_panels, int kernel, double kappa, double* out, hipStream_t s) {
+   if (blocks.empty()) return;
+   DBuf<IntBlock> db;
+   DBuf<double> dp;
+   DBuf<int> dn;
+   db.upload(blocks, s);
+   dp.upload(bpts, s);
+   dn.upload(bpan, s);
+   launch_integral_blocks((int)blocks.size(), db.p, dp.p, dn.p, d_panels, kernel, kappa, out, s);
+   FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+void run_gemms(const std::vector<GemmDesc>& g, hipStream_t s) {
+   if (g.empty()) return;
+   DBuf<GemmDesc> dg;
+   dg.upload(g, s);
+   int maxout = 0;
+   for (auto& d : g) maxout = std::max(maxout, d.M * d.N);
+   int tiles = std::min(64, std::max(1, (maxout + 255) / 256));
+   // batch dimension limited to 65535 per launch
+   for (size_t b0 = 0; b0 < g.size(); b0 += 65535) {
+      int nb = (int)std::min<size_t>(65535, g.size() - b0);
+      launch_gemm_batched(nb, tiles, dg.p + b0, s);
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+void to_f32(const DBuf<double>& a, DBuf<float>& b, hipStream_t s) {
+   b.alloc(a.n);
+   launch_f64_to_f32(a.p, b.p, (long long)a.n, s);
+}
+
+}  // namespace
+
+std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
+                                      const double* pts, int npt, int kernel, double kappa, int maxpanels,
+                                      int gridpoints, double svdeps, bool want32, hipStream_t s) {
+   auto opp = std::make_unique<TreeOp>();
+   TreeOp& op = *opp;
+   op.kernel = kernel;
+   op.kappa = kappa;
+   Tree& t = op.tree;
+   t.kernel = kernel;
+   t.kappa = kappa;
+   t.svdeps = svdeps;
+   const int npan = (int)panels.size();
+   build_tree(t, panels.data(), npan, pts, npt, maxpanels, gridpoints);
+   op.stats.depth = t.depth;
+   op.stats.ncubes = (int)t.cubes.size();
+
+   std::vector<double> ptsT(3 * (size_t)npt);
+   for (int k = 0; k < npt; k++)
+      for (int d = 0; d < 3; d++) ptsT[3 * (size_t)k + d] = pts[3 * (size_t)t.pt_perm[k] + d];
+   op.pts_tree.upload(ptsT, s);
+   op.pan_perm.upload(t.pan_perm, s);
+   op.yfar.alloc(std::max(npt, npan) + 1);
+
+   // ---------------- near field (Cube_fill_D, Cube.c:715-793) ----------------
+   double t0 = now_ms();
+   long long dsize = 0;
+   for (size_t i = 0; i < t.cubes.size(); i++) {
+      const Cube& c = t.cubes[i];
+      if (!c.leaf || c.nq == 0) continue;
+      LeafDesc L;
+      L.q0 = c.q0;
+      L.p = c.nq;
+      L.col_off = (int)op.h_dcol.size();
+      for (int k = 0; k < c.np; k++) op.h_dcol.push_back(c.p0 + k);
+      for (int l : c.loc) {
+         const Cube& o = t.cubes[l];
+         for (int k = 0; k < o.np; k++) op.h_dcol.push_back(o.p0 + k);
+      }
+      L.m = (int)op.h_dcol.size() - L.col_off;
+      L.d_off = dsize;
+      dsize += 2LL * L.p * L.m;
+      op.h_leaves.push_back(L);
+      op.maxm = std::max(op.maxm, L.m);
+      op.maxp = std::max(op.maxp, L.p);
+   }
+   op.nleaf = (int)op.h_leaves.size();
+   op.stats.nleaf = op.nleaf;
+   op.stats.near_entries = dsize;
+   op.leaves.upload(op.h_leaves, s);
+   op.dcol.upload(op.h_dcol, s);
+   op.D64.alloc(std::max<long long>(dsize, 1));
+   {
+      std::vector<int> bpan(op.h_dcol.size());
+      for (size_t k = 0; k < bpan.size(); k++) bpan[k] = t.pan_perm[op.h_dcol[k]];
+      std::vector<IntBlock> blocks;
+      for (const LeafDesc& L : op.h_leaves) {
+         if (L.m == 0) continue;
+         IntBlock b;
+         b.npt = L.p;
+         b.npan = L.m;
+         b.pt_off = L.q0;
+         b.pan_off = L.col_off;
+         b.out_off = L.d_off;
+         b.ld_pt = 1;
+         b.ld_pan = 2 * L.p;
+         b.layer_stride = L.p;
+         push_blocks(blocks, b);
+      }
+      run_integrals(blocks, ptsT, bpan, d_panels, kernel, kappa, op.D64.p, s);
+   }
+   op.stats.setup_ms_near = now_ms() - t0;
+   t0 = now_ms();
+
+   // ---------------- far field ----------------
+   const double root = t.cubes[0].b1[0] - t.cubes[0].b0[0];
+   op.h_pinv.resize(t.depth);
+   std::vector<int> src_slot(t.cubes.size(), -1), tgt_slot(t.cubes.size(), -1);
+   for (int l = 2; l < t.depth; l++) {
+      FarLevel lv;
+      lv.level = l;
+      lv.n = t.gpl[l];
+      const int n = lv.n, N = 2 * n - 1, H = N / 2 + 1;
+      lv.pad = N * N * H;
+      lv.gp3 = n * n * n;
+      const int gp3 = lv.gp3, pad = lv.pad;
+
+      // pseudoinverse of the sphere -> grid Green's matrix (Tree_fill_pinv_Fitting, Tree.c:295-369)
+      {
+         double box = root / std::pow(2.0, l);
+         double h = box / (n - 1);
+         double sc = box * 1.5 * 1, ctr = 0.5 * box;
+         std::vector<double> Q((size_t)NQ * gp3);
+         for (int q = 0; q < NQ; q++) {
+            double mq[3];
+            for (int a = 0; a < 3; a++) mq[a] = SPHERE49[q][a] * sc + ctr;
+            int g = 0;
+            for (int i = 0; i < n; i++)
+               for (int j = 0; j < n; j++)
+                  for (int k = 0; k < n; k++, g++) {
+                     double p0 = i * h - mq[0], p1 = j * h - mq[1], p2 = k * h - mq[2];
+                     double r = std::sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+                     Q[(size_t)q * gp3 + g] = (kernel == YUKAWA) ? std::exp(-kappa * r) / r : (r == 0.0 ? 0.0 : 1.0 / r);
+                  }
+         }
+         op.h_pinv[l].resize((size_t)gp3 * NQ);
+         pinv(Q.data(), NQ, gp3, op.h_pinv[l].data());
+      }
+      // translation spectra (Tree_fill_T, Tree.c:387-445)
+      {
+         std::vector<std::complex<double>> T((size_t)343 * pad);
+         translation_spectra(n, root / std::pow(2.0, l) / (n - 1), kernel, kappa, T.data());
+         lv.T64.upload((const double*)T.data(), 2 * T.size(), s);
+      }
+      DBuf<double> d_pinv;
+      d_pinv.upload(op.h_pinv[l], s);
+
+      // cubes of this level with interactions
+      std::vector<int> cl;
+      for (size_t i = 0; i < t.cubes.size(); i++)
+         if (t.cubes[i].level == l && !t.cubes[i].itr.empty()) cl.push_back((int)i);
+
+      // ---- sampled bases (Cube_fill_U_VT, Cube.c:1104-1412) ----
+      struct Prob {
+         int cube;
+         int isU;       // 0: row basis VT over panels, 1: column basis U over points
+         int pass;
+         int nsamp;     // interacting cubes with points (VT) / panels (U)
+         int rank = -1;
+      };
+      std::vector<Prob> probs;
+      std::vector<std::vector<int>> ipt(cl.size()), ipn(cl.size());
+      for (size_t ci = 0; ci < cl.size(); ci++) {
+         const Cube& c = t.cubes[cl[ci]];
+         for (int o : c.itr) {
+            if (t.cubes[o].nq > 0) ipt[ci].push_back(o);
+            if (t.cubes[o].np > 0) ipn[ci].push_back(o);
+         }
+         if (c.np > 0 && !ipt[ci].empty()) probs.push_back({(int)ci, 0, 1, (int)ipt[ci].size()});
+         if (c.nq > 0 && !ipn[ci].empty()) probs.push_back({(int)ci, 1, 1, (int)ipn[ci].size()});
+      }
+      // final Q storage per problem (device) collected across resampling rounds
+      std::vector<std::shared_ptr<DBuf<double>>> keep;
+      std::vector<const double*> qsrc(probs.size(), nullptr);
+      std::vector<int> pending(probs.size());
+      for (size_t i = 0; i < probs.size(); i++) pending[i] = (int)i;
+      while (!pending.empty()) {
+         std::vector<IntBlock> blocks;
+         std::vector<MgsDesc> mgs;
+         std::vector<double> bpts(ptsT);
+         std::vector<int> bpan(t.pan_perm);
+         long long asz = 0, qsz = 0;
+         for (size_t pi = 0; pi < pending.size(); pi++) {
+            Prob& P = probs[pending[pi]];
+            const Cube& c = t.cubes[cl[P.cube]];
+            const int K = 2 * P.nsamp * P.pass;
+            const int half = P.nsamp * P.pass;
+            MgsDesc m;
+            m.rank_slot = (int)pi;
+            m.a_off = asz;
+            m.q_off = qsz;
+            IntBlock b;
+            b.out_off = asz;
+            b.layer_stride = half;
+            if (!P.isU) {
+               // rows of R: sampled points of interacting cubes, columns: my panels -> A = R^T (np x K)
+               int base = (int)(bpts.size() / 3);
+               for (int p = 0; p < P.pass; p++)
+                  for (int r = 0; r < P.nsamp; r++) {
+                     const Cube& I = t.cubes[ipt[P.cube][r]];
+                     int ci = (int)(&I - &t.cubes[0]);
+                     int off = t.cube_pt_sorted_off[ci];
+                     int idx = t.cube_pt_sorted[off + (unsigned)p * (unsigned)I.nq / (unsigned)P.pass];
+                     for (int d = 0; d < 3; d++) bpts.push_back(pts[3 * (size_t)idx + d]);
+                  }
+               b.npt = half;
+               b.npan = c.np;
+               b.pt_off = base;
+               b.pan_off = c.p0;
+               b.ld_pt = 1;
+               b.ld_pan = K;
+               m.rows = c.np;
+               m.cols = K;
+            } else {
+               int base = (int)bpan.size();
+               for (int p = 0; p < P.pass; p++)
+                  for (int q = 0; q < P.nsamp; q++) {
+                     const Cube& I = t.cubes[ipn[P.cube][q]];
+                     int ci = (int)(&I - &t.cubes[0]);
+                     int off = t.cube_pan_sorted_off[ci];
+                     bpan.push_back(t.cube_pan_sorted[off + (unsigned)p * (unsigned)I.np / (unsigned)P.pass]);
+                  }
+               b.npt = c.nq;
+               b.npan = half;
+               b.pt_off = c.q0;
+               b.pan_off = base;
+               b.ld_pt = K;
+               b.ld_pan = 1;
+               m.rows = c.nq;
+               m.cols = K;
+            }
+            push_blocks(blocks, b);
+            mgs.push_back(m);
+            asz += (long long)m.rows * m.cols;
+            qsz += (long long)m.rows * std::min(m.rows, m.cols);
+         }
+         auto Abuf = std::make_shared<DBuf<double>>();
+         auto Qbuf = std::make_shared<DBuf<double>>();
+         Abuf->alloc(std::max<long long>(asz, 1));
+         Qbuf->alloc(std::max<long long>(qsz, 1));
+         run_integrals(blocks, bpts, bpan, d_panels, kernel, kappa, Abuf->p, s);
+         DBuf<MgsDesc> dm;
+         dm.upload(mgs, s);
+         DBuf<int> dr;
+         dr.alloc(mgs.size());
+         launch_mgs((int)mgs.size(), 0, dm.p, Abuf->p, Qbuf->p, dr.p, svdeps, s);
+         std::vector<int> ranks(mgs.size());
+         FSVD_CHECK(hipMemcpyAsync(ranks.data(), dr.p, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+         std::vector<int> next;
+         for (size_t pi = 0; pi < pending.size(); pi++) {
+            Prob& P = probs[pending[pi]];
+            int r = ranks[pi];
+            if (r < (2 * P.pass * P.nsamp) / 2) {
+               P.rank = r;
+               qsrc[pending[pi]] = Qbuf->p + mgs[pi].q_off;
+            } else {
+               P.pass++;
+               next.push_back(pending[pi]);
+            }
+         }
+         keep.push_back(Qbuf);
+         pending.swap(next);
+      }
+      // ranks into cubes; compact bases into the level buffers
+      std::vector<int> vt_prob(cl.size(), -1), u_prob(cl.size(), -1);
+      for (size_t i = 0; i < probs.size(); i++) {
+         Cube& c = t.cubes[cl[probs[i].cube]];
+         if (!probs[i].isU) {
+            c.rowrank = probs[i].rank;
+            vt_prob[probs[i].cube] = (int)i;
+         } else {
+            c.colrank = probs[i].rank;
+            u_prob[probs[i].cube] = (int)i;
+         }
+      }
+      long long vts = 0, pvs = 0, uts = 0, utis = 0;
+      std::vector<CopyDesc> copies;
+      for (size_t ci = 0; ci < cl.size(); ci++) {
+         Cube& c = t.cubes[cl[ci]];
+         if (c.np > 0 && c.rowrank > 0) {
+            SrcDesc sd;
+            sd.p0 = c.p0;
+            sd.np = c.np;
+            sd.r = c.rowrank;
+            sd.spec_slot = (int)lv.h_src.size();
+            sd.vt_off = vts;
+            sd.pv_off = pvs;
+            vts += (long long)c.rowrank * c.np;
+            pvs += 2LL * c.rowrank * gp3;
+            src_slot[cl[ci]] = sd.spec_slot;
+            lv.h_src.push_back(sd);
+            lv.maxr_src = std::max(lv.maxr_src, c.rowrank);
+         }
+         if (c.nq > 0 && c.colrank > 0) {
+            TgtDesc td;
+            td.q0 = c.q0;
+            td.nq = c.nq;
+            td.r = c.colrank;
+            td.spec_slot = (int)lv.h_tgt.size();
+            td.u_off = uts;
+            td.uti_off = utis;
+            uts += (long long)c.colrank * c.nq;
+            utis += (long long)c.colrank * gp3;
+            tgt_slot[cl[ci]] = td.spec_slot;
+            lv.h_tgt.push_back(td);
+            lv.maxr_tgt = std::max(lv.maxr_tgt, c.colrank);
+         }
+      }
+      lv.VT64.alloc(std::max<long long>(vts, 1));
+      lv.PV64.alloc(std::max<long long>(pvs, 1));
+      lv.UT64.alloc(std::max<long long>(uts, 1));
+      lv.UTI64.alloc(std::max<long long>(utis, 1));
+      for (auto& sd : lv.h_src) {
+         (void)sd;
+      }
+      {
+         size_t si = 0, ti = 0;
+         for (size_t ci = 0; ci < cl.size(); ci++) {
+            Cube& c = t.cubes[cl[ci]];
+            if (c.np > 0 && c.rowrank > 0) {
+               const SrcDesc& sd = lv.h_src[si++];
+               copies.push_back({qsrc[vt_prob[ci]], lv.VT64.p + sd.vt_off, (long long)sd.r * sd.np});
+            }
+            if (c.nq > 0 && c.colrank > 0) {
+               const TgtDesc& td = lv.h_tgt[ti++];
+               copies.push_back({qsrc[u_prob[ci]], lv.UT64.p + td.u_off, (long long)td.r * td.nq});
+            }
+         }
+         DBuf<CopyDesc> dc;
+         dc.upload(copies, s);
+         for (size_t b0 = 0; b0 < copies.size(); b0 += 65535)
+            launch_batched_copy((int)std::min<size_t>(65535, copies.size() - b0), dc.p + b0, s);
+         FSVD_CHECK(hipStreamSynchronize(s));
+      }
+      keep.clear();
+
+      // ---- equivalent densities: PV = P VT^T, UTI = U^T I (Cube.c:795-1002,1414-1522) ----
+      {
+         std::vector<IntBlock> blocks;
+         std::vector<double> bpts(ptsT);
+         std::vector<double> sph;   // sphere points per cube (target side, Green blocks)
+         std::vector<GreenBlock> gblocks;
+         long long qs = 0, ps = 0, qp = 0, ip = 0;
+         struct Aux {
+            long long q_off, p_off;
+         };
+         std::vector<Aux> saux(lv.h_src.size()), taux(lv.h_tgt.size());
+         size_t si = 0, ti = 0;
+         for (size_t ci = 0; ci < cl.size(); ci++) {
+            const Cube& c = t.cubes[cl[ci]];
+            double box = c.b1[0] - c.b0[0];
+            double sc = box * 1.5 * 1;
+            bool is_src = c.np > 0 && c.rowrank > 0, is_tgt = c.nq > 0 && c.colrank > 0;
+            if (!is_src && !is_tgt) continue;
+            int sbase = (int)(bpts.size() / 3);
+            for (int q = 0; q < NQ; q++)
+               for (int a = 0; a < 3; a++) bpts.push_back(SPHERE49[q][a] * sc + c.ctr[a]);
+            if (is_src) {
+               IntBlock b;
+               b.npt = NQ;
+               b.npan = c.np;
+               b.pt_off = sbase;
+               b.pan_off = c.p0;
+               b.out_off = qs;
+               b.ld_pt = c.np;
+               b.ld_pan = 1;
+               b.layer_stride = (long long)NQ * c.np;
+               push_blocks(blocks, b);
+               saux[si] = {qs, ps};
+               qs += 2LL * NQ * c.np;
+               ps += 2LL * gp3 * c.np;
+               si++;
+            }
+            if (is_tgt) {
+               GreenBlock g;
+               g.ndest = c.nq;
+               g.nsrc = NQ;
+               g.dest_off = c.q0;
+               g.src_off = sbase;
+               g.out_off = qp;
+               g.ld_dest = 1;
+               g.ld_src = c.nq;
+               gblocks.push_back(g);
+               taux[ti] = {qp, ip};
+               qp += (long long)NQ * c.nq;
+               ip += (long long)gp3 * c.nq;
+               ti++;
+            }
+         }
+         DBuf<double> Qs, Pb, Qpb, Ib;
+         Qs.alloc(std::max<long long>(qs, 1));
+         Pb.alloc(std::max<long long>(ps, 1));
+         Qpb.alloc(std::max<long long>(qp, 1));
+         Ib.alloc(std::max<long long>(ip, 1));
+         run_integrals(blocks, bpts, t.pan_perm, d_panels, kernel, kappa, Qs.p, s);
+         if (!gblocks.empty()) {
+            DBuf<GreenBlock> dg;
+            DBuf<double> dp;
+            dg.upload(gblocks, s);
+            dp.upload(bpts, s);
+            launch_green_blocks((int)gblocks.size(), dg.p, dp.p, dp.p, kernel, kappa, Qpb.p, s);
+            FSVD_CHECK(hipStreamSynchronize(s));
+         }
+         // P = pinv * Q2S (both layers); I^T = (pinv * Q2S_points)^T
+         std::vector<GemmDesc> g1, g2;
+         for (size_t k = 0; k < lv.h_src.size(); k++) {
+            const SrcDesc& sd = lv.h_src[k];
+            for (int layer = 0; layer < 2; layer++) {
+               GemmDesc d;
+               d.M = gp3;
+               d.N = sd.np;
+               d.K = NQ;
+               d.A = d_pinv.p;
+               d.a_rs = NQ;
+               d.a_cs = 1;
+               d.B = Qs.p + saux[k].q_off + (long long)layer * NQ * sd.np;
+               d.b_rs = sd.np;
+               d.b_cs = 1;
+               d.C = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
+               d.c_rs = sd.np;
+               d.c_cs = 1;
+               g1.push_back(d);
+            }
+         }
+         for (size_t k = 0; k < lv.h_tgt.size(); k++) {
+            const TgtDesc& td = lv.h_tgt[k];
+            GemmDesc d;
+            d.M = td.nq;
+            d.N = gp3;
+            d.K = NQ;
+            d.A = Qpb.p + taux[k].q_off;
+            d.a_rs = 1;
+            d.a_cs = td.nq;
+            d.B = d_pinv.p;
+            d.b_rs = 1;
+            d.b_cs = NQ;
+            d.C = Ib.p + taux[k].p_off;
+            d.c_rs = gp3;
+            d.c_cs = 1;
+            g1.push_back(d);
+         }
+         run_gemms(g1, s);
+         // PV^T (r x gp3) = VT (r x np) * P^T ; UTI (r x gp3) = U^T (r x nq) * I (nq x gp3)
+         for (size_t k = 0; k < lv.h_src.size(); k++) {
+            const SrcDesc& sd = lv.h_src[k];
+            for (int layer = 0; layer < 2; layer++) {
+               GemmDesc d;
+               d.M = sd.r;
+               d.N = gp3;
+               d.K = sd.np;
+               d.A = lv.VT64.p + sd.vt_off;
+               d.a_rs = sd.np;
+               d.a_cs = 1;
+               d.B = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
+               d.b_rs = 1;
+               d.b_cs = sd.np;
+               d.C = lv.PV64.p + sd.pv_off + (long long)layer * sd.r * gp3;
+               d.c_rs = gp3;
+               d.c_cs = 1;
+               g2.push_back(d);
+            }
+         }
+         for (size_t k = 0; k < lv.h_tgt.size(); k++) {
+            const TgtDesc& td = lv.h_tgt[k];
+            GemmDesc d;
+            d.M = td.r;
+            d.N = gp3;
+            d.K = td.nq;
+            d.A = lv.UT64.p + td.u_off;
+            d.a_rs = td.nq;
+            d.a_cs = 1;
+            d.B = Ib.p + taux[k].p_off;
+            d.b_rs = gp3;
+            d.b_cs = 1;
+            d.C = lv.UTI64.p + td.uti_off;
+            d.c_rs = gp3;
+            d.c_cs = 1;
+            g2.push_back(d);
+         }
+         run_gemms(g2, s);
+      }
+      op.stats.far_entries += (long long)(lv.VT64.n + lv.PV64.n + lv.UT64.n + lv.UTI64.n);
+      op.levels.push_back(std::move(lv));
+   }
+
+   op.cube_src_slot = src_slot;
+   op.cube_tgt_slot = tgt_slot;
+   op.cube_leaf.assign(t.cubes.size(), -1);
+   {
+      size_t li = 0;
+      for (size_t i = 0; i < t.cubes.size(); i++)
+         if (t.cubes[i].leaf && t.cubes[i].nq > 0) op.cube_leaf[i] = (int)li++;
+   }
+   // interaction lists (Cube_multiply_T, Cube.c:2062-2111) and their transposes
+   for (FarLevel& lv : op.levels) {
+      for (TgtDesc& td : lv.h_tgt) td.nint = 0;
+      size_t ti = 0;
+      for (size_t i = 0; i < t.cubes.size(); i++) {
+         const Cube& c = t.cubes[i];
+         if (c.level != lv.level || tgt_slot[i] < 0) continue;
+         TgtDesc& td = lv.h_tgt[ti++];
+         td.int_off = (int)lv.h_isrc.size();
+         for (int o : c.itr)
+            if (src_slot[o] >= 0) {
+               lv.h_isrc.push_back(src_slot[o]);
+               lv.h_itoff.push_back(offset_code(c.idx, t.cubes[o].idx));
+            }
+         td.nint = (int)lv.h_isrc.size() - td.int_off;
+         op.stats.translate_macs += (long long)td.nint * lv.pad;
+      }
+      size_t si = 0;
+      lv.h_tgtT.resize(lv.h_src.size());
+      for (size_t i = 0; i < t.cubes.size(); i++) {
+         const Cube& c = t.cubes[i];
+         if (c.level != lv.level || src_slot[i] < 0) continue;
+         TgtDesc& td = lv.h_tgtT[si];
+         std::memset(&td, 0, sizeof(td));
+         td.spec_slot = src_slot[i];
+         td.int_off = (int)lv.h_isrcT.size();
+         for (int o : c.itr)
+            if (tgt_slot[o] >= 0) {
+               lv.h_isrcT.push_back(tgt_slot[o]);
+               lv.h_itoffT.push_back(offset_code(c.idx, t.cubes[o].idx));
+            }
+         td.nint = (int)lv.h_isrcT.size() - td.int_off;
+         si++;
+      }
+      lv.nsrc = (int)lv.h_src.size();
+      lv.ntgt = (int)lv.h_tgt.size();
+      op.stats.spectra_bins += (long long)(lv.nsrc + lv.ntgt) * lv.pad;
+      lv.src.upload(lv.h_src, s);
+      lv.tgt.upload(lv.h_tgt, s);
+      lv.tgtT.upload(lv.h_tgtT, s);
+      lv.isrc.upload(lv.h_isrc, s);
+      lv.itoff.upload(lv.h_itoff, s);
+      lv.isrcT.upload(lv.h_isrcT, s);
+      lv.itoffT.upload(lv.h_itoffT, s);
+      lv.spec.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
+      lv.acc.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
+      lv.specT.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
+      lv.accT.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
+   }
+   if (want32) {
+      to_f32(op.D64, op.D32, s);
+      for (FarLevel& lv : op.levels) {
+         to_f32(lv.T64, lv.T32, s);
+         to_f32(lv.VT64, lv.VT32, s);
+         to_f32(lv.PV64, lv.PV32, s);
+         to_f32(lv.UT64, lv.UT32, s);
+         to_f32(lv.UTI64, lv.UTI32, s);
+      }
+      op.have32 = true;
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   op.stats.setup_ms_far = now_ms() - t0;
+   return opp;
+}
+
+template <typename T>
+static void apply_impl(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s) {
+   const bool f32 = std::is_same<T, float>::value;
+   launch_zero(op.yfar.p, op.tree.npt, s);
+   for (FarLevel& lv : op.levels)
+      if (lv.nsrc)
+         launch_far_source<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
+                              f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, xs, xd, lv.spec.p, lv.maxr_src, s);
+   for (FarLevel& lv : op.levels)
+      if (lv.ntgt)
+         launch_far_translate<T>(lv.ntgt, lv.tgt.p, lv.isrc.p, lv.itoff.p,
+                                 f32 ? (const void*)lv.T32.p : (const void*)lv.T64.p, lv.spec.p, lv.acc.p, lv.pad, s);
+   for (FarLevel& lv : op.levels)
+      if (lv.ntgt)
+         launch_far_target<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
+                              f32 ? (const T*)lv.UTI32.p : (const T*)lv.UTI64.p, lv.acc.p, op.yfar.p, lv.maxr_tgt, s);
+   if (op.nleaf)
+      launch_near<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, xs, xd,
+                     op.yfar.p, y, op.maxm, op.maxp, s);
+}
+
+template <typename T>
+static void apply_part_impl(TreeOp& op, int part, const double* xs, const double* xd, double* y, hipStream_t s) {
+   const bool f32 = std::is_same<T, float>::value;
+   if (part == 0 && op.nleaf)
+      launch_near<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, xs, xd,
+                     op.yfar.p, y, op.maxm, op.maxp, s);
+   for (FarLevel& lv : op.levels) {
+      if (part == 1 && lv.nsrc)
+         launch_far_source<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
+                              f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, xs, xd, lv.spec.p, lv.maxr_src, s);
+      if (part == 2 && lv.ntgt)
+         launch_far_translate<T>(lv.ntgt, lv.tgt.p, lv.isrc.p, lv.itoff.p,
+                                 f32 ? (const void*)lv.T32.p : (const void*)lv.T64.p, lv.spec.p, lv.acc.p, lv.pad, s);
+      if (part == 3 && lv.ntgt)
+         launch_far_target<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
+                              f32 ? (const T*)lv.UTI32.p : (const T*)lv.UTI64.p, lv.acc.p, op.yfar.p, lv.maxr_tgt, s);
+   }
+}
+
+void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s) {
+   if (prec == PREC_F32)
+      apply_part_impl<float>(op, part, xs, xd, y, s);
+   else
+      apply_part_impl<double>(op, part, xs, xd, y, s);
+}
+
+void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s) {
+   if (prec == PREC_F32) {
+      if (!op.have32) throw std::runtime_error("operator has no FP32 copy");
+      apply_impl<float>(op, xs, xd, y, s);
+   } else {
+      apply_impl<double>(op, xs, xd, y, s);
+   }
+}
+
+template <typename T>
+static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipStream_t s) {
+   const bool f32 = std::is_same<T, float>::value;
+   launch_zero(y, op.tree.npan, s);
+   for (FarLevel& lv : op.levels)
+      if (lv.ntgt)
+         launch_farT_source<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
+                               f32 ? (const T*)lv.UTI32.p : (const T*)lv.UTI64.p, x, lv.specT.p, lv.maxr_tgt, s);
+   for (FarLevel& lv : op.levels)
+      if (lv.nsrc)
+         launch_far_translate<T>(lv.nsrc, lv.tgtT.p, lv.isrcT.p, lv.itoffT.p,
+                                 f32 ? (const void*)lv.T32.p : (const void*)lv.T64.p, lv.specT.p, lv.accT.p, lv.pad,
+                                 s);
+   for (FarLevel& lv : op.levels)
+      if (lv.nsrc)
+         launch_farT_target<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
+                               f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, lv.accT.p, layer, y, lv.maxr_src, s);
+   if (op.nleaf)
+      launch_nearT<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, x, layer, y, s);
+}
+
+void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s) {
+   if (prec == PREC_F32) {
+      if (!op.have32) throw std::runtime_error("operator has no FP32 copy");
+      applyT_impl<float>(op, layer, x, y, s);
+   } else {
+      applyT_impl<double>(op, layer, x, y, s);
+   }
+}
+
+}  // namespace fsvd

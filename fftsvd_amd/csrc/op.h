@@ -1,0 +1,154 @@
+// Device-resident compressed operator of one tree, and its region-operator
+// and solver wrappers.
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <type_traits>
+#include <vector>
+
+#include "device.h"
+#include "internal.h"
+
+namespace fsvd {
+
+// A device buffer that frees itself.
+template <typename T>
+struct DBuf {
+   T* p = nullptr;
+   size_t n = 0;
+   DBuf() = default;
+   DBuf(const DBuf&) = delete;
+   DBuf& operator=(const DBuf&) = delete;
+   DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+      o.p = nullptr;
+      o.n = 0;
+   }
+   DBuf& operator=(DBuf&& o) noexcept {
+      if (this != &o) {
+         reset();
+         p = o.p;
+         n = o.n;
+         o.p = nullptr;
+         o.n = 0;
+      }
+      return *this;
+   }
+   ~DBuf() { reset(); }
+   void reset() {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+   }
+   void alloc(size_t count) {
+      reset();
+      n = count;
+      if (count) FSVD_CHECK(hipMalloc(&p, sizeof(T) * count));
+   }
+   void upload(const T* h, size_t count, hipStream_t s) {
+      alloc(count);
+      if (count) FSVD_CHECK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+   }
+   void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+struct FarLevel {
+   int level = 0, n = 0, pad = 0, gp3 = 0;
+   int nsrc = 0, ntgt = 0, maxr_src = 0, maxr_tgt = 0;
+   std::vector<SrcDesc> h_src;
+   std::vector<TgtDesc> h_tgt;
+   std::vector<int> h_isrc, h_itoff;     // target-side interaction lists
+   std::vector<TgtDesc> h_tgtT;          // transposed: panel-side receivers (one per source cube)
+   std::vector<int> h_isrcT, h_itoffT;
+   DBuf<SrcDesc> src;
+   DBuf<TgtDesc> tgt, tgtT;
+   DBuf<int> isrc, itoff, isrcT, itoffT;
+   DBuf<double> T64;     // 343 * pad complex
+   DBuf<float> T32;
+   DBuf<double> spec, acc;       // nsrc*pad, ntgt*pad complex (sized for FP64)
+   DBuf<double> specT, accT;     // ntgt*pad, nsrc*pad complex
+   // compressed blocks of this level (offsets in SrcDesc / TgtDesc)
+   DBuf<double> VT64, PV64, UT64, UTI64;
+   DBuf<float> VT32, PV32, UT32, UTI32;
+};
+
+struct TreeOpStats {
+   long long near_entries = 0;     // stored near-field entries (both layers)
+   long long far_entries = 0;      // VT + PV + U^T + UTI entries
+   long long spectra_bins = 0;     // sum over levels of (#src + #tgt) * pad
+   long long translate_macs = 0;   // sum over targets of #interactions * pad
+   int depth = 0, nleaf = 0, ncubes = 0;
+   double setup_ms_near = 0, setup_ms_far = 0;
+};
+
+struct TreeOp {
+   Tree tree;
+   int kernel = LAPLACE;
+   double kappa = 0.0;
+   bool have32 = false;
+   // geometry
+   DBuf<double> pts_tree;     // tree-order point coordinates
+   DBuf<int> pan_perm;        // tree order -> caller panel index
+   // near field
+   int nleaf = 0, maxm = 0, maxp = 0;
+   std::vector<LeafDesc> h_leaves;
+   std::vector<int> h_dcol;   // tree-order panel positions
+   DBuf<LeafDesc> leaves;
+   DBuf<int> dcol;
+   DBuf<double> D64;
+   DBuf<float> D32;
+   // far field
+   std::vector<FarLevel> levels;
+   std::vector<std::vector<double>> h_pinv;   // per level gp3 x 49 (levels >= 2)
+   DBuf<double> yfar;
+   TreeOpStats stats;
+   std::vector<int> cube_src_slot, cube_tgt_slot, cube_leaf;   // per cube: slot in its level / leaf index, -1
+};
+
+// Builds tree, near field and far field for `npan` panels (caller-order
+// geometry already on the device in `d_panels`) and `npt` points.
+std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
+                                      const double* pts, int npt, int kernel, double kappa, int maxpanels,
+                                      int gridpoints, double svdeps, bool want32, hipStream_t s);
+
+// y_t = S xs_t + D xd_t; all vectors in tree order, device pointers.
+void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s);
+// One class of the apply's launches only (0 near, 1 far source, 2 translate,
+// 3 far target), for per-kernel timing.
+void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s);
+// y_t = A_layer^T x_t (panels, tree order).
+void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s);
+
+// launchers (apply.hip / fill.hip)
+template <typename T>
+void launch_far_source(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const double* xs,
+                       const double* xd, void* spec, int maxr, hipStream_t s);
+template <typename T>
+void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const int* itoff, const void* Tspec,
+                          const void* spec, void* acc, int pad, hipStream_t s);
+template <typename T>
+void launch_far_target(int n, int nblk, const TgtDesc* tgt, const T* UT, const T* UTI, const void* acc,
+                       double* yfar, int maxr, hipStream_t s);
+template <typename T>
+void launch_near(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* xs, const double* xd,
+                 const double* yfar, double* y, int maxm, int maxp, hipStream_t s);
+template <typename T>
+void launch_farT_source(int n, int nblk, const TgtDesc* tgt, const T* UT, const T* UTI, const double* x, void* spec,
+                        int maxr, hipStream_t s);
+template <typename T>
+void launch_farT_target(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const void* acc, int layer,
+                        double* yfar, int maxr, hipStream_t s);
+template <typename T>
+void launch_nearT(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* x, int layer,
+                  double* y, hipStream_t s);
+void launch_zero(double* a, long long n, hipStream_t s);
+void launch_integral_blocks(int nblk, const IntBlock* blocks, const double* bpts, const int* bpan,
+                            const PanelGeom* panels, int kernel, double kappa, double* out, hipStream_t s);
+void launch_green_blocks(int nblk, const GreenBlock* blocks, const double* dest, const double* src, int kernel,
+                         double kappa, double* out, hipStream_t s);
+void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStream_t s);
+void launch_mgs(int nprob, int maxrows, const MgsDesc* descs, double* A, double* Q, int* ranks, double eps,
+                hipStream_t s);
+void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s);
+
+}  // namespace fsvd

@@ -1,0 +1,504 @@
+// Region operator, preconditioner, right-hand side, GMRES and reaction
+// potentials on the GPU.
+#include "solver.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+
+namespace fsvd {
+
+void launch_join(int n, const int* pi, const int* di, const double* sp, const double* sd, const double* x, double* xs,
+                 double* xd, hipStream_t s);
+void launch_split(int n, const int* oi, const double* corr, const double* yt, const double* xd, double* y,
+                  hipStream_t s);
+void launch_precond(int nb, const int* ia, const int* ib, const double* M, const double* b, double* x, hipStream_t s);
+void launch_dot_partial(long long n, int np, const double* a, const double* b, double* partial, hipStream_t s);
+void launch_mgs_step(long long n, int np, const double* pin, double* hout, double* w, const double* v,
+                     const double* vnext, double* pout, hipStream_t s);
+void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, double* v,
+                      hipStream_t s);
+void launch_combine(long long n, int m, const double* y, const double* const* V, double* x, hipStream_t s);
+void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
+void launch_rhs(int ne, const double* cent, const int* out, int nq, const double* qpos, const double* q, double* rhs,
+                hipStream_t s);
+void launch_potentials(int nq, int ne, const PanelGeom* panels, const int* pidx, const double* coef,
+                       const int* self_ok, const double* qpos, double* phi, hipStream_t s);
+void launch_leaf_diag(int nleaf, const LeafDesc* leaves, const double* D, double* ds, double* dd, hipStream_t s);
+
+namespace {
+double now_ms() {
+   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+int partial_blocks(long long n) {
+   long long b = (n + 2047) / 2048;
+   if (b < 1) b = 1;
+   if (b > 512) b = 512;
+   return (int)b;
+}
+}  // namespace
+
+Problem::~Problem() {
+   regions.clear();
+   basis.clear();
+   if (h_pinned) (void)hipHostFree(h_pinned);
+   if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Region layout of the salt-without-Stern and no-salt formulations
+// (reference generateSurfaceOperator, FFTSVDpbeAPI.c:1381-1730): an
+// exterior region over every dielectric surface, one interior region per
+// dielectric (own panels followed by its cavities' panels) holding all the
+// charges, one region per cavity with the exterior kernel.
+std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
+   double t0 = now_ms();
+   auto prp = std::make_unique<Problem>();
+   Problem& pr = *prp;
+   FSVD_CHECK(hipStreamCreate(&pr.stream));
+   hipStream_t s = pr.stream;
+   pr.eps_in = spec.eps_in;
+   pr.qpos = spec.qpos;
+   pr.q = spec.q;
+   const int nd = (int)spec.dielectric_verts.size(), nc = (int)spec.cavity_verts.size();
+   std::vector<int> dstart(nd + 1, 0), cstart(nc + 1, 0);
+   for (int i = 0; i < nd; i++) {
+      int n = (int)(spec.dielectric_verts[i].size() / 9);
+      dstart[i + 1] = dstart[i] + n;
+      for (int k = 0; k < n; k++) {
+         const double* v = &spec.dielectric_verts[i][9 * (size_t)k];
+         PanelGeom g;
+         panel_geometry(v, v + 3, v + 6, &g);
+         pr.panels.push_back(g);
+      }
+   }
+   for (int j = 0; j < nc; j++) {
+      int n = (int)(spec.cavity_verts[j].size() / 9);
+      cstart[j + 1] = cstart[j] + n;
+      for (int k = 0; k < n; k++) {
+         const double* v = &spec.cavity_verts[j][9 * (size_t)k];
+         PanelGeom g;
+         panel_geometry(v, v + 3, v + 6, &g);
+         pr.panels.push_back(g);
+      }
+   }
+   pr.ntotal = (int)pr.panels.size();
+   const int ndp = dstart[nd];
+   const int outer_kernel = spec.kappa > 0.0 ? YUKAWA : LAPLACE;
+
+   // regions in pre-order: root, dielectric 0, its cavities, dielectric 1, ...
+   pr.regions.emplace_back();
+   {
+      Region& R = pr.regions[0];
+      R.kernel = outer_kernel;
+      R.eps = spec.eps_out;
+      R.kappa = spec.kappa > 0.0 ? spec.kappa : 0.0;
+      R.mynpan = 0;
+      for (int k = 0; k < ndp; k++) R.panel_ids.push_back(k);
+   }
+   for (int i = 0; i < nd; i++) {
+      int ri = (int)pr.regions.size();
+      pr.regions.emplace_back();
+      pr.regions[0].child.push_back(ri);
+      {
+         Region& D = pr.regions[ri];
+         D.kernel = LAPLACE;
+         D.eps = spec.eps_in;
+         D.kappa = 0.0;
+         D.parent = 0;
+         D.mynpan = dstart[i + 1] - dstart[i];
+         D.has_charges = true;
+         for (int k = dstart[i]; k < dstart[i + 1]; k++) D.panel_ids.push_back(k);
+      }
+      for (int j = 0; j < nc; j++) {
+         if (spec.cavity_parent[j] != i) continue;
+         int rj = (int)pr.regions.size();
+         pr.regions.emplace_back();
+         pr.regions[ri].child.push_back(rj);
+         Region& C = pr.regions[rj];
+         C.kernel = outer_kernel;
+         C.eps = spec.eps_out;
+         C.kappa = spec.kappa > 0.0 ? spec.kappa : 0.0;
+         C.parent = ri;
+         C.mynpan = cstart[j + 1] - cstart[j];
+         for (int k = cstart[j]; k < cstart[j + 1]; k++) C.panel_ids.push_back(ndp + k);
+         for (int k : C.panel_ids) pr.regions[ri].panel_ids.push_back(k);
+      }
+   }
+   // start indices (SurfaceOperator_initStartIndices, SurfaceOperator.c:39-47)
+   {
+      int index = 0;
+      std::vector<int> stack;
+      std::function<void(int)> walk = [&](int r) {
+         for (int c : pr.regions[r].child) {
+            Region& C = pr.regions[c];
+            C.startphi = index;
+            C.startdphi = index + C.mynpan;
+            index += 2 * C.mynpan;
+            walk(c);
+         }
+      };
+      walk(0);
+      if (index != 2 * pr.ntotal) throw std::runtime_error("SurfaceOperator indexing is WRONG!");
+   }
+   // operators (share identical trees: same kernel, same panels)
+   for (size_t r = 0; r < pr.regions.size(); r++) {
+      Region& R = pr.regions[r];
+      for (size_t o = 0; o < r; o++) {
+         const Region& O = pr.regions[o];
+         if (O.shares_tree_with < 0 && O.kernel == R.kernel && O.kappa == R.kappa && O.panel_ids == R.panel_ids) {
+            R.shares_tree_with = (int)o;
+            break;
+         }
+      }
+      std::vector<PanelGeom> tp(R.panel_ids.size());
+      std::vector<double> cent(3 * R.panel_ids.size());
+      for (size_t k = 0; k < R.panel_ids.size(); k++) {
+         tp[k] = pr.panels[R.panel_ids[k]];
+         for (int d = 0; d < 3; d++) cent[3 * k + d] = tp[k].c[d];
+      }
+      if (R.shares_tree_with < 0) {
+         R.d_panels.upload(tp, s);
+         R.op = build_tree_op(tp, R.d_panels.p, cent.data(), (int)R.panel_ids.size(), R.kernel, R.kappa,
+                              spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32, s);
+         if (R.op->tree.pt_perm != R.op->tree.pan_perm) throw std::runtime_error("point/panel partitions differ");
+      }
+   }
+   // join / split maps in tree order
+   for (size_t r = 0; r < pr.regions.size(); r++) {
+      Region& R = pr.regions[r];
+      TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+      int n = (int)R.panel_ids.size();
+      std::vector<int> pi(n), di(n), oi(n);
+      std::vector<double> sp(n), sd(n), co(n);
+      // tree-local index -> (child region, offset)
+      std::vector<int> owner(n, -1), local(n, 0);
+      for (int l = 0; l < R.mynpan; l++) local[l] = l;
+      int cur = R.mynpan;
+      for (int c : R.child) {
+         for (int j = 0; j < pr.regions[c].mynpan; j++) {
+            owner[cur + j] = c;
+            local[cur + j] = j;
+         }
+         cur += pr.regions[c].mynpan;
+      }
+      for (int k = 0; k < n; k++) {
+         int l = op.tree.pan_perm[k];
+         if (owner[l] < 0) {
+            pi[k] = R.startphi + l;
+            di[k] = R.startdphi + l;
+            sp[k] = 1.0;
+            sd[k] = -1.0;
+            oi[k] = R.startphi + l;
+            co[k] = 0.0;
+         } else {
+            const Region& C = pr.regions[owner[l]];
+            int j = local[l];
+            pi[k] = C.startphi + j;
+            di[k] = C.startdphi + j;
+            sp[k] = -1.0;
+            sd[k] = C.eps / R.eps;
+            oi[k] = C.startdphi + j;
+            co[k] = -4.0 * M_PI;
+         }
+      }
+      R.phi_idx.upload(pi, s);
+      R.dphi_idx.upload(di, s);
+      R.out_idx.upload(oi, s);
+      R.s_phi.upload(sp, s);
+      R.s_dphi.upload(sd, s);
+      R.corr.upload(co, s);
+      R.xs.alloc(n + 1);
+      R.xd.alloc(n + 1);
+      R.yt.alloc(n + 1);
+   }
+   // preconditioner: 2x2 blocks per owned panel from the region's own near-field diagonal
+   // (SurfaceOperator_generatePreconditioner, SurfaceOperator.c:170-197)
+   {
+      std::vector<int> pa, pb;
+      std::vector<double> pm;
+      for (size_t r = 0; r < pr.regions.size(); r++) {
+         Region& R = pr.regions[r];
+         if (R.mynpan == 0) continue;
+         TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+         int n = (int)R.panel_ids.size();
+         DBuf<double> dds, ddd;
+         dds.alloc(n);
+         ddd.alloc(n);
+         launch_leaf_diag(op.nleaf, op.leaves.p, op.D64.p, dds.p, ddd.p, s);
+         std::vector<double> ds(n), dd(n);
+         FSVD_CHECK(hipMemcpyAsync(ds.data(), dds.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipMemcpyAsync(dd.data(), ddd.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+         std::vector<double> dsl(n), ddl(n);
+         for (int k = 0; k < n; k++) {
+            dsl[op.tree.pan_perm[k]] = ds[k];
+            ddl[op.tree.pan_perm[k]] = dd[k];
+         }
+         double rel = R.parent >= 0 ? R.eps / pr.regions[R.parent].eps : 1.0;
+         for (int i = 0; i < R.mynpan; i++) {
+            double a11 = ddl[i], a12 = -dsl[i], a21 = ddl[i], a22 = rel * dsl[i];
+            double det = a11 * a22 - a12 * a21;
+            if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+               throw std::runtime_error("singular preconditioner block");
+            pa.push_back(R.startphi + i);
+            pb.push_back(R.startdphi + i);
+            pm.push_back(a22 / det);
+            pm.push_back(-a12 / det);
+            pm.push_back(-a21 / det);
+            pm.push_back(a11 / det);
+         }
+      }
+      pr.nblocks = (int)pa.size();
+      pr.pc_a.upload(pa, s);
+      pr.pc_b.upload(pb, s);
+      pr.pc_m.upload(pm, s);
+   }
+   pr.d_qpos.upload(pr.qpos, s);
+   pr.d_q.upload(pr.q, s);
+   long long n2 = 2LL * pr.ntotal;
+   pr.w.alloc(n2);
+   pr.r.alloc(n2);
+   pr.tmp.alloc(n2);
+   pr.partial.alloc(2 * 512);
+   pr.hbuf.alloc(1024 + 8);
+   FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned, sizeof(double) * 2048));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   pr.setup_ms = now_ms() - t0;
+   return prp;
+}
+
+void problem_matvec(Problem& pr, int prec, const double* x, double* y) {
+   hipStream_t s = pr.stream;
+   for (Region& R : pr.regions) {
+      int n = (int)R.panel_ids.size();
+      if (n == 0) continue;
+      TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+      launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x, R.xs.p, R.xd.p, s);
+      tree_apply(op, prec, R.xs.p, R.xd.p, R.yt.p, s);
+      launch_split(n, R.out_idx.p, R.corr.p, R.yt.p, R.xd.p, y, s);
+   }
+}
+
+void problem_precond(Problem& pr, const double* b, double* x) {
+   launch_precond(pr.nblocks, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, b, x, pr.stream);
+}
+
+void problem_rhs(Problem& pr, double* rhs) {
+   hipStream_t s = pr.stream;
+   long long n2 = 2LL * pr.ntotal;
+   FSVD_CHECK(hipMemsetAsync(rhs, 0, sizeof(double) * n2, s));
+   for (Region& R : pr.regions) {
+      if (!R.has_charges) continue;
+      std::vector<double> cent;
+      std::vector<int> out;
+      for (int j = 0; j < R.mynpan; j++) {
+         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         out.push_back(R.startphi + j);
+      }
+      for (int c : R.child) {
+         const Region& C = pr.regions[c];
+         for (int j = 0; j < C.mynpan; j++) {
+            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            out.push_back(C.startdphi + j);
+         }
+      }
+      DBuf<double> dc;
+      DBuf<int> dout;
+      dc.upload(cent, s);
+      dout.upload(out, s);
+      launch_rhs((int)out.size(), dc.p, dout.p, (int)pr.q.size(), pr.d_qpos.p, pr.d_q.p, rhs, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+}
+
+static void givens(double a, double b, double* c, double* s) {
+   if (b == 0.0) {
+      *c = 1.0;
+      *s = 0.0;
+   } else {
+      double r1 = std::fabs(a), r2 = std::fabs(b);
+      double L = std::sqrt(r1 * r1 + r2 * r2);
+      *c = a / L;
+      *s = -b / L;
+   }
+}
+static void grot(double c, double s, double* a, double* b) {
+   double a1 = *a, b1 = *b;
+   *a = a1 * c - b1 * s;
+   *b = c * b1 + s * a1;
+}
+
+static double* basis_vec(Problem& pr, int j) {
+   while ((int)pr.basis.size() <= j) {
+      pr.basis.emplace_back(new DBuf<double>());
+      pr.basis.back()->alloc(2LL * pr.ntotal);
+   }
+   return pr.basis[j]->p;
+}
+
+SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, double tol, int maxit,
+                        std::vector<double>* hist) {
+   double t0 = now_ms();
+   SolveInfo info;
+   hipStream_t s = pr.stream;
+   const long long n = 2LL * pr.ntotal;
+   const int np = partial_blocks(n);
+   double* r = pr.r.p;
+   double* w = pr.w.p;
+   double* P0 = pr.partial.p;
+   double* P1 = pr.partial.p + 512;
+   problem_precond(pr, rhs, r);
+   launch_dot_partial(n, np, r, r, P0, s);
+   FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, P0, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   double nr2 = 0.0;
+   for (int k = 0; k < np; k++) nr2 += pr.h_pinned[k];
+   double normr = std::sqrt(nr2);
+   if (normr < tol) {
+      FSVD_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * n, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      info.ms = now_ms() - t0;
+      return info;
+   }
+   const int W = maxit + 1;
+   std::vector<double> H((size_t)W * W, 0.0), c(W, 0.0), sn(W, 0.0), g(W + 1, 0.0), y(W, 0.0);
+   g[0] = normr;
+   launch_normalize(n, np, P0, pr.hbuf.p, r, basis_vec(pr, 0), s);
+   int i;
+   for (i = 0; i < maxit; i++) {
+      double* vi = basis_vec(pr, i);
+      double* vnext_store = basis_vec(pr, i + 1);
+      problem_matvec(pr, prec, vi, w);
+      if (prec == PREC_F32)
+         info.matvecs32++;
+      else
+         info.matvecs64++;
+      problem_precond(pr, w, w);
+      launch_dot_partial(n, np, w, basis_vec(pr, 0), P0, s);
+      for (int j = 0; j <= i; j++) {
+         double* pin = (j % 2 == 0) ? P0 : P1;
+         double* pout = (j % 2 == 0) ? P1 : P0;
+         const double* vn = (j < i) ? basis_vec(pr, j + 1) : nullptr;
+         launch_mgs_step(n, np, pin, pr.hbuf.p + j, w, basis_vec(pr, j), vn, pout, s);
+      }
+      double* plast = ((i + 1) % 2 == 0) ? P0 : P1;
+      launch_normalize(n, np, plast, pr.hbuf.p + i + 1, w, vnext_store, s);
+      FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, pr.hbuf.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      double* Hi = &H[(size_t)i * W];
+      for (int j = 0; j <= i + 1; j++) Hi[j] = pr.h_pinned[j];
+      for (int k = 0; k < i; k++) grot(c[k], sn[k], &Hi[k], &Hi[k + 1]);
+      givens(Hi[i], Hi[i + 1], &c[i], &sn[i]);
+      grot(c[i], sn[i], &Hi[i], &Hi[i + 1]);
+      g[i + 1] = 0.0;
+      grot(c[i], sn[i], &g[i], &g[i + 1]);
+      double res = std::fabs(g[i + 1]) / normr;
+      if (hist) hist->push_back(res);
+      info.final_residual = res;
+      if (res < tol) break;
+   }
+   if (i == maxit) i = maxit - 1;
+   for (int k = 0; k <= i; k++) y[k] = g[k];
+   for (int k = i; k >= 0; k--) {
+      y[k] /= H[(size_t)k * W + k];
+      for (int j = k - 1; j >= 0; j--) y[j] -= H[(size_t)k * W + j] * y[k];
+   }
+   info.iterations = i + 1;
+   {
+      DBuf<double> dy;
+      dy.upload(y.data(), i + 1, s);
+      std::vector<const double*> ptrs(i + 1);
+      for (int j = 0; j <= i; j++) ptrs[j] = basis_vec(pr, j);
+      DBuf<const double*> dp;
+      dp.upload(ptrs, s);
+      launch_combine(n, i + 1, dy.p, dp.p, sol, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   info.ms = now_ms() - t0;
+   return info;
+}
+
+static double precond_norm(Problem& pr, const double* v) {
+   const long long n = 2LL * pr.ntotal;
+   const int np = partial_blocks(n);
+   problem_precond(pr, v, pr.tmp.p);
+   launch_dot_partial(n, np, pr.tmp.p, pr.tmp.p, pr.partial.p, pr.stream);
+   FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, pr.partial.p, sizeof(double) * np, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   double a = 0.0;
+   for (int k = 0; k < np; k++) a += pr.h_pinned[k];
+   return std::sqrt(a);
+}
+
+SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit,
+                        std::vector<double>* hist) {
+   if (mode == PREC_F64) return problem_gmres(pr, PREC_F64, rhs, sol, tol, maxit, hist);
+   double t0 = now_ms();
+   hipStream_t s = pr.stream;
+   const long long n = 2LL * pr.ntotal;
+   SolveInfo tot = problem_gmres(pr, PREC_F32, rhs, sol, tol, maxit, hist);
+   double bn = precond_norm(pr, rhs);
+   DBuf<double> res, e;
+   res.alloc(n);
+   e.alloc(n);
+   for (int it = 0; it < 4; it++) {
+      problem_matvec(pr, PREC_F64, sol, pr.tmp.p);
+      tot.matvecs64++;
+      launch_axpby(n, 1.0, rhs, -1.0, pr.tmp.p, res.p, s);
+      double rn = precond_norm(pr, res.p);
+      tot.final_residual = bn > 0 ? rn / bn : rn;
+      if (!(tot.final_residual >= tol)) break;
+      double inner = std::min(0.5, 0.5 * tol / tot.final_residual);
+      SolveInfo si = problem_gmres(pr, PREC_F32, res.p, e.p, inner, maxit, nullptr);
+      tot.iterations += si.iterations;
+      tot.matvecs32 += si.matvecs32;
+      tot.refinements++;
+      launch_axpby(n, 1.0, sol, 1.0, e.p, sol, s);
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   tot.ms = now_ms() - t0;
+   return tot;
+}
+
+void problem_potentials(Problem& pr, const double* sol, double* phi_out) {
+   hipStream_t s = pr.stream;
+   const int nq = (int)pr.q.size();
+   DBuf<PanelGeom> dall;
+   dall.upload(pr.panels, s);
+   DBuf<double> dphi;
+   dphi.alloc(std::max(nq, 1));
+   FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max(nq, 1), s));
+   for (Region& R : pr.regions) {
+      if (!R.has_charges) continue;
+      std::vector<int> pidx, self;
+      std::vector<double> coef;
+      for (int i = 0; i < R.mynpan; i++) {
+         pidx.push_back(R.panel_ids[i]);
+         coef.push_back(sol[R.startdphi + i]);
+         coef.push_back(sol[R.startphi + i]);
+         self.push_back(1);
+      }
+      for (int c : R.child) {
+         const Region& C = pr.regions[c];
+         for (int i = 0; i < C.mynpan; i++) {
+            pidx.push_back(C.panel_ids[i]);
+            coef.push_back(-C.eps / R.eps * sol[C.startdphi + i]);
+            coef.push_back(-sol[C.startphi + i]);
+            self.push_back(0);
+         }
+      }
+      DBuf<int> dp, dsf;
+      DBuf<double> dc;
+      dp.upload(pidx, s);
+      dsf.upload(self, s);
+      dc.upload(coef, s);
+      launch_potentials(nq, (int)pidx.size(), dall.p, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace fsvd

@@ -1,0 +1,88 @@
+// Nested region operator over 2N unknowns (phi and dphi/dn per panel), its
+// block preconditioner and the GMRES driver, all device resident.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "op.h"
+
+namespace fsvd {
+
+struct SurfaceSpec {
+   // Surfaces in the order the caller numbers its panels: each dielectric
+   // surface, then each dielectric cavity.  Cavity parent = index of the
+   // dielectric that encloses it.
+   std::vector<std::vector<double>> dielectric_verts;   // per dielectric, npan*9
+   std::vector<std::vector<double>> cavity_verts;       // per cavity, npan*9
+   std::vector<int> cavity_parent;
+   std::vector<double> qpos, q;                         // all atoms
+   double eps_in = 4.0, eps_out = 80.0, kappa = 0.0;
+   int maxpanels = 32, gridpoints = 4;
+   double svdeps = 1e-5;
+   bool want32 = true;
+};
+
+struct Region {
+   int kernel = LAPLACE;
+   double eps = 1.0, kappa = 0.0;
+   int mynpan = 0;
+   int startphi = 0, startdphi = 0;
+   int parent = -1;
+   std::vector<int> child;
+   std::vector<int> panel_ids;       // global panel ids of the tree panels (own then children's)
+   std::unique_ptr<TreeOp> op;
+   DBuf<PanelGeom> d_panels;         // tree-local panel geometry
+   DBuf<int> phi_idx, dphi_idx, out_idx;
+   DBuf<double> s_phi, s_dphi, corr;
+   DBuf<double> xs, xd, yt;
+   bool has_charges = false;
+   int shares_tree_with = -1;        // reuse another region's operator (same panels and kernel)
+};
+
+struct SolveInfo {
+   int iterations = 0;
+   int refinements = 0;
+   double final_residual = 0.0;
+   double ms = 0.0;
+   int matvecs64 = 0, matvecs32 = 0;
+};
+
+struct Problem {
+   hipStream_t stream = nullptr;
+   int ntotal = 0;                       // total panels
+   std::vector<PanelGeom> panels;        // global panel geometry
+   std::vector<Region> regions;          // regions[0] = exterior (root)
+   std::vector<double> qpos, q;
+   double eps_in = 4.0;
+   // preconditioner
+   int nblocks = 0;
+   DBuf<int> pc_a, pc_b;
+   DBuf<double> pc_m;
+   // device charges
+   DBuf<double> d_qpos, d_q;
+   // scratch
+   DBuf<double> w, r, tmp, partial, hbuf;
+   std::vector<std::unique_ptr<DBuf<double>>> basis;
+   double* h_pinned = nullptr;
+   double setup_ms = 0.0;
+   ~Problem();
+};
+
+std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec);
+
+// y = A x over the 2N unknowns (reference SurfaceOperator_topmultiply,
+// SurfaceOperator.c:90-97); device pointers.
+void problem_matvec(Problem& pr, int prec, const double* x, double* y);
+void problem_precond(Problem& pr, const double* b, double* x);
+void problem_rhs(Problem& pr, double* rhs_dev);
+// GMRES (reference GMRES_SurfaceOperator, GMRES.c:446-539); device vectors.
+SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, double tol, int maxit,
+                        std::vector<double>* hist);
+// Mixed precision: FP32-operator GMRES with FP64 residual refinement.
+SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit,
+                        std::vector<double>* hist);
+void problem_potentials(Problem& pr, const double* sol_host, double* phi_host);
+
+}  // namespace fsvd

@@ -1,0 +1,92 @@
+/*
+ * Extended C-ABI of libfftsvd_hip.so: array-based entry points with status
+ * codes, precision selection, device-resident benchmarking and the views
+ * the parity tests need.  None of these exist in the reference; each says
+ * which reference routine it mirrors.  Return value: 0 on success,
+ * negative on error (message in fftsvd_last_error()).
+ *
+ * Vectors of unknowns have length 2N in the reference's global order
+ * (per region phi block then dphi/dn block, regions in pre-order,
+ * SurfaceOperator.c:39-47).  Precision: 0 = FP64 storage and arithmetic,
+ * 1 = FP32 storage/arithmetic of the compressed operator with FP64 Krylov
+ * vectors ("mixed").
+ */
+#ifndef FFTSVD_HIP_H
+#define FFTSVD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fftsvd_problem fftsvd_problem;
+
+const char* fftsvd_last_error(void);
+int fftsvd_device_info(char* name, int len, int* num_cu);
+
+/* One dielectric surface (panels as v0,v1,v2 triples, the vertex order the
+ * reference builds from a flipped MSMS face), all charges inside; salt if
+ * kappa > 0 (mirrors generateSurfaceOperator + generateSurfaceOperatorPreconditioner,
+ * FFTSVDpbeAPI.c:1381-1730,1853-1861). */
+int fftsvd_problem_create(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
+                          double eps_in, double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
+                          int want_f32, fftsvd_problem** out);
+/* General surfaces: ndiel dielectric surfaces and ncav cavities, panel
+ * counts in npan_diel / npan_cav, vertices concatenated in that order,
+ * cavity parents index the dielectrics. */
+int fftsvd_problem_create_multi(int ndiel, const int* npan_diel, int ncav, const int* npan_cav,
+                                const int* cav_parent, const double* verts9, const double* qpos, const double* q,
+                                int ncharge, double eps_in, double eps_out, double kappa, int maxpanels,
+                                int gridpoints, double svdeps, int want_f32, fftsvd_problem** out);
+int fftsvd_problem_destroy(fftsvd_problem* p);
+int fftsvd_problem_size(fftsvd_problem* p, int* npanels, int* ncharges);
+
+/* y = A x (SurfaceOperator_topmultiply, SurfaceOperator.c:90-97) */
+int fftsvd_problem_matvec(fftsvd_problem* p, int precision, const double* x, double* y);
+/* x = P^{-1} b (Preconditioner_solve, Preconditioner.c:272-294) */
+int fftsvd_problem_precond(fftsvd_problem* p, const double* b, double* x);
+/* rhs (SurfaceOperator_makeRHS, SurfaceOperator.c:291-321) */
+int fftsvd_problem_rhs(fftsvd_problem* p, double* rhs);
+/* GMRES to tol (GMRES_SurfaceOperator, GMRES.c:446-539).  precision 1
+ * adds FP64 residual refinement; iters (optional) receives the iteration
+ * count, resid_hist (optional, maxit entries) the residual history, info
+ * (optional, 6 doubles): iterations, refinements, final residual, ms,
+ * FP64 matvecs, FP32 matvecs. */
+int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, double* sol, double tol, int maxit,
+                         int* iters, double* resid_hist, double* info);
+/* reaction potentials at the charges (SurfaceOperator.c:323-401 direct branch) */
+int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi);
+/* operator statistics: out[0..15] (see fftsvd_hip.cpp for the layout) */
+int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout);
+/* Times `nrep` surface matvecs on device-resident vectors with HIP events
+ * on the library stream: total ms and the per-kernel-class ms (near, far
+ * source, translate, far target, glue) from a separately timed pass. */
+int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, double* ms_total, double* ms_parts);
+
+/* Per-tree views for parity tests.  which = region index (0 = exterior). */
+int fftsvd_tree_info(fftsvd_problem* p, int which, int* depth, int* ncubes, int* npanels);
+/* info[13] per cube: level, i, j, k, leaf, npan, npt, nloc, nint, rowrank, colrank, parent, dcols */
+int fftsvd_tree_cube_info(fftsvd_problem* p, int which, int cube, int* info);
+/* caller (tree-local) panel indices of the cube in tree order, and of the
+ * leaf's near-field columns */
+int fftsvd_tree_cube_panels(fftsvd_problem* p, int which, int cube, int* pan, int* dcols);
+/* FP64 blocks of one cube: VT (r x np), U (np x r'), PVs/PVd (gp3 x r),
+ * UTI (r' x gp3), Ds/Dd (p x m); rows/columns in tree order. */
+int fftsvd_tree_cube_mats(fftsvd_problem* p, int which, int cube, double* VT, double* U, double* PVs, double* PVd,
+                          double* UTI, double* Ds, double* Dd);
+/* y = S xs + D xd of one tree (Tree_multiplyboth, Tree.c:597-683), caller order */
+int fftsvd_tree_apply(fftsvd_problem* p, int which, int precision, const double* xs, const double* xd, double* y);
+/* y = A_layer^T x of one tree (Tree_multiply_transpose, Tree.c:769-832) */
+int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int layer, const double* x, double* y);
+
+/* Panel integrals on the GPU: slp, dlp (npt x npan) for kernel 0 = Laplace,
+ * 1 = Yukawa (Integration, Integration.c:32-185). */
+int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
+                     double* slp, double* dlp);
+
+/* Precision used by solveSurfaceOperator (0 FP64, 1 mixed); default 0. */
+int fftsvd_set_precision(int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

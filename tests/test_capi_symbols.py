@@ -1,0 +1,38 @@
+"""The C-ABI library loads on a CPU-only host and exports every function
+declared in include/*.h (no compute call is made here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import fftsvd_amd as fa
+
+
+def declared_functions():
+    names = []
+    for h in fa.HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z_0-9\s\*]*?\b([A-Za-z_][A-Za-z_0-9]*)\s*\(", text,
+                             flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "for", "while", "switch", "return", "sizeof"):
+                names.append(name)
+    return sorted(set(names))
+
+
+def test_headers_declare_solve_path():
+    names = declared_functions()
+    for required in ("readSRF", "generateSurfaceOperator", "generateSurfaceOperatorPreconditioner", "generateRHS",
+                     "solveSurfaceOperator", "collectPotentials", "fftsvd_problem_create", "fftsvd_problem_solve",
+                     "fftsvd_tree_apply", "fftsvd_tree_apply_transpose"):
+        assert required in names
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(fa.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(fa.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing

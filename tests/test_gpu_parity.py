@@ -1,0 +1,208 @@
+"""GPU parity: the HIP implementation (through the C-ABI) against the FP64
+CPU oracle on the same seeded inputs.
+
+Gates (FP64 mode): panel integrals <= 1e-12 (relative to the row scale),
+identical compressed operator -> matvec <= 1e-10 relative (the FP64 matvec
+gate), independently built operators agree in every integer decision (tree,
+lists, ranks) and to <= 1e-8 in the matvec, preconditioner / RHS /
+potentials <= 1e-12, GMRES iteration counts within one and energies within
+1e-8 relative.  Mixed mode: solvation energy within 1e-6 relative of FP64.
+"""
+import numpy as np
+import pytest
+
+import fftsvd_amd as fa
+import oracle
+from fftsvd_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def blob(sub=3, seed=7):
+    V, F = meshes.ellipsoid((3.0, 2.2, 1.6), subdivisions=sub, jitter=0.04, seed=seed)
+    return meshes.panels_from_mesh(V, F)
+
+
+def charges(n=6, seed=1):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-0.8, 0.8, size=(n, 3)), rng.uniform(-0.8, 0.8, size=n)
+
+
+@pytest.fixture(scope="module")
+def salt_pair():
+    P = blob(3)
+    qpos, q = charges()
+    g = fa.Problem(P, qpos, q, 4.0, 80.0, 0.124)
+    o = oracle.Problem(P, qpos, q, 4.0, 80.0, 0.124)
+    return P, g, o
+
+
+@pytest.mark.parametrize("kernel,kappa", [(fa.LAPLACE, 0.0), (fa.YUKAWA, 0.124), (fa.YUKAWA, 1.3)])
+def test_integrals_match_oracle(kernel, kappa):
+    P = blob(2, seed=3)
+    rng = np.random.default_rng(0)
+    cent = P.reshape(-1, 3, 3).mean(axis=1)
+    pts = np.vstack([cent[:40], rng.normal(scale=2.0, size=(40, 3))])
+    # exact centroids (self terms) from the oracle's own arithmetic
+    pts[:40] = np.array([oracle.panel_geom(p)["centroid"] for p in P[:40]])
+    gs, gd = fa.integrate(P, pts, kernel, kappa)
+    os_, od = oracle.integrate(P, pts, kernel, kappa)
+    scale_s = np.abs(os_).max(axis=1, keepdims=True)
+    scale_d = np.abs(od).max(axis=1, keepdims=True)
+    assert np.max(np.abs(gs - os_) / scale_s) <= 1e-12
+    assert np.max(np.abs(gd - od) / scale_d) <= 1e-12
+
+
+def _key(c):
+    return (c["level"],) + tuple(c["idx"])
+
+
+def test_tree_structure_and_near_field(salt_pair):
+    P, g, o = salt_pair
+    for which in (0, 1):
+        gt, ot = g.tree(which), o.tree(which)
+        assert gt.depth == ot.depth
+        gc = {_key(c): c for c in gt.cubes()}
+        oc = {_key(c): c for c in ot.cubes()}
+        assert set(gc) == set(oc)
+        gpl = ot.gpl()
+        rank_mismatch = 0
+        for k, c in oc.items():
+            d = gc[k]
+            assert sorted(d["pan"].tolist()) == sorted(c["pan"].tolist())
+            assert d["nint"] == len(c["itr"]) and d["nloc"] == len(c["loc"])
+            rank_mismatch += (d["rowrank"] != c["rowrank"]) + (d["colrank"] != c["colrank"])
+            if c["leaf"] and len(c["pt"]):
+                om = ot.cube_mats(c)
+                gm = gt.cube_mats(d, int(gpl[c["level"]]) ** 3)
+                # identical column order: own panels then local cubes in list order
+                assert d["dcols"].shape[0] == c["dcols"]
+                sc = np.abs(om["Ds"]).max()
+                assert np.abs(gm["Ds"] - om["Ds"]).max() <= 1e-12 * sc
+                sd = np.abs(om["Dd"]).max()
+                assert np.abs(gm["Dd"] - om["Dd"]).max() <= 1e-12 * sd
+        assert rank_mismatch == 0
+
+
+def _import_gpu_operator(g, o, which):
+    """Load the GPU-built compressed blocks into the oracle's tree (mapping
+    tree-order rows/columns back to the oracle's cube orders)."""
+    gt, ot = g.tree(which), o.tree(which)
+    gpl = ot.gpl()
+    gc = {_key(c): c for c in gt.cubes()}
+    for c in ot.cubes():
+        d = gc[_key(c)]
+        gm = gt.cube_mats(d, int(gpl[c["level"]]) ** 3)
+        pos = {int(p): k for k, p in enumerate(d["pan"])}
+        perm = np.array([pos[int(p)] for p in c["pan"]], dtype=int) if len(c["pan"]) else np.zeros(0, int)
+        VT = gm["VT"][:, perm] if d["rowrank"] else None
+        U = gm["U"][perm, :] if d["colrank"] else None
+        Ds = gm["Ds"] if (c["leaf"] and len(c["pt"])) else None
+        Dd = gm["Dd"] if (c["leaf"] and len(c["pt"])) else None
+        ot.set_cube_mats(c["id"], d["rowrank"], d["colrank"], Ds, Dd, VT, U,
+                         gm["PVs"] if d["rowrank"] else None, gm["PVd"] if d["rowrank"] else None,
+                         gm["UTI"] if d["colrank"] else None)
+
+
+def test_tree_apply_identical_operator_fp64(salt_pair):
+    P, g, o = salt_pair
+    rng = np.random.default_rng(5)
+    for which in (0, 1):
+        _import_gpu_operator(g, o, which)
+        xs = rng.uniform(-1, 1, len(P))
+        xd = rng.uniform(-1, 1, len(P))
+        yo = o.tree(which).apply_both(xs, xd)
+        yg = g.tree(which).apply_both(xs, xd, fa.F64)
+        assert np.linalg.norm(yg - yo) <= 1e-10 * np.linalg.norm(yo)
+        # transpose, per layer
+        x = rng.uniform(-1, 1, len(P))
+        for layer in (0, 1):
+            to = o.tree(which).apply_transpose(layer, x)
+            tg = g.tree(which).apply_transpose(layer, x, fa.F64)
+            assert np.linalg.norm(tg - to) <= 1e-10 * np.linalg.norm(to)
+
+
+def test_surface_matvec_identical_operator_fp64(salt_pair):
+    P, g, o = salt_pair
+    for which in (0, 1):
+        _import_gpu_operator(g, o, which)
+    x = np.random.default_rng(9).uniform(-1, 1, 2 * len(P))
+    yo = o.matvec(x)
+    yg = g.matvec(x, fa.F64)
+    assert np.linalg.norm(yg - yo) <= 1e-10 * np.linalg.norm(yo)
+
+
+def test_mixed_matvec_close(salt_pair):
+    P, g, o = salt_pair
+    x = np.random.default_rng(10).uniform(-1, 1, 2 * len(P))
+    y64 = g.matvec(x, fa.F64)
+    y32 = g.matvec(x, fa.MIXED)
+    assert np.linalg.norm(y32 - y64) <= 1e-5 * np.linalg.norm(y64)
+
+
+def test_transpose_dot_product_gpu(salt_pair):
+    P, g, o = salt_pair
+    rng = np.random.default_rng(3)
+    t = g.tree(0)
+    for layer in (0, 1):
+        x = rng.uniform(-1, 1, len(P))
+        y = rng.uniform(-1, 1, len(P))
+        z = np.zeros(len(P))
+        Ax = t.apply_both(x, z) if layer == 0 else t.apply_both(z, x)
+        ATy = t.apply_transpose(layer, y)
+        assert abs(np.dot(Ax, y) - np.dot(x, ATy)) <= 1e-11 * np.linalg.norm(Ax) * np.linalg.norm(y)
+
+
+def test_rhs_precond_potentials(salt_pair):
+    P, g, o = salt_pair
+    ro, rg = o.rhs(), g.rhs()
+    assert np.abs(rg - ro).max() <= 1e-14 * np.abs(ro).max()
+    b = np.random.default_rng(4).normal(size=2 * len(P))
+    po, pg = o.precond_solve(b), g.precond_solve(b)
+    assert np.abs(pg - po).max() <= 1e-12 * np.abs(po).max()
+    sol = np.random.default_rng(6).normal(size=2 * len(P))
+    fo, fg = o.reaction_potentials(sol), g.reaction_potentials(sol)
+    assert np.abs(fg - fo).max() <= 1e-12 * np.abs(fo).max()
+
+
+def test_gmres_and_energy_match_oracle(salt_pair):
+    P, g, o = salt_pair
+    for which in (0, 1):
+        _import_gpu_operator(g, o, which)
+    rhs = o.rhs()
+    so, ito, ho = o.gmres(rhs, 1e-6)
+    sg, itg, hg, info = g.solve(rhs, 1e-6, precision=fa.F64)
+    assert abs(itg - ito) <= 1
+    n = min(len(ho), len(hg))
+    np.testing.assert_allclose(hg[:n], ho[:n], rtol=1e-6)
+    Eo, Eg = o.energy(so), g.energy(sg)
+    assert abs(Eg - Eo) <= 1e-8 * abs(Eo)
+
+
+@pytest.mark.parametrize("kappa", [0.0, 0.124])
+def test_sphere_energies_analytic(kappa):
+    """Born (kappa = 0) and Debye-Hueckel sphere energies on the GPU, FP64
+    and mixed: analytic within 1% (flat-panel discretisation), mixed within
+    1e-6 of FP64."""
+    V, F = meshes.icosphere(2.0, 3)
+    P = meshes.panels_from_mesh(V, F)
+    g = fa.Problem(P, [[0, 0, 0]], [1.0], 4.0, 80.0, kappa)
+    rhs = g.rhs()
+    s64, it64, _, _ = g.solve(rhs, 1e-6, precision=fa.F64)
+    s32, it32, _, info = g.solve(rhs, 1e-6, precision=fa.MIXED)
+    E64 = g.energy(s64, 332.06)
+    E32 = g.energy(s32, 332.06)
+    a = 2.0
+    Ea = 332.06 * 0.5 * (1 / (80 * a * (1 + kappa * a)) - 1 / (4 * a))
+    assert abs(E64 - Ea) < 0.01 * abs(Ea)
+    assert abs(E32 - E64) <= 1e-6 * abs(E64)
+
+
+def test_mixed_energy_blob(salt_pair):
+    P, g, o = salt_pair
+    rhs = g.rhs()
+    s64, _, _, _ = g.solve(rhs, 1e-6, precision=fa.F64)
+    s32, _, _, info = g.solve(rhs, 1e-6, precision=fa.MIXED)
+    E64, E32 = g.energy(s64), g.energy(s32)
+    assert abs(E32 - E64) <= 1e-6 * abs(E64)
+    assert info["final_residual"] < 1e-6
