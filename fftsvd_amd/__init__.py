@@ -241,7 +241,7 @@ class Problem:
         out = np.zeros(16)
         _check(lib().fftsvd_problem_stats(self.h, _d(out), 16))
         keys = ("panels", "regions", "setup_ms", "near_entries", "far_entries", "spectra_bins", "translate_macs",
-                "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32")
+                "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32", "near_cols")
         return dict(zip(keys, out.tolist()))
 
     def bench_matvec(self, precision=F64, nrep=20):

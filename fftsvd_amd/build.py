@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libfftsvd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["geometry.cpp", "tree.cpp", "host_linalg.cpp", "op.cpp", "solver.cpp", "capi.cpp",
+SOURCES = ["geometry.cpp", "tree.cpp", "host_linalg.cpp", "op.cpp", "solver.cpp", "capi.cpp", "mesh.cpp",
            "fill.hip", "apply.hip", "solve.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-x", "hip",

@@ -230,7 +230,7 @@ static TreeOp& region_op(Problem& pr, int which) {
  * entries, 5 spectra bins, 6 translate complex MACs, 7 max depth,
  * 8 leaves, 9 unique trees, 10 algorithmic bytes per FP64 matvec,
  * 11 algorithmic bytes per FP32 matvec, 12 near bytes per FP64 matvec,
- * 13 near bytes per FP32 matvec */
+ * 13 near bytes per FP32 matvec, 14 near-field column indices (all trees) */
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
    API_TRY
    Problem& pr = *p->pr;
@@ -249,6 +249,8 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
             v[4] += op.stats.far_entries;
             v[5] += op.stats.spectra_bins;
             v[6] += op.stats.translate_macs;
+            v[14] += (double)op.h_dcol.size();
+            v[9] += 1;
          }
          continue;
       }
@@ -260,6 +262,7 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
       v[7] = std::max<double>(v[7], op.stats.depth);
       v[8] += op.stats.nleaf;
       v[9] += 1;
+      v[14] += (double)op.h_dcol.size();
    }
    // algorithmic bytes: stored operator entries once, spectra written and
    // read once, vectors (2 gathers + 1 write per point per tree) and glue

@@ -449,13 +449,16 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
       launch_axpby(n, 1.0, rhs, -1.0, pr.tmp.p, res.p, s);
       double rn = precond_norm(pr, res.p);
       tot.final_residual = bn > 0 ? rn / bn : rn;
-      if (!(tot.final_residual >= tol)) break;
+      if (!(tot.final_residual >= tol) || !(rn > 0.0)) break;
+      // correction solve on the normalised residual (GMRES's zero-solution
+      // test is absolute, GMRES.c:461-465), scaled back afterwards
+      launch_axpby(n, 1.0 / rn, res.p, 0.0, res.p, res.p, s);
       double inner = std::min(0.5, 0.5 * tol / tot.final_residual);
       SolveInfo si = problem_gmres(pr, PREC_F32, res.p, e.p, inner, maxit, nullptr);
       tot.iterations += si.iterations;
       tot.matvecs32 += si.matvecs32;
       tot.refinements++;
-      launch_axpby(n, 1.0, sol, 1.0, e.p, sol, s);
+      launch_axpby(n, 1.0, sol, rn, e.p, sol, s);
    }
    FSVD_CHECK(hipStreamSynchronize(s));
    tot.ms = now_ms() - t0;

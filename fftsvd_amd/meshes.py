@@ -83,3 +83,84 @@ def write_vertface(basename, V, F, genus=None):
     with open(basename + ".face", "w") as f:
         for (a, b, c), gg in zip(F, g):
             f.write("%d %d %d %d\n" % (a + 1, b + 1, c + 1, gg))
+
+
+def synthetic_globule(natoms=2000, seed=20261202, step=1.5, minsep=2.4, radius_range=(1.2, 2.0),
+                      charge_range=(-0.8, 0.8), confine=26.0):
+    """Seeded self-avoiding random walk of `natoms` atoms (step 1.5 A, every
+    non-bonded pair at least 2.4 A apart), confined to a sphere of radius `confine` so
+    the chain folds into a globule; when none of 64 random directions is
+    free the walk continues from a random earlier atom.  Radii uniform
+    in radius_range; charges uniform in charge_range, each shifted by
+    (round(sum q) - sum q)/natoms so the net charge is an integer.
+    Returns (xyz (n,3), radii (n,), charges (n,))."""
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((natoms, 3))
+    n = 1
+    cur = 0
+    m2 = minsep * minsep
+    while n < natoms:
+        # 64 candidate directions at once; the first free one is taken
+        d = rng.normal(size=(64, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        cand = pos[cur] + step * d
+        ok = np.ones(64, dtype=bool)
+        if confine is not None:
+            ok &= np.einsum("ij,ij->i", cand, cand) <= confine * confine
+        d2 = ((cand[:, None, :] - pos[None, :n, :]) ** 2).sum(-1)
+        d2[:, cur] = m2  # the bonded predecessor sits one step (1.5 A) away
+        ok &= (d2 >= m2).all(axis=1)
+        hit = np.flatnonzero(ok)
+        if len(hit):
+            pos[n] = cand[hit[0]]
+            cur = n
+            n += 1
+        else:
+            cur = int(rng.integers(n))
+    radii = rng.uniform(radius_range[0], radius_range[1], natoms)
+    q = rng.uniform(charge_range[0], charge_range[1], natoms)
+    q += (np.round(q.sum()) - q.sum()) / natoms
+    return pos, radii, q
+
+
+def ses_mesh(xyz, radii, probe=1.4, spacing=0.6):
+    """Solvent-excluded surface triangulated by the library's mesher
+    (fftsvd_mesh_ses); returns (verts, faces) in the MSMS convention."""
+    import ctypes as C
+
+    from fftsvd_amd import lib
+
+    L = lib()
+    f = L.fftsvd_mesh_ses
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double, C.POINTER(C.POINTER(C.c_double)),
+                  C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
+    L.fftsvd_free.argtypes = [C.c_void_p]
+    L.fftsvd_free.restype = None
+    a = np.ascontiguousarray(np.column_stack([xyz, radii]), dtype=np.float64)
+    pv = C.POINTER(C.c_double)()
+    pf = C.POINTER(C.c_int)()
+    nv, nf = C.c_int(), C.c_int()
+    rc = f(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), probe, spacing, C.byref(pv), C.byref(nv), C.byref(pf),
+           C.byref(nf))
+    if rc != 0:
+        raise RuntimeError("SES mesher failed")
+    V = np.ctypeslib.as_array(pv, shape=(nv.value * 3,)).reshape(-1, 3).copy()
+    F = np.ctypeslib.as_array(pf, shape=(nf.value * 3,)).reshape(-1, 3).copy().astype(np.int64)
+    L.fftsvd_free(C.cast(pv, C.c_void_p))
+    L.fftsvd_free(C.cast(pf, C.c_void_p))
+    return V, F
+
+
+def ses_mesh_target(xyz, radii, target_faces, probe=1.4, spacing=0.6, tol=0.05, max_rounds=6):
+    """SES mesh whose face count is within `tol` of `target_faces`, found by
+    rescaling the grid spacing (faces scale like 1/spacing^2)."""
+    best = None
+    for _ in range(max_rounds):
+        V, F = ses_mesh(xyz, radii, probe, spacing)
+        best = (V, F, spacing)
+        ratio = len(F) / float(target_faces)
+        if abs(ratio - 1.0) <= tol:
+            break
+        spacing *= ratio ** 0.5
+    return best

@@ -83,6 +83,15 @@ int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int
 int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
                      double* slp, double* dlp);
 
+/* Solvent-excluded surface of atoms (x, y, z, r per atom) with a probe of
+ * radius `probe`, contoured on a grid of spacing h; the library's own
+ * replacement for the MSMS run behind the reference's generateFlatSRF
+ * (FFTSVDpbeAPI.c:1034-1040, meshmaker.c:119-270).  Faces are 0-based,
+ * counter-clockwise seen from outside; free both arrays with fftsvd_free. */
+int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, double** verts, int* nverts,
+                    int** faces, int* nfaces);
+void fftsvd_free(void* p);
+
 /* Precision used by solveSurfaceOperator (0 FP64, 1 mixed); default 0. */
 int fftsvd_set_precision(int precision);
 
