@@ -238,14 +238,16 @@ class Problem:
         return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in
 
     def stats(self):
-        out = np.zeros(16)
-        _check(lib().fftsvd_problem_stats(self.h, _d(out), 16))
+        out = np.zeros(24)
+        _check(lib().fftsvd_problem_stats(self.h, _d(out), 24))
         keys = ("panels", "regions", "setup_ms", "near_entries", "far_entries", "spectra_bins", "translate_macs",
-                "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32", "near_cols")
+                "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32", "near_cols",
+                "ut_entries", "vt_entries", "pv_entries", "uti_entries", "t_bins", "leaf_ublocks", "proj_chunks",
+                "points_applied")
         return dict(zip(keys, out.tolist()))
 
     def bench_matvec(self, precision=F64, nrep=20):
         tot = C.c_double()
         parts = np.zeros(4)
         _check(lib().fftsvd_problem_bench_matvec(self.h, precision, nrep, C.byref(tot), _d(parts)))
-        return tot.value / nrep, dict(zip(("near", "far_source", "translate", "far_target"), parts.tolist()))
+        return tot.value / nrep, dict(zip(("leaf", "project_source", "translate", "target"), parts.tolist()))

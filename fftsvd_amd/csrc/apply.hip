@@ -4,16 +4,13 @@
 //     + U UTI IFFT( sum_c T[d-c] . FFT(PV_s VT xs_c + PV_d VT xd_c) )  (far field)
 //
 // Storage precision T is float (mixed mode) or double (FP64 mode); vectors
-// enter and leave in FP64.  Kernels:
-//   k_far_source<n,T>  per source cube: VT projection, PV interpolation onto
-//                      the n^3 grid, zero-padded r2c DFT -> spectrum
-//   k_far_translate<T> per target cube: sum over the interaction list of
-//                      T[offset] * spectrum (complex multiply-accumulate)
-//   k_far_target<n,T>  per target cube: cropped c2r DFT, UTI, U, += y_far
-//   k_near<T>          per leaf: dense block apply, adds y_far, writes y
-// The transposed operator reuses the same pieces with the roles swapped.
+// enter and leave in FP64.  The forward apply is five launches per tree
+// (k_project, k_source, k_translate, k_target, k_leaf; each covers every
+// level).  The transposed operator (k_farT_*, k_far_translate, k_nearT)
+// runs level by level with the roles of the two sides swapped.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "device.h"
@@ -161,55 +158,6 @@ struct FarSizes {
    static constexpr int N = 2 * n - 1, H = N / 2 + 1, PAD = N * N * H, GP3 = n * n * n;
 };
 
-// Source side.  Block per source cube; LDS: twiddles, t_s/t_d (r), grid g,
-// DFT work arrays.
-template <int n, typename T>
-__global__ void __launch_bounds__(256) k_far_source(const SrcDesc* __restrict__ src, const T* __restrict__ VT,
-                                                    const T* __restrict__ PV, const double* __restrict__ xs,
-                                                    const double* __restrict__ xd,
-                                                    typename Cplx<T>::type* __restrict__ spec, int maxr) {
-   using C2 = typename Cplx<T>::type;
-   constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, PAD = FarSizes<n>::PAD, GP3 = FarSizes<n>::GP3;
-   extern __shared__ __align__(16) unsigned char smem[];
-   C2* A = (C2*)smem;                     // n*n*H
-   C2* B = A + n * n * H;                 // n*N*H
-   T* cs = (T*)(B + n * N * H);           // N
-   T* sn = cs + N;                        // N
-   T* g = sn + N;                         // GP3
-   T* ts = g + GP3;                       // maxr
-   T* td = ts + maxr;                     // maxr
-   const SrcDesc d = src[blockIdx.x];
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-   twiddle_table<T, N>(cs, sn);
-   const T* vt = VT + d.vt_off;
-   for (int k = wave; k < d.r; k += nw) {
-      const T* row = vt + (long long)k * d.np;
-      T as = 0, ad = 0;
-      for (int i = lane; i < d.np; i += 64) {
-         T v = row[i];
-         as += v * (T)xs[d.p0 + i];
-         ad += v * (T)xd[d.p0 + i];
-      }
-      as = wave_sum(as);
-      ad = wave_sum(ad);
-      if (lane == 0) {
-         ts[k] = as;
-         td[k] = ad;
-      }
-   }
-   __syncthreads();
-   const T* pvs = PV + d.pv_off;
-   const T* pvd = pvs + (long long)d.r * GP3;
-   for (int q = tid; q < GP3; q += blockDim.x) {
-      T acc = 0;
-      for (int k = 0; k < d.r; k++) acc += pvs[(long long)k * GP3 + q] * ts[k];
-      for (int k = 0; k < d.r; k++) acc += pvd[(long long)k * GP3 + q] * td[k];
-      g[q] = acc;
-   }
-   __syncthreads();
-   grid_forward<n, T>(g, A, B, spec + (long long)d.spec_slot * PAD, cs, sn);
-}
-
 // Translation: S_d[b] = sum_e T[toff_e][b] * G[src_e][b].  One block per
 // target cube, threads over frequency bins.
 template <typename T>
@@ -232,26 +180,173 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
    }
 }
 
-// Target side.  Block per target cube: inverse DFT, UTI, U, accumulate.
+// ---------------------------------------------------------------------------
+// Forward apply, one launch per stage over all levels of a tree:
+//   k_project    chunk of a source cube's panels: partial VT.xs, VT.xd
+//   k_source     source cube: sum of chunk partials, PV interpolation onto
+//                the n^3 grid, zero-padded r2c DFT -> spectrum
+//   k_translate  target cube: sum over the interaction list of T . spectrum
+//   k_target     target cube: cropped c2r DFT, w = UTI h
+//   k_leaf       leaf: near-field block apply plus U w of every target cube
+//                containing the leaf, writes y
+// Every sum runs in a fixed order: the result does not depend on scheduling.
+
+// Rows e of a column-major E x m block (column c at blk + c*E), streamed
+// with unit stride by G = 256/Ec column groups per row chunk; returns the
+// thread's partial sum for (row e, layer inputs x0/x1 selected by e >= split).
+template <typename T>
+__device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int m, int g, int G, const T* xl) {
+   const long long step = (long long)G * E;
+   T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+   int c = g;
+   for (; c + 3 * G < m; c += 4 * G) {
+      const T* pc = colp + (long long)c * E;
+      T d0 = pc[0], d1 = pc[step], d2 = pc[2 * step], d3 = pc[3 * step];
+      a0 += d0 * xl[c];
+      a1 += d1 * xl[c + G];
+      a2 += d2 * xl[c + 2 * G];
+      a3 += d3 * xl[c + 3 * G];
+   }
+   for (; c < m; c += G) a0 += colp[(long long)c * E] * xl[c];
+   return (a0 + a1) + (a2 + a3);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ chunks, const T* __restrict__ VTc,
+                                                 const double* __restrict__ xs, const double* __restrict__ xd,
+                                                 T* __restrict__ tpart) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const ProjChunk c = chunks[blockIdx.x];
+   const int tid = threadIdx.x, m = c.ncol, E = c.r;
+   T* xl = (T*)smem;      // 2*m
+   T* part = xl + 2 * m;  // 2*256
+   for (int i = tid; i < m; i += 256) {
+      xl[i] = (T)xs[c.p0 + i];
+      xl[m + i] = (T)xd[c.p0 + i];
+   }
+   __syncthreads();
+   const T* blk = VTc + c.vt_off;
+   for (int e0 = 0; e0 < E; e0 += 256) {
+      const int Ec = min(256, E - e0), G = 256 / Ec;
+      const int g = tid / Ec, e = e0 + tid - g * Ec;
+      T as = 0, ad = 0;
+      if (g < G) {
+         const T* colp = blk + e;
+         const long long step = (long long)G * E;
+         T s0 = 0, s1 = 0, d0 = 0, d1 = 0;
+         int i = g;
+         for (; i + G < m; i += 2 * G) {
+            T v0 = colp[(long long)i * E], v1 = colp[(long long)i * E + step];
+            s0 += v0 * xl[i];
+            d0 += v0 * xl[m + i];
+            s1 += v1 * xl[i + G];
+            d1 += v1 * xl[m + i + G];
+         }
+         for (; i < m; i += G) {
+            T v = colp[(long long)i * E];
+            s0 += v * xl[i];
+            d0 += v * xl[m + i];
+         }
+         as = s0 + s1;
+         ad = d0 + d1;
+      }
+      part[tid] = as;
+      part[256 + tid] = ad;
+      __syncthreads();
+      if (tid < Ec) {
+         T ss = 0, sd = 0;
+         for (int gg = 0; gg < G; gg++) {
+            ss += part[gg * Ec + tid];
+            sd += part[256 + gg * Ec + tid];
+         }
+         tpart[c.out_off + e0 + tid] = ss;
+         tpart[c.out_off + E + e0 + tid] = sd;
+      }
+      __syncthreads();
+   }
+}
+
 template <int n, typename T>
-__global__ void __launch_bounds__(256) k_far_target(const TgtDesc* __restrict__ tgt, const T* __restrict__ UT,
-                                                    const T* __restrict__ UTI,
-                                                    const typename Cplx<T>::type* __restrict__ acc,
-                                                    double* __restrict__ yfar, int maxr) {
+__device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const T* __restrict__ PV,
+                            typename Cplx<T>::type* __restrict__ spec, unsigned char* smem) {
+   using C2 = typename Cplx<T>::type;
+   constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, GP3 = FarSizes<n>::GP3;
+   C2* A = (C2*)smem;            // n*n*H
+   C2* B = A + n * n * H;        // n*N*H
+   T* cs = (T*)(B + n * N * H);  // N
+   T* sn = cs + N;               // N
+   T* g = sn + N;                // GP3
+   T* t = g + GP3;               // 2r: ts then td
+   const int tid = threadIdx.x, r = d.r;
+   twiddle_table<T, N>(cs, sn);
+   for (int k = tid; k < 2 * r; k += blockDim.x) {
+      T a = 0;
+      for (int j = 0; j < d.nchunk; j++) a += tpart[d.part_off + j * 2 * r + k];
+      t[k] = a;
+   }
+   __syncthreads();
+   const T* pvs = PV + d.pv_off;
+   const T* pvd = pvs + (long long)r * GP3;
+   for (int q = tid; q < GP3; q += blockDim.x) {
+      T acc = 0;
+      for (int k = 0; k < r; k++) acc += pvs[(long long)k * GP3 + q] * t[k];
+      for (int k = 0; k < r; k++) acc += pvd[(long long)k * GP3 + q] * t[r + k];
+      g[q] = acc;
+   }
+   __syncthreads();
+   grid_forward<n, T>(g, A, B, spec + d.spec_off, cs, sn);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_source(const SrcPlan* __restrict__ src, const T* __restrict__ tpart,
+                                                const T* __restrict__ PV, typename Cplx<T>::type* __restrict__ spec) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const SrcPlan d = src[blockIdx.x];
+   switch (d.n) {
+      case 4: source_body<4, T>(d, tpart, PV, spec, smem); break;
+      case 5: source_body<5, T>(d, tpart, PV, spec, smem); break;
+      case 6: source_body<6, T>(d, tpart, PV, spec, smem); break;
+      case 7: source_body<7, T>(d, tpart, PV, spec, smem); break;
+      case 8: source_body<8, T>(d, tpart, PV, spec, smem); break;
+      case 9: source_body<9, T>(d, tpart, PV, spec, smem); break;
+      default: break;
+   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_translate(const TgtPlan* __restrict__ tgt, const int* __restrict__ isrc,
+                                                   const int* __restrict__ itoff,
+                                                   const typename Cplx<T>::type* __restrict__ Tspec,
+                                                   const typename Cplx<T>::type* __restrict__ spec,
+                                                   typename Cplx<T>::type* __restrict__ acc) {
+   const TgtPlan d = tgt[blockIdx.x];
+   const int pad = d.pad;
+   for (int b = threadIdx.x; b < pad; b += blockDim.x) {
+      T re = 0, im = 0;
+      for (int e = 0; e < d.nint; e++) {
+         auto t = Tspec[d.t_off + (long long)itoff[d.int_off + e] * pad + b];
+         auto s = spec[(long long)isrc[d.int_off + e] + b];
+         re += t.x * s.x - t.y * s.y;
+         im += t.x * s.y + t.y * s.x;
+      }
+      acc[d.acc_off + b] = {re, im};
+   }
+}
+
+template <int n, typename T>
+__device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
+                            const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w, unsigned char* smem) {
    using C2 = typename Cplx<T>::type;
    constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, PAD = FarSizes<n>::PAD, GP3 = FarSizes<n>::GP3;
-   extern __shared__ __align__(16) unsigned char smem[];
    C2* S = (C2*)smem;             // PAD
    C2* C1 = S + PAD;              // n*N*H
    C2* Cc = C1 + n * N * H;       // n*n*H
    T* cs = (T*)(Cc + n * n * H);  // N
    T* sn = cs + N;
    T* h = sn + N;                 // GP3
-   T* w = h + GP3;                // maxr
-   const TgtDesc d = tgt[blockIdx.x];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
    twiddle_table<T, N>(cs, sn);
-   for (int b = tid; b < PAD; b += blockDim.x) S[b] = acc[(long long)d.spec_slot * PAD + b];
+   for (int b = tid; b < PAD; b += blockDim.x) S[b] = acc[d.acc_off + b];
    __syncthreads();
    grid_backward<n, T>(S, C1, Cc, h, cs, sn);
    __syncthreads();
@@ -261,54 +356,86 @@ __global__ void __launch_bounds__(256) k_far_target(const TgtDesc* __restrict__ 
       T a = 0;
       for (int q = lane; q < GP3; q += 64) a += row[q] * h[q];
       a = wave_sum(a);
-      if (lane == 0) w[k] = a;
-   }
-   __syncthreads();
-   const T* ut = UT + d.u_off;
-   for (int i = tid; i < d.nq; i += blockDim.x) {
-      T a = 0;
-      for (int k = 0; k < d.r; k++) a += ut[(long long)k * d.nq + i] * w[k];
-      yfar[d.q0 + i] += (double)a;
+      if (lane == 0) w[d.w_off + k] = a;
    }
 }
 
-// Near field: block per leaf.  x gathered to LDS in storage precision; the
-// 2*p*m block is streamed with unit stride and reduced into LDS rows.
 template <typename T>
-__global__ void __launch_bounds__(256) k_near(const LeafDesc* __restrict__ leaves, const int* __restrict__ dcol,
+__global__ void __launch_bounds__(256) k_target(const TgtPlan* __restrict__ tgt, const T* __restrict__ UTI,
+                                                const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const TgtPlan d = tgt[blockIdx.x];
+   switch (d.n) {
+      case 4: target_body<4, T>(d, UTI, acc, w, smem); break;
+      case 5: target_body<5, T>(d, UTI, acc, w, smem); break;
+      case 6: target_body<6, T>(d, UTI, acc, w, smem); break;
+      case 7: target_body<7, T>(d, UTI, acc, w, smem); break;
+      case 8: target_body<8, T>(d, UTI, acc, w, smem); break;
+      case 9: target_body<9, T>(d, UTI, acc, w, smem); break;
+      default: break;
+   }
+}
+
+// Leaf output: y = D_s xs + D_d xd (interleaved 2p x m block) + sum over the
+// leaf's far-field blocks of U w.  Thread (g, e) owns block row e and the
+// columns c = g, g+G, ...; the G partial sums of each row are added in group
+// order, then the two layers and the far field.
+template <typename T>
+__global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leaves, const int* __restrict__ dcol,
                                               const T* __restrict__ D, const double* __restrict__ xs,
-                                              const double* __restrict__ xd, const double* __restrict__ yfar,
+                                              const double* __restrict__ xd, const LeafU* __restrict__ ublk,
+                                              const T* __restrict__ UT, const T* __restrict__ w,
                                               double* __restrict__ y) {
    extern __shared__ __align__(16) unsigned char smem[];
    const LeafDesc L = leaves[blockIdx.x];
-   T* xg = (T*)smem;        // 2*m
-   T* yr = xg + 2 * L.m;    // p
-   const int tid = threadIdx.x, nt = blockDim.x;
-   for (int c = tid; c < L.m; c += nt) {
+   const int E = 2 * L.p, m = L.m, p = L.p;
+   T* xg = (T*)smem;       // 2*m
+   T* part = xg + 2 * m;   // 256
+   T* rows = part + 256;   // 2p near rows, then p far rows
+   const int tid = threadIdx.x;
+   for (int c = tid; c < m; c += 256) {
       int col = dcol[L.col_off + c];
       xg[c] = (T)xs[col];
-      xg[L.m + c] = (T)xd[col];
+      xg[m + c] = (T)xd[col];
    }
-   for (int r = tid; r < L.p; r += nt) yr[r] = 0;
    __syncthreads();
-   const int E = 2 * L.p;
-   const long long total = (long long)E * L.m;
    const T* blk = D + L.d_off;
-   int c = tid / E, r2 = tid - (tid / E) * E;
-   const int dc = nt / E, dr = nt - dc * E;
-   for (long long e = tid; e < total; e += nt) {
-      int layer = r2 >= L.p;
-      int row = r2 - layer * L.p;
-      atomicAdd(&yr[row], blk[e] * xg[layer * L.m + c]);
-      c += dc;
-      r2 += dr;
-      if (r2 >= E) {
-         r2 -= E;
-         c++;
+   for (int e0 = 0; e0 < E; e0 += 256) {
+      const int Ec = min(256, E - e0), G = 256 / Ec;
+      const int g = tid / Ec, e = e0 + tid - g * Ec;
+      T acc = 0;
+      if (g < G) acc = stream_rows<T>(blk + e, E, m, g, G, xg + (e >= p ? m : 0));
+      part[tid] = acc;
+      __syncthreads();
+      if (tid < Ec) {
+         T sum = 0;
+         for (int gg = 0; gg < G; gg++) sum += part[gg * Ec + tid];
+         rows[e0 + tid] = sum;
       }
+      __syncthreads();
    }
-   __syncthreads();
-   for (int r = tid; r < L.p; r += nt) y[L.q0 + r] = (double)yr[r] + yfar[L.q0 + r];
+   // far field: rows e < p of every block, the k index split over G groups
+   for (int e0 = 0; e0 < p; e0 += 256) {
+      const int Ec = min(256, p - e0), G = 256 / Ec;
+      const int g = tid / Ec, e = e0 + tid - g * Ec;
+      T acc = 0;
+      if (g < G)
+         for (int u = 0; u < L.nu; u++) {
+            const LeafU b = ublk[L.u0 + u];
+            const T* col = UT + b.u_off + e;
+            const T* wb = w + b.w_off;
+            for (int k = g; k < b.r; k += G) acc += col[(long long)k * b.ld] * wb[k];
+         }
+      part[tid] = acc;
+      __syncthreads();
+      if (tid < Ec) {
+         T sum = 0;
+         for (int gg = 0; gg < G; gg++) sum += part[gg * Ec + tid];
+         rows[E + e0 + tid] = sum;
+      }
+      __syncthreads();
+   }
+   for (int r = tid; r < p; r += 256) y[L.q0 + r] = (double)((rows[r] + rows[p + r]) + rows[E + r]);
 }
 
 __global__ void k_zero(double* __restrict__ a, long long n) {
@@ -439,18 +566,6 @@ static void check_lds(size_t lds) {
 }
 
 template <typename T>
-void launch_far_source(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const double* xs,
-                       const double* xd, void* spec, int maxr, hipStream_t s) {
-   size_t lds = far_source_lds<T>(n, maxr);
-   check_lds(lds);
-#define CALL(NN)                                                                                                    \
-   hipLaunchKernelGGL((k_far_source<NN, T>), dim3(nblk), dim3(256), lds, s, src, VT, PV, xs, xd,                  \
-                      (typename Cplx<T>::type*)spec, maxr)
-   FSVD_DISPATCH_N(n, CALL)
-#undef CALL
-}
-
-template <typename T>
 void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const int* itoff, const void* Tspec,
                           const void* spec, void* acc, int pad, hipStream_t s) {
    hipLaunchKernelGGL((k_far_translate<T>), dim3(nblk), dim3(256), 0, s, tgt, isrc, itoff,
@@ -459,23 +574,80 @@ void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const i
 }
 
 template <typename T>
-void launch_far_target(int n, int nblk, const TgtDesc* tgt, const T* UT, const T* UTI, const void* acc,
-                       double* yfar, int maxr, hipStream_t s) {
-   size_t lds = far_target_lds<T>(n, maxr);
-   check_lds(lds);
-#define CALL(NN)                                                                                                    \
-   hipLaunchKernelGGL((k_far_target<NN, T>), dim3(nblk), dim3(256), lds, s, tgt, UT, UTI,                          \
-                      (const typename Cplx<T>::type*)acc, yfar, maxr)
-   FSVD_DISPATCH_N(n, CALL)
-#undef CALL
+size_t source_lds(int maxn, int maxr) {
+   size_t m = 0;
+   for (int n = 4; n <= maxn; n++) m = std::max(m, far_source_lds<T>(n, maxr));
+   return m;
+}
+template <typename T>
+size_t target_lds(int maxn) {
+   size_t m = 0;
+   for (int n = 4; n <= maxn; n++) m = std::max(m, far_target_lds<T>(n, 0));
+   return m;
 }
 
 template <typename T>
-void launch_near(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* xs, const double* xd,
-                 const double* yfar, double* y, int maxm, int maxp, hipStream_t s) {
-   size_t lds = sizeof(T) * (size_t)(2 * maxm + maxp) + 64;
-   check_lds(lds);
-   hipLaunchKernelGGL((k_near<T>), dim3(nleaf), dim3(256), lds, s, leaves, dcol, D, xs, xd, yfar, y);
+void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
+   using C2 = typename Cplx<T>::type;
+   if (a.max_n > 9) throw std::runtime_error("grid points per side above 9");
+   switch (stage) {
+      case 0: {
+         size_t lds = sizeof(T) * (size_t)(2 * a.proj_maxcols + 512) + 64;
+         check_lds(lds);
+         if (a.nchunk)
+            hipLaunchKernelGGL((k_project<T>), dim3(a.nchunk), dim3(256), lds, s, a.chunks, (const T*)a.VTc, a.xs,
+                               a.xd, (T*)a.tpart);
+         break;
+      }
+      case 1: {
+         size_t lds = source_lds<T>(a.max_n, a.maxr_src);
+         check_lds(lds);
+         if (a.nsrc)
+            hipLaunchKernelGGL((k_source<T>), dim3(a.nsrc), dim3(256), lds, s, a.src, (const T*)a.tpart,
+                               (const T*)a.PV, (C2*)a.spec);
+         break;
+      }
+      case 2:
+         if (a.ntgt)
+            hipLaunchKernelGGL((k_translate<T>), dim3(a.ntgt), dim3(256), 0, s, a.tgt, a.isrc, a.itoff,
+                               (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+         break;
+      case 3: {
+         size_t lds = target_lds<T>(a.max_n);
+         check_lds(lds);
+         if (a.ntgt)
+            hipLaunchKernelGGL((k_target<T>), dim3(a.ntgt), dim3(256), lds, s, a.tgt, (const T*)a.UTI,
+                               (const C2*)a.acc, (T*)a.w);
+         break;
+      }
+      case 4: {
+         size_t lds = sizeof(T) * (size_t)(2 * a.maxm + 256 + 3 * a.maxp) + 64;
+         check_lds(lds);
+         if (a.nleaf)
+            hipLaunchKernelGGL((k_leaf<T>), dim3(a.nleaf), dim3(256), lds, s, a.leaves, a.dcol, (const T*)a.D, a.xs,
+                               a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y);
+         break;
+      }
+      default: break;
+   }
+}
+
+template <typename T>
+__global__ void k_transpose_batched(const TransDesc* __restrict__ d) {
+   const TransDesc t = d[blockIdx.x];
+   const T* src = (const T*)t.src;
+   T* dst = (T*)t.dst;
+   const int total = t.rows * t.cols;
+   for (int e = threadIdx.x; e < total; e += blockDim.x) {
+      int j = e / t.rows, i = e - j * t.rows;
+      dst[e] = src[(long long)i * t.cols + j];
+   }
+}
+
+template <typename T>
+void launch_transpose_batched(int n, const TransDesc* d, hipStream_t s) {
+   for (int b0 = 0; b0 < n; b0 += 65535)
+      hipLaunchKernelGGL((k_transpose_batched<T>), dim3(std::min(65535, n - b0)), dim3(256), 0, s, d + b0);
 }
 
 template <typename T>
@@ -514,15 +686,11 @@ void launch_zero(double* a, long long n, hipStream_t s) {
    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, a, n);
 }
 
-#define FSVD_INST(T)                                                                                                   \
-   template void launch_far_source<T>(int, int, const SrcDesc*, const T*, const T*, const double*, const double*, void*, \
-                                      int, hipStream_t);                                                              \
-   template void launch_far_translate<T>(int, const TgtDesc*, const int*, const int*, const void*, const void*, void*, \
-                                         int, hipStream_t);                                                           \
-   template void launch_far_target<T>(int, int, const TgtDesc*, const T*, const T*, const void*, double*, int,        \
-                                      hipStream_t);                                                                   \
-   template void launch_near<T>(int, const LeafDesc*, const int*, const T*, const double*, const double*,              \
-                                const double*, double*, int, int, hipStream_t);                                       \
+#define FSVD_INST(T)                                                                                           \
+   template void launch_far_translate<T>(int, const TgtDesc*, const int*, const int*, const void*, const void*,      \
+                                         void*, int, hipStream_t);                                                   \
+   template void launch_apply_stage<T>(int, const ApplyArgs&, hipStream_t);                                          \
+   template void launch_transpose_batched<T>(int, const TransDesc*, hipStream_t);                                    \
    template void launch_farT_source<T>(int, int, const TgtDesc*, const T*, const T*, const double*, void*, int,       \
                                        hipStream_t);                                                                  \
    template void launch_farT_target<T>(int, int, const SrcDesc*, const T*, const T*, const void*, int, double*, int,  \

@@ -234,27 +234,20 @@ static TreeOp& region_op(Problem& pr, int which) {
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
    API_TRY
    Problem& pr = *p->pr;
-   double v[16] = {0};
+   double v[24] = {0};
    v[0] = pr.ntotal;
    v[1] = (double)pr.regions.size();
    v[2] = pr.setup_ms;
-   double glue = 0;
+   double glue = 0, vec = 0;
    for (Region& R : pr.regions) {
-      int n = (int)R.panel_ids.size();
+      const int n = (int)R.panel_ids.size();
+      if (n == 0) continue;
       glue += 8.0 * (2 * n /*gather*/ + 3 * n /*split*/);
-      if (R.shares_tree_with >= 0 || !R.op) {
-         if (R.shares_tree_with >= 0) {
-            TreeOp& op = *pr.regions[R.shares_tree_with].op;
-            v[3] += op.stats.near_entries;
-            v[4] += op.stats.far_entries;
-            v[5] += op.stats.spectra_bins;
-            v[6] += op.stats.translate_macs;
-            v[14] += (double)op.h_dcol.size();
-            v[9] += 1;
-         }
-         continue;
-      }
-      TreeOp& op = *R.op;
+      vec += 8.0 * 3 * n;
+      TreeOp* opp = R.shares_tree_with >= 0 ? pr.regions[R.shares_tree_with].op.get() : R.op.get();
+      if (!opp) continue;
+      TreeOp& op = *opp;
+      // every region applies its tree once per matvec
       v[3] += op.stats.near_entries;
       v[4] += op.stats.far_entries;
       v[5] += op.stats.spectra_bins;
@@ -263,16 +256,24 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
       v[8] += op.stats.nleaf;
       v[9] += 1;
       v[14] += (double)op.h_dcol.size();
+      for (const FarLevel& lv : op.levels) {
+         v[15] += (double)lv.UT64.n;
+         v[16] += (double)lv.VT64.n;
+         v[17] += (double)lv.PV64.n;
+         v[18] += (double)lv.UTI64.n;
+         v[19] += (double)lv.T64.n / 2;
+      }
+      v[20] += op.plan.nublk;
+      v[21] += op.plan.nchunk;
+      v[22] += (double)n;
    }
    // algorithmic bytes: stored operator entries once, spectra written and
    // read once, vectors (2 gathers + 1 write per point per tree) and glue
-   double vec = 0;
-   for (Region& R : pr.regions) vec += 8.0 * 3 * R.panel_ids.size();
    v[10] = 8.0 * (v[3] + v[4]) + 16.0 * 2 * v[5] + vec + glue;
    v[11] = 4.0 * (v[3] + v[4]) + 8.0 * 2 * v[5] + vec + glue;
    v[12] = 8.0 * v[3];
    v[13] = 4.0 * v[3];
-   for (int i = 0; i < nout && i < 16; i++) out[i] = v[i];
+   for (int i = 0; i < nout && i < 24; i++) out[i] = v[i];
    API_CATCH
 }
 

@@ -36,22 +36,34 @@ struct GemmDesc {
    int a_rs, a_cs, b_rs, b_cs, c_rs, c_cs;
 };
 
-// Pivoted-MGS basis problem: A (rows x cols, row-major, modified in place)
-// -> rank and the first `rank` orthonormal pivot columns at q_off
-// (column-major, column k at q_off + k*rows).
+// Pivoted-MGS basis problem: A (rows x cols, column-major, column j at
+// a_off + j*rows, modified in place) -> rank and the first `rank`
+// orthonormal pivot columns at q_off (column k at q_off + k*rows).
+// seg lanes of a wave share one column; the problem's columns occupy the
+// update grid's waves wave0.. and its squared column norms sit at
+// ss_off.. (filled by the host scheduler).
 struct MgsDesc {
    int rows, cols;
    long long a_off;
    long long q_off;
    int rank_slot;
+   int seg, wave0, ss_off;
+};
+
+// Per-problem progress of the stepwise MGS.
+struct MgsState {
+   double fro;
+   int rank, done;
 };
 
 // Near-field leaf: rows q0..q0+p-1 (tree-order points), m columns listed at
 // dcol[col_off..], interleaved column-major block of 2*p*m values at d_off
 // (column c: p single-layer then p double-layer entries).
+// u0/nu: the leaf's far-field output blocks (LeafU) in the apply plan.
 struct LeafDesc {
    int q0, p, m, col_off;
    long long d_off;
+   int u0, nu;
 };
 
 // Far-field source cube (panels side) at one level.
@@ -69,6 +81,66 @@ struct TgtDesc {
    int int_off, nint;  // into the interaction arrays (source spectrum slot, T offset index)
    long long u_off;    // U^T: r x nq row-major
    long long uti_off;  // UTI: r x gp3 row-major
+};
+
+// ---- apply plan: one launch per stage over every level of a tree ----
+
+// Columns [c0, c0+ncol) of a source cube's row basis, stored panel-major
+// (column i holds the r coefficients of panel p0+i) at vt_off; the block
+// writes 2r partial projections (layer 0 then layer 1) at out_off.
+struct ProjChunk {
+   long long vt_off;
+   int p0, ncol, r, out_off;
+};
+
+// Source cube: n grid points per side, rank r, its chunk partials at
+// part_off + j*2r (j < nchunk), PVs then PVd (r x gp3 each, k-major) at
+// pv_off, forward spectrum (complex index) at spec_off.
+struct SrcPlan {
+   int n, r, part_off, nchunk;
+   long long pv_off;
+   long long spec_off;
+};
+
+// Target cube: interactions int_off..+nint (absolute source spectrum
+// offsets, T offset codes), its level's T spectra at t_off (offset code c
+// at t_off + c*pad), accumulated spectrum at acc_off, UTI (r x gp3) at
+// uti_off, r outputs w at w_off.
+struct TgtPlan {
+   int n, r, pad, nint, int_off, w_off;
+   long long t_off, acc_off, uti_off;
+};
+
+// Far-field output block of a leaf: rows of one target cube's U^T
+// (r x ld, k-major) starting at the leaf's first point, weights at w_off.
+struct LeafU {
+   long long u_off;
+   int ld, r, w_off, pad_;
+};
+
+// Element-wise transpose job: dst (cols x rows, row-major) = src^T.
+struct TransDesc {
+   const void* src;
+   void* dst;
+   int rows, cols;
+};
+
+// Everything one stage launch of the forward apply needs (device pointers;
+// matrix pointers are of the storage type of the launch).
+struct ApplyArgs {
+   int nchunk, nsrc, ntgt, nleaf;
+   int proj_maxcols, maxr_src, max_n, maxm, maxp;
+   const ProjChunk* chunks;
+   const SrcPlan* src;
+   const TgtPlan* tgt;
+   const int *isrc, *itoff;
+   const LeafDesc* leaves;
+   const int* dcol;
+   const LeafU* ublk;
+   const void *VTc, *PV, *Tsp, *UTI, *UT, *D;
+   void *tpart, *spec, *acc, *w;
+   const double *xs, *xd;
+   double* y;
 };
 
 }  // namespace fsvd

@@ -3,8 +3,12 @@
 // All of it runs in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "device.h"
 #include "integrals.h"
+#include "op.h"
 
 namespace fsvd {
 
@@ -61,120 +65,186 @@ __global__ void __launch_bounds__(256) k_gemm_batched(const GemmDesc* __restrict
 
 // Column basis by pivoted MGS with two re-orthogonalisations and the rank
 // cut ||col|| < eps*||A||_F (reference Matrix_columnbasis_pivotedmgs,
-// Matrix.c:215-307).  One workgroup per problem; every thread owns whole
-// columns, so each column's dot products and norms are sequential sums over
-// the rows (the reference's summation order).
-__global__ void __launch_bounds__(256) k_pivoted_mgs(const MgsDesc* __restrict__ descs, double* __restrict__ Abuf,
-                                                     double* __restrict__ Qbuf, int* __restrict__ ranks,
-                                                     double eps) {
-   const MgsDesc d = descs[blockIdx.x];
-   double* A = Abuf + d.a_off;
-   double* Q = Qbuf + d.q_off;
-   const int rows = d.rows, cols = d.cols;
-   const int tid = threadIdx.x, nt = blockDim.x;
-   extern __shared__ double sh[];
-   double* red_v = sh;                     // nt
-   int* red_i = (int*)(red_v + nt);        // nt
-   __shared__ double s_fro;
-   __shared__ int s_best;
-   __shared__ double s_max;
-   __shared__ int s_done;
+// Matrix.c:215-307), run as one pair of launches per pivot step over every
+// problem of a batch:
+//   k_mgs_pivot   workgroup per problem: argmax of the column norms (ties to
+//                 the lowest index), rank cut, column swap, q_k = a_k / |a_k|
+//   k_mgs_update  seg lanes per remaining column: two passes of
+//                 a -= (q_k . a) q_k, then the fresh squared norm of a
+// Columns are contiguous (column-major A), so each segment reads and writes
+// its column with unit stride; the dot products are segment reductions.
 
-   // Frobenius norm: per-column sums, then the column sums in order.
-   double part = 0.0;
-   for (int j = tid; j < cols; j += nt) {
-      double s = 0.0;
-      for (int i = 0; i < rows; i++) {
-         double v = A[(long long)i * cols + j];
-         s += v * v;
-      }
-      part += s;
+__device__ __forceinline__ double seg_sum(double v, int S) {
+   for (int o = S >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+   return v;
+}
+
+template <int NR>
+__global__ void __launch_bounds__(256) k_mgs_update(const MgsDesc* __restrict__ descs, int nprob, int total_waves,
+                                                    double* __restrict__ Abuf, const double* __restrict__ Qbuf,
+                                                    double* __restrict__ ss, const MgsState* __restrict__ st,
+                                                    int k) {
+   const int gw = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+   if (gw >= total_waves) return;
+   int lo = 0, hi = nprob - 1;
+   while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].wave0 <= gw) lo = mid;
+      else hi = mid - 1;
    }
-   red_v[tid] = part;
-   __syncthreads();
-   if (tid == 0) {
-      double f = 0.0;
-      for (int t = 0; t < nt; t++) f += red_v[t];
-      s_fro = sqrt(f);
-      s_done = 0;
+   const MgsDesc d = descs[lo];
+   if (k >= 0 && st[lo].done) return;  // uniform over the wave
+   const int lane = threadIdx.x & 63, S = d.seg, sub = lane / S, sl = lane - sub * S;
+   const int j = (gw - d.wave0) * (64 / S) + sub;
+   const bool active = j < d.cols && j > k;
+   const int rows = d.rows;
+   double* a = Abuf + d.a_off + (long long)(active ? j : 0) * rows;
+   const double* q = Qbuf + d.q_off + (long long)(k > 0 ? k : 0) * rows;
+   const int nr = (rows + S - 1) / S;
+   double s2 = 0.0;
+   if (nr <= NR) {
+      double v[NR];
+#pragma unroll
+      for (int t = 0; t < NR; t++) {
+         int i = sl + t * S;
+         v[t] = (active && t < nr && i < rows) ? a[i] : 0.0;
+      }
+      if (k >= 0) {
+         for (int pass = 0; pass < 2; pass++) {
+            double r = 0.0;
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+               int i = sl + t * S;
+               if (t < nr && i < rows) r = fma(q[i], v[t], r);
+            }
+            r = seg_sum(r, S);
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+               int i = sl + t * S;
+               if (t < nr && i < rows) v[t] = fma(-q[i], r, v[t]);
+            }
+         }
+         if (active) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+               int i = sl + t * S;
+               if (t < nr && i < rows) a[i] = v[t];
+            }
+         }
+      }
+#pragma unroll
+      for (int t = 0; t < NR; t++) s2 = fma(v[t], v[t], s2);
+   } else {
+      // long columns (S == 64): stream the column three times
+      if (k >= 0) {
+         double r1 = 0.0;
+         for (int i = sl; i < rows; i += S) r1 = fma(q[i], active ? a[i] : 0.0, r1);
+         r1 = seg_sum(r1, S);
+         double r2 = 0.0;
+         for (int i = sl; i < rows; i += S) r2 = fma(q[i], fma(-q[i], r1, active ? a[i] : 0.0), r2);
+         r2 = seg_sum(r2, S);
+         for (int i = sl; i < rows; i += S) {
+            double v = fma(-q[i], r2, fma(-q[i], r1, active ? a[i] : 0.0));
+            if (active) a[i] = v;
+            s2 = fma(v, v, s2);
+         }
+      } else {
+         for (int i = sl; i < rows; i += S) {
+            double v = active ? a[i] : 0.0;
+            s2 = fma(v, v, s2);
+         }
+      }
    }
-   __syncthreads();
-   const double thresh = eps * s_fro;
-   int rank = 0;
-   for (int k = 0; k < cols; k++) {
-      // at most `rows` orthonormal columns exist: the remaining ones are
-      // rounding noise, far below any positive cut
-      if (k >= rows) {
-         rank = k;
-         break;
-      }
-      // column norms for l >= k, argmax with ties to the lowest index
-      double bv = 0.0;
-      int bi = 0x7fffffff;
-      for (int l = k + tid; l < cols; l += nt) {
-         double s = 0.0;
-         for (int i = 0; i < rows; i++) {
-            double v = A[(long long)i * cols + l];
-            s += v * v;
-         }
-         s = sqrt(s);
-         if (s > bv || (s == bv && l < bi)) {
-            bv = s;
-            bi = l;
-         }
-      }
-      red_v[tid] = bv;
-      red_i[tid] = bi;
+   s2 = seg_sum(s2, S);
+   if (active && sl == 0) ss[d.ss_off + j] = s2;
+}
+
+__global__ void __launch_bounds__(256) k_mgs_pivot(const MgsDesc* __restrict__ descs, double* __restrict__ Abuf,
+                                                   double* __restrict__ Qbuf, const double* __restrict__ ss,
+                                                   MgsState* __restrict__ st, int k, double eps) {
+   const int p = blockIdx.x;
+   if (st[p].done) return;
+   const MgsDesc d = descs[p];
+   const int tid = threadIdx.x, nt = blockDim.x;
+   __shared__ double rv[256];
+   __shared__ int ri[256];
+   double fro;
+   if (k == 0) {
+      double part = 0.0;
+      for (int j = tid; j < d.cols; j += nt) part += ss[d.ss_off + j];
+      rv[tid] = part;
       __syncthreads();
       for (int w = nt / 2; w > 0; w >>= 1) {
-         if (tid < w) {
-            double ov = red_v[tid + w];
-            int oi = red_i[tid + w];
-            if (ov > red_v[tid] || (ov == red_v[tid] && oi < red_i[tid])) {
-               red_v[tid] = ov;
-               red_i[tid] = oi;
-            }
-         }
+         if (tid < w) rv[tid] += rv[tid + w];
          __syncthreads();
       }
-      if (tid == 0) {
-         double mx = red_v[0];
-         int best = red_i[0];
-         if (!(mx > 0.0)) best = k;  // all remaining columns zero: reference keeps index 0 -> k
-         s_max = mx;
-         s_best = best;
-         if (k >= 1 && mx < thresh) s_done = 1;
-      }
+      fro = sqrt(rv[0]);
       __syncthreads();
-      if (s_done) {
-         rank = k;
-         break;
-      }
-      const int best = s_best;
-      const double mx = s_max;
-      if (best != k)
-         for (int i = tid; i < rows; i += nt) {
-            double t = A[(long long)i * cols + best];
-            A[(long long)i * cols + best] = A[(long long)i * cols + k];
-            A[(long long)i * cols + k] = t;
-         }
-      __syncthreads();
-      for (int i = tid; i < rows; i += nt) Q[(long long)k * rows + i] = A[(long long)i * cols + k] / mx;
-      __syncthreads();
-      const double* q = Q + (long long)k * rows;
-      if (k < cols - 1) {
-         for (int j = k + 1 + tid; j < cols; j += nt) {
-            for (int pass = 0; pass < 2; pass++) {
-               double r = 0.0;
-               for (int i = 0; i < rows; i++) r += q[i] * A[(long long)i * cols + j];
-               for (int i = 0; i < rows; i++) A[(long long)i * cols + j] -= q[i] * r;
-            }
-         }
-      }
-      __syncthreads();
-      if (k == cols - 1) rank = cols;
+      if (tid == 0) st[p].fro = fro;
+   } else {
+      fro = st[p].fro;
    }
-   if (tid == 0) ranks[d.rank_slot] = rank;
+   if (k >= d.cols || k >= d.rows) {
+      // every column taken, or `rows` orthonormal columns found (the rest is
+      // rounding noise far below any positive cut)
+      if (tid == 0) {
+         st[p].rank = k;
+         st[p].done = 1;
+      }
+      return;
+   }
+   double bv = 0.0;
+   int bi = 0x7fffffff;
+   for (int j = k + tid; j < d.cols; j += nt) {
+      double v = sqrt(ss[d.ss_off + j]);
+      if (v > bv || (v == bv && j < bi)) {
+         bv = v;
+         bi = j;
+      }
+   }
+   rv[tid] = bv;
+   ri[tid] = bi;
+   __syncthreads();
+   for (int w = nt / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+         double ov = rv[tid + w];
+         int oi = ri[tid + w];
+         if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
+            rv[tid] = ov;
+            ri[tid] = oi;
+         }
+      }
+      __syncthreads();
+   }
+   const double mx = rv[0];
+   const int best = mx > 0.0 ? ri[0] : k;
+   if (k >= 1 && mx < eps * fro) {
+      if (tid == 0) {
+         st[p].rank = k;
+         st[p].done = 1;
+      }
+      return;
+   }
+   double* A = Abuf + d.a_off;
+   double* ak = A + (long long)k * d.rows;
+   double* ab = A + (long long)best * d.rows;
+   double* q = Qbuf + d.q_off + (long long)k * d.rows;
+   for (int i = tid; i < d.rows; i += nt) {
+      double vb = ab[i], vk = ak[i];
+      ab[i] = vk;
+      ak[i] = vb;
+      q[i] = vb / mx;
+   }
+   if (tid == 0) st[p].rank = k + 1;
+}
+
+__global__ void k_mgs_init(MgsState* __restrict__ st, int n) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) {
+      st[i].fro = 0.0;
+      st[i].rank = 0;
+      st[i].done = 0;
+   }
 }
 
 __global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
@@ -203,11 +273,50 @@ void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStr
       hipLaunchKernelGGL(k_gemm_batched, dim3(maxtiles, nbatch), dim3(256), 0, s, descs);
 }
 
-void launch_mgs(int nprob, int maxrows, const MgsDesc* descs, double* A, double* Q, int* ranks, double eps,
-                hipStream_t s) {
-   (void)maxrows;
-   size_t lds = sizeof(double) * 256 + sizeof(int) * 256 + 64;
-   if (nprob > 0) hipLaunchKernelGGL(k_pivoted_mgs, dim3(nprob), dim3(256), lds, s, descs, A, Q, ranks, eps);
+void run_mgs(std::vector<MgsDesc>& descs, double* A, double* Q, std::vector<int>& ranks, double eps,
+             hipStream_t s) {
+   constexpr int NR = 16;
+   const int nprob = (int)descs.size();
+   ranks.assign(nprob, 0);
+   if (nprob == 0) return;
+   int waves = 0, ncol = 0, kmax = 0;
+   for (auto& d : descs) {
+      int S = 4;
+      while (S < 64 && (d.rows + S - 1) / S > NR) S <<= 1;
+      d.seg = S;
+      d.wave0 = waves;
+      d.ss_off = ncol;
+      waves += (d.cols + 64 / S - 1) / (64 / S);
+      ncol += d.cols;
+      kmax = std::max(kmax, std::min(d.rows, d.cols));
+   }
+   DBuf<MgsDesc> dd;
+   dd.upload(descs, s);
+   DBuf<double> ss;
+   ss.alloc(std::max(ncol, 1));
+   DBuf<MgsState> st;
+   st.alloc(nprob);
+   hipLaunchKernelGGL(k_mgs_init, dim3((nprob + 255) / 256), dim3(256), 0, s, st.p, nprob);
+   const int blocks = (waves + 3) / 4;
+   hipLaunchKernelGGL((k_mgs_update<NR>), dim3(blocks), dim3(256), 0, s, dd.p, nprob, waves, A, Q, ss.p, st.p, -1);
+   MgsState* h = nullptr;
+   FSVD_CHECK(hipHostMalloc((void**)&h, sizeof(MgsState) * nprob, hipHostMallocDefault));
+   for (int k = 0; k <= kmax; k++) {
+      hipLaunchKernelGGL(k_mgs_pivot, dim3(nprob), dim3(256), 0, s, dd.p, A, Q, ss.p, st.p, k, eps);
+      if (k < kmax)
+         hipLaunchKernelGGL((k_mgs_update<NR>), dim3(blocks), dim3(256), 0, s, dd.p, nprob, waves, A, Q, ss.p, st.p,
+                            k);
+      if ((k & 7) == 7 || k == kmax) {
+         FSVD_CHECK(hipMemcpyAsync(h, st.p, sizeof(MgsState) * nprob, hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+         bool all = true;
+         for (int i = 0; i < nprob && all; i++) all = h[i].done != 0;
+         if (all) break;
+      }
+   }
+   FSVD_CHECK(hipGetLastError());
+   for (int i = 0; i < nprob; i++) ranks[descs[i].rank_slot] = h[i].rank;
+   FSVD_CHECK(hipHostFree(h));
 }
 
 void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s) {

@@ -103,6 +103,157 @@ void to_f32(const DBuf<double>& a, DBuf<float>& b, hipStream_t s) {
 
 }  // namespace
 
+// Concatenates every level's far-field blocks into single buffers with
+// absolute offsets (row bases re-laid panel-major for the projection),
+// splits source cubes into projection chunks, and attaches to every leaf
+// the U blocks of the target cubes that contain it.
+static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
+   constexpr int CHUNK = 16384;   // basis entries per projection block
+   ApplyPlan& P = op.plan;
+   const Tree& t = op.tree;
+   const size_t L = op.levels.size();
+   long long vt_n = 0, pv_n = 0, t_n = 0, uti_n = 0, ut_n = 0;
+   for (const FarLevel& lv : op.levels) {
+      vt_n += (long long)lv.VT64.n;
+      pv_n += (long long)lv.PV64.n;
+      t_n += (long long)lv.T64.n;
+      uti_n += (long long)lv.UTI64.n;
+      ut_n += (long long)lv.UT64.n;
+   }
+   P.VTc64.alloc(std::max<long long>(vt_n, 1));
+   P.PV64.alloc(std::max<long long>(pv_n, 1));
+   P.Tsp64.alloc(std::max<long long>(t_n, 2));
+   P.UTI64.alloc(std::max<long long>(uti_n, 1));
+   P.UT64.alloc(std::max<long long>(ut_n, 1));
+
+   std::vector<ProjChunk> chunks;
+   std::vector<SrcPlan> src;
+   std::vector<TgtPlan> tgt;
+   std::vector<int> isrc, itoff;
+   std::vector<TransDesc> tr;
+   std::vector<long long> lv_ut(L);
+   std::vector<std::vector<int>> lv_w(L);
+   long long vt_b = 0, pv_b = 0, t_b = 0, uti_b = 0, ut_b = 0, spec_b = 0, acc_b = 0;
+   int tpart_n = 0, w_n = 0;
+   for (size_t li = 0; li < L; li++) {
+      const FarLevel& lv = op.levels[li];
+      P.max_n = std::max(P.max_n, lv.n);
+      for (const SrcDesc& sd : lv.h_src) {
+         SrcPlan sp;
+         sp.n = lv.n;
+         sp.r = sd.r;
+         sp.part_off = tpart_n;
+         const int cpc = std::max(1, CHUNK / sd.r);
+         sp.nchunk = 0;
+         for (int c0 = 0; c0 < sd.np; c0 += cpc) {
+            ProjChunk c;
+            c.vt_off = vt_b + sd.vt_off + (long long)c0 * sd.r;
+            c.p0 = sd.p0 + c0;
+            c.ncol = std::min(cpc, sd.np - c0);
+            c.r = sd.r;
+            c.out_off = tpart_n;
+            tpart_n += 2 * sd.r;
+            chunks.push_back(c);
+            sp.nchunk++;
+            P.proj_maxcols = std::max(P.proj_maxcols, c.ncol);
+         }
+         sp.pv_off = pv_b + sd.pv_off;
+         sp.spec_off = spec_b + (long long)sd.spec_slot * lv.pad;
+         src.push_back(sp);
+         P.maxr_src = std::max(P.maxr_src, sd.r);
+         tr.push_back({lv.VT64.p + sd.vt_off, P.VTc64.p + vt_b + sd.vt_off, sd.r, sd.np});
+      }
+      lv_ut[li] = ut_b;
+      lv_w[li].resize(lv.h_tgt.size());
+      for (size_t k = 0; k < lv.h_tgt.size(); k++) {
+         const TgtDesc& td = lv.h_tgt[k];
+         TgtPlan tp;
+         tp.n = lv.n;
+         tp.r = td.r;
+         tp.pad = lv.pad;
+         tp.nint = td.nint;
+         tp.int_off = (int)isrc.size();
+         tp.w_off = w_n;
+         tp.t_off = t_b / 2;
+         tp.acc_off = acc_b + (long long)td.spec_slot * lv.pad;
+         tp.uti_off = uti_b + td.uti_off;
+         for (int e = 0; e < td.nint; e++) {
+            long long so = spec_b + (long long)lv.h_isrc[td.int_off + e] * lv.pad;
+            if (so > 0x7fffffffLL) throw std::runtime_error("spectrum offsets exceed 32 bits");
+            isrc.push_back((int)so);
+            itoff.push_back(lv.h_itoff[td.int_off + e]);
+         }
+         lv_w[li][k] = w_n;
+         w_n += td.r;
+         tgt.push_back(tp);
+      }
+      auto cp = [&](double* dst, const DBuf<double>& b) {
+         if (b.n) FSVD_CHECK(hipMemcpyAsync(dst, b.p, sizeof(double) * b.n, hipMemcpyDeviceToDevice, s));
+      };
+      cp(P.PV64.p + pv_b, lv.PV64);
+      cp(P.Tsp64.p + t_b, lv.T64);
+      cp(P.UTI64.p + uti_b, lv.UTI64);
+      cp(P.UT64.p + ut_b, lv.UT64);
+      vt_b += (long long)lv.VT64.n;
+      pv_b += (long long)lv.PV64.n;
+      t_b += (long long)lv.T64.n;
+      uti_b += (long long)lv.UTI64.n;
+      ut_b += (long long)lv.UT64.n;
+      spec_b += (long long)lv.nsrc * lv.pad;
+      acc_b += (long long)lv.ntgt * lv.pad;
+   }
+   {
+      DBuf<TransDesc> dtr;
+      dtr.upload(tr, s);
+      launch_transpose_batched<double>((int)tr.size(), dtr.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   // leaf -> far-field output blocks, finest level first
+   std::vector<int> leaf_cube(op.nleaf, -1);
+   for (size_t i = 0; i < op.cube_leaf.size(); i++)
+      if (op.cube_leaf[i] >= 0) leaf_cube[op.cube_leaf[i]] = (int)i;
+   std::vector<LeafU> ublk;
+   for (int li = 0; li < op.nleaf; li++) {
+      LeafDesc& Ld = op.h_leaves[li];
+      Ld.u0 = (int)ublk.size();
+      for (int a = leaf_cube[li]; a >= 0 && t.cubes[a].level >= 2; a = t.cubes[a].parent) {
+         const int lvl = t.cubes[a].level - 2, slot = op.cube_tgt_slot[a];
+         if (lvl >= (int)L || slot < 0) continue;
+         const TgtDesc& td = op.levels[lvl].h_tgt[slot];
+         LeafU u;
+         u.u_off = lv_ut[lvl] + td.u_off + (Ld.q0 - td.q0);
+         u.ld = td.nq;
+         u.r = td.r;
+         u.w_off = lv_w[lvl][slot];
+         u.pad_ = 0;
+         ublk.push_back(u);
+      }
+      Ld.nu = (int)ublk.size() - Ld.u0;
+   }
+   op.leaves.upload(op.h_leaves, s);
+   P.nchunk = (int)chunks.size();
+   P.nsrc = (int)src.size();
+   P.ntgt = (int)tgt.size();
+   P.nublk = (int)ublk.size();
+   P.chunks.upload(chunks, s);
+   P.src.upload(src, s);
+   P.tgt.upload(tgt, s);
+   P.isrc.upload(isrc, s);
+   P.itoff.upload(itoff, s);
+   P.ublk.upload(ublk, s);
+   P.tpart.alloc(std::max(tpart_n, 1));
+   P.spec.alloc(2 * std::max<long long>(spec_b, 1));
+   P.acc.alloc(2 * std::max<long long>(acc_b, 1));
+   P.w.alloc(std::max(w_n, 1));
+   if (want32) {
+      to_f32(P.VTc64, P.VTc32, s);
+      to_f32(P.PV64, P.PV32, s);
+      to_f32(P.Tsp64, P.Tsp32, s);
+      to_f32(P.UTI64, P.UTI32, s);
+      to_f32(P.UT64, P.UT32, s);
+   }
+}
+
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s) {
@@ -143,6 +294,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       L.m = (int)op.h_dcol.size() - L.col_off;
       L.d_off = dsize;
+      L.u0 = 0;
+      L.nu = 0;
       dsize += 2LL * L.p * L.m;
       op.h_leaves.push_back(L);
       op.maxm = std::max(op.maxm, L.m);
@@ -277,12 +430,14 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
                      int idx = t.cube_pt_sorted[off + (unsigned)p * (unsigned)I.nq / (unsigned)P.pass];
                      for (int d = 0; d < 3; d++) bpts.push_back(pts[3 * (size_t)idx + d]);
                   }
+               // column-major A (np x K): sample s of layer L at column L*half + s
                b.npt = half;
                b.npan = c.np;
                b.pt_off = base;
                b.pan_off = c.p0;
-               b.ld_pt = 1;
-               b.ld_pan = K;
+               b.ld_pt = c.np;
+               b.ld_pan = 1;
+               b.layer_stride = (long long)half * c.np;
                m.rows = c.np;
                m.cols = K;
             } else {
@@ -298,8 +453,9 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
                b.npan = half;
                b.pt_off = c.q0;
                b.pan_off = base;
-               b.ld_pt = K;
-               b.ld_pan = 1;
+               b.ld_pt = 1;
+               b.ld_pan = c.nq;
+               b.layer_stride = (long long)half * c.nq;
                m.rows = c.nq;
                m.cols = K;
             }
@@ -313,14 +469,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          Abuf->alloc(std::max<long long>(asz, 1));
          Qbuf->alloc(std::max<long long>(qsz, 1));
          run_integrals(blocks, bpts, bpan, d_panels, kernel, kappa, Abuf->p, s);
-         DBuf<MgsDesc> dm;
-         dm.upload(mgs, s);
-         DBuf<int> dr;
-         dr.alloc(mgs.size());
-         launch_mgs((int)mgs.size(), 0, dm.p, Abuf->p, Qbuf->p, dr.p, svdeps, s);
-         std::vector<int> ranks(mgs.size());
-         FSVD_CHECK(hipMemcpyAsync(ranks.data(), dr.p, sizeof(int) * ranks.size(), hipMemcpyDeviceToHost, s));
-         FSVD_CHECK(hipStreamSynchronize(s));
+         std::vector<int> ranks;
+         run_mgs(mgs, Abuf->p, Qbuf->p, ranks, svdeps, s);
          std::vector<int> next;
          for (size_t pi = 0; pi < pending.size(); pi++) {
             Prob& P = probs[pending[pi]];
@@ -626,48 +776,68 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       op.have32 = true;
    }
+   build_apply_plan(op, want32, s);
    FSVD_CHECK(hipStreamSynchronize(s));
    op.stats.setup_ms_far = now_ms() - t0;
    return opp;
 }
 
 template <typename T>
-static void apply_impl(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s) {
+static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, double* y) {
    const bool f32 = std::is_same<T, float>::value;
-   launch_zero(op.yfar.p, op.tree.npt, s);
-   for (FarLevel& lv : op.levels)
-      if (lv.nsrc)
-         launch_far_source<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
-                              f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, xs, xd, lv.spec.p, lv.maxr_src, s);
-   for (FarLevel& lv : op.levels)
-      if (lv.ntgt)
-         launch_far_translate<T>(lv.ntgt, lv.tgt.p, lv.isrc.p, lv.itoff.p,
-                                 f32 ? (const void*)lv.T32.p : (const void*)lv.T64.p, lv.spec.p, lv.acc.p, lv.pad, s);
-   for (FarLevel& lv : op.levels)
-      if (lv.ntgt)
-         launch_far_target<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
-                              f32 ? (const T*)lv.UTI32.p : (const T*)lv.UTI64.p, lv.acc.p, op.yfar.p, lv.maxr_tgt, s);
-   if (op.nleaf)
-      launch_near<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, xs, xd,
-                     op.yfar.p, y, op.maxm, op.maxp, s);
+   ApplyPlan& P = op.plan;
+   ApplyArgs a;
+   a.nchunk = P.nchunk;
+   a.nsrc = P.nsrc;
+   a.ntgt = P.ntgt;
+   a.nleaf = op.nleaf;
+   a.proj_maxcols = P.proj_maxcols;
+   a.maxr_src = P.maxr_src;
+   a.max_n = P.max_n;
+   a.maxm = op.maxm;
+   a.maxp = op.maxp;
+   a.chunks = P.chunks.p;
+   a.src = P.src.p;
+   a.tgt = P.tgt.p;
+   a.isrc = P.isrc.p;
+   a.itoff = P.itoff.p;
+   a.leaves = op.leaves.p;
+   a.dcol = op.dcol.p;
+   a.ublk = P.ublk.p;
+   a.VTc = f32 ? (const void*)P.VTc32.p : (const void*)P.VTc64.p;
+   a.PV = f32 ? (const void*)P.PV32.p : (const void*)P.PV64.p;
+   a.Tsp = f32 ? (const void*)P.Tsp32.p : (const void*)P.Tsp64.p;
+   a.UTI = f32 ? (const void*)P.UTI32.p : (const void*)P.UTI64.p;
+   a.UT = f32 ? (const void*)P.UT32.p : (const void*)P.UT64.p;
+   a.D = f32 ? (const void*)op.D32.p : (const void*)op.D64.p;
+   a.tpart = P.tpart.p;
+   a.spec = P.spec.p;
+   a.acc = P.acc.p;
+   a.w = P.w.p;
+   a.xs = xs;
+   a.xd = xd;
+   a.y = y;
+   return a;
+}
+
+template <typename T>
+static void apply_impl(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s) {
+   const ApplyArgs a = make_args<T>(op, xs, xd, y);
+   for (int stage = 0; stage < 5; stage++) launch_apply_stage<T>(stage, a, s);
 }
 
 template <typename T>
 static void apply_part_impl(TreeOp& op, int part, const double* xs, const double* xd, double* y, hipStream_t s) {
-   const bool f32 = std::is_same<T, float>::value;
-   if (part == 0 && op.nleaf)
-      launch_near<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, xs, xd,
-                     op.yfar.p, y, op.maxm, op.maxp, s);
-   for (FarLevel& lv : op.levels) {
-      if (part == 1 && lv.nsrc)
-         launch_far_source<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
-                              f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, xs, xd, lv.spec.p, lv.maxr_src, s);
-      if (part == 2 && lv.ntgt)
-         launch_far_translate<T>(lv.ntgt, lv.tgt.p, lv.isrc.p, lv.itoff.p,
-                                 f32 ? (const void*)lv.T32.p : (const void*)lv.T64.p, lv.spec.p, lv.acc.p, lv.pad, s);
-      if (part == 3 && lv.ntgt)
-         launch_far_target<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
-                              f32 ? (const T*)lv.UTI32.p : (const T*)lv.UTI64.p, lv.acc.p, op.yfar.p, lv.maxr_tgt, s);
+   const ApplyArgs a = make_args<T>(op, xs, xd, y);
+   switch (part) {
+      case 0: launch_apply_stage<T>(4, a, s); break;
+      case 1:
+         launch_apply_stage<T>(0, a, s);
+         launch_apply_stage<T>(1, a, s);
+         break;
+      case 2: launch_apply_stage<T>(2, a, s); break;
+      case 3: launch_apply_stage<T>(3, a, s); break;
+      default: break;
    }
 }
 

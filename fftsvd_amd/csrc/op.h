@@ -81,6 +81,21 @@ struct TreeOpStats {
    double setup_ms_near = 0, setup_ms_far = 0;
 };
 
+// Forward-apply plan of a tree: every level's blocks concatenated, with
+// absolute offsets, so each stage is one launch.
+struct ApplyPlan {
+   int nchunk = 0, nsrc = 0, ntgt = 0, nublk = 0;
+   int proj_maxcols = 0, maxr_src = 0, max_n = 4;
+   DBuf<ProjChunk> chunks;
+   DBuf<SrcPlan> src;
+   DBuf<TgtPlan> tgt;
+   DBuf<int> isrc, itoff;
+   DBuf<LeafU> ublk;
+   DBuf<double> tpart, spec, acc, w;   // work vectors, sized for FP64
+   DBuf<double> VTc64, PV64, Tsp64, UTI64, UT64;
+   DBuf<float> VTc32, PV32, Tsp32, UTI32, UT32;
+};
+
 struct TreeOp {
    Tree tree;
    int kernel = LAPLACE;
@@ -103,6 +118,7 @@ struct TreeOp {
    DBuf<double> yfar;
    TreeOpStats stats;
    std::vector<int> cube_src_slot, cube_tgt_slot, cube_leaf;   // per cube: slot in its level / leaf index, -1
+   ApplyPlan plan;
 };
 
 // Builds tree, near field and far field for `npan` panels (caller-order
@@ -113,25 +129,23 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
 
 // y_t = S xs_t + D xd_t; all vectors in tree order, device pointers.
 void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s);
-// One class of the apply's launches only (0 near, 1 far source, 2 translate,
-// 3 far target), for per-kernel timing.
+// One stage of the apply only (0 leaf: near field + U w, 1 projection and
+// source transforms, 2 translation, 3 target transforms), for per-kernel
+// timing.
 void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s);
 // y_t = A_layer^T x_t (panels, tree order).
 void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s);
 
 // launchers (apply.hip / fill.hip)
 template <typename T>
-void launch_far_source(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const double* xs,
-                       const double* xd, void* spec, int maxr, hipStream_t s);
-template <typename T>
 void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const int* itoff, const void* Tspec,
                           const void* spec, void* acc, int pad, hipStream_t s);
+// One stage of the forward apply (0 project, 1 source, 2 translate,
+// 3 target, 4 leaf) over every level of a tree.
 template <typename T>
-void launch_far_target(int n, int nblk, const TgtDesc* tgt, const T* UT, const T* UTI, const void* acc,
-                       double* yfar, int maxr, hipStream_t s);
+void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s);
 template <typename T>
-void launch_near(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* xs, const double* xd,
-                 const double* yfar, double* y, int maxm, int maxp, hipStream_t s);
+void launch_transpose_batched(int n, const TransDesc* d, hipStream_t s);
 template <typename T>
 void launch_farT_source(int n, int nblk, const TgtDesc* tgt, const T* UT, const T* UTI, const double* x, void* spec,
                         int maxr, hipStream_t s);
@@ -147,8 +161,10 @@ void launch_integral_blocks(int nblk, const IntBlock* blocks, const double* bpts
 void launch_green_blocks(int nblk, const GreenBlock* blocks, const double* dest, const double* src, int kernel,
                          double kappa, double* out, hipStream_t s);
 void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStream_t s);
-void launch_mgs(int nprob, int maxrows, const MgsDesc* descs, double* A, double* Q, int* ranks, double eps,
-                hipStream_t s);
+// Pivoted MGS of every problem in `descs` (fills seg/wave0/ss_off);
+// ranks[rank_slot] receives each problem's rank.  Synchronous.
+void run_mgs(std::vector<MgsDesc>& descs, double* A, double* Q, std::vector<int>& ranks, double eps,
+             hipStream_t s);
 void launch_f64_to_f32(const double* a, float* b, long long n, hipStream_t s);
 
 }  // namespace fsvd

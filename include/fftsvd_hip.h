@@ -55,11 +55,18 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
                          int* iters, double* resid_hist, double* info);
 /* reaction potentials at the charges (SurfaceOperator.c:323-401 direct branch) */
 int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi);
-/* operator statistics: out[0..15] (see fftsvd_hip.cpp for the layout) */
+/* Operator statistics, up to 24 doubles: 0 panels, 1 regions, 2 setup ms,
+ * 3 near entries, 4 far entries, 5 spectrum bins, 6 translation MACs,
+ * 7 depth, 8 leaves, 9 tree applications per matvec, 10/11 algorithmic
+ * bytes per matvec (FP64 / FP32 storage), 12/13 near-field bytes, 14 near
+ * column indices, 15 U entries, 16 VT entries, 17 PV entries, 18 UTI
+ * entries, 19 T spectrum bins, 20 leaf far-field blocks, 21 projection
+ * chunks, 22 points applied (all counts summed over tree applications). */
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout);
 /* Times `nrep` surface matvecs on device-resident vectors with HIP events
- * on the library stream: total ms and the per-kernel-class ms (near, far
- * source, translate, far target, glue) from a separately timed pass. */
+ * on the library stream: total ms, and in ms_parts[0..3] the per-stage ms
+ * (leaf: near field + far-field output; projection + source transforms;
+ * translation; target transforms) from separately timed passes. */
 int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, double* ms_total, double* ms_parts);
 
 /* Per-tree views for parity tests.  which = region index (0 = exterior). */
