@@ -63,6 +63,38 @@ def cpu_baseline(seed, budget_s=25.0):
     return {"panels": len(P), "matvecs_per_s": 1.0 / dt, "setup_s": setup, "ms_per_matvec": dt * 1e3}
 
 
+STAGE_KERNELS = {"leaf": "k_leaf<float>", "project_source": "k_project<float> + k_source<float>",
+                 "translate": "k_translate_group<float, 32>", "target": "k_target<float>"}
+
+
+def stage_bytes(st):
+    """Algorithmic HBM bytes per matvec of each forward-apply stage in mixed
+    mode (4-byte operator entries, 8-byte complex spectrum bins, FP64
+    vectors)."""
+    n = st["points_applied"]
+    return {
+        # D blocks, their column indices, the U rows, x (both layers) once, y once
+        "leaf": 4.0 * st["near_entries"] + 4.0 * st["near_cols"] + 4.0 * st["ut_entries"] + 24.0 * n,
+        # VT and PV once, x (both layers) once, forward spectra written
+        "project_source": 4.0 * st["vt_entries"] + 4.0 * st["pv_entries"] + 16.0 * n + 8.0 * st["src_bins"],
+        # T spectra and source spectra read once, accumulated spectra written
+        "translate": 8.0 * st["t_bins"] + 8.0 * st["src_bins"] + 8.0 * st["tgt_bins"],
+        # accumulated spectra read, UTI once
+        "target": 8.0 * st["tgt_bins"] + 4.0 * st["uti_entries"],
+    }
+
+
+def stage_rooflines(st, parts_ms):
+    out = {}
+    for k, b in stage_bytes(st).items():
+        ms = parts_ms[k]
+        ach = b / (ms * 1e-3) / 1e9
+        out[k] = {"kernel": STAGE_KERNELS[k], "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": ach / HBM_PEAK_GBS, "bytes_per_matvec": b, "ms": ms,
+                  "launches_per_matvec": st["trees"] * (2 if k == "project_source" else 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,15 +156,13 @@ def main():
     ms64, parts64 = prob.bench_matvec(fa.F64, max(10, args.steps // 4))
     print("# matvec mixed %.3f ms, fp64 %.3f ms; parts mixed %s" % (ms32, ms64, parts32), flush=True)
 
-    # roofline of the dominant kernel (near-field block apply, HBM bound)
-    # per matvec: every stored FP32 near-field entry once, its column index
-    # list once, and per tree xs, xd, y_far read + y written (FP64)
-    near_bytes = st["near_bytes_f32"] + 4.0 * st["near_cols"] + 32.0 * len(P) * st["trees"]
-    near_ms = parts32["near"]
-    achieved = near_bytes / (near_ms * 1e-3) / 1e9
-    roofline = {"kernel": "k_near<float>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "bytes_per_launch_set": near_bytes, "ms": near_ms}
+    # rooflines: algorithmic HBM bytes of each apply stage (FP32 storage,
+    # every stored entry read once, spectra written and read once, vectors
+    # read/written once) over its HIP-event time on the library stream
+    rf = stage_rooflines(st, parts32)
+    dom = max(rf, key=lambda k: rf[k]["ms"])
+    roofline = dict(rf[dom])
+    roofline.update({"stage": dom, "traffic": None})
 
     cpu = None
     if not args.no_cpu:
@@ -171,6 +201,7 @@ def main():
         "solve": solve,
         "operator": st,
         "roofline": roofline,
+        "roofline_stages": rf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

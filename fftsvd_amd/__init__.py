@@ -238,12 +238,13 @@ class Problem:
         return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in
 
     def stats(self):
-        out = np.zeros(24)
-        _check(lib().fftsvd_problem_stats(self.h, _d(out), 24))
+        out = np.zeros(32)
+        _check(lib().fftsvd_problem_stats(self.h, _d(out), 32))
         keys = ("panels", "regions", "setup_ms", "near_entries", "far_entries", "spectra_bins", "translate_macs",
                 "depth", "leaves", "trees", "bytes_f64", "bytes_f32", "near_bytes_f64", "near_bytes_f32", "near_cols",
                 "ut_entries", "vt_entries", "pv_entries", "uti_entries", "t_bins", "leaf_ublocks", "proj_chunks",
-                "points_applied")
+                "points_applied", "src_bins", "tgt_bins", "ms_tree", "ms_near", "ms_pinv_T", "ms_sampling",
+                "ms_mgs", "ms_bases", "ms_plan")
         return dict(zip(keys, out.tolist()))
 
     def bench_matvec(self, precision=F64, nrep=20):

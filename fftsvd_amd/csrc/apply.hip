@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <type_traits>
 
 #include "device.h"
 
@@ -35,6 +36,11 @@ __device__ __forceinline__ T wave_sum(T v) {
    return v;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs.  Launches whose blocks
+// share data across tree-order neighbours use a host-built block map
+// (xcd_block_map): block b runs logical block map[b], and each XCD gets a
+// contiguous tree-order range of about equal work, so neighbouring work
+// shares one L2.  -1 marks padding blocks.
 // e^{-2 pi i m / N} table
 template <typename T, int N>
 __device__ __forceinline__ void twiddle_table(T* cs, T* sn) {
@@ -313,23 +319,53 @@ __global__ void __launch_bounds__(256) k_source(const SrcPlan* __restrict__ src,
    }
 }
 
+// Translation: block per target cube, threads over frequency bins; blocks
+// follow a host-built map that gives each XCD a contiguous tree-order range
+// of about equal work.  The interaction indices are uniform over the block
+// (scalar loads); four interactions are in flight per iteration.  Summation
+// order: two interleaved partial sums over the interaction list, added at
+// the end.
 template <typename T>
-__global__ void __launch_bounds__(256) k_translate(const TgtPlan* __restrict__ tgt, const int* __restrict__ isrc,
-                                                   const int* __restrict__ itoff,
+__global__ void __launch_bounds__(256) k_translate(const TgtPlan* __restrict__ tgt, const int* __restrict__ bmap,
+                                                   const int* __restrict__ isrc, const int* __restrict__ itoff,
                                                    const typename Cplx<T>::type* __restrict__ Tspec,
                                                    const typename Cplx<T>::type* __restrict__ spec,
                                                    typename Cplx<T>::type* __restrict__ acc) {
-   const TgtPlan d = tgt[blockIdx.x];
+   using C2 = typename Cplx<T>::type;
+   const int ti = bmap[blockIdx.x];
+   if (ti < 0) return;
+   const TgtPlan d = tgt[ti];
    const int pad = d.pad;
+   const int* is = isrc + d.int_off;
+   const int* io = itoff + d.int_off;
    for (int b = threadIdx.x; b < pad; b += blockDim.x) {
-      T re = 0, im = 0;
-      for (int e = 0; e < d.nint; e++) {
-         auto t = Tspec[d.t_off + (long long)itoff[d.int_off + e] * pad + b];
-         auto s = spec[(long long)isrc[d.int_off + e] + b];
-         re += t.x * s.x - t.y * s.y;
-         im += t.x * s.y + t.y * s.x;
+      const C2* Tb = Tspec + d.t_off + b;
+      const C2* Sb = spec + b;
+      T re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+      int e = 0;
+      for (; e + 3 < d.nint; e += 4) {
+         const C2 t0 = Tb[(long long)io[e] * pad], s0 = Sb[is[e]];
+         const C2 t1 = Tb[(long long)io[e + 1] * pad], s1 = Sb[is[e + 1]];
+         const C2 t2 = Tb[(long long)io[e + 2] * pad], s2 = Sb[is[e + 2]];
+         const C2 t3 = Tb[(long long)io[e + 3] * pad], s3 = Sb[is[e + 3]];
+         re0 += t0.x * s0.x - t0.y * s0.y;
+         im0 += t0.x * s0.y + t0.y * s0.x;
+         re1 += t1.x * s1.x - t1.y * s1.y;
+         im1 += t1.x * s1.y + t1.y * s1.x;
+         re0 += t2.x * s2.x - t2.y * s2.y;
+         im0 += t2.x * s2.y + t2.y * s2.x;
+         re1 += t3.x * s3.x - t3.y * s3.y;
+         im1 += t3.x * s3.y + t3.y * s3.x;
       }
-      acc[d.acc_off + b] = {re, im};
+      for (; e < d.nint; e++) {
+         const C2 t0 = Tb[(long long)io[e] * pad], s0 = Sb[is[e]];
+         re0 += t0.x * s0.x - t0.y * s0.y;
+         im0 += t0.x * s0.y + t0.y * s0.x;
+      }
+      C2 out;
+      out.x = re0 + re1;
+      out.y = im0 + im1;
+      acc[d.acc_off + b] = out;
    }
 }
 
@@ -385,9 +421,11 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
                                               const T* __restrict__ D, const double* __restrict__ xs,
                                               const double* __restrict__ xd, const LeafU* __restrict__ ublk,
                                               const T* __restrict__ UT, const T* __restrict__ w,
-                                              double* __restrict__ y) {
+                                              double* __restrict__ y, const int* __restrict__ bmap) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const LeafDesc L = leaves[blockIdx.x];
+   const int li = bmap[blockIdx.x];
+   if (li < 0) return;
+   const LeafDesc L = leaves[li];
    const int E = 2 * L.p, m = L.m, p = L.p;
    T* xg = (T*)smem;       // 2*m
    T* part = xg + 2 * m;   // 256
@@ -561,8 +599,9 @@ size_t far_target_lds(int n, int maxr) {
       default: throw std::runtime_error("unsupported grid size"); \
    }
 
+// gfx950: 160 KiB of LDS per workgroup
 static void check_lds(size_t lds) {
-   if (lds > 65536) throw std::runtime_error("LDS request above 64 KiB per workgroup");
+   if (lds > 163840) throw std::runtime_error("LDS request above 160 KiB per workgroup");
 }
 
 template <typename T>
@@ -609,8 +648,8 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       }
       case 2:
          if (a.ntgt)
-            hipLaunchKernelGGL((k_translate<T>), dim3(a.ntgt), dim3(256), 0, s, a.tgt, a.isrc, a.itoff,
-                               (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+            hipLaunchKernelGGL((k_translate<T>), dim3(a.nmap_tgt), dim3(256), 0, s, a.tgt, a.map_tgt, a.isrc,
+                               a.itoff, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
          break;
       case 3: {
          size_t lds = target_lds<T>(a.max_n);
@@ -624,8 +663,8 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          size_t lds = sizeof(T) * (size_t)(2 * a.maxm + 256 + 3 * a.maxp) + 64;
          check_lds(lds);
          if (a.nleaf)
-            hipLaunchKernelGGL((k_leaf<T>), dim3(a.nleaf), dim3(256), lds, s, a.leaves, a.dcol, (const T*)a.D, a.xs,
-                               a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y);
+            hipLaunchKernelGGL((k_leaf<T>), dim3(a.nmap_leaf), dim3(256), lds, s, a.leaves, a.dcol,
+                               (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y, a.map_leaf);
          break;
       }
       default: break;

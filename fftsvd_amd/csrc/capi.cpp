@@ -234,7 +234,7 @@ static TreeOp& region_op(Problem& pr, int which) {
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
    API_TRY
    Problem& pr = *p->pr;
-   double v[24] = {0};
+   double v[32] = {0};
    v[0] = pr.ntotal;
    v[1] = (double)pr.regions.size();
    v[2] = pr.setup_ms;
@@ -262,8 +262,12 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
          v[17] += (double)lv.PV64.n;
          v[18] += (double)lv.UTI64.n;
          v[19] += (double)lv.T64.n / 2;
+         v[23] += (double)lv.nsrc * lv.pad;
+         v[24] += (double)lv.ntgt * lv.pad;
       }
       v[20] += op.plan.nublk;
+      if (R.shares_tree_with < 0)
+         for (int k = 0; k < 7; k++) v[25 + k] += op.stats.phase_ms[k];
       v[21] += op.plan.nchunk;
       v[22] += (double)n;
    }
@@ -273,7 +277,7 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
    v[11] = 4.0 * (v[3] + v[4]) + 8.0 * 2 * v[5] + vec + glue;
    v[12] = 8.0 * v[3];
    v[13] = 4.0 * v[3];
-   for (int i = 0; i < nout && i < 24; i++) out[i] = v[i];
+   for (int i = 0; i < nout && i < 32; i++) out[i] = v[i];
    API_CATCH
 }
 

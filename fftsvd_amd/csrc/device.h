@@ -47,7 +47,7 @@ struct MgsDesc {
    long long a_off;
    long long q_off;
    int rank_slot;
-   int seg, wave0, ss_off;
+   int seg, wave0, ss_off, pid;   // pid: index of the problem's MgsState
 };
 
 // Per-problem progress of the stepwise MGS.
@@ -103,9 +103,9 @@ struct SrcPlan {
 };
 
 // Target cube: interactions int_off..+nint (absolute source spectrum
-// offsets, T offset codes), its level's T spectra at t_off (offset code c
-// at t_off + c*pad), accumulated spectrum at acc_off, UTI (r x gp3) at
-// uti_off, r outputs w at w_off.
+// offsets, T offset codes), its level's T spectra
+// at t_off (offset code c at t_off + c*pad), accumulated spectrum at
+// acc_off, UTI (r x gp3) at uti_off, r outputs w at w_off.
 struct TgtPlan {
    int n, r, pad, nint, int_off, w_off;
    long long t_off, acc_off, uti_off;
@@ -134,6 +134,8 @@ struct ApplyArgs {
    const SrcPlan* src;
    const TgtPlan* tgt;
    const int *isrc, *itoff;
+   int nmap_tgt, nmap_leaf;           // XCD block maps (grid sizes)
+   const int *map_tgt, *map_leaf;
    const LeafDesc* leaves;
    const int* dcol;
    const LeafU* ublk;

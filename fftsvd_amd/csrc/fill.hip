@@ -47,19 +47,59 @@ __global__ void __launch_bounds__(256) k_green_blocks(const GreenBlock* __restri
    }
 }
 
-// C = A*B, one thread per output element, sum over k in increasing order.
-// grid.x = output tiles of 256, grid.y = batch.
+// C = A*B with arbitrary element strides, one descriptor per grid.y.
+// 32x32 output tiles (grid.x strides over a problem's tiles), K staged in
+// LDS chunks of 32; each thread owns a 2x2 block of the tile.  Each output
+// is summed over k in increasing order.  Tile loads read along whichever
+// operand dimension has unit stride.
 __global__ void __launch_bounds__(256) k_gemm_batched(const GemmDesc* __restrict__ descs) {
    const GemmDesc d = descs[blockIdx.y];
-   const int total = d.M * d.N;
-   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-      int i = e / d.N;
-      int j = e - i * d.N;
-      const double* a = d.A + (long long)i * d.a_rs;
-      const double* b = d.B + (long long)j * d.b_cs;
-      double s = 0.0;
-      for (int k = 0; k < d.K; k++) s += a[(long long)k * d.a_cs] * b[(long long)k * d.b_rs];
-      d.C[(long long)i * d.c_rs + (long long)j * d.c_cs] = s;
+   __shared__ double As[32][33], Bs[32][33];   // As[i][k], Bs[k][j]
+   const int tm = (d.M + 31) / 32, tn = (d.N + 31) / 32;
+   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+   for (int tile = blockIdx.x; tile < tm * tn; tile += gridDim.x) {
+      const int i0 = (tile / tn) * 32, j0 = (tile % tn) * 32;
+      double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+      for (int k0 = 0; k0 < d.K; k0 += 32) {
+         for (int e = tid; e < 1024; e += 256) {
+            int ii, kk;
+            if (d.a_cs == 1) {
+               ii = e >> 5;
+               kk = e & 31;
+            } else {
+               kk = e >> 5;
+               ii = e & 31;
+            }
+            const int gi = i0 + ii, gk = k0 + kk;
+            As[ii][kk] = (gi < d.M && gk < d.K) ? d.A[(long long)gi * d.a_rs + (long long)gk * d.a_cs] : 0.0;
+            int jj;
+            if (d.b_rs == 1) {
+               jj = e >> 5;
+               kk = e & 31;
+            } else {
+               kk = e >> 5;
+               jj = e & 31;
+            }
+            const int gj = j0 + jj, gk2 = k0 + kk;
+            Bs[kk][jj] = (gj < d.N && gk2 < d.K) ? d.B[(long long)gk2 * d.b_rs + (long long)gj * d.b_cs] : 0.0;
+         }
+         __syncthreads();
+#pragma unroll 8
+         for (int kk = 0; kk < 32; kk++) {
+            const double a0 = As[2 * ty][kk], a1 = As[2 * ty + 1][kk];
+            const double b0 = Bs[kk][2 * tx], b1 = Bs[kk][2 * tx + 1];
+            c00 += a0 * b0;
+            c01 += a0 * b1;
+            c10 += a1 * b0;
+            c11 += a1 * b1;
+         }
+         __syncthreads();
+      }
+      const int i = i0 + 2 * ty, j = j0 + 2 * tx;
+      if (i < d.M && j < d.N) d.C[(long long)i * d.c_rs + (long long)j * d.c_cs] = c00;
+      if (i < d.M && j + 1 < d.N) d.C[(long long)i * d.c_rs + (long long)(j + 1) * d.c_cs] = c01;
+      if (i + 1 < d.M && j < d.N) d.C[(long long)(i + 1) * d.c_rs + (long long)j * d.c_cs] = c10;
+      if (i + 1 < d.M && j + 1 < d.N) d.C[(long long)(i + 1) * d.c_rs + (long long)(j + 1) * d.c_cs] = c11;
    }
 }
 
@@ -93,7 +133,7 @@ __global__ void __launch_bounds__(256) k_mgs_update(const MgsDesc* __restrict__ 
       else hi = mid - 1;
    }
    const MgsDesc d = descs[lo];
-   if (k >= 0 && st[lo].done) return;  // uniform over the wave
+   if (k >= 0 && st[d.pid].done) return;  // uniform over the wave
    const int lane = threadIdx.x & 63, S = d.seg, sub = lane / S, sl = lane - sub * S;
    const int j = (gw - d.wave0) * (64 / S) + sub;
    const bool active = j < d.cols && j > k;
@@ -275,37 +315,51 @@ void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStr
 
 void run_mgs(std::vector<MgsDesc>& descs, double* A, double* Q, std::vector<int>& ranks, double eps,
              hipStream_t s) {
-   constexpr int NR = 16;
+   // two update classes: columns up to 16*seg rows (seg <= 64) held in 16
+   // registers per lane, longer ones in 32 (streamed beyond 2048 rows)
+   constexpr int NRS = 16, NRL = 32;
    const int nprob = (int)descs.size();
    ranks.assign(nprob, 0);
    if (nprob == 0) return;
-   int waves = 0, ncol = 0, kmax = 0;
-   for (auto& d : descs) {
+   std::vector<MgsDesc> cls[2];
+   int waves[2] = {0, 0}, ncol = 0, kmax = 0;
+   for (int p = 0; p < nprob; p++) {
+      MgsDesc& d = descs[p];
+      const int c = d.rows > 64 * NRS ? 1 : 0;
+      const int NR = c ? NRL : NRS;
       int S = 4;
       while (S < 64 && (d.rows + S - 1) / S > NR) S <<= 1;
       d.seg = S;
-      d.wave0 = waves;
+      d.wave0 = waves[c];
       d.ss_off = ncol;
-      waves += (d.cols + 64 / S - 1) / (64 / S);
+      d.pid = p;
+      waves[c] += (d.cols + 64 / S - 1) / (64 / S);
       ncol += d.cols;
       kmax = std::max(kmax, std::min(d.rows, d.cols));
+      cls[c].push_back(d);
    }
-   DBuf<MgsDesc> dd;
+   DBuf<MgsDesc> dd, dc[2];
    dd.upload(descs, s);
+   for (int c = 0; c < 2; c++) dc[c].upload(cls[c], s);
    DBuf<double> ss;
    ss.alloc(std::max(ncol, 1));
    DBuf<MgsState> st;
    st.alloc(nprob);
    hipLaunchKernelGGL(k_mgs_init, dim3((nprob + 255) / 256), dim3(256), 0, s, st.p, nprob);
-   const int blocks = (waves + 3) / 4;
-   hipLaunchKernelGGL((k_mgs_update<NR>), dim3(blocks), dim3(256), 0, s, dd.p, nprob, waves, A, Q, ss.p, st.p, -1);
+   auto update = [&](int k) {
+      if (!cls[0].empty())
+         hipLaunchKernelGGL((k_mgs_update<NRS>), dim3((waves[0] + 3) / 4), dim3(256), 0, s, dc[0].p,
+                            (int)cls[0].size(), waves[0], A, Q, ss.p, st.p, k);
+      if (!cls[1].empty())
+         hipLaunchKernelGGL((k_mgs_update<NRL>), dim3((waves[1] + 3) / 4), dim3(256), 0, s, dc[1].p,
+                            (int)cls[1].size(), waves[1], A, Q, ss.p, st.p, k);
+   };
+   update(-1);
    MgsState* h = nullptr;
    FSVD_CHECK(hipHostMalloc((void**)&h, sizeof(MgsState) * nprob, hipHostMallocDefault));
    for (int k = 0; k <= kmax; k++) {
       hipLaunchKernelGGL(k_mgs_pivot, dim3(nprob), dim3(256), 0, s, dd.p, A, Q, ss.p, st.p, k, eps);
-      if (k < kmax)
-         hipLaunchKernelGGL((k_mgs_update<NR>), dim3(blocks), dim3(256), 0, s, dd.p, nprob, waves, A, Q, ss.p, st.p,
-                            k);
+      if (k < kmax) update(k);
       if ((k & 7) == 7 || k == kmax) {
          FSVD_CHECK(hipMemcpyAsync(h, st.p, sizeof(MgsState) * nprob, hipMemcpyDeviceToHost, s));
          FSVD_CHECK(hipStreamSynchronize(s));

@@ -85,9 +85,9 @@ void run_gemms(const std::vector<GemmDesc>& g, hipStream_t s) {
    if (g.empty()) return;
    DBuf<GemmDesc> dg;
    dg.upload(g, s);
-   int maxout = 0;
-   for (auto& d : g) maxout = std::max(maxout, d.M * d.N);
-   int tiles = std::min(64, std::max(1, (maxout + 255) / 256));
+   int maxtiles = 0;
+   for (auto& d : g) maxtiles = std::max(maxtiles, ((d.M + 31) / 32) * ((d.N + 31) / 32));
+   int tiles = std::min(256, std::max(1, maxtiles));
    // batch dimension limited to 65535 per launch
    for (size_t b0 = 0; b0 < g.size(); b0 += 65535) {
       int nb = (int)std::min<size_t>(65535, g.size() - b0);
@@ -102,6 +102,26 @@ void to_f32(const DBuf<double>& a, DBuf<float>& b, hipStream_t s) {
 }
 
 }  // namespace
+
+std::vector<int> xcd_block_map(const std::vector<double>& work) {
+   const int n = (int)work.size();
+   double total = 0;
+   for (double w : work) total += w;
+   std::vector<std::vector<int>> part(8);
+   double acc = 0;
+   for (int i = 0; i < n; i++) {
+      int x = total > 0 ? (int)(8.0 * (acc + 0.5 * work[i]) / total) : (int)(8LL * i / std::max(1, n));
+      x = std::min(7, std::max(0, x));
+      part[x].push_back(i);
+      acc += work[i];
+   }
+   size_t len = 0;
+   for (auto& p : part) len = std::max(len, p.size());
+   std::vector<int> map(8 * len, -1);
+   for (int x = 0; x < 8; x++)
+      for (size_t k = 0; k < part[x].size(); k++) map[8 * k + x] = part[x][k];
+   return map;
+}
 
 // Concatenates every level's far-field blocks into single buffers with
 // absolute offsets (row bases re-laid panel-major for the projection),
@@ -236,10 +256,23 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.ntgt = (int)tgt.size();
    P.nublk = (int)ublk.size();
    P.chunks.upload(chunks, s);
-   P.src.upload(src, s);
-   P.tgt.upload(tgt, s);
    P.isrc.upload(isrc, s);
    P.itoff.upload(itoff, s);
+   {
+      std::vector<double> wt(tgt.size()), wl(op.h_leaves.size());
+      for (size_t k = 0; k < tgt.size(); k++) wt[k] = (double)tgt[k].pad * (tgt[k].nint + 4);
+      for (size_t k = 0; k < wl.size(); k++) {
+         const LeafDesc& Ld = op.h_leaves[k];
+         wl[k] = 2.0 * Ld.p * Ld.m + 64.0;
+      }
+      std::vector<int> mt = xcd_block_map(wt), ml = xcd_block_map(wl);
+      P.nmap_tgt = (int)mt.size();
+      P.nmap_leaf = (int)ml.size();
+      P.map_tgt.upload(mt, s);
+      P.map_leaf.upload(ml, s);
+   }
+   P.src.upload(src, s);
+   P.tgt.upload(tgt, s);
    P.ublk.upload(ublk, s);
    P.tpart.alloc(std::max(tpart_n, 1));
    P.spec.alloc(2 * std::max<long long>(spec_b, 1));
@@ -256,7 +289,8 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
 
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
-                                      int gridpoints, double svdeps, bool want32, hipStream_t s) {
+                                      int gridpoints, double svdeps, bool want32, hipStream_t s,
+                                      const Tree* pre) {
    auto opp = std::make_unique<TreeOp>();
    TreeOp& op = *opp;
    op.kernel = kernel;
@@ -266,7 +300,16 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    t.kappa = kappa;
    t.svdeps = svdeps;
    const int npan = (int)panels.size();
-   build_tree(t, panels.data(), npan, pts, npt, maxpanels, gridpoints);
+   double tp = now_ms();
+   if (pre) {
+      t = *pre;
+   } else {
+      build_tree(t, panels.data(), npan, pts, npt, maxpanels, gridpoints);
+   }
+   t.kernel = kernel;
+   t.kappa = kappa;
+   t.svdeps = svdeps;
+   op.stats.phase_ms[0] += now_ms() - tp;
    op.stats.depth = t.depth;
    op.stats.ncubes = (int)t.cubes.size();
 
@@ -326,7 +369,9 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       run_integrals(blocks, ptsT, bpan, d_panels, kernel, kappa, op.D64.p, s);
    }
+   FSVD_CHECK(hipStreamSynchronize(s));
    op.stats.setup_ms_near = now_ms() - t0;
+   op.stats.phase_ms[1] += op.stats.setup_ms_near;
    t0 = now_ms();
 
    // ---------------- far field ----------------
@@ -342,6 +387,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       lv.gp3 = n * n * n;
       const int gp3 = lv.gp3, pad = lv.pad;
 
+      tp = now_ms();
       // pseudoinverse of the sphere -> grid Green's matrix (Tree_fill_pinv_Fitting, Tree.c:295-369)
       {
          double box = root / std::pow(2.0, l);
@@ -371,6 +417,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       DBuf<double> d_pinv;
       d_pinv.upload(op.h_pinv[l], s);
+      op.stats.phase_ms[2] += now_ms() - tp;
+      tp = now_ms();
 
       // cubes of this level with interactions
       std::vector<int> cl;
@@ -469,8 +517,13 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          Abuf->alloc(std::max<long long>(asz, 1));
          Qbuf->alloc(std::max<long long>(qsz, 1));
          run_integrals(blocks, bpts, bpan, d_panels, kernel, kappa, Abuf->p, s);
+         FSVD_CHECK(hipStreamSynchronize(s));
+         op.stats.phase_ms[3] += now_ms() - tp;
+         tp = now_ms();
          std::vector<int> ranks;
          run_mgs(mgs, Abuf->p, Qbuf->p, ranks, svdeps, s);
+         op.stats.phase_ms[4] += now_ms() - tp;
+         tp = now_ms();
          std::vector<int> next;
          for (size_t pi = 0; pi < pending.size(); pi++) {
             Prob& P = probs[pending[pi]];
@@ -486,6 +539,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          keep.push_back(Qbuf);
          pending.swap(next);
       }
+      op.stats.phase_ms[3] += now_ms() - tp;
+      tp = now_ms();
       // ranks into cubes; compact bases into the level buffers
       std::vector<int> vt_prob(cl.size(), -1), u_prob(cl.size(), -1);
       for (size_t i = 0; i < probs.size(); i++) {
@@ -704,6 +759,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          }
          run_gemms(g2, s);
       }
+      FSVD_CHECK(hipStreamSynchronize(s));
+      op.stats.phase_ms[5] += now_ms() - tp;
       op.stats.far_entries += (long long)(lv.VT64.n + lv.PV64.n + lv.UT64.n + lv.UTI64.n);
       op.levels.push_back(std::move(lv));
    }
@@ -716,6 +773,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       for (size_t i = 0; i < t.cubes.size(); i++)
          if (t.cubes[i].leaf && t.cubes[i].nq > 0) op.cube_leaf[i] = (int)li++;
    }
+   tp = now_ms();
    // interaction lists (Cube_multiply_T, Cube.c:2062-2111) and their transposes
    for (FarLevel& lv : op.levels) {
       for (TgtDesc& td : lv.h_tgt) td.nint = 0;
@@ -778,6 +836,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    }
    build_apply_plan(op, want32, s);
    FSVD_CHECK(hipStreamSynchronize(s));
+   op.stats.phase_ms[6] += now_ms() - tp;
    op.stats.setup_ms_far = now_ms() - t0;
    return opp;
 }
@@ -801,6 +860,10 @@ static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, doubl
    a.tgt = P.tgt.p;
    a.isrc = P.isrc.p;
    a.itoff = P.itoff.p;
+   a.nmap_tgt = P.nmap_tgt;
+   a.map_tgt = P.map_tgt.p;
+   a.nmap_leaf = P.nmap_leaf;
+   a.map_leaf = P.map_leaf.p;
    a.leaves = op.leaves.p;
    a.dcol = op.dcol.p;
    a.ublk = P.ublk.p;

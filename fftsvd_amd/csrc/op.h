@@ -79,6 +79,10 @@ struct TreeOpStats {
    long long translate_macs = 0;   // sum over targets of #interactions * pad
    int depth = 0, nleaf = 0, ncubes = 0;
    double setup_ms_near = 0, setup_ms_far = 0;
+   // host wall time of the setup phases: 0 tree, 1 near-field fill,
+   // 2 pinv + T spectra, 3 sampling integrals, 4 pivoted MGS, 5 bases,
+   // PV and UTI, 6 lists, FP32 copies and apply plan
+   double phase_ms[8] = {0};
 };
 
 // Forward-apply plan of a tree: every level's blocks concatenated, with
@@ -90,6 +94,8 @@ struct ApplyPlan {
    DBuf<SrcPlan> src;
    DBuf<TgtPlan> tgt;
    DBuf<int> isrc, itoff;
+   DBuf<int> map_tgt, map_leaf;   // XCD block maps
+   int nmap_tgt = 0, nmap_leaf = 0;
    DBuf<LeafU> ublk;
    DBuf<double> tpart, spec, acc, w;   // work vectors, sized for FP64
    DBuf<double> VTc64, PV64, Tsp64, UTI64, UT64;
@@ -123,9 +129,12 @@ struct TreeOp {
 
 // Builds tree, near field and far field for `npan` panels (caller-order
 // geometry already on the device in `d_panels`) and `npt` points.
+// `pre`, if given, is the octree of exactly these panels and points (it
+// depends on neither the kernel nor kappa) and is copied instead of rebuilt.
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
-                                      int gridpoints, double svdeps, bool want32, hipStream_t s);
+                                      int gridpoints, double svdeps, bool want32, hipStream_t s,
+                                      const Tree* pre = nullptr);
 
 // y_t = S xs_t + D xd_t; all vectors in tree order, device pointers.
 void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s);
@@ -135,6 +144,11 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
 void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s);
 // y_t = A_layer^T x_t (panels, tree order).
 void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s);
+
+// Block map of a launch over n logical blocks with the given work: block b
+// of the returned grid runs logical block map[b] (-1: idle); XCD x (blocks
+// b = x mod 8) gets a contiguous range of about 1/8 of the total work.
+std::vector<int> xcd_block_map(const std::vector<double>& work);
 
 // launchers (apply.hip / fill.hip)
 template <typename T>

@@ -5,7 +5,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <functional>
+#include <thread>
 #include <stdexcept>
 
 namespace fsvd {
@@ -44,6 +46,11 @@ Problem::~Problem() {
    regions.clear();
    basis.clear();
    if (h_pinned) (void)hipHostFree(h_pinned);
+   for (int k = 0; k < MAX_AUX; k++) {
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+      if (aux[k]) (void)hipStreamDestroy(aux[k]);
+   }
+   if (ev_fork) (void)hipEventDestroy(ev_fork);
    if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -142,27 +149,87 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
       walk(0);
       if (index != 2 * pr.ntotal) throw std::runtime_error("SurfaceOperator indexing is WRONG!");
    }
-   // operators (share identical trees: same kernel, same panels)
-   for (size_t r = 0; r < pr.regions.size(); r++) {
-      Region& R = pr.regions[r];
-      for (size_t o = 0; o < r; o++) {
-         const Region& O = pr.regions[o];
-         if (O.shares_tree_with < 0 && O.kernel == R.kernel && O.kappa == R.kappa && O.panel_ids == R.panel_ids) {
-            R.shares_tree_with = (int)o;
-            break;
+   // operators (share identical trees: same kernel, same panels).  The
+   // octree depends only on the panels, so it is built once per distinct
+   // panel set; the distinct operators are then built concurrently, one host
+   // thread and one stream each.
+   {
+      std::vector<int> todo;
+      for (size_t r = 0; r < pr.regions.size(); r++) {
+         Region& R = pr.regions[r];
+         for (size_t o = 0; o < r; o++) {
+            const Region& O = pr.regions[o];
+            if (O.shares_tree_with < 0 && O.kernel == R.kernel && O.kappa == R.kappa && O.panel_ids == R.panel_ids) {
+               R.shares_tree_with = (int)o;
+               break;
+            }
          }
+         if (R.shares_tree_with < 0) todo.push_back((int)r);
       }
-      std::vector<PanelGeom> tp(R.panel_ids.size());
-      std::vector<double> cent(3 * R.panel_ids.size());
-      for (size_t k = 0; k < R.panel_ids.size(); k++) {
-         tp[k] = pr.panels[R.panel_ids[k]];
-         for (int d = 0; d < 3; d++) cent[3 * k + d] = tp[k].c[d];
+      const size_t nt = todo.size();
+      std::vector<std::vector<PanelGeom>> tp(nt);
+      std::vector<std::vector<double>> cent(nt);
+      std::vector<int> tree_src(nt, -1);
+      std::vector<Tree> trees(nt);
+      for (size_t k = 0; k < nt; k++) {
+         Region& R = pr.regions[todo[k]];
+         tp[k].resize(R.panel_ids.size());
+         cent[k].resize(3 * R.panel_ids.size());
+         for (size_t i = 0; i < R.panel_ids.size(); i++) {
+            tp[k][i] = pr.panels[R.panel_ids[i]];
+            for (int d = 0; d < 3; d++) cent[k][3 * i + d] = tp[k][i].c[d];
+         }
+         R.d_panels.upload(tp[k], s);
+         for (size_t o = 0; o < k; o++)
+            if (tree_src[o] < 0 && pr.regions[todo[o]].panel_ids == R.panel_ids) {
+               tree_src[k] = (int)o;
+               break;
+            }
+         if (tree_src[k] < 0)
+            build_tree(trees[k], tp[k].data(), (int)tp[k].size(), cent[k].data(), (int)tp[k].size(),
+                       spec.maxpanels, spec.gridpoints);
       }
-      if (R.shares_tree_with < 0) {
-         R.d_panels.upload(tp, s);
-         R.op = build_tree_op(tp, R.d_panels.p, cent.data(), (int)R.panel_ids.size(), R.kernel, R.kappa,
-                              spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32, s);
-         if (R.op->tree.pt_perm != R.op->tree.pan_perm) throw std::runtime_error("point/panel partitions differ");
+      FSVD_CHECK(hipStreamSynchronize(s));
+      std::vector<hipStream_t> streams(nt);
+      for (size_t k = 0; k < nt; k++) FSVD_CHECK(hipStreamCreate(&streams[k]));
+      std::vector<std::exception_ptr> err(nt);
+      std::vector<std::thread> th;
+      for (size_t k = 0; k < nt; k++)
+         th.emplace_back([&, k]() {
+            try {
+               Region& R = pr.regions[todo[k]];
+               const Tree& pre = trees[tree_src[k] < 0 ? k : (size_t)tree_src[k]];
+               R.op = build_tree_op(tp[k], R.d_panels.p, cent[k].data(), (int)tp[k].size(), R.kernel, R.kappa,
+                                    spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32, streams[k], &pre);
+               FSVD_CHECK(hipStreamSynchronize(streams[k]));
+            } catch (...) {
+               err[k] = std::current_exception();
+            }
+         });
+      for (auto& x : th) x.join();
+      for (size_t k = 0; k < nt; k++) (void)hipStreamDestroy(streams[k]);
+      for (size_t k = 0; k < nt; k++)
+         if (err[k]) std::rethrow_exception(err[k]);
+      for (int r : todo)
+         if (pr.regions[r].op->tree.pt_perm != pr.regions[r].op->tree.pan_perm)
+            throw std::runtime_error("point/panel partitions differ");
+   }
+   // matvec streams: regions that share an operator share its work buffers,
+   // so they run on the same stream; distinct operators round-robin
+   {
+      std::vector<int> op_lane(pr.regions.size(), -1);
+      int next = 0;
+      for (size_t r = 0; r < pr.regions.size(); r++) {
+         Region& R = pr.regions[r];
+         const int owner = R.shares_tree_with >= 0 ? R.shares_tree_with : (int)r;
+         if (op_lane[owner] < 0) op_lane[owner] = (next++) % (Problem::MAX_AUX + 1);
+         R.lane = op_lane[owner];
+         pr.naux = std::max(pr.naux, R.lane);
+      }
+      FSVD_CHECK(hipEventCreateWithFlags(&pr.ev_fork, hipEventDisableTiming));
+      for (int k = 0; k < pr.naux; k++) {
+         FSVD_CHECK(hipStreamCreate(&pr.aux[k]));
+         FSVD_CHECK(hipEventCreateWithFlags(&pr.ev_join[k], hipEventDisableTiming));
       }
    }
    // join / split maps in tree order
@@ -269,15 +336,27 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    return prp;
 }
 
+// Every region writes a disjoint set of rows of y (the phi rows of its own
+// panels and the dphi rows of its children's panels), so regions on
+// different streams never touch the same output.
 void problem_matvec(Problem& pr, int prec, const double* x, double* y) {
    hipStream_t s = pr.stream;
+   if (pr.naux > 0) {
+      FSVD_CHECK(hipEventRecord(pr.ev_fork, s));
+      for (int k = 0; k < pr.naux; k++) FSVD_CHECK(hipStreamWaitEvent(pr.aux[k], pr.ev_fork, 0));
+   }
    for (Region& R : pr.regions) {
       int n = (int)R.panel_ids.size();
       if (n == 0) continue;
+      hipStream_t rs = R.lane == 0 ? s : pr.aux[R.lane - 1];
       TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
-      launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x, R.xs.p, R.xd.p, s);
-      tree_apply(op, prec, R.xs.p, R.xd.p, R.yt.p, s);
-      launch_split(n, R.out_idx.p, R.corr.p, R.yt.p, R.xd.p, y, s);
+      launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x, R.xs.p, R.xd.p, rs);
+      tree_apply(op, prec, R.xs.p, R.xd.p, R.yt.p, rs);
+      launch_split(n, R.out_idx.p, R.corr.p, R.yt.p, R.xd.p, y, rs);
+   }
+   for (int k = 0; k < pr.naux; k++) {
+      FSVD_CHECK(hipEventRecord(pr.ev_join[k], pr.aux[k]));
+      FSVD_CHECK(hipStreamWaitEvent(s, pr.ev_join[k], 0));
    }
 }
 

@@ -39,6 +39,7 @@ struct Region {
    DBuf<double> xs, xd, yt;
    bool has_charges = false;
    int shares_tree_with = -1;        // reuse another region's operator (same panels and kernel)
+   int lane = 0;                     // matvec stream: 0 = main, k > 0 = aux[k-1]
 };
 
 struct SolveInfo {
@@ -51,6 +52,12 @@ struct SolveInfo {
 
 struct Problem {
    hipStream_t stream = nullptr;
+   // the distinct tree operators run concurrently: operator k of the matvec
+   // on stream 0 (k = 0) or aux[k-1]; forked from / joined into `stream`
+   static constexpr int MAX_AUX = 3;
+   hipStream_t aux[MAX_AUX] = {nullptr, nullptr, nullptr};
+   hipEvent_t ev_fork = nullptr, ev_join[MAX_AUX] = {nullptr, nullptr, nullptr};
+   int naux = 0;
    int ntotal = 0;                       // total panels
    std::vector<PanelGeom> panels;        // global panel geometry
    std::vector<Region> regions;          // regions[0] = exterior (root)

@@ -23,8 +23,6 @@ struct Builder {
 
    explicit Builder(Tree& tt, const PanelGeom* p, const double* q) : t(tt), P(p), pts(q) {}
 
-   static bool side(double v, double c, int half) { return half == 0 ? (v < c) : (v >= c); }
-
    int build(std::vector<int>&& mypan, std::vector<int>&& mypt, int level, const int idx[3], const double b0[3],
              const double b1[3], int parent) {
       int id = (int)t.cubes.size();
@@ -47,18 +45,30 @@ struct Builder {
       }
       if (t.cubes[id].leaf) return id;
       double ctr[3] = {t.cubes[id].ctr[0], t.cubes[id].ctr[1], t.cubes[id].ctr[2]};
+      // child k = 4*cx + 2*cy + cz with c* = 0 for v < ctr, 1 for v >= ctr
+      // (a coordinate that compares neither way joins no child)
+      std::vector<int> bp[8], bq[8];
+      auto child_of = [&](const double* g) {
+         int k = 0;
+         for (int d = 0; d < 3; d++) {
+            if (g[d] >= ctr[d]) k |= 4 >> d;
+            else if (!(g[d] < ctr[d])) return -1;
+         }
+         return k;
+      };
+      for (int i : pan[id]) {
+         int k = child_of(P[i].c);
+         if (k >= 0) bp[k].push_back(i);
+      }
+      for (int i : pt[id]) {
+         int k = child_of(pts + 3 * (size_t)i);
+         if (k >= 0) bq[k].push_back(i);
+      }
       for (int cx = 0; cx <= 1; cx++)
          for (int cy = 0; cy <= 1; cy++)
             for (int cz = 0; cz <= 1; cz++) {
-               std::vector<int> cp, cq;
-               for (int i : pan[id]) {
-                  const double* g = P[i].c;
-                  if (side(g[0], ctr[0], cx) && side(g[1], ctr[1], cy) && side(g[2], ctr[2], cz)) cp.push_back(i);
-               }
-               for (int i : pt[id]) {
-                  const double* g = pts + 3 * (size_t)i;
-                  if (side(g[0], ctr[0], cx) && side(g[1], ctr[1], cy) && side(g[2], ctr[2], cz)) cq.push_back(i);
-               }
+               std::vector<int>& cp = bp[cx * 4 + cy * 2 + cz];
+               std::vector<int>& cq = bq[cx * 4 + cy * 2 + cz];
                if (cp.empty() && cq.empty()) continue;
                double cb0[3] = {b0[0], b0[1], b0[2]}, cb1[3] = {ctr[0], ctr[1], ctr[2]};
                int sel[3] = {cx, cy, cz};

@@ -55,13 +55,16 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
                          int* iters, double* resid_hist, double* info);
 /* reaction potentials at the charges (SurfaceOperator.c:323-401 direct branch) */
 int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi);
-/* Operator statistics, up to 24 doubles: 0 panels, 1 regions, 2 setup ms,
+/* Operator statistics, up to 32 doubles: 0 panels, 1 regions, 2 setup ms,
  * 3 near entries, 4 far entries, 5 spectrum bins, 6 translation MACs,
  * 7 depth, 8 leaves, 9 tree applications per matvec, 10/11 algorithmic
  * bytes per matvec (FP64 / FP32 storage), 12/13 near-field bytes, 14 near
  * column indices, 15 U entries, 16 VT entries, 17 PV entries, 18 UTI
  * entries, 19 T spectrum bins, 20 leaf far-field blocks, 21 projection
- * chunks, 22 points applied (all counts summed over tree applications). */
+ * chunks, 22 points applied, 23 source spectrum bins, 24 target spectrum
+ * bins (all counts summed over tree applications), 25..31 host wall ms of
+ * the setup phases summed over built trees: tree, near field, pinv + T
+ * spectra, sampling integrals, pivoted MGS, bases/PV/UTI, lists + plan. */
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout);
 /* Times `nrep` surface matvecs on device-resident vectors with HIP events
  * on the library stream: total ms, and in ms_parts[0..3] the per-stage ms

@@ -1,5 +1,5 @@
 """Pins the oracle's panel integrals to mathematics the reference ships no
-golden vectors for (the reference has no tests):
+golden vectors for (the reference ships no tests):
 
 * Laplace single/double layer (FlatIntegration.c:295-429) vs an independent
   Duffy-transformed brute-force quadrature, and the double layer vs the
