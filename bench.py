@@ -164,6 +164,9 @@ def main():
     roofline = dict(rf[dom])
     roofline.update({"stage": dom, "traffic": None})
 
+    # measured copy bandwidth of this GPU (1 GiB copied 20 times)
+    copy_gbs = fa.copy_bandwidth(1 << 30, 20)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -202,6 +205,7 @@ def main():
         "operator": st,
         "roofline": roofline,
         "roofline_stages": rf,
+        "hbm_copy_GBs": copy_gbs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -38,6 +38,7 @@ _SIGS = {
     "fftsvd_problem_solve": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _dp]),
     "fftsvd_problem_potentials": (C.c_int, [_vp, _dp, _dp]),
     "fftsvd_problem_stats": (C.c_int, [_vp, _dp, C.c_int]),
+    "fftsvd_copy_bandwidth": (C.c_int, [C.c_longlong, C.c_int, _dp]),
     "fftsvd_problem_bench_matvec": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "fftsvd_tree_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
     "fftsvd_tree_cube_info": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
@@ -92,6 +93,13 @@ def device_info():
     cu = C.c_int()
     _check(lib().fftsvd_device_info(name, 256, C.byref(cu)))
     return name.value.decode(), cu.value
+
+
+def copy_bandwidth(nbytes=1 << 30, nrep=20):
+    """Device-to-device copy rate in GB/s (bytes read + written per second)."""
+    g = C.c_double()
+    _check(lib().fftsvd_copy_bandwidth(int(nbytes), int(nrep), C.byref(g)))
+    return g.value
 
 
 def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):

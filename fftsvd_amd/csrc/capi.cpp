@@ -42,6 +42,10 @@ static int g_precision = 0;
    }                                              \
    return 0;
 
+namespace fsvd {
+void launch_copy(long long n4, const void* a, void* b, hipStream_t s);
+}
+
 extern "C" {
 
 /* Parameters (reference FFTSVDpbeAPI.c:18-34) */
@@ -72,6 +76,35 @@ int fftsvd_set_precision(int precision) {
    }
    g_precision = precision;
    return 0;
+}
+
+int fftsvd_copy_bandwidth(long long bytes, int nrep, double* gbs) {
+   API_TRY
+   if (bytes < 32 || nrep < 1) throw std::runtime_error("copy probe: bytes >= 32 and nrep >= 1 required");
+   const long long n4 = bytes / 32;
+   DBuf<double> a, b;
+   a.alloc(4 * (size_t)n4);
+   b.alloc(4 * (size_t)n4);
+   hipStream_t s;
+   FSVD_CHECK(hipStreamCreate(&s));
+   FSVD_CHECK(hipMemsetAsync(a.p, 0, 32 * (size_t)n4, s));
+   launch_copy(n4, a.p, b.p, s);
+   hipEvent_t e0, e1;
+   FSVD_CHECK(hipEventCreate(&e0));
+   FSVD_CHECK(hipEventCreate(&e1));
+   FSVD_CHECK(hipEventRecord(e0, s));
+   for (int r = 0; r < nrep; r++) launch_copy(n4, a.p, b.p, s);
+   FSVD_CHECK(hipEventRecord(e1, s));
+   FSVD_CHECK(hipEventSynchronize(e1));
+   float ms = 0;
+   FSVD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   FSVD_CHECK(hipStreamSynchronize(s));
+   (void)hipStreamDestroy(s);
+   // bytes read + bytes written per copy
+   if (gbs) *gbs = 2.0 * 32.0 * (double)n4 * nrep / (ms * 1e-3) / 1e9;
+   API_CATCH
 }
 
 int fftsvd_device_info(char* name, int len, int* num_cu) {

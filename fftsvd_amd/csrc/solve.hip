@@ -74,30 +74,6 @@ __device__ __forceinline__ double sum_partials(const double* partial, int np) {
    return h;
 }
 
-// One modified Gram-Schmidt step fused with the next reduction:
-//   h = sum(pin); hout[0] = h; w -= h*v; pout = partials of <w, vnext>
-// (vnext == nullptr: partials of <w, w>).
-__global__ void __launch_bounds__(256) k_mgs_step(long long n, const double* __restrict__ pin, int np,
-                                                  double* __restrict__ hout, double* __restrict__ w,
-                                                  const double* __restrict__ v, const double* __restrict__ vnext,
-                                                  double* __restrict__ pout) {
-   __shared__ double red[8];
-   __shared__ double hs;
-   if (threadIdx.x == 0) hs = sum_partials(pin, np);
-   __syncthreads();
-   const double h = hs;
-   if (blockIdx.x == 0 && threadIdx.x == 0) hout[0] = h;
-   double s = 0.0;
-   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-      double wi = w[i] - h * v[i];
-      w[i] = wi;
-      double o = vnext ? vnext[i] : wi;
-      s += wi * o;
-   }
-   s = block_sum(s, red);
-   if (threadIdx.x == 0) pout[blockIdx.x] = s;
-}
-
 // v = w / sqrt(sum(pin)); hout[0] = the norm.
 __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __restrict__ pin, int np,
                                                    double* __restrict__ hout, const double* __restrict__ w,
@@ -110,6 +86,63 @@ __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __
    const double inv = 1.0 / h;
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
       v[i] = w[i] * inv;
+}
+
+// Classical Gram-Schmidt pass over m basis vectors (two passes make CGS2).
+// Block b owns the contiguous chunk b of the vectors; partial[j*np + b] =
+// chunk sum of V_j . w, four columns per sweep so w is read once per four.
+__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int np, const double* const* __restrict__ V,
+                                                  const double* __restrict__ w, double* __restrict__ partial) {
+   __shared__ double red[8];
+   const long long chunk = (n + np - 1) / np;
+   const long long i0 = blockIdx.x * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+   for (int j0 = 0; j0 < m; j0 += 4) {
+      const int nj = m - j0 < 4 ? m - j0 : 4;
+      const double* v0 = V[j0];
+      const double* v1 = nj > 1 ? V[j0 + 1] : v0;
+      const double* v2 = nj > 2 ? V[j0 + 2] : v0;
+      const double* v3 = nj > 3 ? V[j0 + 3] : v0;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+         const double wi = w[i];
+         s0 += v0[i] * wi;
+         s1 += v1[i] * wi;
+         s2 += v2[i] * wi;
+         s3 += v3[i] * wi;
+      }
+      s0 = block_sum(s0, red);
+      s1 = block_sum(s1, red);
+      s2 = block_sum(s2, red);
+      s3 = block_sum(s3, red);
+      if (threadIdx.x == 0) {
+         partial[(long long)j0 * np + blockIdx.x] = s0;
+         if (nj > 1) partial[(long long)(j0 + 1) * np + blockIdx.x] = s1;
+         if (nj > 2) partial[(long long)(j0 + 2) * np + blockIdx.x] = s2;
+         if (nj > 3) partial[(long long)(j0 + 3) * np + blockIdx.x] = s3;
+      }
+   }
+}
+
+// h[j] = sum of column j's np partials in block order; hacc[j] = h[j]
+// (first pass) or hacc[j] + h[j] (second pass).
+__global__ void k_multidot_finish(int m, int np, const double* __restrict__ partial, double* __restrict__ h,
+                                  double* __restrict__ hacc, int add) {
+   const int j = blockIdx.x * blockDim.x + threadIdx.x;
+   if (j >= m) return;
+   double s = 0.0;
+   for (int k = 0; k < np; k++) s += partial[(long long)j * np + k];
+   h[j] = s;
+   hacc[j] = add ? hacc[j] + s : s;
+}
+
+// w -= sum_j h[j] V_j, summed over j in order.
+__global__ void k_multiaxpy(long long n, int m, const double* __restrict__ h, const double* const* __restrict__ V,
+                            double* __restrict__ w) {
+   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += h[j] * V[j][i];
+      w[i] -= s;
+   }
 }
 
 // x = sum_j y[j] * V[j]  (basis vectors at stride ld), in order of j.
@@ -196,6 +229,12 @@ __global__ void k_integrate_all(int kernel, double kappa, const PanelGeom* __res
    dlp[e] = d;
 }
 
+// Plain copy (bandwidth probe and counter calibration).
+__global__ void k_copy(long long n4, const double4* __restrict__ a, double4* __restrict__ b) {
+   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+      b[i] = a[i];
+}
+
 // ---------------------------------------------------------------------------
 static inline int nblocks(long long n, int per = 256, int cap = 65535) {
    long long b = (n + per - 1) / per;
@@ -218,9 +257,19 @@ void launch_precond(int nb, const int* ia, const int* ib, const double* M, const
 void launch_dot_partial(long long n, int np, const double* a, const double* b, double* partial, hipStream_t s) {
    hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(256), 0, s, n, a, b, partial);
 }
-void launch_mgs_step(long long n, int np, const double* pin, double* hout, double* w, const double* v,
-                     const double* vnext, double* pout, hipStream_t s) {
-   hipLaunchKernelGGL(k_mgs_step, dim3(np), dim3(256), 0, s, n, pin, np, hout, w, v, vnext, pout);
+void launch_copy(long long n4, const void* a, void* b, hipStream_t s) {
+   if (n4 > 0)
+      hipLaunchKernelGGL(k_copy, dim3(nblocks(n4, 256, 8192)), dim3(256), 0, s, n4, (const double4*)a, (double4*)b);
+}
+void launch_multidot(long long n, int m, int np, const double* const* V, const double* w, double* partial,
+                     hipStream_t s) {
+   if (m > 0) hipLaunchKernelGGL(k_multidot, dim3(np), dim3(256), 0, s, n, m, np, V, w, partial);
+}
+void launch_multidot_finish(int m, int np, const double* partial, double* h, double* hacc, int add, hipStream_t s) {
+   if (m > 0) hipLaunchKernelGGL(k_multidot_finish, dim3((m + 63) / 64), dim3(64), 0, s, m, np, partial, h, hacc, add);
+}
+void launch_multiaxpy(long long n, int m, const double* h, const double* const* V, double* w, hipStream_t s) {
+   if (m > 0) hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, h, V, w);
 }
 void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, double* v,
                       hipStream_t s) {

@@ -18,11 +18,13 @@ void launch_split(int n, const int* oi, const double* corr, const double* yt, co
                   hipStream_t s);
 void launch_precond(int nb, const int* ia, const int* ib, const double* M, const double* b, double* x, hipStream_t s);
 void launch_dot_partial(long long n, int np, const double* a, const double* b, double* partial, hipStream_t s);
-void launch_mgs_step(long long n, int np, const double* pin, double* hout, double* w, const double* v,
-                     const double* vnext, double* pout, hipStream_t s);
 void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, double* v,
                       hipStream_t s);
 void launch_combine(long long n, int m, const double* y, const double* const* V, double* x, hipStream_t s);
+void launch_multidot(long long n, int m, int np, const double* const* V, const double* w, double* partial,
+                     hipStream_t s);
+void launch_multidot_finish(int m, int np, const double* partial, double* h, double* hacc, int add, hipStream_t s);
+void launch_multiaxpy(long long n, int m, const double* h, const double* const* V, double* w, hipStream_t s);
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
 void launch_rhs(int ne, const double* cent, const int* out, int nq, const double* qpos, const double* q, double* rhs,
                 hipStream_t s);
@@ -417,6 +419,19 @@ static double* basis_vec(Problem& pr, int j) {
    return pr.basis[j]->p;
 }
 
+// Device array of the basis pointers v_0..v_{m-1} (grown in steps of 64).
+static const double* const* basis_ptrs(Problem& pr, int m) {
+   if ((int)pr.basis_ptr_n < m) {
+      const int cap = ((m + 63) / 64) * 64;
+      std::vector<const double*> p(cap);
+      for (int j = 0; j < cap; j++) p[j] = basis_vec(pr, j);
+      pr.d_basis_ptr.upload(p, pr.stream);
+      FSVD_CHECK(hipStreamSynchronize(pr.stream));
+      pr.basis_ptr_n = cap;
+   }
+   return pr.d_basis_ptr.p;
+}
+
 SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, double tol, int maxit,
                         std::vector<double>* hist) {
    double t0 = now_ms();
@@ -427,7 +442,7 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
    double* r = pr.r.p;
    double* w = pr.w.p;
    double* P0 = pr.partial.p;
-   double* P1 = pr.partial.p + 512;
+   if (maxit < 1 || maxit > 2040) throw std::runtime_error("GMRES: maxit must be in 1..2040");
    problem_precond(pr, rhs, r);
    launch_dot_partial(n, np, r, r, P0, s);
    FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, P0, sizeof(double) * np, hipMemcpyDeviceToHost, s));
@@ -443,6 +458,9 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
    }
    const int W = maxit + 1;
    std::vector<double> H((size_t)W * W, 0.0), c(W, 0.0), sn(W, 0.0), g(W + 1, 0.0), y(W, 0.0);
+   if ((long long)pr.partial_m.n < (long long)(maxit + 1) * np) pr.partial_m.alloc((size_t)(maxit + 1) * np);
+   if ((long long)pr.htmp.n < maxit + 2) pr.htmp.alloc(maxit + 2);
+   if ((long long)pr.hbuf.n < maxit + 2) pr.hbuf.alloc(maxit + 2);
    g[0] = normr;
    launch_normalize(n, np, P0, pr.hbuf.p, r, basis_vec(pr, 0), s);
    int i;
@@ -455,15 +473,19 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
       else
          info.matvecs64++;
       problem_precond(pr, w, w);
-      launch_dot_partial(n, np, w, basis_vec(pr, 0), P0, s);
-      for (int j = 0; j <= i; j++) {
-         double* pin = (j % 2 == 0) ? P0 : P1;
-         double* pout = (j % 2 == 0) ? P1 : P0;
-         const double* vn = (j < i) ? basis_vec(pr, j + 1) : nullptr;
-         launch_mgs_step(n, np, pin, pr.hbuf.p + j, w, basis_vec(pr, j), vn, pout, s);
+      // Arnoldi column by classical Gram-Schmidt with one re-orthogonalisation
+      // (CGS2): two passes of h = V^T w, w -= V h over v_0..v_i.  The
+      // reference orthogonalises by modified Gram-Schmidt (GMRES.c:470-490);
+      // both give the same H up to rounding, CGS2 in 8 launches per
+      // iteration instead of i+3.
+      const double* const* V = basis_ptrs(pr, i + 2);
+      for (int pass = 0; pass < 2; pass++) {
+         launch_multidot(n, i + 1, np, V, w, pr.partial_m.p, s);
+         launch_multidot_finish(i + 1, np, pr.partial_m.p, pr.htmp.p, pr.hbuf.p, pass, s);
+         launch_multiaxpy(n, i + 1, pr.htmp.p, V, w, s);
       }
-      double* plast = ((i + 1) % 2 == 0) ? P0 : P1;
-      launch_normalize(n, np, plast, pr.hbuf.p + i + 1, w, vnext_store, s);
+      launch_dot_partial(n, np, w, w, P0, s);
+      launch_normalize(n, np, P0, pr.hbuf.p + i + 1, w, vnext_store, s);
       FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, pr.hbuf.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
       FSVD_CHECK(hipStreamSynchronize(s));
       double* Hi = &H[(size_t)i * W];

@@ -70,8 +70,10 @@ struct Problem {
    // device charges
    DBuf<double> d_qpos, d_q;
    // scratch
-   DBuf<double> w, r, tmp, partial, hbuf;
+   DBuf<double> w, r, tmp, partial, hbuf, htmp, partial_m;
    std::vector<std::unique_ptr<DBuf<double>>> basis;
+   DBuf<const double*> d_basis_ptr;     // device copy of the basis pointers
+   int basis_ptr_n = 0;
    double* h_pinned = nullptr;
    double setup_ms = 0.0;
    ~Problem();

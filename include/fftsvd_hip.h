@@ -22,6 +22,10 @@ typedef struct fftsvd_problem fftsvd_problem;
 
 const char* fftsvd_last_error(void);
 int fftsvd_device_info(char* name, int len, int* num_cu);
+/* Device-to-device copy of `bytes` bytes repeated nrep times on a private
+ * stream; gbs = (read + written bytes) / time.  A bandwidth probe and the
+ * reference kernel for calibrating memory counters. */
+int fftsvd_copy_bandwidth(long long bytes, int nrep, double* gbs);
 
 /* One dielectric surface (panels as v0,v1,v2 triples, the vertex order the
  * reference builds from a flipped MSMS face), all charges inside; salt if
