@@ -64,7 +64,7 @@ def cpu_baseline(seed, budget_s=25.0):
 
 
 STAGE_KERNELS = {"leaf": "k_leaf<float>", "project_source": "k_project<float> + k_source<float>",
-                 "translate": "k_translate_group<float, 32>", "target": "k_target<float>"}
+                 "translate": "k_translate<float, 8>", "target": "k_target<float>"}
 
 
 def stage_bytes(st):

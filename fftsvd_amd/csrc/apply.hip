@@ -191,7 +191,8 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
 //   k_project    chunk of a source cube's panels: partial VT.xs, VT.xd
 //   k_source     source cube: sum of chunk partials, PV interpolation onto
 //                the n^3 grid, zero-padded r2c DFT -> spectrum
-//   k_translate  target cube: sum over the interaction list of T . spectrum
+//   k_translate  tile of <= 32 target cubes x a bin chunk: sum over each
+//                interaction list of T . spectrum, from LDS
 //   k_target     target cube: cropped c2r DFT, w = UTI h
 //   k_leaf       leaf: near-field block apply plus U w of every target cube
 //                containing the leaf, writes y
@@ -319,53 +320,99 @@ __global__ void __launch_bounds__(256) k_source(const SrcPlan* __restrict__ src,
    }
 }
 
-// Translation: block per target cube, threads over frequency bins; blocks
-// follow a host-built map that gives each XCD a contiguous tree-order range
-// of about equal work.  The interaction indices are uniform over the block
-// (scalar loads); four interactions are in flight per iteration.  Summation
-// order: two interleaved partial sums over the interaction list, added at
-// the end.
-template <typename T>
-__global__ void __launch_bounds__(256) k_translate(const TgtPlan* __restrict__ tgt, const int* __restrict__ bmap,
-                                                   const int* __restrict__ isrc, const int* __restrict__ itoff,
-                                                   const typename Cplx<T>::type* __restrict__ Tspec,
-                                                   const typename Cplx<T>::type* __restrict__ spec,
-                                                   typename Cplx<T>::type* __restrict__ acc) {
+// Translation by tiles: a block owns BC frequency bins of a run of tiles
+// (each up to TR_TILE neighbouring target cubes of one level).  It stages
+// the 343 x BC T spectra of those bins in LDS once, then per tile the BC
+// bins of every source spectrum the tile's targets interact with, and
+// thread (target j, bin b) sums its interaction list from LDS.  Per target
+// the sum runs in interaction-list order.
+template <typename T, int BC>
+__global__ void __launch_bounds__(TR_TILE * BC) k_translate(const int4* __restrict__ tasks,
+                                                            const TrTile* __restrict__ tiles,
+                                                            const GTgt* __restrict__ gtgt,
+                                                            const int* __restrict__ gsrc, const int* __restrict__ gint,
+                                                            const typename Cplx<T>::type* __restrict__ Tspec,
+                                                            const typename Cplx<T>::type* __restrict__ spec,
+                                                            typename Cplx<T>::type* __restrict__ acc) {
    using C2 = typename Cplx<T>::type;
-   const int ti = bmap[blockIdx.x];
-   if (ti < 0) return;
-   const TgtPlan d = tgt[ti];
-   const int pad = d.pad;
-   const int* is = isrc + d.int_off;
-   const int* io = itoff + d.int_off;
-   for (int b = threadIdx.x; b < pad; b += blockDim.x) {
-      const C2* Tb = Tspec + d.t_off + b;
-      const C2* Sb = spec + b;
-      T re0 = 0, im0 = 0, re1 = 0, im1 = 0;
-      int e = 0;
-      for (; e + 3 < d.nint; e += 4) {
-         const C2 t0 = Tb[(long long)io[e] * pad], s0 = Sb[is[e]];
-         const C2 t1 = Tb[(long long)io[e + 1] * pad], s1 = Sb[is[e + 1]];
-         const C2 t2 = Tb[(long long)io[e + 2] * pad], s2 = Sb[is[e + 2]];
-         const C2 t3 = Tb[(long long)io[e + 3] * pad], s3 = Sb[is[e + 3]];
-         re0 += t0.x * s0.x - t0.y * s0.y;
-         im0 += t0.x * s0.y + t0.y * s0.x;
-         re1 += t1.x * s1.x - t1.y * s1.y;
-         im1 += t1.x * s1.y + t1.y * s1.x;
-         re0 += t2.x * s2.x - t2.y * s2.y;
-         im0 += t2.x * s2.y + t2.y * s2.x;
-         re1 += t3.x * s3.x - t3.y * s3.y;
-         im1 += t3.x * s3.y + t3.y * s3.x;
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int4 tk = tasks[blockIdx.x];   // first tile, tile count, first bin
+   const int b0 = tk.z, tid = threadIdx.x, nthr = TR_TILE * BC;
+   const int pad = tiles[tk.x].pad;
+   const long long t_off = tiles[tk.x].t_off;
+   C2* Ts = (C2*)smem;        // 343 * BC
+   C2* Ss = Ts + 343 * BC;    // sources * BC
+   // staging: eight loads in flight per thread before their LDS stores
+   constexpr int U = 8;
+   for (int i0 = tid; i0 < 343 * BC; i0 += U * nthr) {
+      C2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+         const int i = i0 + u * nthr, c = i / BC, b = i - c * BC;
+         v[u].x = 0;
+         v[u].y = 0;
+         if (i < 343 * BC && b0 + b < pad) v[u] = Tspec[t_off + (long long)c * pad + b0 + b];
       }
-      for (; e < d.nint; e++) {
-         const C2 t0 = Tb[(long long)io[e] * pad], s0 = Sb[is[e]];
-         re0 += t0.x * s0.x - t0.y * s0.y;
-         im0 += t0.x * s0.y + t0.y * s0.x;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+         if (i0 + u * nthr < 343 * BC) Ts[i0 + u * nthr] = v[u];
+   }
+   const int j = tid / BC, b = tid - j * BC;
+   for (int ti = tk.x; ti < tk.x + tk.y; ti++) {
+      const TrTile tl = tiles[ti];
+      __syncthreads();   // the previous tile's sums are done with Ss
+      const int ns = tl.nsrc * BC;
+      for (int i0 = tid; i0 < ns; i0 += U * nthr) {
+         int so[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            const int i = i0 + u * nthr;
+            so[u] = i < ns ? gsrc[tl.src_off + i / BC] : 0;
+         }
+         C2 v[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            const int i = i0 + u * nthr, bb = i - (i / BC) * BC;
+            v[u].x = 0;
+            v[u].y = 0;
+            if (i < ns && b0 + bb < pad) v[u] = spec[(long long)so[u] + b0 + bb];
+         }
+#pragma unroll
+         for (int u = 0; u < U; u++)
+            if (i0 + u * nthr < ns) Ss[i0 + u * nthr] = v[u];
       }
-      C2 out;
-      out.x = re0 + re1;
-      out.y = im0 + im1;
-      acc[d.acc_off + b] = out;
+      __syncthreads();
+      if (j < tl.nt && b0 + b < pad) {
+         const GTgt tt = gtgt[tl.tgt0 + j];
+         const int* it = gint + tt.int_off;
+         T re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+         int e = 0;
+         for (; e + 3 < tt.nint; e += 4) {
+            const int v0 = it[e], v1 = it[e + 1], v2 = it[e + 2], v3 = it[e + 3];
+            const C2 t0 = Ts[(v0 & 511) * BC + b], s0 = Ss[(v0 >> 9) * BC + b];
+            const C2 t1 = Ts[(v1 & 511) * BC + b], s1 = Ss[(v1 >> 9) * BC + b];
+            const C2 t2 = Ts[(v2 & 511) * BC + b], s2 = Ss[(v2 >> 9) * BC + b];
+            const C2 t3 = Ts[(v3 & 511) * BC + b], s3 = Ss[(v3 >> 9) * BC + b];
+            re0 += t0.x * s0.x - t0.y * s0.y;
+            im0 += t0.x * s0.y + t0.y * s0.x;
+            re1 += t1.x * s1.x - t1.y * s1.y;
+            im1 += t1.x * s1.y + t1.y * s1.x;
+            re0 += t2.x * s2.x - t2.y * s2.y;
+            im0 += t2.x * s2.y + t2.y * s2.x;
+            re1 += t3.x * s3.x - t3.y * s3.y;
+            im1 += t3.x * s3.y + t3.y * s3.x;
+         }
+         for (; e < tt.nint; e++) {
+            const int v0 = it[e];
+            const C2 t0 = Ts[(v0 & 511) * BC + b], s0 = Ss[(v0 >> 9) * BC + b];
+            re0 += t0.x * s0.x - t0.y * s0.y;
+            im0 += t0.x * s0.y + t0.y * s0.x;
+         }
+         C2 out;
+         out.x = re0 + re1;
+         out.y = im0 + im1;
+         acc[tt.acc_off + b0 + b] = out;
+      }
    }
 }
 
@@ -646,11 +693,26 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
                                (const T*)a.PV, (C2*)a.spec);
          break;
       }
-      case 2:
-         if (a.ntgt)
-            hipLaunchKernelGGL((k_translate<T>), dim3(a.nmap_tgt), dim3(256), 0, s, a.tgt, a.map_tgt, a.isrc,
-                               a.itoff, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+      case 2: {
+         // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
+         constexpr bool f32 = std::is_same<T, float>::value;
+         constexpr int BC = f32 ? TR_BINS_F32 : TR_BINS_F64;
+         const size_t lds_max = sizeof(C2) * (size_t)(343 + TR_MAX_SRC) * BC;
+         const size_t lds = sizeof(C2) * (size_t)(343 + a.tr_max_src) * BC;
+         check_lds(lds_max);
+         if (a.tr_max_src > TR_MAX_SRC) throw std::runtime_error("translation tile above its LDS bound");
+         static bool attr = false;
+         if (!attr) {
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+            attr = true;
+         }
+         const int nt = f32 ? a.ntask_f : a.ntask_d;
+         if (nt)
+            hipLaunchKernelGGL((k_translate<T, BC>), dim3(nt), dim3(TR_TILE * BC), lds, s, f32 ? a.task_f : a.task_d,
+                               a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
          break;
+      }
       case 3: {
          size_t lds = target_lds<T>(a.max_n);
          check_lds(lds);

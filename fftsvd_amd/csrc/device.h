@@ -111,6 +111,32 @@ struct TgtPlan {
    long long t_off, acc_off, uti_off;
 };
 
+// Translation tiles.  A block handles TR_BINS_F32 (FP32) or TR_BINS_F64
+// (FP64) frequency bins of up to TR_TILE tree-order neighbouring targets;
+// tiles grow while their source union stays within TR_SRC_BUDGET, so the
+// block's LDS ((343 + sources) spectra of 64 bytes of bins) stays at
+// 38 KiB and four blocks share a CU.  TR_MAX_SRC bounds a single target's
+// sources (LDS limit 160 KiB).
+constexpr int TR_BINS_F32 = 8, TR_BINS_F64 = 4;
+constexpr int TR_TILE = 32;
+constexpr int TR_SRC_BUDGET = 256;
+// Tiles per translation block (they share the block's staged T spectra).
+constexpr int TR_RUN = 3;
+constexpr int TR_MAX_SRC = 2216;
+
+// Translation tile: nt <= TR_TILE consecutive (tree-order) target cubes of one
+// level, GTgt entries at tgt0, their union of nsrc source spectra
+// (absolute offsets at src_off).  Each target's interactions are packed as
+// (T offset code | local source index << 9) in interaction-list order.
+struct TrTile {
+   int nt, tgt0, nsrc, src_off, pad, pad_;
+   long long t_off;
+};
+struct GTgt {
+   long long acc_off;
+   int int_off, nint;
+};
+
 // Far-field output block of a leaf: rows of one target cube's U^T
 // (r x ld, k-major) starting at the leaf's first point, weights at w_off.
 struct LeafU {
@@ -133,9 +159,13 @@ struct ApplyArgs {
    const ProjChunk* chunks;
    const SrcPlan* src;
    const TgtPlan* tgt;
-   const int *isrc, *itoff;
-   int nmap_tgt, nmap_leaf;           // XCD block maps (grid sizes)
-   const int *map_tgt, *map_leaf;
+   int ntask_f, ntask_d, tr_max_src;  // translation tasks (first tile, tiles, first bin), FP32 / FP64
+   const int4 *task_f, *task_d;
+   const TrTile* tiles;
+   const GTgt* gtgt;
+   const int *gsrc, *gint;
+   int nmap_leaf;                    // XCD block map of the leaf launch
+   const int* map_leaf;
    const LeafDesc* leaves;
    const int* dcol;
    const LeafU* ublk;

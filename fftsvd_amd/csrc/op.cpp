@@ -10,6 +10,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
+#include <iterator>
 #include <stdexcept>
 
 #include "host_linalg.h"
@@ -256,19 +258,103 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.ntgt = (int)tgt.size();
    P.nublk = (int)ublk.size();
    P.chunks.upload(chunks, s);
-   P.isrc.upload(isrc, s);
-   P.itoff.upload(itoff, s);
+   // translation tiles: runs of up to TR_TILE consecutive targets of a level
+   // (tree order, so spatially compact) with the union of their sources;
+   // a run whose union exceeds TR_MAX_SRC is halved until it fits
    {
-      std::vector<double> wt(tgt.size()), wl(op.h_leaves.size());
-      for (size_t k = 0; k < tgt.size(); k++) wt[k] = (double)tgt[k].pad * (tgt[k].nint + 4);
+      std::vector<TrTile> tiles;
+      std::vector<GTgt> gtgt;
+      std::vector<int> gsrc, gint;
+      std::vector<int4> tf, td;
+      size_t tbase = 0;
+      long long tb = 0;
+      for (size_t li = 0; li < L; li++) {
+         const FarLevel& lv = op.levels[li];
+         const int nl = (int)lv.h_tgt.size();
+         std::function<void(int, int)> make = [&](int k0, int k1) {
+            std::vector<int> so;
+            for (int k = k0; k < k1; k++) {
+               const TgtPlan& tp = tgt[tbase + k];
+               so.insert(so.end(), isrc.begin() + tp.int_off, isrc.begin() + tp.int_off + tp.nint);
+            }
+            std::sort(so.begin(), so.end());
+            so.erase(std::unique(so.begin(), so.end()), so.end());
+            if ((int)so.size() > TR_MAX_SRC && k1 - k0 > 1) {
+               const int km = (k0 + k1) / 2;
+               make(k0, km);
+               make(km, k1);
+               return;
+            }
+            if ((int)so.size() > TR_MAX_SRC) throw std::runtime_error("translation tile exceeds its LDS bound");
+            TrTile tl;
+            tl.nt = k1 - k0;
+            tl.tgt0 = (int)gtgt.size();
+            tl.nsrc = (int)so.size();
+            tl.src_off = (int)gsrc.size();
+            tl.pad = lv.pad;
+            tl.pad_ = 0;
+            tl.t_off = tb / 2;
+            gsrc.insert(gsrc.end(), so.begin(), so.end());
+            P.tr_max_src = std::max(P.tr_max_src, tl.nsrc);
+            for (int k = k0; k < k1; k++) {
+               const TgtPlan& tp = tgt[tbase + k];
+               GTgt g;
+               g.acc_off = tp.acc_off;
+               g.int_off = (int)gint.size();
+               g.nint = tp.nint;
+               for (int e = 0; e < tp.nint; e++) {
+                  const int loc = (int)(std::lower_bound(so.begin(), so.end(), isrc[tp.int_off + e]) - so.begin());
+                  gint.push_back(itoff[tp.int_off + e] | (loc << 9));
+               }
+               gtgt.push_back(g);
+            }
+            tiles.push_back(tl);
+         };
+         // grow each tile while its source union stays within the budget
+         const int tile_begin = (int)tiles.size();
+         for (int k0 = 0; k0 < nl;) {
+            std::vector<int> un;
+            int k1 = k0;
+            while (k1 < nl && k1 - k0 < TR_TILE) {
+               const TgtPlan& tp = tgt[tbase + k1];
+               std::vector<int> mine(isrc.begin() + tp.int_off, isrc.begin() + tp.int_off + tp.nint), nu;
+               std::sort(mine.begin(), mine.end());
+               std::set_union(un.begin(), un.end(), mine.begin(), mine.end(), std::back_inserter(nu));
+               if (k1 > k0 && (int)nu.size() > TR_SRC_BUDGET) break;
+               un.swap(nu);
+               k1++;
+            }
+            make(k0, k1);
+            k0 = k1;
+         }
+         // tasks: runs of TR_RUN tiles x one bin chunk
+         const int tile_end = (int)tiles.size();
+         for (int r0 = tile_begin; r0 < tile_end; r0 += TR_RUN) {
+            const int nr = std::min(TR_RUN, tile_end - r0);
+            for (int b0 = 0; b0 < lv.pad; b0 += TR_BINS_F32) tf.push_back(make_int4(r0, nr, b0, 0));
+            for (int b0 = 0; b0 < lv.pad; b0 += TR_BINS_F64) td.push_back(make_int4(r0, nr, b0, 0));
+         }
+         tbase += lv.h_tgt.size();
+         tb += (long long)lv.T64.n;
+      }
+      P.ntask_f = (int)tf.size();
+      P.ntask_d = (int)td.size();
+      P.task_f.upload(tf, s);
+      P.task_d.upload(td, s);
+      P.tiles.upload(tiles, s);
+      P.gtgt.upload(gtgt, s);
+      P.gsrc.upload(gsrc, s);
+      P.gint.upload(gint, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   {
+      std::vector<double> wl(op.h_leaves.size());
       for (size_t k = 0; k < wl.size(); k++) {
          const LeafDesc& Ld = op.h_leaves[k];
          wl[k] = 2.0 * Ld.p * Ld.m + 64.0;
       }
-      std::vector<int> mt = xcd_block_map(wt), ml = xcd_block_map(wl);
-      P.nmap_tgt = (int)mt.size();
+      std::vector<int> ml = xcd_block_map(wl);
       P.nmap_leaf = (int)ml.size();
-      P.map_tgt.upload(mt, s);
       P.map_leaf.upload(ml, s);
    }
    P.src.upload(src, s);
@@ -858,10 +944,15 @@ static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, doubl
    a.chunks = P.chunks.p;
    a.src = P.src.p;
    a.tgt = P.tgt.p;
-   a.isrc = P.isrc.p;
-   a.itoff = P.itoff.p;
-   a.nmap_tgt = P.nmap_tgt;
-   a.map_tgt = P.map_tgt.p;
+   a.ntask_f = P.ntask_f;
+   a.ntask_d = P.ntask_d;
+   a.tr_max_src = P.tr_max_src;
+   a.task_f = P.task_f.p;
+   a.task_d = P.task_d.p;
+   a.tiles = P.tiles.p;
+   a.gtgt = P.gtgt.p;
+   a.gsrc = P.gsrc.p;
+   a.gint = P.gint.p;
    a.nmap_leaf = P.nmap_leaf;
    a.map_leaf = P.map_leaf.p;
    a.leaves = op.leaves.p;

@@ -93,9 +93,13 @@ struct ApplyPlan {
    DBuf<ProjChunk> chunks;
    DBuf<SrcPlan> src;
    DBuf<TgtPlan> tgt;
-   DBuf<int> isrc, itoff;
-   DBuf<int> map_tgt, map_leaf;   // XCD block maps
-   int nmap_tgt = 0, nmap_leaf = 0;
+   int ntask_f = 0, ntask_d = 0, tr_max_src = 0;
+   DBuf<int4> task_f, task_d;
+   DBuf<TrTile> tiles;
+   DBuf<GTgt> gtgt;
+   DBuf<int> gsrc, gint;
+   DBuf<int> map_leaf;   // XCD block map of the leaf launch
+   int nmap_leaf = 0;
    DBuf<LeafU> ublk;
    DBuf<double> tpart, spec, acc, w;   // work vectors, sized for FP64
    DBuf<double> VTc64, PV64, Tsp64, UTI64, UT64;
