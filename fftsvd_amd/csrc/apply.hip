@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <type_traits>
 
@@ -701,12 +702,11 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          const size_t lds = sizeof(C2) * (size_t)(343 + a.tr_max_src) * BC;
          check_lds(lds_max);
          if (a.tr_max_src > TR_MAX_SRC) throw std::runtime_error("translation tile above its LDS bound");
-         static bool attr = false;
-         if (!attr) {
+         static std::once_flag attr;
+         std::call_once(attr, [lds_max] {
             FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-            attr = true;
-         }
+         });
          const int nt = f32 ? a.ntask_f : a.ntask_d;
          if (nt)
             hipLaunchKernelGGL((k_translate<T, BC>), dim3(nt), dim3(TR_TILE * BC), lds, s, f32 ? a.task_f : a.task_d,

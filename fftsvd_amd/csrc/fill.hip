@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "device.h"
@@ -107,6 +110,7 @@ __global__ void __launch_bounds__(256) k_gemm_batched(const GemmDesc* __restrict
 // cut ||col|| < eps*||A||_F (reference Matrix_columnbasis_pivotedmgs,
 // Matrix.c:215-307), run as one pair of launches per pivot step over every
 // problem of a batch:
+// (problems small enough for LDS run whole in k_mgs_lds instead)
 //   k_mgs_pivot   workgroup per problem: argmax of the column norms (ties to
 //                 the lowest index), rank cut, column swap, q_k = a_k / |a_k|
 //   k_mgs_update  seg lanes per remaining column: two passes of
@@ -202,9 +206,9 @@ __global__ void __launch_bounds__(256) k_mgs_update(const MgsDesc* __restrict__ 
 __global__ void __launch_bounds__(256) k_mgs_pivot(const MgsDesc* __restrict__ descs, double* __restrict__ Abuf,
                                                    double* __restrict__ Qbuf, const double* __restrict__ ss,
                                                    MgsState* __restrict__ st, int k, double eps) {
-   const int p = blockIdx.x;
+   const MgsDesc d = descs[blockIdx.x];
+   const int p = d.pid;
    if (st[p].done) return;
-   const MgsDesc d = descs[p];
    const int tid = threadIdx.x, nt = blockDim.x;
    __shared__ double rv[256];
    __shared__ int ri[256];
@@ -278,6 +282,110 @@ __global__ void __launch_bounds__(256) k_mgs_pivot(const MgsDesc* __restrict__ d
    if (tid == 0) st[p].rank = k + 1;
 }
 
+// Whole pivoted MGS of one small problem in LDS (one workgroup per
+// problem): A is loaded once, with an odd column stride against bank
+// conflicts; each thread owns whole columns, so every dot product and norm
+// is a serial sum over the rows, as in the reference.  Writes the pivot
+// columns q_k to Q and the rank to st.
+__global__ void __launch_bounds__(256) k_mgs_lds(const MgsDesc* __restrict__ descs, const int* __restrict__ list,
+                                                 const double* __restrict__ Abuf, double* __restrict__ Qbuf,
+                                                 MgsState* __restrict__ st, double eps) {
+   extern __shared__ double sm[];
+   __shared__ double rv[256];
+   __shared__ int ri[256];
+   const int p = list[blockIdx.x];
+   const MgsDesc d = descs[p];
+   const int rows = d.rows, cols = d.cols, lda = rows | 1, tid = threadIdx.x, nt = blockDim.x;
+   double* As = sm;                         // lda * cols
+   double* cn = As + (size_t)lda * cols;    // cols: squared column norms
+   const double* A = Abuf + d.a_off;
+   for (int e = tid; e < rows * cols; e += nt) {
+      const int j = e / rows, i = e - j * rows;
+      As[j * lda + i] = A[e];
+   }
+   __syncthreads();
+   double part = 0.0;
+   for (int j = tid; j < cols; j += nt) {
+      const double* a = As + j * lda;
+      double sq = 0.0;
+      for (int i = 0; i < rows; i++) sq += a[i] * a[i];
+      cn[j] = sq;
+      part += sq;
+   }
+   rv[tid] = part;
+   __syncthreads();
+   for (int w = nt / 2; w > 0; w >>= 1) {
+      if (tid < w) rv[tid] += rv[tid + w];
+      __syncthreads();
+   }
+   const double fro = sqrt(rv[0]);
+   __syncthreads();
+   int rank = 0;
+   for (int k = 0;; k++) {
+      if (k >= cols || k >= rows) {
+         rank = k;
+         break;
+      }
+      double bv = 0.0;
+      int bi = 0x7fffffff;
+      for (int j = k + tid; j < cols; j += nt) {
+         const double v = sqrt(cn[j]);
+         if (v > bv || (v == bv && j < bi)) {
+            bv = v;
+            bi = j;
+         }
+      }
+      rv[tid] = bv;
+      ri[tid] = bi;
+      __syncthreads();
+      for (int w = nt / 2; w > 0; w >>= 1) {
+         if (tid < w) {
+            const double ov = rv[tid + w];
+            const int oi = ri[tid + w];
+            if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
+               rv[tid] = ov;
+               ri[tid] = oi;
+            }
+         }
+         __syncthreads();
+      }
+      const double mx = rv[0];
+      const int best = mx > 0.0 ? ri[0] : k;
+      __syncthreads();
+      if (k >= 1 && mx < eps * fro) {
+         rank = k;
+         break;
+      }
+      double* ak = As + k * lda;
+      double* ab = As + best * lda;
+      double* q = Qbuf + d.q_off + (long long)k * rows;
+      for (int i = tid; i < rows; i += nt) {
+         const double vb = ab[i], vk = ak[i];
+         ab[i] = vk;
+         ak[i] = vb / mx;   // column k now holds q_k
+         q[i] = vb / mx;
+      }
+      __syncthreads();
+      for (int j = k + 1 + tid; j < cols; j += nt) {
+         double* a = As + j * lda;
+         for (int pass = 0; pass < 2; pass++) {
+            double r = 0.0;
+            for (int i = 0; i < rows; i++) r += ak[i] * a[i];
+            for (int i = 0; i < rows; i++) a[i] -= ak[i] * r;
+         }
+         double sq = 0.0;
+         for (int i = 0; i < rows; i++) sq += a[i] * a[i];
+         cn[j] = sq;
+      }
+      __syncthreads();
+   }
+   if (tid == 0) {
+      st[p].fro = fro;
+      st[p].rank = rank;
+      st[p].done = 1;
+   }
+}
+
 __global__ void k_mgs_init(MgsState* __restrict__ st, int n) {
    int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i < n) {
@@ -315,37 +423,62 @@ void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStr
 
 void run_mgs(std::vector<MgsDesc>& descs, double* A, double* Q, std::vector<int>& ranks, double eps,
              hipStream_t s) {
-   // two update classes: columns up to 16*seg rows (seg <= 64) held in 16
-   // registers per lane, longer ones in 32 (streamed beyond 2048 rows)
+   // Problems whose matrix fits in 96 KiB of LDS run whole in k_mgs_lds.
+   // The rest go step by step over two update classes: columns up to 16*seg
+   // rows (seg <= 64) held in 16 registers per lane, longer ones in 32
+   // (streamed beyond 2048 rows).
    constexpr int NRS = 16, NRL = 32;
+   constexpr size_t LDS_MAX = 96 * 1024;
    const int nprob = (int)descs.size();
    ranks.assign(nprob, 0);
    if (nprob == 0) return;
-   std::vector<MgsDesc> cls[2];
+   std::vector<MgsDesc> cls[2], step;
+   std::vector<int> small;
+   size_t lds_small = 0;
    int waves[2] = {0, 0}, ncol = 0, kmax = 0;
    for (int p = 0; p < nprob; p++) {
       MgsDesc& d = descs[p];
+      d.pid = p;
+      d.ss_off = ncol;
+      ncol += d.cols;
+      const size_t lds = sizeof(double) * ((size_t)(d.rows | 1) + 1) * d.cols;
+      if (lds <= LDS_MAX) {
+         small.push_back(p);
+         lds_small = std::max(lds_small, lds);
+         continue;
+      }
       const int c = d.rows > 64 * NRS ? 1 : 0;
       const int NR = c ? NRL : NRS;
       int S = 4;
       while (S < 64 && (d.rows + S - 1) / S > NR) S <<= 1;
       d.seg = S;
       d.wave0 = waves[c];
-      d.ss_off = ncol;
-      d.pid = p;
       waves[c] += (d.cols + 64 / S - 1) / (64 / S);
-      ncol += d.cols;
       kmax = std::max(kmax, std::min(d.rows, d.cols));
       cls[c].push_back(d);
+      step.push_back(d);
    }
-   DBuf<MgsDesc> dd, dc[2];
-   dd.upload(descs, s);
+   DBuf<MgsDesc> dall, dd, dc[2];
+   dall.upload(descs, s);
+   dd.upload(step, s);
    for (int c = 0; c < 2; c++) dc[c].upload(cls[c], s);
    DBuf<double> ss;
    ss.alloc(std::max(ncol, 1));
    DBuf<MgsState> st;
    st.alloc(nprob);
    hipLaunchKernelGGL(k_mgs_init, dim3((nprob + 255) / 256), dim3(256), 0, s, st.p, nprob);
+   DBuf<int> dsmall;
+   if (!small.empty()) {
+      static std::once_flag attr;
+      std::call_once(attr, [] {
+         FSVD_CHECK(hipFuncSetAttribute((const void*)k_mgs_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)LDS_MAX));
+      });
+      dsmall.upload(small, s);
+      for (size_t b0 = 0; b0 < small.size(); b0 += 65535)
+         hipLaunchKernelGGL(k_mgs_lds, dim3((unsigned)std::min<size_t>(65535, small.size() - b0)), dim3(256),
+                            lds_small, s, dall.p, dsmall.p + b0, A, Q, st.p, eps);
+   }
    auto update = [&](int k) {
       if (!cls[0].empty())
          hipLaunchKernelGGL((k_mgs_update<NRS>), dim3((waves[0] + 3) / 4), dim3(256), 0, s, dc[0].p,
@@ -354,22 +487,41 @@ void run_mgs(std::vector<MgsDesc>& descs, double* A, double* Q, std::vector<int>
          hipLaunchKernelGGL((k_mgs_update<NRL>), dim3((waves[1] + 3) / 4), dim3(256), 0, s, dc[1].p,
                             (int)cls[1].size(), waves[1], A, Q, ss.p, st.p, k);
    };
-   update(-1);
    MgsState* h = nullptr;
    FSVD_CHECK(hipHostMalloc((void**)&h, sizeof(MgsState) * nprob, hipHostMallocDefault));
-   for (int k = 0; k <= kmax; k++) {
-      hipLaunchKernelGGL(k_mgs_pivot, dim3(nprob), dim3(256), 0, s, dd.p, A, Q, ss.p, st.p, k, eps);
-      if (k < kmax) update(k);
-      if ((k & 7) == 7 || k == kmax) {
-         FSVD_CHECK(hipMemcpyAsync(h, st.p, sizeof(MgsState) * nprob, hipMemcpyDeviceToHost, s));
-         FSVD_CHECK(hipStreamSynchronize(s));
-         bool all = true;
-         for (int i = 0; i < nprob && all; i++) all = h[i].done != 0;
-         if (all) break;
+   int steps_done = 0;
+   if (!step.empty()) {
+      update(-1);
+      for (int k = 0; k <= kmax; k++) {
+         steps_done = k + 1;
+         hipLaunchKernelGGL(k_mgs_pivot, dim3((unsigned)step.size()), dim3(256), 0, s, dd.p, A, Q, ss.p, st.p, k,
+                            eps);
+         if (k < kmax) update(k);
+         if ((k & 7) == 7 || k == kmax) {
+            FSVD_CHECK(hipMemcpyAsync(h, st.p, sizeof(MgsState) * nprob, hipMemcpyDeviceToHost, s));
+            FSVD_CHECK(hipStreamSynchronize(s));
+            bool all = true;
+            for (const MgsDesc& d : step) all = all && h[d.pid].done != 0;
+            if (all) break;
+         }
       }
    }
+   FSVD_CHECK(hipMemcpyAsync(h, st.p, sizeof(MgsState) * nprob, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
    FSVD_CHECK(hipGetLastError());
    for (int i = 0; i < nprob; i++) ranks[descs[i].rank_slot] = h[i].rank;
+   if (const char* e = std::getenv("FFTSVD_TIMING"); e && e[0] == '1') {
+      int mr = 0, mc = 0, rk = 0;
+      double bytes = 0;
+      for (const MgsDesc& d : step) {
+         mr = std::max(mr, d.rows);
+         mc = std::max(mc, d.cols);
+         rk = std::max(rk, h[d.pid].rank);
+         bytes += 8.0 * d.rows * d.cols;
+      }
+      std::fprintf(stderr, "[fftsvd mgs] %d problems: %zu in LDS, %zu stepwise (max %d x %d, %.1f MB, max rank %d, steps %d)\n",
+                   nprob, small.size(), step.size(), mr, mc, bytes / 1e6, rk, steps_done);
+   }
    FSVD_CHECK(hipHostFree(h));
 }
 

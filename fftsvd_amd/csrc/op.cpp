@@ -6,6 +6,9 @@
 // that fold them into PV / UTI.  The host decides the tree, the sampling
 // pattern and the per-level pseudoinverses and translation spectra.
 #include <algorithm>
+#include <exception>
+#include <thread>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -36,6 +39,20 @@ struct CopyDesc {
 
 inline int offset_code(const int* a, const int* b) {
    return (a[0] - b[0] + 3) * 49 + (a[1] - b[1] + 3) * 7 + (a[2] - b[2] + 3);
+}
+
+// FFTSVD_TIMING=1 prints the host wall time of setup sub-steps to stderr.
+static bool timing_on() {
+   static const bool on = [] {
+      const char* e = std::getenv("FFTSVD_TIMING");
+      return e && e[0] == '1';
+   }();
+   return on;
+}
+static void timing_mark(const char* what, double& t) {
+   const double n = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+   if (timing_on()) std::fprintf(stderr, "[fftsvd setup] %-28s %8.2f ms\n", what, n - t);
+   t = n;
 }
 
 double now_ms() {
@@ -131,6 +148,7 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
 // the U blocks of the target cubes that contain it.
 static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    constexpr int CHUNK = 16384;   // basis entries per projection block
+   double tm = now_ms();
    ApplyPlan& P = op.plan;
    const Tree& t = op.tree;
    const size_t L = op.levels.size();
@@ -230,6 +248,8 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       launch_transpose_batched<double>((int)tr.size(), dtr.p, s);
       FSVD_CHECK(hipStreamSynchronize(s));
    }
+   timing_mark("plan: concatenate + VT^T", tm);
+   timing_mark("plan: leaf U blocks start", tm);
    // leaf -> far-field output blocks, finest level first
    std::vector<int> leaf_cube(op.nleaf, -1);
    for (size_t i = 0; i < op.cube_leaf.size(); i++)
@@ -258,6 +278,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.ntgt = (int)tgt.size();
    P.nublk = (int)ublk.size();
    P.chunks.upload(chunks, s);
+   timing_mark("plan: leaf U blocks", tm);
    // translation tiles: runs of up to TR_TILE consecutive targets of a level
    // (tree order, so spatially compact) with the union of their sources;
    // a run whose union exceeds TR_MAX_SRC is halved until it fits
@@ -267,18 +288,29 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       std::vector<int> gsrc, gint;
       std::vector<int4> tf, td;
       size_t tbase = 0;
-      long long tb = 0;
+      long long tb = 0, sbase = 0;
       for (size_t li = 0; li < L; li++) {
          const FarLevel& lv = op.levels[li];
          const int nl = (int)lv.h_tgt.size();
+         // sources of this level by slot: spec offset = sbase + slot * pad
+         std::vector<int> stamp(lv.nsrc, -1), loc(lv.nsrc, 0);
+         auto slot_of = [&](int so) { return (int)(((long long)so - sbase) / lv.pad); };
+         int epoch = 0;
          std::function<void(int, int)> make = [&](int k0, int k1) {
+            // union of the tiles' sources in order of first appearance
+            const int mark = -2 - (epoch++);
             std::vector<int> so;
             for (int k = k0; k < k1; k++) {
                const TgtPlan& tp = tgt[tbase + k];
-               so.insert(so.end(), isrc.begin() + tp.int_off, isrc.begin() + tp.int_off + tp.nint);
+               for (int e = 0; e < tp.nint; e++) {
+                  const int sl = slot_of(isrc[tp.int_off + e]);
+                  if (stamp[sl] != mark) {
+                     stamp[sl] = mark;
+                     loc[sl] = (int)so.size();
+                     so.push_back(isrc[tp.int_off + e]);
+                  }
+               }
             }
-            std::sort(so.begin(), so.end());
-            so.erase(std::unique(so.begin(), so.end()), so.end());
             if ((int)so.size() > TR_MAX_SRC && k1 - k0 > 1) {
                const int km = (k0 + k1) / 2;
                make(k0, km);
@@ -302,26 +334,25 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
                g.acc_off = tp.acc_off;
                g.int_off = (int)gint.size();
                g.nint = tp.nint;
-               for (int e = 0; e < tp.nint; e++) {
-                  const int loc = (int)(std::lower_bound(so.begin(), so.end(), isrc[tp.int_off + e]) - so.begin());
-                  gint.push_back(itoff[tp.int_off + e] | (loc << 9));
-               }
+               for (int e = 0; e < tp.nint; e++)
+                  gint.push_back(itoff[tp.int_off + e] | (loc[slot_of(isrc[tp.int_off + e])] << 9));
                gtgt.push_back(g);
             }
             tiles.push_back(tl);
          };
          // grow each tile while its source union stays within the budget
+         // (union size tracked with a stamp per source slot)
          const int tile_begin = (int)tiles.size();
          for (int k0 = 0; k0 < nl;) {
-            std::vector<int> un;
-            int k1 = k0;
+            int k1 = k0, un = 0;
             while (k1 < nl && k1 - k0 < TR_TILE) {
                const TgtPlan& tp = tgt[tbase + k1];
-               std::vector<int> mine(isrc.begin() + tp.int_off, isrc.begin() + tp.int_off + tp.nint), nu;
-               std::sort(mine.begin(), mine.end());
-               std::set_union(un.begin(), un.end(), mine.begin(), mine.end(), std::back_inserter(nu));
-               if (k1 > k0 && (int)nu.size() > TR_SRC_BUDGET) break;
-               un.swap(nu);
+               int add = 0;
+               for (int e = 0; e < tp.nint; e++) add += stamp[slot_of(isrc[tp.int_off + e])] != k0;
+               // sources within one list are distinct
+               if (k1 > k0 && un + add > TR_SRC_BUDGET) break;
+               for (int e = 0; e < tp.nint; e++) stamp[slot_of(isrc[tp.int_off + e])] = k0;
+               un += add;
                k1++;
             }
             make(k0, k1);
@@ -336,6 +367,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          }
          tbase += lv.h_tgt.size();
          tb += (long long)lv.T64.n;
+         sbase += (long long)lv.nsrc * lv.pad;
       }
       P.ntask_f = (int)tf.size();
       P.ntask_d = (int)td.size();
@@ -360,6 +392,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.src.upload(src, s);
    P.tgt.upload(tgt, s);
    P.ublk.upload(ublk, s);
+   timing_mark("plan: tiles + maps", tm);
    P.tpart.alloc(std::max(tpart_n, 1));
    P.spec.alloc(2 * std::max<long long>(spec_b, 1));
    P.acc.alloc(2 * std::max<long long>(acc_b, 1));
@@ -464,7 +497,14 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    const double root = t.cubes[0].b1[0] - t.cubes[0].b0[0];
    op.h_pinv.resize(t.depth);
    std::vector<int> src_slot(t.cubes.size(), -1), tgt_slot(t.cubes.size(), -1);
-   for (int l = 2; l < t.depth; l++) {
+   // Levels are independent until the interaction lists: each is built by
+   // its own host thread on its own stream (level l -> op.levels[l-2]).
+   const int nlev = std::max(0, t.depth - 2);
+   op.levels.resize(nlev);
+   std::vector<std::array<double, 8>> lev_ph(nlev);
+   std::vector<long long> lev_far(nlev, 0);
+   auto build_level = [&](int l, hipStream_t s) {
+      double* ph = lev_ph[l - 2].data();
       FarLevel lv;
       lv.level = l;
       lv.n = t.gpl[l];
@@ -473,7 +513,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       lv.gp3 = n * n * n;
       const int gp3 = lv.gp3, pad = lv.pad;
 
-      tp = now_ms();
+      double tp = now_ms();
+      double tl = tp;
       // pseudoinverse of the sphere -> grid Green's matrix (Tree_fill_pinv_Fitting, Tree.c:295-369)
       {
          double box = root / std::pow(2.0, l);
@@ -503,7 +544,8 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       DBuf<double> d_pinv;
       d_pinv.upload(op.h_pinv[l], s);
-      op.stats.phase_ms[2] += now_ms() - tp;
+      ph[2] += now_ms() - tp;
+      timing_mark("level: pinv + T", tl);
       tp = now_ms();
 
       // cubes of this level with interactions
@@ -604,11 +646,13 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          Qbuf->alloc(std::max<long long>(qsz, 1));
          run_integrals(blocks, bpts, bpan, d_panels, kernel, kappa, Abuf->p, s);
          FSVD_CHECK(hipStreamSynchronize(s));
-         op.stats.phase_ms[3] += now_ms() - tp;
+         ph[3] += now_ms() - tp;
          tp = now_ms();
+         timing_mark("level: sampling", tl);
          std::vector<int> ranks;
          run_mgs(mgs, Abuf->p, Qbuf->p, ranks, svdeps, s);
-         op.stats.phase_ms[4] += now_ms() - tp;
+         timing_mark("level: pivoted MGS", tl);
+         ph[4] += now_ms() - tp;
          tp = now_ms();
          std::vector<int> next;
          for (size_t pi = 0; pi < pending.size(); pi++) {
@@ -625,7 +669,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          keep.push_back(Qbuf);
          pending.swap(next);
       }
-      op.stats.phase_ms[3] += now_ms() - tp;
+      ph[3] += now_ms() - tp;
       tp = now_ms();
       // ranks into cubes; compact bases into the level buffers
       std::vector<int> vt_prob(cl.size(), -1), u_prob(cl.size(), -1);
@@ -700,6 +744,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       keep.clear();
 
+      timing_mark("level: compaction", tl);
       // ---- equivalent densities: PV = P VT^T, UTI = U^T I (Cube.c:795-1002,1414-1522) ----
       {
          std::vector<IntBlock> blocks;
@@ -846,9 +891,33 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          run_gemms(g2, s);
       }
       FSVD_CHECK(hipStreamSynchronize(s));
-      op.stats.phase_ms[5] += now_ms() - tp;
-      op.stats.far_entries += (long long)(lv.VT64.n + lv.PV64.n + lv.UT64.n + lv.UTI64.n);
-      op.levels.push_back(std::move(lv));
+      ph[5] += now_ms() - tp;
+      timing_mark("level: PV, UTI", tl);
+      lev_far[l - 2] = (long long)(lv.VT64.n + lv.PV64.n + lv.UT64.n + lv.UTI64.n);
+      op.levels[l - 2] = std::move(lv);
+   };
+   {
+      std::vector<std::exception_ptr> err(nlev);
+      std::vector<std::thread> th;
+      for (int l = 2; l < t.depth; l++)
+         th.emplace_back([&, l]() {
+            hipStream_t ls = nullptr;
+            try {
+               FSVD_CHECK(hipStreamCreate(&ls));
+               build_level(l, ls);
+               FSVD_CHECK(hipStreamSynchronize(ls));
+            } catch (...) {
+               err[l - 2] = std::current_exception();
+            }
+            if (ls) (void)hipStreamDestroy(ls);
+         });
+      for (auto& x : th) x.join();
+      for (auto& e : err)
+         if (e) std::rethrow_exception(e);
+      for (int k = 0; k < nlev; k++) {
+         for (int q = 0; q < 8; q++) op.stats.phase_ms[q] += lev_ph[k][q];
+         op.stats.far_entries += lev_far[k];
+      }
    }
 
    op.cube_src_slot = src_slot;
@@ -860,6 +929,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          if (t.cubes[i].leaf && t.cubes[i].nq > 0) op.cube_leaf[i] = (int)li++;
    }
    tp = now_ms();
+   double tm = tp;
    // interaction lists (Cube_multiply_T, Cube.c:2062-2111) and their transposes
    for (FarLevel& lv : op.levels) {
       for (TgtDesc& td : lv.h_tgt) td.nint = 0;
@@ -909,6 +979,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       lv.specT.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
       lv.accT.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
    }
+   timing_mark("interaction lists", tm);
    if (want32) {
       to_f32(op.D64, op.D32, s);
       for (FarLevel& lv : op.levels) {
@@ -920,8 +991,11 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       op.have32 = true;
    }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   timing_mark("FP32 copies", tm);
    build_apply_plan(op, want32, s);
    FSVD_CHECK(hipStreamSynchronize(s));
+   timing_mark("apply plan", tm);
    op.stats.phase_ms[6] += now_ms() - tp;
    op.stats.setup_ms_far = now_ms() - t0;
    return opp;
