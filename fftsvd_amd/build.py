@@ -15,10 +15,12 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfftsvd_hip.so")
+PBE = os.path.join(LIBDIR, "fftsvd_pbe")          # drop-in driver built against the C interface
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["geometry.cpp", "tree.cpp", "host_linalg.cpp", "op.cpp", "solver.cpp", "capi.cpp", "mesh.cpp",
+SOURCES = ["geometry.cpp", "tree.cpp", "host_linalg.cpp", "op.cpp", "solver.cpp", "capi.cpp", "mesh.cpp", "frontend.cpp",
            "fill.hip", "apply.hip", "solve.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-x", "hip",
@@ -43,13 +45,22 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"]
+    # -Bsymbolic-functions: calls inside the library bind to its own
+    # definitions (error() in particular, which glibc also exports)
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-Bsymbolic-functions", "-o", tmp] + objs + \
+        ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     os.replace(tmp, LIB)
+    # the reference-style driver, plain C against include/FFTSVDpbeAPI.h
+    cmd = ["gcc", "-O2", "-std=c99", "-Wall", "-I", INCLUDE, os.path.join(HERE, "apps", "fftsvd_pbe.c"), "-L", LIBDIR,
+           "-lfftsvd_hip", "-Wl,-rpath,$ORIGIN", "-o", PBE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", PBE)
     return LIB
 
 

@@ -121,7 +121,20 @@ typedef struct {
    real charge;
 } CRGentry;
 
-/* Parameters (reference FFTSVDpbeAPI.h:60-89, defaults FFTSVDpbeAPI.c:18-34) */
+/* DelPhi unit-number file names (reference FFTSVDpbeAPI.h:60-69, defaults
+ * "fort.10" .. "fort.20", FFTSVDpbeAPI.c:5-14) */
+extern const char* delphiparameterfilename;
+extern const char* delphisizfilename;
+extern const char* delphicrgfilename;
+extern const char* delphipdbinputfilename;
+extern const char* delphiphioutputfilename;
+extern const char* delphipdbcollectfilename;
+extern const char* delphifrcfilename;
+extern const char* delphiphiinputfilename;
+extern const char* delphipdboutputfilename;
+extern const char* delphisrffilename;
+
+/* Parameters (reference FFTSVDpbeAPI.h:70-89, defaults FFTSVDpbeAPI.c:18-34) */
 extern real dielectricvertexdensity;
 extern real saltvertexdensity;
 extern real innerdielectric;
@@ -143,6 +156,35 @@ extern unsigned int usecurved;
 /* Utility (reference FFTSVDpbeAPI.h:93-96; FFTSVDpbeAPI.c:38-93) */
 void error(char* message, ...);
 void warning(char* message, ...);
+
+void removeWhitespace(char* s);   /* reference FFTSVDpbeAPI.h:94; FFTSVDpbeAPI.c:64-79 */
+int yesno(char* s);               /* reference FFTSVDpbeAPI.h:95; FFTSVDpbeAPI.c:81-92 */
+
+/* Input (reference FFTSVDpbeAPI.h:99-105; FFTSVDpbeAPI.c:94-500): the
+ * DelPhi-style parameter file (NAME=value, ENERGY(G), OUT(FRC|MODPDB|PHI)),
+ * PDB / CHARMM CRD / XYZR coordinates and SIZ / CRG tables.  Arrays are
+ * calloc'ed; the caller frees them. */
+void readParams(const char* filename);
+void checkParams();
+void readPDB(const char* filename, unsigned int* numPDBentries, PDBentry** PDBentries);
+void readCRD(const char* filename, unsigned int* numPDBentries, PDBentry** PDBentries);
+void readXYZR(const char* filename, unsigned int* numPDBentries, PDBentry** PDBentries);
+void readSIZ(const char* filename, unsigned int* numSIZentries, SIZentry** SIZentries);
+void readCRG(const char* filename, unsigned int* numCRGentries, CRGentry** CRGentries);
+
+/* Output (reference FFTSVDpbeAPI.h:109-110; FFTSVDpbeAPI.c:504-565) */
+void writeMODPDB(const char* filename, PDBentry* PDBentries, unsigned int numPDBentries);
+void writeFRC(const char* filename, PDBentry* PDBentries, unsigned int numPDBentries);
+
+/* Molecule handling (reference FFTSVDpbeAPI.h:114-116; FFTSVDpbeAPI.c:569-672,
+ * 1034-1049).  generateFlatSRF meshes the molecule with this library's SES
+ * mesher (the reference runs its meshmaker/MSMS) and writes the .srf and
+ * <srf>.d0.vert/.face; generateCurvedSRF reports that curved panels are not
+ * supported. */
+void assignRadiiCharges(PDBentry* PDBentries, unsigned int numPDBentries, SIZentry* SIZentries,
+                        unsigned int numSIZentries, CRGentry* CRGentries, unsigned int numCRGentries);
+void generateFlatSRF(const char* pdbfilename, const char* sizfilename, const char* srffilename);
+void generateCurvedSRF(const char* pdbfilename, const char* sizfilename, const char* srffilename);
 
 /* Surfaces: reads an .srf file and its .vert/.face surfaces
  * (reference FFTSVDpbeAPI.h:118-121; FFTSVDpbeAPI.c:776-1032).  Flat
