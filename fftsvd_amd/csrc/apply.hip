@@ -204,16 +204,23 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
 // thread's partial sum for (row e, layer inputs x0/x1 selected by e >= split).
 template <typename T>
 __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int m, int g, int G, const T* xl) {
+   // eight columns in flight per thread
    const long long step = (long long)G * E;
    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
    int c = g;
-   for (; c + 3 * G < m; c += 4 * G) {
+   for (; c + 7 * G < m; c += 8 * G) {
       const T* pc = colp + (long long)c * E;
-      T d0 = pc[0], d1 = pc[step], d2 = pc[2 * step], d3 = pc[3 * step];
-      a0 += d0 * xl[c];
-      a1 += d1 * xl[c + G];
-      a2 += d2 * xl[c + 2 * G];
-      a3 += d3 * xl[c + 3 * G];
+      T d[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) d[u] = pc[u * step];
+      a0 += d[0] * xl[c];
+      a1 += d[1] * xl[c + G];
+      a2 += d[2] * xl[c + 2 * G];
+      a3 += d[3] * xl[c + 3 * G];
+      a0 += d[4] * xl[c + 4 * G];
+      a1 += d[5] * xl[c + 5 * G];
+      a2 += d[6] * xl[c + 6 * G];
+      a3 += d[7] * xl[c + 7 * G];
    }
    for (; c < m; c += G) a0 += colp[(long long)c * E] * xl[c];
    return (a0 + a1) + (a2 + a3);
@@ -243,12 +250,17 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
          const long long step = (long long)G * E;
          T s0 = 0, s1 = 0, d0 = 0, d1 = 0;
          int i = g;
-         for (; i + G < m; i += 2 * G) {
-            T v0 = colp[(long long)i * E], v1 = colp[(long long)i * E + step];
+         for (; i + 3 * G < m; i += 4 * G) {
+            const T v0 = colp[(long long)i * E], v1 = colp[(long long)i * E + step];
+            const T v2 = colp[(long long)i * E + 2 * step], v3 = colp[(long long)i * E + 3 * step];
             s0 += v0 * xl[i];
             d0 += v0 * xl[m + i];
             s1 += v1 * xl[i + G];
             d1 += v1 * xl[m + i + G];
+            s0 += v2 * xl[i + 2 * G];
+            d0 += v2 * xl[m + i + 2 * G];
+            s1 += v3 * xl[i + 3 * G];
+            d1 += v3 * xl[m + i + 3 * G];
          }
          for (; i < m; i += G) {
             T v = colp[(long long)i * E];

@@ -89,50 +89,89 @@ __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __
 }
 
 // Classical Gram-Schmidt pass over m basis vectors (two passes make CGS2).
-// Block b owns the contiguous chunk b of the vectors; partial[j*np + b] =
-// chunk sum of V_j . w, four columns per sweep so w is read once per four.
-__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int np, const double* const* __restrict__ V,
+// Block (b, jg) owns the contiguous chunk b of the vectors and the columns
+// 4jg..4jg+3: partial[j*nch + b] = chunk sum of V_j . w.
+__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const double* const* __restrict__ V,
                                                   const double* __restrict__ w, double* __restrict__ partial) {
    __shared__ double red[8];
-   const long long chunk = (n + np - 1) / np;
+   const long long chunk = (n + nch - 1) / nch;
    const long long i0 = blockIdx.x * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
-   for (int j0 = 0; j0 < m; j0 += 4) {
-      const int nj = m - j0 < 4 ? m - j0 : 4;
-      const double* v0 = V[j0];
-      const double* v1 = nj > 1 ? V[j0 + 1] : v0;
-      const double* v2 = nj > 2 ? V[j0 + 2] : v0;
-      const double* v3 = nj > 3 ? V[j0 + 3] : v0;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-         const double wi = w[i];
-         s0 += v0[i] * wi;
-         s1 += v1[i] * wi;
-         s2 += v2[i] * wi;
-         s3 += v3[i] * wi;
-      }
-      s0 = block_sum(s0, red);
-      s1 = block_sum(s1, red);
-      s2 = block_sum(s2, red);
-      s3 = block_sum(s3, red);
-      if (threadIdx.x == 0) {
-         partial[(long long)j0 * np + blockIdx.x] = s0;
-         if (nj > 1) partial[(long long)(j0 + 1) * np + blockIdx.x] = s1;
-         if (nj > 2) partial[(long long)(j0 + 2) * np + blockIdx.x] = s2;
-         if (nj > 3) partial[(long long)(j0 + 3) * np + blockIdx.x] = s3;
-      }
+   const int j0 = blockIdx.y * 4;
+   const int nj = m - j0 < 4 ? m - j0 : 4;
+   const double* v0 = V[j0];
+   const double* v1 = nj > 1 ? V[j0 + 1] : v0;
+   const double* v2 = nj > 2 ? V[j0 + 2] : v0;
+   const double* v3 = nj > 3 ? V[j0 + 3] : v0;
+   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+   for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const double wi = w[i];
+      s0 += v0[i] * wi;
+      s1 += v1[i] * wi;
+      s2 += v2[i] * wi;
+      s3 += v3[i] * wi;
+   }
+   s0 = block_sum(s0, red);
+   s1 = block_sum(s1, red);
+   s2 = block_sum(s2, red);
+   s3 = block_sum(s3, red);
+   if (threadIdx.x == 0) {
+      partial[(long long)j0 * nch + blockIdx.x] = s0;
+      if (nj > 1) partial[(long long)(j0 + 1) * nch + blockIdx.x] = s1;
+      if (nj > 2) partial[(long long)(j0 + 2) * nch + blockIdx.x] = s2;
+      if (nj > 3) partial[(long long)(j0 + 3) * nch + blockIdx.x] = s3;
    }
 }
 
-// h[j] = sum of column j's np partials in block order; hacc[j] = h[j]
-// (first pass) or hacc[j] + h[j] (second pass).
-__global__ void k_multidot_finish(int m, int np, const double* __restrict__ partial, double* __restrict__ h,
-                                  double* __restrict__ hacc, int add) {
-   const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// h[j] = sum of column j's nch partials (one wave per column, lanes over
+// the partials, then a fixed-order wave reduction); hacc[j] = h[j] (first
+// pass) or hacc[j] + h[j] (second pass).
+__global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const double* __restrict__ partial,
+                                                        double* __restrict__ h, double* __restrict__ hacc, int add) {
+   const int j = blockIdx.x;
    if (j >= m) return;
-   double s = 0.0;
-   for (int k = 0; k < np; k++) s += partial[(long long)j * np + k];
-   h[j] = s;
-   hacc[j] = add ? hacc[j] + s : s;
+   double sum = 0.0;
+   for (int k = threadIdx.x; k < nch; k += 64) sum += partial[(long long)j * nch + k];
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+   if (threadIdx.x == 0) {
+      h[j] = sum;
+      hacc[j] = add ? hacc[j] + sum : sum;
+   }
+}
+
+// One Arnoldi column's Givens step on the device (reference GMRES.c:3-22,
+// 495-510): the previous rotations applied to H[:, i] = hcol, the new
+// rotation from (H[i,i], H[i+1,i]), g updated, |g[i+1]|/normr recorded.
+// One thread; H is column-major with leading dimension W.
+__global__ void k_arnoldi_givens(int i, int W, const double* __restrict__ hcol, double* __restrict__ H,
+                                 double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                                 double* __restrict__ hist, double normr) {
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   double* Hi = H + (long long)i * W;
+   for (int j = 0; j <= i + 1; j++) Hi[j] = hcol[j];
+   for (int k = 0; k < i; k++) {
+      const double a = Hi[k], b = Hi[k + 1];
+      Hi[k] = a * cs[k] - b * sn[k];
+      Hi[k + 1] = cs[k] * b + sn[k] * a;
+   }
+   double c = 1.0, s = 0.0;
+   if (Hi[i + 1] != 0.0) {
+      const double r1 = fabs(Hi[i]), r2 = fabs(Hi[i + 1]);
+      const double L = sqrt(r1 * r1 + r2 * r2);
+      c = Hi[i] / L;
+      s = -Hi[i + 1] / L;
+   }
+   cs[i] = c;
+   sn[i] = s;
+   {
+      const double a = Hi[i], b = Hi[i + 1];
+      Hi[i] = a * c - b * s;
+      Hi[i + 1] = c * b + s * a;
+   }
+   const double gi = g[i];
+   g[i] = gi * c;   // g[i+1] was zero
+   g[i + 1] = s * gi;
+   hist[i] = fabs(g[i + 1]) / normr;
 }
 
 // w -= sum_j h[j] V_j, summed over j in order.
@@ -261,12 +300,16 @@ void launch_copy(long long n4, const void* a, void* b, hipStream_t s) {
    if (n4 > 0)
       hipLaunchKernelGGL(k_copy, dim3(nblocks(n4, 256, 8192)), dim3(256), 0, s, n4, (const double4*)a, (double4*)b);
 }
-void launch_multidot(long long n, int m, int np, const double* const* V, const double* w, double* partial,
+void launch_multidot(long long n, int m, int nch, const double* const* V, const double* w, double* partial,
                      hipStream_t s) {
-   if (m > 0) hipLaunchKernelGGL(k_multidot, dim3(np), dim3(256), 0, s, n, m, np, V, w, partial);
+   if (m > 0) hipLaunchKernelGGL(k_multidot, dim3(nch, (m + 3) / 4), dim3(256), 0, s, n, m, nch, V, w, partial);
 }
-void launch_multidot_finish(int m, int np, const double* partial, double* h, double* hacc, int add, hipStream_t s) {
-   if (m > 0) hipLaunchKernelGGL(k_multidot_finish, dim3((m + 63) / 64), dim3(64), 0, s, m, np, partial, h, hacc, add);
+void launch_multidot_finish(int m, int nch, const double* partial, double* h, double* hacc, int add, hipStream_t s) {
+   if (m > 0) hipLaunchKernelGGL(k_multidot_finish, dim3(m), dim3(64), 0, s, m, nch, partial, h, hacc, add);
+}
+void launch_arnoldi_givens(int i, int W, const double* hcol, double* H, double* cs, double* sn, double* g,
+                           double* hist, double normr, hipStream_t s) {
+   hipLaunchKernelGGL(k_arnoldi_givens, dim3(1), dim3(64), 0, s, i, W, hcol, H, cs, sn, g, hist, normr);
 }
 void launch_multiaxpy(long long n, int m, const double* h, const double* const* V, double* w, hipStream_t s) {
    if (m > 0) hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, h, V, w);

@@ -24,6 +24,8 @@ void launch_combine(long long n, int m, const double* y, const double* const* V,
 void launch_multidot(long long n, int m, int np, const double* const* V, const double* w, double* partial,
                      hipStream_t s);
 void launch_multidot_finish(int m, int np, const double* partial, double* h, double* hacc, int add, hipStream_t s);
+void launch_arnoldi_givens(int i, int W, const double* hcol, double* H, double* cs, double* sn, double* g,
+                           double* hist, double normr, hipStream_t s);
 void launch_multiaxpy(long long n, int m, const double* h, const double* const* V, double* w, hipStream_t s);
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
 void launch_rhs(int ne, const double* cent, const int* out, int nq, const double* qpos, const double* q, double* rhs,
@@ -394,23 +396,6 @@ void problem_rhs(Problem& pr, double* rhs) {
    }
 }
 
-static void givens(double a, double b, double* c, double* s) {
-   if (b == 0.0) {
-      *c = 1.0;
-      *s = 0.0;
-   } else {
-      double r1 = std::fabs(a), r2 = std::fabs(b);
-      double L = std::sqrt(r1 * r1 + r2 * r2);
-      *c = a / L;
-      *s = -b / L;
-   }
-}
-static void grot(double c, double s, double* a, double* b) {
-   double a1 = *a, b1 = *b;
-   *a = a1 * c - b1 * s;
-   *b = c * b1 + s * a1;
-}
-
 static double* basis_vec(Problem& pr, int j) {
    while ((int)pr.basis.size() <= j) {
       pr.basis.emplace_back(new DBuf<double>());
@@ -457,13 +442,30 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
       return info;
    }
    const int W = maxit + 1;
-   std::vector<double> H((size_t)W * W, 0.0), c(W, 0.0), sn(W, 0.0), g(W + 1, 0.0), y(W, 0.0);
-   if ((long long)pr.partial_m.n < (long long)(maxit + 1) * np) pr.partial_m.alloc((size_t)(maxit + 1) * np);
+   // multi-dot chunks: about 1024 elements each, at most 256
+   const int nch = (int)std::max<long long>(1, std::min<long long>(256, (n + 1023) / 1024));
+   if ((long long)pr.partial_m.n < (long long)(maxit + 1) * nch) pr.partial_m.alloc((size_t)(maxit + 1) * nch);
    if ((long long)pr.htmp.n < maxit + 2) pr.htmp.alloc(maxit + 2);
    if ((long long)pr.hbuf.n < maxit + 2) pr.hbuf.alloc(maxit + 2);
-   g[0] = normr;
+   // Givens state on the device: H (W x W, column-major), rotations, g, and
+   // the residual history; the host reads the history every CHECK iterations
+   // and stops at the first iteration below tol, so the iteration count and
+   // the solution are those of a per-iteration check.
+   constexpr int CHECK = 4;
+   if ((long long)pr.gm_H.n < (long long)W * W) pr.gm_H.alloc((size_t)W * W);
+   if ((long long)pr.gm_v.n < 4LL * (W + 1)) pr.gm_v.alloc(4 * (size_t)(W + 1));
+   double* dH = pr.gm_H.p;
+   double* dcs = pr.gm_v.p;
+   double* dsn = dcs + (W + 1);
+   double* dg = dsn + (W + 1);
+   double* dhist = dg + (W + 1);
+   {
+      std::vector<double> g0(W + 1, 0.0);
+      g0[0] = normr;
+      FSVD_CHECK(hipMemcpyAsync(dg, g0.data(), sizeof(double) * (W + 1), hipMemcpyHostToDevice, s));
+   }
    launch_normalize(n, np, P0, pr.hbuf.p, r, basis_vec(pr, 0), s);
-   int i;
+   int i, last = -1, done = -1;
    for (i = 0; i < maxit; i++) {
       double* vi = basis_vec(pr, i);
       double* vnext_store = basis_vec(pr, i + 1);
@@ -480,27 +482,37 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
       // iteration instead of i+3.
       const double* const* V = basis_ptrs(pr, i + 2);
       for (int pass = 0; pass < 2; pass++) {
-         launch_multidot(n, i + 1, np, V, w, pr.partial_m.p, s);
-         launch_multidot_finish(i + 1, np, pr.partial_m.p, pr.htmp.p, pr.hbuf.p, pass, s);
+         launch_multidot(n, i + 1, nch, V, w, pr.partial_m.p, s);
+         launch_multidot_finish(i + 1, nch, pr.partial_m.p, pr.htmp.p, pr.hbuf.p, pass, s);
          launch_multiaxpy(n, i + 1, pr.htmp.p, V, w, s);
       }
       launch_dot_partial(n, np, w, w, P0, s);
       launch_normalize(n, np, P0, pr.hbuf.p + i + 1, w, vnext_store, s);
-      FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, pr.hbuf.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
-      FSVD_CHECK(hipStreamSynchronize(s));
-      double* Hi = &H[(size_t)i * W];
-      for (int j = 0; j <= i + 1; j++) Hi[j] = pr.h_pinned[j];
-      for (int k = 0; k < i; k++) grot(c[k], sn[k], &Hi[k], &Hi[k + 1]);
-      givens(Hi[i], Hi[i + 1], &c[i], &sn[i]);
-      grot(c[i], sn[i], &Hi[i], &Hi[i + 1]);
-      g[i + 1] = 0.0;
-      grot(c[i], sn[i], &g[i], &g[i + 1]);
-      double res = std::fabs(g[i + 1]) / normr;
-      if (hist) hist->push_back(res);
-      info.final_residual = res;
-      if (res < tol) break;
+      launch_arnoldi_givens(i, W, pr.hbuf.p, dH, dcs, dsn, dg, dhist, normr, s);
+      if ((i + 1) % CHECK == 0 || i == maxit - 1) {
+         const int a0 = last + 1;
+         FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, dhist + a0, sizeof(double) * (i + 1 - a0), hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+         for (int k = a0; k <= i; k++) {
+            const double res = pr.h_pinned[k - a0];
+            if (hist) hist->push_back(res);
+            info.final_residual = res;
+            if (res < tol) {
+               done = k;
+               break;
+            }
+         }
+         last = i;
+         if (done >= 0) break;
+      }
    }
-   if (i == maxit) i = maxit - 1;
+   if (done >= 0) i = done;
+   if (i >= maxit) i = maxit - 1;
+   // back substitution on the host over the rotated H and g
+   std::vector<double> H((size_t)W * (i + 1)), g(i + 1), y(i + 1);
+   FSVD_CHECK(hipMemcpyAsync(H.data(), dH, sizeof(double) * (size_t)W * (i + 1), hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipMemcpyAsync(g.data(), dg, sizeof(double) * (i + 1), hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
    for (int k = 0; k <= i; k++) y[k] = g[k];
    for (int k = i; k >= 0; k--) {
       y[k] /= H[(size_t)k * W + k];

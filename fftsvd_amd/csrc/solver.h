@@ -73,6 +73,7 @@ struct Problem {
    DBuf<double> w, r, tmp, partial, hbuf, htmp, partial_m;
    std::vector<std::unique_ptr<DBuf<double>>> basis;
    DBuf<const double*> d_basis_ptr;     // device copy of the basis pointers
+   DBuf<double> gm_H, gm_v;             // GMRES Givens state (H; rotations, g, history)
    int basis_ptr_n = 0;
    double* h_pinned = nullptr;
    double setup_ms = 0.0;
