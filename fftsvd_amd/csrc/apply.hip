@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
 //   k_project    chunk of a source cube's panels: partial VT.xs, VT.xd
 //   k_source     source cube: sum of chunk partials, PV interpolation onto
 //                the n^3 grid, zero-padded r2c DFT -> spectrum
-//   k_translate  tile of <= 32 target cubes x a bin chunk: sum over each
+//   k_translate  tile of <= TR_TILE target cubes x a bin chunk: sum over each
 //                interaction list of T . spectrum, from LDS
 //   k_target     target cube: cropped c2r DFT, w = UTI h
 //   k_leaf       leaf: near-field block apply plus U w of every target cube
@@ -296,7 +296,8 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
    T* cs = (T*)(B + n * N * H);  // N
    T* sn = cs + N;               // N
    T* g = sn + N;                // GP3
-   T* t = g + GP3;               // 2r: ts then td
+   T* part = g + GP3;            // KG * GP3 partial grid values
+   T* t = part + 256;            // 2r: ts then td
    const int tid = threadIdx.x, r = d.r;
    twiddle_table<T, N>(cs, sn);
    for (int k = tid; k < 2 * r; k += blockDim.x) {
@@ -305,15 +306,31 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
       t[k] = a;
    }
    __syncthreads();
-   const T* pvs = PV + d.pv_off;
-   const T* pvd = pvs + (long long)r * GP3;
-   for (int q = tid; q < GP3; q += blockDim.x) {
-      T acc = 0;
-      for (int k = 0; k < r; k++) acc += pvs[(long long)k * GP3 + q] * t[k];
-      for (int k = 0; k < r; k++) acc += pvd[(long long)k * GP3 + q] * t[r + k];
-      g[q] = acc;
+   // g = PV t over the 2r rows (PVs then PVd, k-major): thread (kg, q) sums
+   // the rows k = kg, kg + KG, ... so all 256 threads stream PV
+   constexpr int KG = GP3 >= 256 ? 1 : 256 / GP3;
+   const T* pv = PV + d.pv_off;
+   const int R2 = 2 * r;
+   for (int q0 = 0; q0 < GP3; q0 += 256 / KG) {
+      const int kg = tid / (256 / KG), q = q0 + tid - kg * (256 / KG);
+      T a0 = 0, a1 = 0;
+      if (kg < KG && q < GP3) {
+         int k = kg;
+         for (; k + KG < R2; k += 2 * KG) {
+            a0 += pv[(long long)k * GP3 + q] * t[k];
+            a1 += pv[(long long)(k + KG) * GP3 + q] * t[k + KG];
+         }
+         if (k < R2) a0 += pv[(long long)k * GP3 + q] * t[k];
+      }
+      if (kg < KG) part[kg * (256 / KG) + (q - q0)] = a0 + a1;
+      __syncthreads();
+      if (tid < 256 / KG && q0 + tid < GP3) {
+         T sum = 0;
+         for (int gg = 0; gg < KG; gg++) sum += part[gg * (256 / KG) + tid];
+         g[q0 + tid] = sum;
+      }
+      __syncthreads();
    }
-   __syncthreads();
    grid_forward<n, T>(g, A, B, spec + d.spec_off, cs, sn);
 }
 
@@ -333,28 +350,69 @@ __global__ void __launch_bounds__(256) k_source(const SrcPlan* __restrict__ src,
    }
 }
 
-// Translation by tiles: a block owns BC frequency bins of a run of tiles
-// (each up to TR_TILE neighbouring target cubes of one level).  It stages
-// the 343 x BC T spectra of those bins in LDS once, then per tile the BC
-// bins of every source spectrum the tile's targets interact with, and
-// thread (target j, bin b) sums its interaction list from LDS.  Per target
-// the sum runs in interaction-list order.
-template <typename T, int BC>
-__global__ void __launch_bounds__(TR_TILE * BC) k_translate(const int4* __restrict__ tasks,
-                                                            const TrTile* __restrict__ tiles,
-                                                            const GTgt* __restrict__ gtgt,
-                                                            const int* __restrict__ gsrc, const int* __restrict__ gint,
-                                                            const typename Cplx<T>::type* __restrict__ Tspec,
-                                                            const typename Cplx<T>::type* __restrict__ spec,
-                                                            typename Cplx<T>::type* __restrict__ acc) {
+// Translation by tiles: a block owns BC frequency bins (one TR_ROW-byte row
+// per spectrum) of a run of tiles (each up to TR_TILE neighbouring target
+// cubes of one level).  It stages the 343 T rows of those bins in LDS once,
+// then per tile the rows of every source spectrum the tile's targets
+// interact with.  Four lanes share a target, each owning 16 bytes of the
+// row (two FP32 bins or one FP64 bin), and sum its interaction list from
+// LDS in interaction-list order; the packed interaction word holds the two
+// row byte offsets, so an address is one AND/OR or shift/OR away.
+// Lane share of a TR_ROW-byte row: LPT lanes per target, each reading
+// TR_ROW / LPT bytes (NB bins).
+template <typename T, int LPT>
+struct TrVec;
+template <>
+struct TrVec<float, 8> {
+   using type = float2;   // bin l
+   static constexpr int NB = 1;
+};
+template <>
+struct TrVec<float, 4> {
+   using type = float4;   // bins 2l and 2l+1
+   static constexpr int NB = 2;
+};
+template <>
+struct TrVec<double, 4> {
+   using type = double2;  // bin l
+   static constexpr int NB = 1;
+};
+
+template <typename T, int LPT>
+__device__ __forceinline__ void tr_mac(const typename TrVec<T, LPT>::type& t, const typename TrVec<T, LPT>::type& v,
+                                       T* re, T* im) {
+   // (t * v) accumulated per bin as four FMAs
+   re[0] = fma(t.x, v.x, re[0]);
+   re[0] = fma(-t.y, v.y, re[0]);
+   im[0] = fma(t.x, v.y, im[0]);
+   im[0] = fma(t.y, v.x, im[0]);
+   if constexpr (TrVec<T, LPT>::NB == 2) {
+      re[1] = fma(t.z, v.z, re[1]);
+      re[1] = fma(-t.w, v.w, re[1]);
+      im[1] = fma(t.z, v.w, im[1]);
+      im[1] = fma(t.w, v.z, im[1]);
+   }
+}
+
+template <typename T, int BC, int LPT>
+__global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restrict__ tasks,
+                                                           const TrTile* __restrict__ tiles,
+                                                           const GTgt* __restrict__ gtgt,
+                                                           const int* __restrict__ gsrc, const int* __restrict__ gint,
+                                                           const typename Cplx<T>::type* __restrict__ Tspec,
+                                                           const typename Cplx<T>::type* __restrict__ spec,
+                                                           typename Cplx<T>::type* __restrict__ acc) {
    using C2 = typename Cplx<T>::type;
+   using V = typename TrVec<T, LPT>::type;
+   constexpr int NB = TrVec<T, LPT>::NB;
+   static_assert(BC * sizeof(C2) == TR_ROW && NB * sizeof(C2) * LPT == TR_ROW, "translation row layout");
    extern __shared__ __align__(16) unsigned char smem[];
    const int4 tk = tasks[blockIdx.x];   // first tile, tile count, first bin
-   const int b0 = tk.z, tid = threadIdx.x, nthr = TR_TILE * BC;
+   const int b0 = tk.z, tid = threadIdx.x, nthr = TR_TILE * LPT;
    const int pad = tiles[tk.x].pad;
    const long long t_off = tiles[tk.x].t_off;
-   C2* Ts = (C2*)smem;        // 343 * BC
-   C2* Ss = Ts + 343 * BC;    // sources * BC
+   C2* Ts = (C2*)smem;        // 343 rows
+   C2* Ss = Ts + 343 * BC;    // source rows
    // staging: eight loads in flight per thread before their LDS stores
    constexpr int U = 8;
    for (int i0 = tid; i0 < 343 * BC; i0 += U * nthr) {
@@ -370,7 +428,11 @@ __global__ void __launch_bounds__(TR_TILE * BC) k_translate(const int4* __restri
       for (int u = 0; u < U; u++)
          if (i0 + u * nthr < 343 * BC) Ts[i0 + u * nthr] = v[u];
    }
-   const int j = tid / BC, b = tid - j * BC;
+   const int j = tid / LPT, l = tid % LPT;
+   const int bb = l * NB;   // lane's first bin in the row
+   // lane's bytes of T row 0 and source row 0
+   const unsigned char* Tl = smem + l * (TR_ROW / LPT);
+   const unsigned char* Sl = Tl + 343 * TR_ROW;
    for (int ti = tk.x; ti < tk.x + tk.y; ti++) {
       const TrTile tl = tiles[ti];
       __syncthreads();   // the previous tile's sums are done with Ss
@@ -385,46 +447,54 @@ __global__ void __launch_bounds__(TR_TILE * BC) k_translate(const int4* __restri
          C2 v[U];
 #pragma unroll
          for (int u = 0; u < U; u++) {
-            const int i = i0 + u * nthr, bb = i - (i / BC) * BC;
+            const int i = i0 + u * nthr, b = i - (i / BC) * BC;
             v[u].x = 0;
             v[u].y = 0;
-            if (i < ns && b0 + bb < pad) v[u] = spec[(long long)so[u] + b0 + bb];
+            if (i < ns && b0 + b < pad) v[u] = spec[(long long)so[u] + b0 + b];
          }
 #pragma unroll
          for (int u = 0; u < U; u++)
             if (i0 + u * nthr < ns) Ss[i0 + u * nthr] = v[u];
       }
       __syncthreads();
-      if (j < tl.nt && b0 + b < pad) {
+      if (j < tl.nt && b0 + bb < pad) {
          const GTgt tt = gtgt[tl.tgt0 + j];
          const int* it = gint + tt.int_off;
-         T re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+         T re0[NB] = {}, im0[NB] = {}, re1[NB] = {}, im1[NB] = {};
          int e = 0;
+         // the next four interaction words load while the current four are
+         // summed (a list is followed by at least four readable words)
+         int4 q = *(const int4*)it;
          for (; e + 3 < tt.nint; e += 4) {
-            const int v0 = it[e], v1 = it[e + 1], v2 = it[e + 2], v3 = it[e + 3];
-            const C2 t0 = Ts[(v0 & 511) * BC + b], s0 = Ss[(v0 >> 9) * BC + b];
-            const C2 t1 = Ts[(v1 & 511) * BC + b], s1 = Ss[(v1 >> 9) * BC + b];
-            const C2 t2 = Ts[(v2 & 511) * BC + b], s2 = Ss[(v2 >> 9) * BC + b];
-            const C2 t3 = Ts[(v3 & 511) * BC + b], s3 = Ss[(v3 >> 9) * BC + b];
-            re0 += t0.x * s0.x - t0.y * s0.y;
-            im0 += t0.x * s0.y + t0.y * s0.x;
-            re1 += t1.x * s1.x - t1.y * s1.y;
-            im1 += t1.x * s1.y + t1.y * s1.x;
-            re0 += t2.x * s2.x - t2.y * s2.y;
-            im0 += t2.x * s2.y + t2.y * s2.x;
-            re1 += t3.x * s3.x - t3.y * s3.y;
-            im1 += t3.x * s3.y + t3.y * s3.x;
+            int4 qn = *(const int4*)(it + e + 4);
+            const unsigned u0 = (unsigned)q.x, u1 = (unsigned)q.y, u2 = (unsigned)q.z, u3 = (unsigned)q.w;
+            const V t0 = *(const V*)(Tl + (u0 & 0xffffu)), s0 = *(const V*)(Sl + (u0 >> 16));
+            const V t1 = *(const V*)(Tl + (u1 & 0xffffu)), s1 = *(const V*)(Sl + (u1 >> 16));
+            const V t2 = *(const V*)(Tl + (u2 & 0xffffu)), s2 = *(const V*)(Sl + (u2 >> 16));
+            const V t3 = *(const V*)(Tl + (u3 & 0xffffu)), s3 = *(const V*)(Sl + (u3 >> 16));
+            tr_mac<T, LPT>(t0, s0, re0, im0);
+            tr_mac<T, LPT>(t1, s1, re1, im1);
+            tr_mac<T, LPT>(t2, s2, re0, im0);
+            tr_mac<T, LPT>(t3, s3, re1, im1);
+            // consumed here, after the sums: keeps the read-ahead load from
+            // being moved to the top of the next iteration
+            asm volatile("" : "+v"(qn.x), "+v"(qn.y), "+v"(qn.z), "+v"(qn.w));
+            q = qn;
          }
          for (; e < tt.nint; e++) {
-            const int v0 = it[e];
-            const C2 t0 = Ts[(v0 & 511) * BC + b], s0 = Ss[(v0 >> 9) * BC + b];
-            re0 += t0.x * s0.x - t0.y * s0.y;
-            im0 += t0.x * s0.y + t0.y * s0.x;
+            const unsigned u0 = (unsigned)it[e];
+            const V t0 = *(const V*)(Tl + (u0 & 0xffffu)), s0 = *(const V*)(Sl + (u0 >> 16));
+            tr_mac<T, LPT>(t0, s0, re0, im0);
          }
-         C2 out;
-         out.x = re0 + re1;
-         out.y = im0 + im1;
-         acc[tt.acc_off + b0 + b] = out;
+#pragma unroll
+         for (int k = 0; k < NB; k++) {
+            if (b0 + bb + k < pad) {
+               C2 out;
+               out.x = re0[k] + re1[k];
+               out.y = im0[k] + im1[k];
+               acc[tt.acc_off + b0 + bb + k] = out;
+            }
+         }
       }
    }
 }
@@ -440,19 +510,23 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
    T* cs = (T*)(Cc + n * n * H);  // N
    T* sn = cs + N;
    T* h = sn + N;                 // GP3
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+   const int tid = threadIdx.x;
    twiddle_table<T, N>(cs, sn);
    for (int b = tid; b < PAD; b += blockDim.x) S[b] = acc[d.acc_off + b];
    __syncthreads();
    grid_backward<n, T>(S, C1, Cc, h, cs, sn);
    __syncthreads();
    const T* uti = UTI + d.uti_off;
-   for (int k = wave; k < d.r; k += nw) {
+   // w = UTI h: 16 lanes per row, 16 rows per pass (256 threads), so each
+   // lane has few dependent loads in flight per row
+   const int kr = tid >> 4, ql = tid & 15;
+   for (int k = kr; k < d.r; k += 16) {
       const T* row = uti + (long long)k * GP3;
       T a = 0;
-      for (int q = lane; q < GP3; q += 64) a += row[q] * h[q];
-      a = wave_sum(a);
-      if (lane == 0) w[d.w_off + k] = a;
+      for (int q = ql; q < GP3; q += 16) a += row[q] * h[q];
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      if (ql == 0) w[d.w_off + k] = a;
    }
 }
 
@@ -640,7 +714,8 @@ __global__ void __launch_bounds__(256) k_nearT(const LeafDesc* __restrict__ leav
 template <typename T>
 size_t far_source_lds(int n, int maxr) {
    int N = 2 * n - 1, H = N / 2 + 1;
-   return sizeof(typename Cplx<T>::type) * (size_t)(n * n * H + n * N * H) + sizeof(T) * (size_t)(2 * N + n * n * n + 2 * maxr) + 64;
+   return sizeof(typename Cplx<T>::type) * (size_t)(n * n * H + n * N * H) +
+          sizeof(T) * (size_t)(2 * N + n * n * n + 256 + 2 * maxr) + 64;
 }
 template <typename T>
 size_t far_target_lds(int n, int maxr) {
@@ -710,18 +785,20 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
          constexpr bool f32 = std::is_same<T, float>::value;
          constexpr int BC = f32 ? TR_BINS_F32 : TR_BINS_F64;
+         constexpr int LPT = f32 ? TR_LPT_F32 : TR_LPT_F64;
          const size_t lds_max = sizeof(C2) * (size_t)(343 + TR_MAX_SRC) * BC;
          const size_t lds = sizeof(C2) * (size_t)(343 + a.tr_max_src) * BC;
          check_lds(lds_max);
          if (a.tr_max_src > TR_MAX_SRC) throw std::runtime_error("translation tile above its LDS bound");
          static std::once_flag attr;
          std::call_once(attr, [lds_max] {
-            FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC>,
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC, LPT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
          });
          const int nt = f32 ? a.ntask_f : a.ntask_d;
          if (nt)
-            hipLaunchKernelGGL((k_translate<T, BC>), dim3(nt), dim3(TR_TILE * BC), lds, s, f32 ? a.task_f : a.task_d,
+            hipLaunchKernelGGL((k_translate<T, BC, LPT>), dim3(nt), dim3(TR_TILE * LPT), lds, s,
+                               f32 ? a.task_f : a.task_d,
                                a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
          break;
       }

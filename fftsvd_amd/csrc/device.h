@@ -112,22 +112,30 @@ struct TgtPlan {
 };
 
 // Translation tiles.  A block handles TR_BINS_F32 (FP32) or TR_BINS_F64
-// (FP64) frequency bins of up to TR_TILE tree-order neighbouring targets;
-// tiles grow while their source union stays within TR_SRC_BUDGET, so the
-// block's LDS ((343 + sources) spectra of 64 bytes of bins) stays at
-// 38 KiB and four blocks share a CU.  TR_MAX_SRC bounds a single target's
-// sources (LDS limit 160 KiB).
+// (FP64) frequency bins, one 64-byte row per spectrum, of up to TR_TILE
+// tree-order neighbouring targets; tiles grow while their source union
+// stays within TR_SRC_BUDGET, so the block's LDS ((343 + sources) rows)
+// stays under 62 KiB and two blocks (32 FP32 waves) share a CU.  Larger
+// tiles reuse each staged source row for more targets: on the 100k-panel
+// globule, 32-target tiles ran the FP32 translation in 0.30 ms, 64 in
+// 0.23 ms and 128 in 0.20 ms.  TR_MAX_SRC bounds a single target's sources
+// (local source row offsets must fit 16 bits).
 constexpr int TR_BINS_F32 = 8, TR_BINS_F64 = 4;
-constexpr int TR_TILE = 32;
-constexpr int TR_SRC_BUDGET = 256;
+// lanes per target: FP32 one bin (8 bytes) per lane, FP64 one bin (16 bytes)
+constexpr int TR_LPT_F32 = 8, TR_LPT_F64 = 4;
+constexpr int TR_ROW = 64;
+constexpr int TR_TILE = 128;
+constexpr int TR_SRC_BUDGET = 640;
 // Tiles per translation block (they share the block's staged T spectra).
 constexpr int TR_RUN = 3;
-constexpr int TR_MAX_SRC = 2216;
+constexpr int TR_MAX_SRC = 1023;
 
 // Translation tile: nt <= TR_TILE consecutive (tree-order) target cubes of one
 // level, GTgt entries at tgt0, their union of nsrc source spectra
 // (absolute offsets at src_off).  Each target's interactions are packed as
-// (T offset code | local source index << 9) in interaction-list order.
+// byte offsets of LDS rows, (T code * TR_ROW) | (local source * TR_ROW) << 16,
+// in interaction-list order, from a 16-byte aligned int_off; every list is
+// followed by at least four readable words (the next list or padding).
 struct TrTile {
    int nt, tgt0, nsrc, src_off, pad, pad_;
    long long t_off;

@@ -332,10 +332,12 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
                const TgtPlan& tp = tgt[tbase + k];
                GTgt g;
                g.acc_off = tp.acc_off;
+               while (gint.size() % 4) gint.push_back(0);
                g.int_off = (int)gint.size();
                g.nint = tp.nint;
                for (int e = 0; e < tp.nint; e++)
-                  gint.push_back(itoff[tp.int_off + e] | (loc[slot_of(isrc[tp.int_off + e])] << 9));
+                  gint.push_back((int)((unsigned)(itoff[tp.int_off + e] * TR_ROW) |
+                                       ((unsigned)(loc[slot_of(isrc[tp.int_off + e])] * TR_ROW) << 16)));
                gtgt.push_back(g);
             }
             tiles.push_back(tl);
@@ -369,6 +371,8 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          tb += (long long)lv.T64.n;
          sbase += (long long)lv.nsrc * lv.pad;
       }
+      // words the translation kernel may read ahead past the last list
+      for (int k = 0; k < 8; k++) gint.push_back(0);
       P.ntask_f = (int)tf.size();
       P.ntask_d = (int)td.size();
       P.task_f.upload(tf, s);
