@@ -42,10 +42,29 @@ __device__ __forceinline__ T wave_sum(T v) {
 // (xcd_block_map): block b runs logical block map[b], and each XCD gets a
 // contiguous tree-order range of about equal work, so neighbouring work
 // shares one L2.  -1 marks padding blocks.
+// The threads that share one cube's work: a whole block (__syncthreads
+// between phases) or one wavefront.  A wavefront's LDS accesses are
+// processed in issue order, so between its phases only the compiler's
+// reordering has to be fenced.
+struct BlockGroup {
+   int tid, nt;
+   __device__ void sync() const { __syncthreads(); }
+};
+struct WaveGroup {
+   int tid;
+   static constexpr int nt = 64;
+   __device__ void sync() const {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+   }
+};
+__device__ __forceinline__ BlockGroup block_group() { return BlockGroup{(int)threadIdx.x, (int)blockDim.x}; }
+
 // e^{-2 pi i m / N} table
-template <typename T, int N>
-__device__ __forceinline__ void twiddle_table(T* cs, T* sn) {
-   for (int m = threadIdx.x; m < N; m += blockDim.x) {
+template <typename T, int N, typename G = BlockGroup>
+__device__ __forceinline__ void twiddle_table(T* cs, T* sn, const G& gr = block_group()) {
+   for (int m = gr.tid; m < N; m += gr.nt) {
       double s, c;
       sincospi(2.0 * m / N, &s, &c);
       cs[m] = (T)c;
@@ -58,11 +77,11 @@ __device__ __forceinline__ void twiddle_table(T* cs, T* sn) {
 // (reference FFT_forwardGridTransform, FFT.c:66-113).  g: n^3 in LDS,
 // out: N*N*H complex in global memory.  Work arrays A (n*n*H) and
 // B (n*N*H) in LDS.
-template <int n, typename T>
+template <int n, typename T, typename G = BlockGroup>
 __device__ void grid_forward(const T* g, typename Cplx<T>::type* A, typename Cplx<T>::type* B,
-                             typename Cplx<T>::type* out, const T* cs, const T* sn) {
+                             typename Cplx<T>::type* out, const T* cs, const T* sn, const G& gr = block_group()) {
    constexpr int N = 2 * n - 1, H = N / 2 + 1, OFF = n / 2;
-   const int tid = threadIdx.x, nt = blockDim.x;
+   const int tid = gr.tid, nt = gr.nt;
    // along k
    for (int e = tid; e < n * n * H; e += nt) {
       int c = e % H, ij = e / H;
@@ -77,7 +96,7 @@ __device__ void grid_forward(const T* g, typename Cplx<T>::type* A, typename Cpl
       A[e].x = re;
       A[e].y = im;
    }
-   __syncthreads();
+   gr.sync();
    // along j
    for (int e = tid; e < n * N * H; e += nt) {
       int c = e % H, b = (e / H) % N, i = e / (H * N);
@@ -92,7 +111,7 @@ __device__ void grid_forward(const T* g, typename Cplx<T>::type* A, typename Cpl
       B[e].x = re;
       B[e].y = im;
    }
-   __syncthreads();
+   gr.sync();
    // along i
    for (int e = tid; e < N * N * H; e += nt) {
       int c = e % H, b = (e / H) % N, a = e / (H * N);
@@ -113,11 +132,12 @@ __device__ void grid_forward(const T* g, typename Cplx<T>::type* A, typename Cpl
 // grid at offset n/2 and scaled by 1/N^3 (reference
 // FFT_backwardGridTransform, FFT.c:117-150).  S in LDS (N*N*H),
 // C1 (n*N*H), C2 (n*n*H) in LDS, h (n^3) in LDS.
-template <int n, typename T>
+template <int n, typename T, typename G = BlockGroup>
 __device__ void grid_backward(const typename Cplx<T>::type* S, typename Cplx<T>::type* C1,
-                              typename Cplx<T>::type* C2, T* h, const T* cs, const T* sn) {
+                              typename Cplx<T>::type* C2, T* h, const T* cs, const T* sn,
+                              const G& gr = block_group()) {
    constexpr int N = 2 * n - 1, H = N / 2 + 1, OFF = n / 2;
-   const int tid = threadIdx.x, nt = blockDim.x;
+   const int tid = gr.tid, nt = gr.nt;
    for (int e = tid; e < n * N * H; e += nt) {
       int c = e % H, b = (e / H) % N, i = e / (H * N);
       T re = 0, im = 0;
@@ -131,7 +151,7 @@ __device__ void grid_backward(const typename Cplx<T>::type* S, typename Cplx<T>:
       C1[e].x = re;
       C1[e].y = im;
    }
-   __syncthreads();
+   gr.sync();
    for (int e = tid; e < n * n * H; e += nt) {
       int c = e % H, j = (e / H) % n, i = e / (H * n);
       T re = 0, im = 0;
@@ -145,7 +165,7 @@ __device__ void grid_backward(const typename Cplx<T>::type* S, typename Cplx<T>:
       C2[e].x = re;
       C2[e].y = im;
    }
-   __syncthreads();
+   gr.sync();
    const T scale = (T)(1.0 / ((double)N * N * N));
    for (int e = tid; e < n * n * n; e += nt) {
       int k = e % n, ij = e / n;
@@ -226,69 +246,78 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
    return (a0 + a1) + (a2 + a3);
 }
 
+// Projection: one chunk of a source cube's panels per wavefront (four per
+// block), tpart[out_off + e] = sum_i VT[e, i] xs[i] and
+// tpart[out_off + r + e] = sum_i VT[e, i] xd[i] over the chunk's columns.
+// Lane (g, e) owns row e and the columns g, g + G, ... (G = 64 / rows of the
+// row chunk); the G partial sums of a row are added in group order.
 template <typename T>
-__global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ chunks, const T* __restrict__ VTc,
-                                                 const double* __restrict__ xs, const double* __restrict__ xd,
-                                                 T* __restrict__ tpart) {
+__global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ chunks, int nchunk,
+                                                 const T* __restrict__ VTc, const double* __restrict__ xs,
+                                                 const double* __restrict__ xd, T* __restrict__ tpart, int wave_lds) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const ProjChunk c = chunks[blockIdx.x];
-   const int tid = threadIdx.x, m = c.ncol, E = c.r;
-   T* xl = (T*)smem;      // 2*m
-   T* part = xl + 2 * m;  // 2*256
-   for (int i = tid; i < m; i += 256) {
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ci = blockIdx.x * 4 + wave;
+   if (ci >= nchunk) return;
+   const WaveGroup gr{lane};
+   const ProjChunk c = chunks[ci];
+   const int m = c.ncol, E = c.r;
+   T* part = (T*)(smem + (size_t)wave * wave_lds);   // 2*64
+   T* xl = part + 128;                                // 2*m
+   for (int i = lane; i < m; i += 64) {
       xl[i] = (T)xs[c.p0 + i];
       xl[m + i] = (T)xd[c.p0 + i];
    }
-   __syncthreads();
+   gr.sync();
    const T* blk = VTc + c.vt_off;
-   for (int e0 = 0; e0 < E; e0 += 256) {
-      const int Ec = min(256, E - e0), G = 256 / Ec;
-      const int g = tid / Ec, e = e0 + tid - g * Ec;
+   for (int e0 = 0; e0 < E; e0 += 64) {
+      const int Ec = min(64, E - e0), G = 64 / Ec;
+      const int g = lane / Ec, e = e0 + lane - g * Ec;
       T as = 0, ad = 0;
       if (g < G) {
          const T* colp = blk + e;
          const long long step = (long long)G * E;
          T s0 = 0, s1 = 0, d0 = 0, d1 = 0;
          int i = g;
-         for (; i + 3 * G < m; i += 4 * G) {
-            const T v0 = colp[(long long)i * E], v1 = colp[(long long)i * E + step];
-            const T v2 = colp[(long long)i * E + 2 * step], v3 = colp[(long long)i * E + 3 * step];
-            s0 += v0 * xl[i];
-            d0 += v0 * xl[m + i];
-            s1 += v1 * xl[i + G];
-            d1 += v1 * xl[m + i + G];
-            s0 += v2 * xl[i + 2 * G];
-            d0 += v2 * xl[m + i + 2 * G];
-            s1 += v3 * xl[i + 3 * G];
-            d1 += v3 * xl[m + i + 3 * G];
+         // eight columns in flight per lane
+         for (; i + 7 * G < m; i += 8 * G) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = colp[(long long)i * E + u * step];
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) {
+               s0 += v[u] * xl[i + u * G];
+               d0 += v[u] * xl[m + i + u * G];
+               s1 += v[u + 1] * xl[i + (u + 1) * G];
+               d1 += v[u + 1] * xl[m + i + (u + 1) * G];
+            }
          }
          for (; i < m; i += G) {
-            T v = colp[(long long)i * E];
+            const T v = colp[(long long)i * E];
             s0 += v * xl[i];
             d0 += v * xl[m + i];
          }
          as = s0 + s1;
          ad = d0 + d1;
       }
-      part[tid] = as;
-      part[256 + tid] = ad;
-      __syncthreads();
-      if (tid < Ec) {
+      part[lane] = as;
+      part[64 + lane] = ad;
+      gr.sync();
+      if (lane < Ec) {
          T ss = 0, sd = 0;
          for (int gg = 0; gg < G; gg++) {
-            ss += part[gg * Ec + tid];
-            sd += part[256 + gg * Ec + tid];
+            ss += part[gg * Ec + lane];
+            sd += part[64 + gg * Ec + lane];
          }
-         tpart[c.out_off + e0 + tid] = ss;
-         tpart[c.out_off + E + e0 + tid] = sd;
+         tpart[c.out_off + e0 + lane] = ss;
+         tpart[c.out_off + E + e0 + lane] = sd;
       }
-      __syncthreads();
+      gr.sync();
    }
 }
 
-template <int n, typename T>
+template <int n, typename T, typename G>
 __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const T* __restrict__ PV,
-                            typename Cplx<T>::type* __restrict__ spec, unsigned char* smem) {
+                            typename Cplx<T>::type* __restrict__ spec, unsigned char* smem, const G& gr) {
    using C2 = typename Cplx<T>::type;
    constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, GP3 = FarSizes<n>::GP3;
    C2* A = (C2*)smem;            // n*n*H
@@ -296,58 +325,109 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
    T* cs = (T*)(B + n * N * H);  // N
    T* sn = cs + N;               // N
    T* g = sn + N;                // GP3
-   T* part = g + GP3;            // KG * GP3 partial grid values
-   T* t = part + 256;            // 2r: ts then td
-   const int tid = threadIdx.x, r = d.r;
-   twiddle_table<T, N>(cs, sn);
-   for (int k = tid; k < 2 * r; k += blockDim.x) {
+   T* part = g + GP3;            // partial grid values (block groups)
+   const int tid = gr.tid, nt = gr.nt, r = d.r;
+   T* t = part + (nt > GP3 ? nt : 0);   // 2r: ts then td
+   twiddle_table<T, N>(cs, sn, gr);
+   // sum of the chunk partials in chunk order, eight loads in flight
+   for (int k = tid; k < 2 * r; k += nt) {
+      const T* pp = tpart + d.part_off + k;
       T a = 0;
-      for (int j = 0; j < d.nchunk; j++) a += tpart[d.part_off + j * 2 * r + k];
+      int j = 0;
+      for (; j + 7 < d.nchunk; j += 8) {
+         T v[8];
+#pragma unroll
+         for (int u = 0; u < 8; u++) v[u] = pp[(long long)(j + u) * 2 * r];
+#pragma unroll
+         for (int u = 0; u < 8; u++) a += v[u];
+      }
+      for (; j < d.nchunk; j++) a += pp[(long long)j * 2 * r];
       t[k] = a;
    }
-   __syncthreads();
+   gr.sync();
    // g = PV t over the 2r rows (PVs then PVd, k-major): thread (kg, q) sums
-   // the rows k = kg, kg + KG, ... so all 256 threads stream PV
-   constexpr int KG = GP3 >= 256 ? 1 : 256 / GP3;
+   // the rows k = kg, kg + KG, ... so every thread streams PV
    const T* pv = PV + d.pv_off;
    const int R2 = 2 * r;
-   for (int q0 = 0; q0 < GP3; q0 += 256 / KG) {
-      const int kg = tid / (256 / KG), q = q0 + tid - kg * (256 / KG);
-      T a0 = 0, a1 = 0;
-      if (kg < KG && q < GP3) {
+   const int KG = nt >= GP3 ? nt / GP3 : 1, QW = nt / KG;
+   for (int q0 = 0; q0 < GP3; q0 += QW) {
+      const int kg = tid / QW, q = q0 + tid - kg * QW;
+      T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      if (q < GP3) {
+         // eight rows in flight per thread
          int k = kg;
-         for (; k + KG < R2; k += 2 * KG) {
-            a0 += pv[(long long)k * GP3 + q] * t[k];
-            a1 += pv[(long long)(k + KG) * GP3 + q] * t[k + KG];
+         for (; k + 7 * KG < R2; k += 8 * KG) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = pv[(long long)(k + u * KG) * GP3 + q];
+            a0 += v[0] * t[k];
+            a1 += v[1] * t[k + KG];
+            a2 += v[2] * t[k + 2 * KG];
+            a3 += v[3] * t[k + 3 * KG];
+            a0 += v[4] * t[k + 4 * KG];
+            a1 += v[5] * t[k + 5 * KG];
+            a2 += v[6] * t[k + 6 * KG];
+            a3 += v[7] * t[k + 7 * KG];
          }
-         if (k < R2) a0 += pv[(long long)k * GP3 + q] * t[k];
+         for (; k < R2; k += KG) a0 += pv[(long long)k * GP3 + q] * t[k];
       }
-      if (kg < KG) part[kg * (256 / KG) + (q - q0)] = a0 + a1;
-      __syncthreads();
-      if (tid < 256 / KG && q0 + tid < GP3) {
-         T sum = 0;
-         for (int gg = 0; gg < KG; gg++) sum += part[gg * (256 / KG) + tid];
-         g[q0 + tid] = sum;
+      const T sum = (a0 + a1) + (a2 + a3);
+      if (KG == 1) {
+         if (q < GP3) g[q] = sum;
+      } else {
+         part[kg * QW + (q - q0)] = sum;
+         gr.sync();
+         if (tid < QW && q0 + tid < GP3) {
+            T acc = 0;
+            for (int gg = 0; gg < KG; gg++) acc += part[gg * QW + tid];
+            g[q0 + tid] = acc;
+         }
       }
-      __syncthreads();
+      gr.sync();
    }
-   grid_forward<n, T>(g, A, B, spec + d.spec_off, cs, sn);
+   grid_forward<n, T>(g, A, B, spec + d.spec_off, cs, sn, gr);
 }
 
+// LDS of one source cube's work (block groups of nt threads; a wave group
+// of 64 lanes uses no partial-sum rows when GP3 = 64)
 template <typename T>
-__global__ void __launch_bounds__(256) k_source(const SrcPlan* __restrict__ src, const T* __restrict__ tpart,
+__host__ __device__ constexpr size_t source_cube_lds(int n, int maxr, int nt) {
+   return sizeof(typename Cplx<T>::type) * (size_t)(n * n * ((2 * n - 1) / 2 + 1) + n * (2 * n - 1) * ((2 * n - 1) / 2 + 1)) +
+          sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n + (nt > n * n * n ? nt : 0) + 2 * maxr) + 64;
+}
+
+// Source cubes with more than four grid points per side, one block of
+// SRC_BIG_NT threads each (these cubes are few, so each gets many loads in
+// flight).
+constexpr int SRC_BIG_NT = 1024;
+template <typename T>
+__global__ void __launch_bounds__(SRC_BIG_NT) k_source(const SrcPlan* __restrict__ src, const T* __restrict__ tpart,
                                                 const T* __restrict__ PV, typename Cplx<T>::type* __restrict__ spec) {
    extern __shared__ __align__(16) unsigned char smem[];
    const SrcPlan d = src[blockIdx.x];
+   const BlockGroup gr = block_group();
    switch (d.n) {
-      case 4: source_body<4, T>(d, tpart, PV, spec, smem); break;
-      case 5: source_body<5, T>(d, tpart, PV, spec, smem); break;
-      case 6: source_body<6, T>(d, tpart, PV, spec, smem); break;
-      case 7: source_body<7, T>(d, tpart, PV, spec, smem); break;
-      case 8: source_body<8, T>(d, tpart, PV, spec, smem); break;
-      case 9: source_body<9, T>(d, tpart, PV, spec, smem); break;
+      case 5: source_body<5, T>(d, tpart, PV, spec, smem, gr); break;
+      case 6: source_body<6, T>(d, tpart, PV, spec, smem, gr); break;
+      case 7: source_body<7, T>(d, tpart, PV, spec, smem, gr); break;
+      case 8: source_body<8, T>(d, tpart, PV, spec, smem, gr); break;
+      case 9: source_body<9, T>(d, tpart, PV, spec, smem, gr); break;
       default: break;
    }
+}
+
+// Source cubes of four grid points per side (n^3 = 64, one per lane): one
+// cube per wavefront, four per block, each wave in its own LDS slice of
+// wave_lds bytes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_source4(const SrcPlan* __restrict__ src, int nsrc, const T* __restrict__ tpart,
+                                                 const T* __restrict__ PV, typename Cplx<T>::type* __restrict__ spec,
+                                                 int wave_lds) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int wave = threadIdx.x >> 6, i = blockIdx.x * 4 + wave;
+   if (i >= nsrc) return;
+   const SrcPlan d = src[i];
+   source_body<4, T>(d, tpart, PV, spec, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
 }
 
 // Translation by tiles: a block owns BC frequency bins (one TR_ROW-byte row
@@ -499,9 +579,10 @@ __global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restr
    }
 }
 
-template <int n, typename T>
+template <int n, typename T, typename G>
 __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
-                            const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w, unsigned char* smem) {
+                            const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w, unsigned char* smem,
+                            const G& gr) {
    using C2 = typename Cplx<T>::type;
    constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, PAD = FarSizes<n>::PAD, GP3 = FarSizes<n>::GP3;
    C2* S = (C2*)smem;             // PAD
@@ -510,40 +591,77 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
    T* cs = (T*)(Cc + n * n * H);  // N
    T* sn = cs + N;
    T* h = sn + N;                 // GP3
-   const int tid = threadIdx.x;
-   twiddle_table<T, N>(cs, sn);
-   for (int b = tid; b < PAD; b += blockDim.x) S[b] = acc[d.acc_off + b];
-   __syncthreads();
-   grid_backward<n, T>(S, C1, Cc, h, cs, sn);
-   __syncthreads();
+   const int tid = gr.tid, nt = gr.nt;
+   twiddle_table<T, N>(cs, sn, gr);
+   for (int b = tid; b < PAD; b += nt) S[b] = acc[d.acc_off + b];
+   gr.sync();
+   grid_backward<n, T>(S, C1, Cc, h, cs, sn, gr);
+   gr.sync();
    const T* uti = UTI + d.uti_off;
-   // w = UTI h: 16 lanes per row, 16 rows per pass (256 threads), so each
-   // lane has few dependent loads in flight per row
-   const int kr = tid >> 4, ql = tid & 15;
-   for (int k = kr; k < d.r; k += 16) {
-      const T* row = uti + (long long)k * GP3;
-      T a = 0;
-      for (int q = ql; q < GP3; q += 16) a += row[q] * h[q];
+   // w = UTI h: 16 lanes per row, nt/16 rows per pass, two rows per lane in
+   // flight
+   const int kr = tid >> 4, ql = tid & 15, RP = nt >> 4;
+   for (int k = kr; k < d.r; k += 2 * RP) {
+      const T* row0 = uti + (long long)k * GP3;
+      const T* row1 = row0 + (long long)RP * GP3;
+      const bool two = k + RP < d.r;
+      T a = 0, b = 0;
 #pragma unroll
-      for (int o = 8; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      if (ql == 0) w[d.w_off + k] = a;
+      for (int q = ql; q < GP3; q += 16) {
+         a += row0[q] * h[q];
+         if (two) b += row1[q] * h[q];
+      }
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+         a += __shfl_xor(a, o, 64);
+         b += __shfl_xor(b, o, 64);
+      }
+      if (ql == 0) {
+         w[d.w_off + k] = a;
+         if (two) w[d.w_off + k + RP] = b;
+      }
    }
 }
 
+// LDS of one target cube's work
 template <typename T>
-__global__ void __launch_bounds__(256) k_target(const TgtPlan* __restrict__ tgt, const T* __restrict__ UTI,
+__host__ __device__ constexpr size_t target_cube_lds(int n) {
+   return sizeof(typename Cplx<T>::type) *
+              (size_t)((2 * n - 1) * (2 * n - 1) * ((2 * n - 1) / 2 + 1) + n * (2 * n - 1) * ((2 * n - 1) / 2 + 1) +
+                       n * n * ((2 * n - 1) / 2 + 1)) +
+          sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n) + 64;
+}
+
+// Target cubes with more than four grid points per side, one block of
+// TGT_BIG_NT threads each.
+constexpr int TGT_BIG_NT = 1024;
+template <typename T>
+__global__ void __launch_bounds__(TGT_BIG_NT) k_target(const TgtPlan* __restrict__ tgt, const T* __restrict__ UTI,
                                                 const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w) {
    extern __shared__ __align__(16) unsigned char smem[];
    const TgtPlan d = tgt[blockIdx.x];
+   const BlockGroup gr = block_group();
    switch (d.n) {
-      case 4: target_body<4, T>(d, UTI, acc, w, smem); break;
-      case 5: target_body<5, T>(d, UTI, acc, w, smem); break;
-      case 6: target_body<6, T>(d, UTI, acc, w, smem); break;
-      case 7: target_body<7, T>(d, UTI, acc, w, smem); break;
-      case 8: target_body<8, T>(d, UTI, acc, w, smem); break;
-      case 9: target_body<9, T>(d, UTI, acc, w, smem); break;
+      case 5: target_body<5, T>(d, UTI, acc, w, smem, gr); break;
+      case 6: target_body<6, T>(d, UTI, acc, w, smem, gr); break;
+      case 7: target_body<7, T>(d, UTI, acc, w, smem, gr); break;
+      case 8: target_body<8, T>(d, UTI, acc, w, smem, gr); break;
+      case 9: target_body<9, T>(d, UTI, acc, w, smem, gr); break;
       default: break;
    }
+}
+
+// Target cubes of four grid points per side: one cube per wavefront, four
+// per block, each wave in its own LDS slice of wave_lds bytes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_target4(const TgtPlan* __restrict__ tgt, int ntgt, const T* __restrict__ UTI,
+                                                 const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w,
+                                                 int wave_lds) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int wave = threadIdx.x >> 6, i = blockIdx.x * 4 + wave;
+   if (i >= ntgt) return;
+   const TgtPlan d = tgt[i];
+   target_body<4, T>(d, UTI, acc, w, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
 }
 
 // Leaf output: y = D_s xs + D_d xd (interleaved 2p x m block) + sum over the
@@ -748,37 +866,33 @@ void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const i
 }
 
 template <typename T>
-size_t source_lds(int maxn, int maxr) {
-   size_t m = 0;
-   for (int n = 4; n <= maxn; n++) m = std::max(m, far_source_lds<T>(n, maxr));
-   return m;
-}
-template <typename T>
-size_t target_lds(int maxn) {
-   size_t m = 0;
-   for (int n = 4; n <= maxn; n++) m = std::max(m, far_target_lds<T>(n, 0));
-   return m;
-}
-
-template <typename T>
 void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
    using C2 = typename Cplx<T>::type;
    if (a.max_n > 9) throw std::runtime_error("grid points per side above 9");
    switch (stage) {
       case 0: {
-         size_t lds = sizeof(T) * (size_t)(2 * a.proj_maxcols + 512) + 64;
-         check_lds(lds);
+         const int wl = (int)((sizeof(T) * (size_t)(2 * a.proj_maxcols + 128) + 15) / 16 * 16);
+         check_lds(4 * (size_t)wl);
          if (a.nchunk)
-            hipLaunchKernelGGL((k_project<T>), dim3(a.nchunk), dim3(256), lds, s, a.chunks, (const T*)a.VTc, a.xs,
-                               a.xd, (T*)a.tpart);
+            hipLaunchKernelGGL((k_project<T>), dim3((a.nchunk + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.chunks,
+                               a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart, wl);
          break;
       }
       case 1: {
-         size_t lds = source_lds<T>(a.max_n, a.maxr_src);
-         check_lds(lds);
-         if (a.nsrc)
-            hipLaunchKernelGGL((k_source<T>), dim3(a.nsrc), dim3(256), lds, s, a.src, (const T*)a.tpart,
+         if (a.nsrc_big) {
+            size_t lds = 0;
+            for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, source_cube_lds<T>(n, a.maxr_src, SRC_BIG_NT));
+            check_lds(lds);
+            hipLaunchKernelGGL((k_source<T>), dim3(a.nsrc_big), dim3(SRC_BIG_NT), lds, s, a.src, (const T*)a.tpart,
                                (const T*)a.PV, (C2*)a.spec);
+         }
+         const int n4 = a.nsrc - a.nsrc_big;
+         if (n4) {
+            const int wl = (int)((source_cube_lds<T>(4, a.maxr_src, 64) + 15) / 16 * 16);
+            check_lds(4 * (size_t)wl);
+            hipLaunchKernelGGL((k_source4<T>), dim3((n4 + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.src + a.nsrc_big, n4,
+                               (const T*)a.tpart, (const T*)a.PV, (C2*)a.spec, wl);
+         }
          break;
       }
       case 2: {
@@ -803,11 +917,20 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          break;
       }
       case 3: {
-         size_t lds = target_lds<T>(a.max_n);
-         check_lds(lds);
-         if (a.ntgt)
-            hipLaunchKernelGGL((k_target<T>), dim3(a.ntgt), dim3(256), lds, s, a.tgt, (const T*)a.UTI,
+         if (a.ntgt_big) {
+            size_t lds = 0;
+            for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, target_cube_lds<T>(n));
+            check_lds(lds);
+            hipLaunchKernelGGL((k_target<T>), dim3(a.ntgt_big), dim3(TGT_BIG_NT), lds, s, a.tgt, (const T*)a.UTI,
                                (const C2*)a.acc, (T*)a.w);
+         }
+         const int n4 = a.ntgt - a.ntgt_big;
+         if (n4) {
+            const int wl = (int)((target_cube_lds<T>(4) + 15) / 16 * 16);
+            check_lds(4 * (size_t)wl);
+            hipLaunchKernelGGL((k_target4<T>), dim3((n4 + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.tgt + a.ntgt_big, n4,
+                               (const T*)a.UTI, (const C2*)a.acc, (T*)a.w, wl);
+         }
          break;
       }
       case 4: {

@@ -163,6 +163,7 @@ struct TransDesc {
 // matrix pointers are of the storage type of the launch).
 struct ApplyArgs {
    int nchunk, nsrc, ntgt, nleaf;
+   int nsrc_big, ntgt_big;  // src / tgt entries with n > 4 come first
    int proj_maxcols, maxr_src, max_n, maxm, maxp;
    const ProjChunk* chunks;
    const SrcPlan* src;

@@ -147,7 +147,9 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
 // splits source cubes into projection chunks, and attaches to every leaf
 // the U blocks of the target cubes that contain it.
 static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
-   constexpr int CHUNK = 16384;   // basis entries per projection block
+   // basis entries per projection chunk (one wavefront), at most
+   // PROJ_MAXCOLS panels
+   constexpr int CHUNK = 4096, PROJ_MAXCOLS = 512;
    double tm = now_ms();
    ApplyPlan& P = op.plan;
    const Tree& t = op.tree;
@@ -183,7 +185,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          sp.n = lv.n;
          sp.r = sd.r;
          sp.part_off = tpart_n;
-         const int cpc = std::max(1, CHUNK / sd.r);
+         const int cpc = std::max(1, std::min(PROJ_MAXCOLS, CHUNK / sd.r));
          sp.nchunk = 0;
          for (int c0 = 0; c0 < sd.np; c0 += cpc) {
             ProjChunk c;
@@ -393,8 +395,21 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       P.nmap_leaf = (int)ml.size();
       P.map_leaf.upload(ml, s);
    }
-   P.src.upload(src, s);
-   P.tgt.upload(tgt, s);
+   {
+      // cubes of four grid points per side (one per wavefront) after the
+      // larger ones (one per block): each class is one contiguous launch
+      auto big_first = [](auto v) {
+         std::stable_partition(v.begin(), v.end(), [](const auto& e) { return e.n > 4; });
+         return v;
+      };
+      const std::vector<SrcPlan> src_o = big_first(src);
+      const std::vector<TgtPlan> tgt_o = big_first(tgt);
+      P.nsrc_big = (int)std::count_if(src.begin(), src.end(), [](const SrcPlan& e) { return e.n > 4; });
+      P.ntgt_big = (int)std::count_if(tgt.begin(), tgt.end(), [](const TgtPlan& e) { return e.n > 4; });
+      P.src.upload(src_o, s);
+      P.tgt.upload(tgt_o, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
    P.ublk.upload(ublk, s);
    timing_mark("plan: tiles + maps", tm);
    P.tpart.alloc(std::max(tpart_n, 1));
@@ -1013,6 +1028,8 @@ static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, doubl
    a.nchunk = P.nchunk;
    a.nsrc = P.nsrc;
    a.ntgt = P.ntgt;
+   a.nsrc_big = P.nsrc_big;
+   a.ntgt_big = P.ntgt_big;
    a.nleaf = op.nleaf;
    a.proj_maxcols = P.proj_maxcols;
    a.maxr_src = P.maxr_src;

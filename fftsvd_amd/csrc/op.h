@@ -89,6 +89,7 @@ struct TreeOpStats {
 // absolute offsets, so each stage is one launch.
 struct ApplyPlan {
    int nchunk = 0, nsrc = 0, ntgt = 0, nublk = 0;
+   int nsrc_big = 0, ntgt_big = 0;   // cubes with n > 4, first in src / tgt
    int proj_maxcols = 0, maxr_src = 0, max_n = 4;
    DBuf<ProjChunk> chunks;
    DBuf<SrcPlan> src;
