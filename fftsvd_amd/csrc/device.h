@@ -115,19 +115,20 @@ struct TgtPlan {
 // (FP64) frequency bins, one 64-byte row per spectrum, of up to TR_TILE
 // tree-order neighbouring targets; tiles grow while their source union
 // stays within TR_SRC_BUDGET, so the block's LDS ((343 + sources) rows)
-// stays under 62 KiB and two blocks (32 FP32 waves) share a CU.  Larger
+// stays under 72 KiB and two blocks (32 FP32 waves) share a CU.  Larger
 // tiles reuse each staged source row for more targets: on the 100k-panel
-// globule, 32-target tiles ran the FP32 translation in 0.30 ms, 64 in
-// 0.23 ms and 128 in 0.20 ms.  TR_MAX_SRC bounds a single target's sources
-// (local source row offsets must fit 16 bits).
+// globule (both trees), 32-target tiles ran the FP32 translation in
+// 0.30 ms, 64 in 0.23 ms and 128 in 0.20 ms; one tile per block instead of
+// three (more blocks, shorter tail) then 0.16 ms.  TR_MAX_SRC bounds a
+// single target's sources (local source row offsets must fit 16 bits).
 constexpr int TR_BINS_F32 = 8, TR_BINS_F64 = 4;
 // lanes per target: FP32 one bin (8 bytes) per lane, FP64 one bin (16 bytes)
 constexpr int TR_LPT_F32 = 8, TR_LPT_F64 = 4;
 constexpr int TR_ROW = 64;
 constexpr int TR_TILE = 128;
-constexpr int TR_SRC_BUDGET = 640;
+constexpr int TR_SRC_BUDGET = 800;
 // Tiles per translation block (they share the block's staged T spectra).
-constexpr int TR_RUN = 3;
+constexpr int TR_RUN = 1;
 constexpr int TR_MAX_SRC = 1023;
 
 // Translation tile: nt <= TR_TILE consecutive (tree-order) target cubes of one

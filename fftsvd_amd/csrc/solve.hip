@@ -11,13 +11,14 @@ namespace fsvd {
 // Joined potentials of one region in its tree order (reference
 // SurfaceOperator_joinpotentials, SurfaceOperator.c:51-65):
 // xd = s_phi * x[phi_idx] (double-layer input), xs = s_dphi * x[dphi_idx].
+template <typename XT>
 __global__ void k_join(int n, const int* __restrict__ phi_idx, const int* __restrict__ dphi_idx,
-                       const double* __restrict__ s_phi, const double* __restrict__ s_dphi, const double* __restrict__ x,
+                       const double* __restrict__ s_phi, const double* __restrict__ s_dphi, const XT* __restrict__ x,
                        double* __restrict__ xs, double* __restrict__ xd) {
    int k = blockIdx.x * blockDim.x + threadIdx.x;
    if (k < n) {
-      xd[k] = s_phi[k] * x[phi_idx[k]];
-      xs[k] = s_dphi[k] * x[dphi_idx[k]];
+      xd[k] = s_phi[k] * (double)x[phi_idx[k]];
+      xs[k] = s_dphi[k] * (double)x[dphi_idx[k]];
    }
 }
 
@@ -74,51 +75,70 @@ __device__ __forceinline__ double sum_partials(const double* partial, int np) {
    return h;
 }
 
+// The same sum by one wavefront: lane l adds partials l, l+64, ... in order,
+// then a fixed butterfly; every lane of every block gets the same value.
+__device__ __forceinline__ double wave_sum_partials(const double* partial, int np) {
+   const int lane = threadIdx.x & 63;
+   double h = 0.0;
+   for (int k = lane; k < np; k += 64) h += partial[k];
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+   return h;
+}
+
 // v = w / sqrt(sum(pin)); hout[0] = the norm.
+template <typename VT>
 __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __restrict__ pin, int np,
                                                    double* __restrict__ hout, const double* __restrict__ w,
-                                                   double* __restrict__ v) {
+                                                   VT* __restrict__ v) {
    __shared__ double hs;
-   if (threadIdx.x == 0) hs = sqrt(sum_partials(pin, np));
+   if (threadIdx.x < 64) {
+      const double t = sqrt(wave_sum_partials(pin, np));
+      if (threadIdx.x == 0) hs = t;
+   }
    __syncthreads();
    const double h = hs;
    if (blockIdx.x == 0 && threadIdx.x == 0) hout[0] = h;
    const double inv = 1.0 / h;
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-      v[i] = w[i] * inv;
+      v[i] = (VT)(w[i] * inv);
 }
 
 // Classical Gram-Schmidt pass over m basis vectors (two passes make CGS2).
 // Block (b, jg) owns the contiguous chunk b of the vectors and the columns
-// 4jg..4jg+3: partial[j*nch + b] = chunk sum of V_j . w.
-__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const double* const* __restrict__ V,
+// MD_COLS*jg ..: partial[j*nch + b] = chunk sum of V_j . w.
+constexpr int MD_COLS = 8;
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const VT* const* __restrict__ V,
                                                   const double* __restrict__ w, double* __restrict__ partial) {
-   __shared__ double red[8];
+   __shared__ double red[4][MD_COLS];
    const long long chunk = (n + nch - 1) / nch;
    const long long i0 = blockIdx.x * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
-   const int j0 = blockIdx.y * 4;
-   const int nj = m - j0 < 4 ? m - j0 : 4;
-   const double* v0 = V[j0];
-   const double* v1 = nj > 1 ? V[j0 + 1] : v0;
-   const double* v2 = nj > 2 ? V[j0 + 2] : v0;
-   const double* v3 = nj > 3 ? V[j0 + 3] : v0;
-   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+   const int j0 = blockIdx.y * MD_COLS;
+   const int nj = m - j0 < MD_COLS ? m - j0 : MD_COLS;
+   const VT* v[MD_COLS];
+#pragma unroll
+   for (int u = 0; u < MD_COLS; u++) v[u] = V[j0 + (u < nj ? u : 0)];
+   double acc[MD_COLS];
+#pragma unroll
+   for (int u = 0; u < MD_COLS; u++) acc[u] = 0.0;
    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
       const double wi = w[i];
-      s0 += v0[i] * wi;
-      s1 += v1[i] * wi;
-      s2 += v2[i] * wi;
-      s3 += v3[i] * wi;
+#pragma unroll
+      for (int u = 0; u < MD_COLS; u++) acc[u] += (double)v[u][i] * wi;
    }
-   s0 = block_sum(s0, red);
-   s1 = block_sum(s1, red);
-   s2 = block_sum(s2, red);
-   s3 = block_sum(s3, red);
-   if (threadIdx.x == 0) {
-      partial[(long long)j0 * nch + blockIdx.x] = s0;
-      if (nj > 1) partial[(long long)(j0 + 1) * nch + blockIdx.x] = s1;
-      if (nj > 2) partial[(long long)(j0 + 2) * nch + blockIdx.x] = s2;
-      if (nj > 3) partial[(long long)(j0 + 3) * nch + blockIdx.x] = s3;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+   for (int u = 0; u < MD_COLS; u++) {
+      double t = acc[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (lane == 0) red[wave][u] = t;
+   }
+   __syncthreads();
+   if (threadIdx.x < nj) {
+      const int u = threadIdx.x;
+      partial[(long long)(j0 + u) * nch + blockIdx.x] = ((red[0][u] + red[1][u]) + red[2][u]) + red[3][u];
    }
 }
 
@@ -142,54 +162,80 @@ __global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const do
 // One Arnoldi column's Givens step on the device (reference GMRES.c:3-22,
 // 495-510): the previous rotations applied to H[:, i] = hcol, the new
 // rotation from (H[i,i], H[i+1,i]), g updated, |g[i+1]|/normr recorded.
-// One thread; H is column-major with leading dimension W.
-__global__ void k_arnoldi_givens(int i, int W, const double* __restrict__ hcol, double* __restrict__ H,
-                                 double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
-                                 double* __restrict__ hist, double normr) {
-   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: the column and the rotations are staged in LDS in parallel,
+// lane 0 applies the rotations in order, the column is written back in
+// parallel.  H is column-major with leading dimension W.
+__global__ void __launch_bounds__(64) k_arnoldi_givens(int i, int W, const double* __restrict__ hcol,
+                                                       double* __restrict__ H, double* __restrict__ cs,
+                                                       double* __restrict__ sn, double* __restrict__ g,
+                                                       double* __restrict__ hist, double normr) {
+   extern __shared__ double gsm[];
+   double* hs = gsm;            // i + 2
+   double* cl = hs + i + 2;     // i
+   double* sl = cl + i;         // i
+   const int lane = threadIdx.x;
+   for (int j = lane; j <= i + 1; j += 64) hs[j] = hcol[j];
+   for (int k = lane; k < i; k += 64) {
+      cl[k] = cs[k];
+      sl[k] = sn[k];
+   }
+   __syncthreads();
+   if (lane == 0) {
+      double a = hs[0];
+      for (int k = 0; k < i; k++) {
+         const double b = hs[k + 1], c = cl[k], s = sl[k];
+         hs[k] = a * c - b * s;
+         a = c * b + s * a;
+      }
+      double b = hs[i + 1];
+      double c = 1.0, s = 0.0;
+      if (b != 0.0) {
+         const double r1 = fabs(a), r2 = fabs(b);
+         const double L = sqrt(r1 * r1 + r2 * r2);
+         c = a / L;
+         s = -b / L;
+      }
+      cs[i] = c;
+      sn[i] = s;
+      hs[i] = a * c - b * s;
+      hs[i + 1] = c * b + s * a;
+      const double gi = g[i];
+      g[i] = gi * c;   // g[i+1] was zero
+      g[i + 1] = s * gi;
+      hist[i] = fabs(g[i + 1]) / normr;
+   }
+   __syncthreads();
    double* Hi = H + (long long)i * W;
-   for (int j = 0; j <= i + 1; j++) Hi[j] = hcol[j];
-   for (int k = 0; k < i; k++) {
-      const double a = Hi[k], b = Hi[k + 1];
-      Hi[k] = a * cs[k] - b * sn[k];
-      Hi[k + 1] = cs[k] * b + sn[k] * a;
-   }
-   double c = 1.0, s = 0.0;
-   if (Hi[i + 1] != 0.0) {
-      const double r1 = fabs(Hi[i]), r2 = fabs(Hi[i + 1]);
-      const double L = sqrt(r1 * r1 + r2 * r2);
-      c = Hi[i] / L;
-      s = -Hi[i + 1] / L;
-   }
-   cs[i] = c;
-   sn[i] = s;
-   {
-      const double a = Hi[i], b = Hi[i + 1];
-      Hi[i] = a * c - b * s;
-      Hi[i + 1] = c * b + s * a;
-   }
-   const double gi = g[i];
-   g[i] = gi * c;   // g[i+1] was zero
-   g[i + 1] = s * gi;
-   hist[i] = fabs(g[i + 1]) / normr;
+   for (int j = lane; j <= i + 1; j += 64) Hi[j] = hs[j];
 }
 
 // w -= sum_j h[j] V_j, summed over j in order.
-__global__ void k_multiaxpy(long long n, int m, const double* __restrict__ h, const double* const* __restrict__ V,
+template <typename VT>
+__global__ void k_multiaxpy(long long n, int m, const double* __restrict__ h, const VT* const* __restrict__ V,
                             double* __restrict__ w) {
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
       double s = 0.0;
-      for (int j = 0; j < m; j++) s += h[j] * V[j][i];
+      int j = 0;
+      // eight basis values in flight
+      for (; j + 7 < m; j += 8) {
+         double v[8];
+#pragma unroll
+         for (int u = 0; u < 8; u++) v[u] = (double)V[j + u][i];
+#pragma unroll
+         for (int u = 0; u < 8; u++) s += h[j + u] * v[u];
+      }
+      for (; j < m; j++) s += h[j] * (double)V[j][i];
       w[i] -= s;
    }
 }
 
-// x = sum_j y[j] * V[j]  (basis vectors at stride ld), in order of j.
-__global__ void k_combine(long long n, int m, const double* __restrict__ y, const double* const* __restrict__ V,
+// x = sum_j y[j] * V[j], in order of j.
+template <typename VT>
+__global__ void k_combine(long long n, int m, const double* __restrict__ y, const VT* const* __restrict__ V,
                           double* __restrict__ x) {
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
       double s = 0.0;
-      for (int j = 0; j < m; j++) s += y[j] * V[j][i];
+      for (int j = 0; j < m; j++) s += y[j] * (double)V[j][i];
       x[i] = s;
    }
 }
@@ -282,10 +328,15 @@ static inline int nblocks(long long n, int per = 256, int cap = 65535) {
    return (int)b;
 }
 
-void launch_join(int n, const int* pi, const int* di, const double* sp, const double* sd, const double* x, double* xs,
+template <typename XT>
+void launch_join(int n, const int* pi, const int* di, const double* sp, const double* sd, const XT* x, double* xs,
                  double* xd, hipStream_t s) {
-   if (n > 0) hipLaunchKernelGGL(k_join, dim3(nblocks(n)), dim3(256), 0, s, n, pi, di, sp, sd, x, xs, xd);
+   if (n > 0) hipLaunchKernelGGL((k_join<XT>), dim3(nblocks(n)), dim3(256), 0, s, n, pi, di, sp, sd, x, xs, xd);
 }
+template void launch_join<double>(int, const int*, const int*, const double*, const double*, const double*, double*,
+                                  double*, hipStream_t);
+template void launch_join<float>(int, const int*, const int*, const double*, const double*, const float*, double*,
+                                 double*, hipStream_t);
 void launch_split(int n, const int* oi, const double* corr, const double* yt, const double* xd, double* y,
                   hipStream_t s) {
    if (n > 0) hipLaunchKernelGGL(k_split, dim3(nblocks(n)), dim3(256), 0, s, n, oi, corr, yt, xd, y);
@@ -300,27 +351,42 @@ void launch_copy(long long n4, const void* a, void* b, hipStream_t s) {
    if (n4 > 0)
       hipLaunchKernelGGL(k_copy, dim3(nblocks(n4, 256, 8192)), dim3(256), 0, s, n4, (const double4*)a, (double4*)b);
 }
-void launch_multidot(long long n, int m, int nch, const double* const* V, const double* w, double* partial,
+template <typename VT>
+void launch_multidot(long long n, int m, int nch, const VT* const* V, const double* w, double* partial,
                      hipStream_t s) {
-   if (m > 0) hipLaunchKernelGGL(k_multidot, dim3(nch, (m + 3) / 4), dim3(256), 0, s, n, m, nch, V, w, partial);
+   if (m > 0)
+      hipLaunchKernelGGL((k_multidot<VT>), dim3(nch, (m + MD_COLS - 1) / MD_COLS), dim3(256), 0, s, n, m, nch, V, w,
+                         partial);
 }
 void launch_multidot_finish(int m, int nch, const double* partial, double* h, double* hacc, int add, hipStream_t s) {
    if (m > 0) hipLaunchKernelGGL(k_multidot_finish, dim3(m), dim3(64), 0, s, m, nch, partial, h, hacc, add);
 }
+
 void launch_arnoldi_givens(int i, int W, const double* hcol, double* H, double* cs, double* sn, double* g,
                            double* hist, double normr, hipStream_t s) {
-   hipLaunchKernelGGL(k_arnoldi_givens, dim3(1), dim3(64), 0, s, i, W, hcol, H, cs, sn, g, hist, normr);
+   // LDS: the column (i + 2) and the i previous rotations; i + 2 <= W <= 2041
+   const size_t lds = sizeof(double) * (size_t)(3 * i + 2);
+   hipLaunchKernelGGL(k_arnoldi_givens, dim3(1), dim3(64), lds, s, i, W, hcol, H, cs, sn, g, hist, normr);
 }
-void launch_multiaxpy(long long n, int m, const double* h, const double* const* V, double* w, hipStream_t s) {
-   if (m > 0) hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, h, V, w);
+template <typename VT>
+void launch_multiaxpy(long long n, int m, const double* h, const VT* const* V, double* w, hipStream_t s) {
+   if (m > 0) hipLaunchKernelGGL((k_multiaxpy<VT>), dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, h, V, w);
 }
-void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, double* v,
-                      hipStream_t s) {
-   hipLaunchKernelGGL(k_normalize, dim3(np), dim3(256), 0, s, n, pin, np, hout, w, v);
+template <typename VT>
+void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, VT* v, hipStream_t s) {
+   hipLaunchKernelGGL((k_normalize<VT>), dim3(np), dim3(256), 0, s, n, pin, np, hout, w, v);
 }
-void launch_combine(long long n, int m, const double* y, const double* const* V, double* x, hipStream_t s) {
-   hipLaunchKernelGGL(k_combine, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, y, V, x);
+template <typename VT>
+void launch_combine(long long n, int m, const double* y, const VT* const* V, double* x, hipStream_t s) {
+   hipLaunchKernelGGL((k_combine<VT>), dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, y, V, x);
 }
+#define FSVD_KRYLOV_INST(VT)                                                                                        \
+   template void launch_multidot<VT>(long long, int, int, const VT* const*, const double*, double*, hipStream_t); \
+   template void launch_multiaxpy<VT>(long long, int, const double*, const VT* const*, double*, hipStream_t);     \
+   template void launch_normalize<VT>(long long, int, const double*, double*, const double*, VT*, hipStream_t);  \
+   template void launch_combine<VT>(long long, int, const double*, const VT* const*, double*, hipStream_t);
+FSVD_KRYLOV_INST(double)
+FSVD_KRYLOV_INST(float)
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s) {
    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, a, x, b, y, z);
 }

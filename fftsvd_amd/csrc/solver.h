@@ -71,10 +71,14 @@ struct Problem {
    DBuf<double> d_qpos, d_q;
    // scratch
    DBuf<double> w, r, tmp, partial, hbuf, htmp, partial_m;
+   // Krylov bases: FP64 for FP64 solves, FP32 for the FP32-operator solves
+   // of the mixed mode; device copies of their pointers
    std::vector<std::unique_ptr<DBuf<double>>> basis;
-   DBuf<const double*> d_basis_ptr;     // device copy of the basis pointers
+   std::vector<std::unique_ptr<DBuf<float>>> basis32;
+   DBuf<const double*> d_basis_ptr;
+   DBuf<const float*> d_basis_ptr32;
+   int basis_ptr_n = 0, basis_ptr_n32 = 0;
    DBuf<double> gm_H, gm_v;             // GMRES Givens state (H; rotations, g, history)
-   int basis_ptr_n = 0;
    double* h_pinned = nullptr;
    double setup_ms = 0.0;
    ~Problem();
@@ -85,6 +89,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec);
 // y = A x over the 2N unknowns (reference SurfaceOperator_topmultiply,
 // SurfaceOperator.c:90-97); device pointers.
 void problem_matvec(Problem& pr, int prec, const double* x, double* y);
+void problem_matvec(Problem& pr, int prec, const float* x, double* y);
 void problem_precond(Problem& pr, const double* b, double* x);
 void problem_rhs(Problem& pr, double* rhs_dev);
 // GMRES (reference GMRES_SurfaceOperator, GMRES.c:446-539); device vectors.
