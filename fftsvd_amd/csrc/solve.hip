@@ -106,7 +106,10 @@ __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __
 
 // Classical Gram-Schmidt pass over m basis vectors (two passes make CGS2).
 // Block (b, jg) owns the contiguous chunk b of the vectors and the columns
-// MD_COLS*jg ..: partial[j*nch + b] = chunk sum of V_j . w.
+// MD_COLS*jg ..: partial[j*nch + b] = chunk sum of V_j . w.  (Finishing
+// the sums in the last block to arrive would need a device-scope release
+// fence in every block, which writes back the L2 of each XCD: far slower
+// than the separate finish launch.)
 constexpr int MD_COLS = 8;
 template <typename VT>
 __global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const VT* const* __restrict__ V,
@@ -159,27 +162,31 @@ __global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const do
    }
 }
 
-// One Arnoldi column's Givens step on the device (reference GMRES.c:3-22,
-// 495-510): the previous rotations applied to H[:, i] = hcol, the new
-// rotation from (H[i,i], H[i+1,i]), g updated, |g[i+1]|/normr recorded.
-// One wave: the column and the rotations are staged in LDS in parallel,
-// lane 0 applies the rotations in order, the column is written back in
-// parallel.  H is column-major with leading dimension W.
-__global__ void __launch_bounds__(64) k_arnoldi_givens(int i, int W, const double* __restrict__ hcol,
-                                                       double* __restrict__ H, double* __restrict__ cs,
-                                                       double* __restrict__ sn, double* __restrict__ g,
-                                                       double* __restrict__ hist, double normr) {
-   extern __shared__ double gsm[];
+// One Arnoldi column's Givens step (reference GMRES.c:3-22, 495-510): the
+// previous rotations applied to H[:, i] = hcol, the new rotation from
+// (H[i,i], H[i+1,i]), g updated, |g[i+1]|/normr recorded.  Run by the first
+// wave of a block: the column and the rotations are staged in LDS in
+// parallel, lane 0 applies the rotations in order, the column is written
+// back in parallel.  H is column-major with leading dimension W.
+__device__ __forceinline__ void wave_lds_sync() {
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ void arnoldi_givens(int i, int W, const double* __restrict__ hcol, double hlast, double* __restrict__ H,
+                               double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                               double* __restrict__ hist, double normr, double* gsm) {
    double* hs = gsm;            // i + 2
    double* cl = hs + i + 2;     // i
    double* sl = cl + i;         // i
-   const int lane = threadIdx.x;
-   for (int j = lane; j <= i + 1; j += 64) hs[j] = hcol[j];
+   const int lane = threadIdx.x & 63;
+   for (int j = lane; j <= i; j += 64) hs[j] = hcol[j];
+   if (lane == 0) hs[i + 1] = hlast;
    for (int k = lane; k < i; k += 64) {
       cl[k] = cs[k];
       sl[k] = sn[k];
    }
-   __syncthreads();
+   wave_lds_sync();
    if (lane == 0) {
       double a = hs[0];
       for (int k = 0; k < i; k++) {
@@ -204,15 +211,46 @@ __global__ void __launch_bounds__(64) k_arnoldi_givens(int i, int W, const doubl
       g[i + 1] = s * gi;
       hist[i] = fabs(g[i + 1]) / normr;
    }
-   __syncthreads();
+   wave_lds_sync();
    double* Hi = H + (long long)i * W;
    for (int j = lane; j <= i + 1; j += 64) Hi[j] = hs[j];
 }
 
-// w -= sum_j h[j] V_j, summed over j in order.
+// End of Arnoldi step i: nrm = sqrt(sum of the np partials of |w|^2),
+// v = w / nrm, H[i+1, i] = nrm (hcol[i+1]), and block 0's first wave does
+// the Givens step of column i.
 template <typename VT>
-__global__ void k_multiaxpy(long long n, int m, const double* __restrict__ h, const VT* const* __restrict__ V,
-                            double* __restrict__ w) {
+__global__ void __launch_bounds__(256) k_normalize_givens(long long n, const double* __restrict__ pin, int np,
+                                                          const double* __restrict__ w, VT* __restrict__ v, int i,
+                                                          int W, double* __restrict__ hcol, double* __restrict__ H,
+                                                          double* __restrict__ cs, double* __restrict__ sn,
+                                                          double* __restrict__ g, double* __restrict__ hist,
+                                                          double normr) {
+   extern __shared__ double gsm[];
+   __shared__ double hs;
+   if (threadIdx.x < 64) {
+      const double t = sqrt(wave_sum_partials(pin, np));
+      if (threadIdx.x == 0) hs = t;
+   }
+   __syncthreads();
+   const double nrm = hs;
+   const double inv = 1.0 / nrm;
+   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+      v[k] = (VT)(w[k] * inv);
+   if (blockIdx.x == 0 && threadIdx.x < 64) {
+      if (threadIdx.x == 0) hcol[i + 1] = nrm;
+      arnoldi_givens(i, W, hcol, nrm, H, cs, sn, g, hist, normr, gsm);
+   }
+}
+
+// w -= sum_j h[j] V_j, summed over j in order.
+// With pnorm: pnorm[block] = this block's sum of the updated w^2.
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const double* __restrict__ h,
+                                                   const VT* const* __restrict__ V, double* __restrict__ w,
+                                                   double* __restrict__ pnorm) {
+   __shared__ double red[8];
+   double nn = 0.0;
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
       double s = 0.0;
       int j = 0;
@@ -225,7 +263,13 @@ __global__ void k_multiaxpy(long long n, int m, const double* __restrict__ h, co
          for (int u = 0; u < 8; u++) s += h[j + u] * v[u];
       }
       for (; j < m; j++) s += h[j] * (double)V[j][i];
-      w[i] -= s;
+      const double wn = w[i] - s;
+      w[i] = wn;
+      nn += wn * wn;
+   }
+   if (pnorm) {
+      nn = block_sum(nn, red);
+      if (threadIdx.x == 0) pnorm[blockIdx.x] = nn;
    }
 }
 
@@ -361,20 +405,25 @@ void launch_multidot(long long n, int m, int nch, const VT* const* V, const doub
 void launch_multidot_finish(int m, int nch, const double* partial, double* h, double* hacc, int add, hipStream_t s) {
    if (m > 0) hipLaunchKernelGGL(k_multidot_finish, dim3(m), dim3(64), 0, s, m, nch, partial, h, hacc, add);
 }
-
-void launch_arnoldi_givens(int i, int W, const double* hcol, double* H, double* cs, double* sn, double* g,
-                           double* hist, double normr, hipStream_t s) {
-   // LDS: the column (i + 2) and the i previous rotations; i + 2 <= W <= 2041
-   const size_t lds = sizeof(double) * (size_t)(3 * i + 2);
-   hipLaunchKernelGGL(k_arnoldi_givens, dim3(1), dim3(64), lds, s, i, W, hcol, H, cs, sn, g, hist, normr);
-}
+int multiaxpy_blocks(long long n) { return nblocks(n, 256, 4096); }
 template <typename VT>
-void launch_multiaxpy(long long n, int m, const double* h, const VT* const* V, double* w, hipStream_t s) {
-   if (m > 0) hipLaunchKernelGGL((k_multiaxpy<VT>), dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, h, V, w);
+void launch_multiaxpy(long long n, int m, const double* h, const VT* const* V, double* w, double* pnorm,
+                      hipStream_t s) {
+   if (m > 0)
+      hipLaunchKernelGGL((k_multiaxpy<VT>), dim3(multiaxpy_blocks(n)), dim3(256), 0, s, n, m, h, V, w, pnorm);
 }
 template <typename VT>
 void launch_normalize(long long n, int np, const double* pin, double* hout, const double* w, VT* v, hipStream_t s) {
    hipLaunchKernelGGL((k_normalize<VT>), dim3(np), dim3(256), 0, s, n, pin, np, hout, w, v);
+}
+template <typename VT>
+void launch_normalize_givens(long long n, int np, const double* pin, const double* w, VT* v, int i, int W,
+                             double* hcol, double* H, double* cs, double* sn, double* g, double* hist, double normr,
+                             hipStream_t s) {
+   // LDS: the column (i + 2) and the i previous rotations; i + 2 <= W <= 2041
+   const size_t lds = sizeof(double) * (size_t)(3 * i + 2);
+   hipLaunchKernelGGL((k_normalize_givens<VT>), dim3(nblocks(n, 256, 512)), dim3(256), lds, s, n, pin, np, w, v, i,
+                      W, hcol, H, cs, sn, g, hist, normr);
 }
 template <typename VT>
 void launch_combine(long long n, int m, const double* y, const VT* const* V, double* x, hipStream_t s) {
@@ -382,7 +431,11 @@ void launch_combine(long long n, int m, const double* y, const VT* const* V, dou
 }
 #define FSVD_KRYLOV_INST(VT)                                                                                        \
    template void launch_multidot<VT>(long long, int, int, const VT* const*, const double*, double*, hipStream_t); \
-   template void launch_multiaxpy<VT>(long long, int, const double*, const VT* const*, double*, hipStream_t);     \
+   template void launch_multiaxpy<VT>(long long, int, const double*, const VT* const*, double*, double*,          \
+                                      hipStream_t);                                                              \
+   template void launch_normalize_givens<VT>(long long, int, const double*, const double*, VT*, int, int,        \
+                                             double*, double*, double*, double*, double*, double*, double,        \
+                                             hipStream_t);                                                        \
    template void launch_normalize<VT>(long long, int, const double*, double*, const double*, VT*, hipStream_t);  \
    template void launch_combine<VT>(long long, int, const double*, const VT* const*, double*, hipStream_t);
 FSVD_KRYLOV_INST(double)

@@ -25,12 +25,16 @@ void launch_normalize(long long n, int np, const double* pin, double* hout, cons
 template <typename VT>
 void launch_combine(long long n, int m, const double* y, const VT* const* V, double* x, hipStream_t s);
 template <typename VT>
-void launch_multidot(long long n, int m, int np, const VT* const* V, const double* w, double* partial, hipStream_t s);
-void launch_multidot_finish(int m, int np, const double* partial, double* h, double* hacc, int add, hipStream_t s);
-void launch_arnoldi_givens(int i, int W, const double* hcol, double* H, double* cs, double* sn, double* g,
-                           double* hist, double normr, hipStream_t s);
+void launch_multidot(long long n, int m, int nch, const VT* const* V, const double* w, double* partial, hipStream_t s);
+void launch_multidot_finish(int m, int nch, const double* partial, double* h, double* hacc, int add, hipStream_t s);
+int multiaxpy_blocks(long long n);
 template <typename VT>
-void launch_multiaxpy(long long n, int m, const double* h, const VT* const* V, double* w, hipStream_t s);
+void launch_multiaxpy(long long n, int m, const double* h, const VT* const* V, double* w, double* pnorm,
+                      hipStream_t s);
+template <typename VT>
+void launch_normalize_givens(long long n, int np, const double* pin, const double* w, VT* v, int i, int W,
+                             double* hcol, double* H, double* cs, double* sn, double* g, double* hist, double normr,
+                             hipStream_t s);
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
 void launch_rhs(int ne, const double* cent, const int* out, int nq, const double* qpos, const double* q, double* rhs,
                 hipStream_t s);
@@ -497,6 +501,8 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
    if ((long long)pr.partial_m.n < (long long)(maxit + 1) * nch) pr.partial_m.alloc((size_t)(maxit + 1) * nch);
    if ((long long)pr.htmp.n < maxit + 2) pr.htmp.alloc(maxit + 2);
    if ((long long)pr.hbuf.n < maxit + 2) pr.hbuf.alloc(maxit + 2);
+   const int nax = multiaxpy_blocks(n);
+   if ((long long)pr.pnorm.n < nax) pr.pnorm.alloc(nax);
    // Givens state on the device: H (W x W, column-major), rotations, g, and
    // the residual history; the host reads the history every CHECK iterations
    // and stops at the first iteration below tol, so the iteration count and
@@ -536,11 +542,11 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
       for (int pass = 0; pass < 2; pass++) {
          launch_multidot<VT>(n, i + 1, nch, V, w, pr.partial_m.p, s);
          launch_multidot_finish(i + 1, nch, pr.partial_m.p, pr.htmp.p, pr.hbuf.p, pass, s);
-         launch_multiaxpy<VT>(n, i + 1, pr.htmp.p, V, w, s);
+         launch_multiaxpy<VT>(n, i + 1, pr.htmp.p, V, w, pass ? pr.pnorm.p : nullptr, s);
       }
-      launch_dot_partial(n, np, w, w, P0, s);
-      launch_normalize(n, np, P0, pr.hbuf.p + i + 1, w, vnext_store, s);
-      launch_arnoldi_givens(i, W, pr.hbuf.p, dH, dcs, dsn, dg, dhist, normr, s);
+      // |w|, v_{i+1} = w/|w|, H[i+1,i] and the Givens step of column i
+      launch_normalize_givens(n, nax, pr.pnorm.p, w, vnext_store, i, W, pr.hbuf.p, dH, dcs, dsn, dg, dhist, normr,
+                              s);
       if ((i + 1) % CHECK == 0 || i == maxit - 1) {
          const int a0 = last + 1;
          FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, dhist + a0, sizeof(double) * (i + 1 - a0), hipMemcpyDeviceToHost, s));

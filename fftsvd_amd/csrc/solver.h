@@ -79,6 +79,7 @@ struct Problem {
    DBuf<const float*> d_basis_ptr32;
    int basis_ptr_n = 0, basis_ptr_n32 = 0;
    DBuf<double> gm_H, gm_v;             // GMRES Givens state (H; rotations, g, history)
+   DBuf<double> pnorm;                  // block partials of |w|^2 from the last multi-axpy
    double* h_pinned = nullptr;
    double setup_ms = 0.0;
    ~Problem();
