@@ -63,8 +63,29 @@ def cpu_baseline(seed, budget_s=25.0):
     return {"panels": len(P), "matvecs_per_s": 1.0 / dt, "setup_s": setup, "ms_per_matvec": dt * 1e3}
 
 
-STAGE_KERNELS = {"leaf": "k_leaf<float>", "project_source": "k_project<float> + k_source<float>",
-                 "translate": "k_translate<float, 8>", "target": "k_target<float>"}
+STAGE_KERNELS = {"leaf": "k_leaf<float>",
+                 "project_source": "k_project<float> + k_source<float> + k_source4<float>",
+                 "translate": "k_translate<float, 8, 8>", "target": "k_target<float> + k_target4<float>"}
+STAGE_LAUNCHES = {"leaf": 1, "project_source": 3, "translate": 1, "target": 2}   # per tree
+# LDS of MI355X: 256 CUs x 128 bytes per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+# calibrated HBM bytes per dispatch from the committed rocprofv3 PMC summary
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "apply_kernels_globule100k_hbm_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the copy-kernel calibration) from the committed PMC summary, or
+    None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("TRAFFIC ") and line[8:].rsplit(" ", 1)[0] == kernel:
+                    return float(line.rsplit(" ", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def stage_bytes(st):
@@ -91,7 +112,7 @@ def stage_rooflines(st, parts_ms):
         ach = b / (ms * 1e-3) / 1e9
         out[k] = {"kernel": STAGE_KERNELS[k], "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": ach / HBM_PEAK_GBS, "bytes_per_matvec": b, "ms": ms,
-                  "launches_per_matvec": st["trees"] * (2 if k == "project_source" else 1)}
+                  "launches_per_matvec": st["trees"] * STAGE_LAUNCHES[k]}
     return out
 
 
@@ -163,6 +184,19 @@ def main():
     dom = max(rf, key=lambda k: rf[k]["ms"])
     roofline = dict(rf[dom])
     roofline.update({"stage": dom, "traffic": None})
+    if STAGE_LAUNCHES[dom] == 1:
+        tr = measured_traffic(roofline["kernel"])
+        if tr is not None:
+            roofline.update({"traffic": tr, "traffic_unit": "bytes per launch",
+                             "traffic_source": "profiles/" + os.path.basename(TRAFFIC_FILE),
+                             "algorithmic_bytes_per_launch": roofline["bytes_per_matvec"] / st["trees"]})
+    # the translation is bound by LDS, not HBM: every complex multiply-add
+    # reads one T bin and one source bin (2 x 8 bytes) from LDS
+    lds_bytes = 16.0 * st["translate_macs"]
+    lds_rate = lds_bytes / (parts32["translate"] * 1e-3) / 1e9
+    roofline_lds = {"kernel": STAGE_KERNELS["translate"], "bound": "lds", "achieved": lds_rate,
+                    "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds_rate / LDS_PEAK_GBS,
+                    "lds_bytes_per_matvec": lds_bytes, "complex_macs_per_matvec": st["translate_macs"]}
 
     # measured copy bandwidth of this GPU (1 GiB copied 20 times)
     copy_gbs = fa.copy_bandwidth(1 << 30, 20)
@@ -196,7 +230,8 @@ def main():
         "config": {"workload": "synthetic globule, 2000-atom seeded self-avoiding walk, SES 1.4 A probe, "
                                "~100k flat panels, eps 4/80, kappa 0.124, GMRES tol 1e-6",
                    "panels": len(P), "atoms": len(xyz), "seed": args.seed, "mesh_spacing_A": h,
-                   "precision": "FP32 compressed operator, FP64 Krylov vectors and residual refinement"},
+                   "precision": "FP32 compressed operator and Krylov basis; FP64 work vectors, Hessenberg "
+                                "and residual refinement"},
         "fp64_matvecs_per_s": 1000.0 / ms64,
         "wall_step_check_ms": wall32 * 1e3 / args.steps,
         "matvec_parts_ms": {"mixed": parts32, "fp64": parts64},
@@ -205,6 +240,7 @@ def main():
         "operator": st,
         "roofline": roofline,
         "roofline_stages": rf,
+        "roofline_translate_lds": roofline_lds,
         "hbm_copy_GBs": copy_gbs,
         "cpu_baseline": cpu,
     }
