@@ -242,7 +242,16 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
       a2 += d[6] * xl[c + 6 * G];
       a3 += d[7] * xl[c + 7 * G];
    }
-   for (; c < m; c += G) a0 += colp[(long long)c * E] * xl[c];
+   if (c < m) {
+      // the last partial batch, all of its loads in flight together
+      const T* pc = colp + (long long)c * E;
+      T d[7];
+#pragma unroll
+      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? pc[u * step] : (T)0;
+#pragma unroll
+      for (int u = 0; u < 7; u++)
+         if (c + u * G < m) a0 += d[u] * xl[c + u * G];
+   }
    return (a0 + a1) + (a2 + a3);
 }
 
@@ -263,9 +272,23 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
    const int m = c.ncol, E = c.r;
    T* part = (T*)(smem + (size_t)wave * wave_lds);   // 2*64
    T* xl = part + 128;                                // 2*m
-   for (int i = lane; i < m; i += 64) {
-      xl[i] = (T)xs[c.p0 + i];
-      xl[m + i] = (T)xd[c.p0 + i];
+   // x staged four columns per lane at a time, loads before stores
+   for (int i0 = lane; i0 < m; i0 += 256) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const int i = i0 + 64 * u;
+         a[u] = i < m ? xs[c.p0 + i] : 0.0;
+         b[u] = i < m ? xd[c.p0 + i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const int i = i0 + 64 * u;
+         if (i < m) {
+            xl[i] = (T)a[u];
+            xl[m + i] = (T)b[u];
+         }
+      }
    }
    gr.sync();
    const T* blk = VTc + c.vt_off;
@@ -291,10 +314,17 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
                d1 += v[u + 1] * xl[m + i + (u + 1) * G];
             }
          }
-         for (; i < m; i += G) {
-            const T v = colp[(long long)i * E];
-            s0 += v * xl[i];
-            d0 += v * xl[m + i];
+         if (i < m) {
+            // the last partial batch, all of its loads in flight together
+            T v[7];
+#pragma unroll
+            for (int u = 0; u < 7; u++) v[u] = i + u * G < m ? colp[(long long)i * E + u * step] : (T)0;
+#pragma unroll
+            for (int u = 0; u < 7; u++)
+               if (i + u * G < m) {
+                  s0 += v[u] * xl[i + u * G];
+                  d0 += v[u] * xl[m + i + u * G];
+               }
          }
          as = s0 + s1;
          ad = d0 + d1;
@@ -341,7 +371,14 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
 #pragma unroll
          for (int u = 0; u < 8; u++) a += v[u];
       }
-      for (; j < d.nchunk; j++) a += pp[(long long)j * 2 * r];
+      if (j < d.nchunk) {
+         T v[7];
+#pragma unroll
+         for (int u = 0; u < 7; u++) v[u] = j + u < d.nchunk ? pp[(long long)(j + u) * 2 * r] : (T)0;
+#pragma unroll
+         for (int u = 0; u < 7; u++)
+            if (j + u < d.nchunk) a += v[u];
+      }
       t[k] = a;
    }
    gr.sync();
@@ -369,7 +406,15 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
             a2 += v[6] * t[k + 6 * KG];
             a3 += v[7] * t[k + 7 * KG];
          }
-         for (; k < R2; k += KG) a0 += pv[(long long)k * GP3 + q] * t[k];
+         if (k < R2) {
+            // the last partial batch, all of its loads in flight together
+            T v[7];
+#pragma unroll
+            for (int u = 0; u < 7; u++) v[u] = k + u * KG < R2 ? pv[(long long)(k + u * KG) * GP3 + q] : (T)0;
+#pragma unroll
+            for (int u = 0; u < 7; u++)
+               if (k + u * KG < R2) a0 += v[u] * t[k + u * KG];
+         }
       }
       const T sum = (a0 + a1) + (a2 + a3);
       if (KG == 1) {
@@ -593,7 +638,20 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
    T* h = sn + N;                 // GP3
    const int tid = gr.tid, nt = gr.nt;
    twiddle_table<T, N>(cs, sn, gr);
-   for (int b = tid; b < PAD; b += nt) S[b] = acc[d.acc_off + b];
+   // four bins per thread at a time, loads before stores
+   for (int b0 = tid; b0 < PAD; b0 += 4 * nt) {
+      C2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const int b = b0 + u * nt;
+         if (b < PAD) v[u] = acc[d.acc_off + b];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const int b = b0 + u * nt;
+         if (b < PAD) S[b] = v[u];
+      }
+   }
    gr.sync();
    grid_backward<n, T>(S, C1, Cc, h, cs, sn, gr);
    gr.sync();
@@ -714,7 +772,18 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
             const LeafU b = ublk[L.u0 + u];
             const T* col = UT + b.u_off + e;
             const T* wb = w + b.w_off;
-            for (int k = g; k < b.r; k += G) acc += col[(long long)k * b.ld] * wb[k];
+            // up to eight k in flight (a masked last batch)
+            for (int k = g; k < b.r; k += 8 * G) {
+               T cv[8], wv[8];
+#pragma unroll
+               for (int v = 0; v < 8; v++) {
+                  const int kk = k + v * G;
+                  cv[v] = kk < b.r ? col[(long long)kk * b.ld] : (T)0;
+                  wv[v] = kk < b.r ? wb[kk] : (T)0;
+               }
+#pragma unroll
+               for (int v = 0; v < 8; v++) acc += cv[v] * wv[v];
+            }
          }
       part[tid] = acc;
       __syncthreads();

@@ -262,7 +262,15 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const dou
 #pragma unroll
          for (int u = 0; u < 8; u++) s += h[j + u] * v[u];
       }
-      for (; j < m; j++) s += h[j] * (double)V[j][i];
+      if (j < m) {
+         // the last partial batch, all of its loads in flight together
+         double v[7];
+#pragma unroll
+         for (int u = 0; u < 7; u++) v[u] = j + u < m ? (double)V[j + u][i] : 0.0;
+#pragma unroll
+         for (int u = 0; u < 7; u++)
+            if (j + u < m) s += h[j + u] * v[u];
+      }
       const double wn = w[i] - s;
       w[i] = wn;
       nn += wn * wn;
@@ -279,7 +287,15 @@ __global__ void k_combine(long long n, int m, const double* __restrict__ y, cons
                           double* __restrict__ x) {
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
       double s = 0.0;
-      for (int j = 0; j < m; j++) s += y[j] * (double)V[j][i];
+      for (int j0 = 0; j0 < m; j0 += 8) {
+         // eight basis values in flight, summed in order of j
+         double v[8];
+#pragma unroll
+         for (int u = 0; u < 8; u++) v[u] = j0 + u < m ? (double)V[j0 + u][i] : 0.0;
+#pragma unroll
+         for (int u = 0; u < 8; u++)
+            if (j0 + u < m) s += y[j0 + u] * v[u];
+      }
       x[i] = s;
    }
 }
