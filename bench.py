@@ -63,10 +63,9 @@ def cpu_baseline(seed, budget_s=25.0):
     return {"panels": len(P), "matvecs_per_s": 1.0 / dt, "setup_s": setup, "ms_per_matvec": dt * 1e3}
 
 
-STAGE_KERNELS = {"leaf": "k_leaf<float>",
-                 "project_source": "k_project<float> + k_source<float> + k_source4<float>",
-                 "translate": "k_translate<float, 8, 8>", "target": "k_target<float> + k_target4<float>"}
-STAGE_LAUNCHES = {"leaf": 1, "project_source": 3, "translate": 1, "target": 2}   # per tree
+STAGE_KERNELS = {"leaf": "k_leaf<float>", "project_source": "k_project<float> + k_source<float>",
+                 "translate": "k_translate<float, 8, 8>", "target": "k_target<float>"}
+STAGE_LAUNCHES = {"leaf": 1, "project_source": 2, "translate": 1, "target": 1}   # per tree
 # LDS of MI355X: 256 CUs x 128 bytes per clock x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
 # calibrated HBM bytes per dispatch from the committed rocprofv3 PMC summary

@@ -441,35 +441,32 @@ __host__ __device__ constexpr size_t source_cube_lds(int n, int maxr, int nt) {
           sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n + (nt > n * n * n ? nt : 0) + 2 * maxr) + 64;
 }
 
-// Source cubes with more than four grid points per side, one block of
-// SRC_BIG_NT threads each (these cubes are few, so each gets many loads in
-// flight).
-constexpr int SRC_BIG_NT = 1024;
+// Source cubes, one launch: the nbig cubes with more than four grid points
+// per side first, one block of CUBE_NT threads each (these cubes are few,
+// so each gets many loads in flight), then the cubes of four grid points
+// per side (n^3 = 64, one per lane), one per wavefront, CUBE_NT/64 per
+// block, each wave in its own LDS slice of wave_lds bytes.  The small cubes
+// run while the large ones finish.
+constexpr int CUBE_NT = 512;
 template <typename T>
-__global__ void __launch_bounds__(SRC_BIG_NT) k_source(const SrcPlan* __restrict__ src, const T* __restrict__ tpart,
-                                                const T* __restrict__ PV, typename Cplx<T>::type* __restrict__ spec) {
+__global__ void __launch_bounds__(CUBE_NT) k_source(const SrcPlan* __restrict__ src, int nbig, int nsrc,
+                                                    const T* __restrict__ tpart, const T* __restrict__ PV,
+                                                    typename Cplx<T>::type* __restrict__ spec, int wave_lds) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const SrcPlan d = src[blockIdx.x];
-   const BlockGroup gr = block_group();
-   switch (d.n) {
-      case 5: source_body<5, T>(d, tpart, PV, spec, smem, gr); break;
-      case 6: source_body<6, T>(d, tpart, PV, spec, smem, gr); break;
-      case 7: source_body<7, T>(d, tpart, PV, spec, smem, gr); break;
-      case 8: source_body<8, T>(d, tpart, PV, spec, smem, gr); break;
-      case 9: source_body<9, T>(d, tpart, PV, spec, smem, gr); break;
-      default: break;
+   if ((int)blockIdx.x < nbig) {
+      const SrcPlan d = src[blockIdx.x];
+      const BlockGroup gr = block_group();
+      switch (d.n) {
+         case 5: source_body<5, T>(d, tpart, PV, spec, smem, gr); break;
+         case 6: source_body<6, T>(d, tpart, PV, spec, smem, gr); break;
+         case 7: source_body<7, T>(d, tpart, PV, spec, smem, gr); break;
+         case 8: source_body<8, T>(d, tpart, PV, spec, smem, gr); break;
+         case 9: source_body<9, T>(d, tpart, PV, spec, smem, gr); break;
+         default: break;
+      }
+      return;
    }
-}
-
-// Source cubes of four grid points per side (n^3 = 64, one per lane): one
-// cube per wavefront, four per block, each wave in its own LDS slice of
-// wave_lds bytes.
-template <typename T>
-__global__ void __launch_bounds__(256) k_source4(const SrcPlan* __restrict__ src, int nsrc, const T* __restrict__ tpart,
-                                                 const T* __restrict__ PV, typename Cplx<T>::type* __restrict__ spec,
-                                                 int wave_lds) {
-   extern __shared__ __align__(16) unsigned char smem[];
-   const int wave = threadIdx.x >> 6, i = blockIdx.x * 4 + wave;
+   const int wave = threadIdx.x >> 6, i = nbig + ((int)blockIdx.x - nbig) * (CUBE_NT / 64) + wave;
    if (i >= nsrc) return;
    const SrcPlan d = src[i];
    source_body<4, T>(d, tpart, PV, spec, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
@@ -690,33 +687,29 @@ __host__ __device__ constexpr size_t target_cube_lds(int n) {
           sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n) + 64;
 }
 
-// Target cubes with more than four grid points per side, one block of
-// TGT_BIG_NT threads each.
-constexpr int TGT_BIG_NT = 1024;
+// Target cubes, one launch: the nbig cubes with more than four grid points
+// per side first, one block of CUBE_NT threads each, then the cubes of four
+// grid points per side, one per wavefront in its own LDS slice.
 template <typename T>
-__global__ void __launch_bounds__(TGT_BIG_NT) k_target(const TgtPlan* __restrict__ tgt, const T* __restrict__ UTI,
-                                                const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w) {
+__global__ void __launch_bounds__(CUBE_NT) k_target(const TgtPlan* __restrict__ tgt, int nbig, int ntgt,
+                                                    const T* __restrict__ UTI,
+                                                    const typename Cplx<T>::type* __restrict__ acc,
+                                                    T* __restrict__ w, int wave_lds) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const TgtPlan d = tgt[blockIdx.x];
-   const BlockGroup gr = block_group();
-   switch (d.n) {
-      case 5: target_body<5, T>(d, UTI, acc, w, smem, gr); break;
-      case 6: target_body<6, T>(d, UTI, acc, w, smem, gr); break;
-      case 7: target_body<7, T>(d, UTI, acc, w, smem, gr); break;
-      case 8: target_body<8, T>(d, UTI, acc, w, smem, gr); break;
-      case 9: target_body<9, T>(d, UTI, acc, w, smem, gr); break;
-      default: break;
+   if ((int)blockIdx.x < nbig) {
+      const TgtPlan d = tgt[blockIdx.x];
+      const BlockGroup gr = block_group();
+      switch (d.n) {
+         case 5: target_body<5, T>(d, UTI, acc, w, smem, gr); break;
+         case 6: target_body<6, T>(d, UTI, acc, w, smem, gr); break;
+         case 7: target_body<7, T>(d, UTI, acc, w, smem, gr); break;
+         case 8: target_body<8, T>(d, UTI, acc, w, smem, gr); break;
+         case 9: target_body<9, T>(d, UTI, acc, w, smem, gr); break;
+         default: break;
+      }
+      return;
    }
-}
-
-// Target cubes of four grid points per side: one cube per wavefront, four
-// per block, each wave in its own LDS slice of wave_lds bytes.
-template <typename T>
-__global__ void __launch_bounds__(256) k_target4(const TgtPlan* __restrict__ tgt, int ntgt, const T* __restrict__ UTI,
-                                                 const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w,
-                                                 int wave_lds) {
-   extern __shared__ __align__(16) unsigned char smem[];
-   const int wave = threadIdx.x >> 6, i = blockIdx.x * 4 + wave;
+   const int wave = threadIdx.x >> 6, i = nbig + ((int)blockIdx.x - nbig) * (CUBE_NT / 64) + wave;
    if (i >= ntgt) return;
    const TgtPlan d = tgt[i];
    target_body<4, T>(d, UTI, acc, w, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
@@ -948,20 +941,19 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          break;
       }
       case 1: {
-         if (a.nsrc_big) {
-            size_t lds = 0;
-            for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, source_cube_lds<T>(n, a.maxr_src, SRC_BIG_NT));
-            check_lds(lds);
-            hipLaunchKernelGGL((k_source<T>), dim3(a.nsrc_big), dim3(SRC_BIG_NT), lds, s, a.src, (const T*)a.tpart,
-                               (const T*)a.PV, (C2*)a.spec);
-         }
-         const int n4 = a.nsrc - a.nsrc_big;
-         if (n4) {
-            const int wl = (int)((source_cube_lds<T>(4, a.maxr_src, 64) + 15) / 16 * 16);
-            check_lds(4 * (size_t)wl);
-            hipLaunchKernelGGL((k_source4<T>), dim3((n4 + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.src + a.nsrc_big, n4,
+         const int wl = (int)((source_cube_lds<T>(4, a.maxr_src, 64) + 15) / 16 * 16);
+         size_t lds = (size_t)(CUBE_NT / 64) * wl;
+         for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, source_cube_lds<T>(n, a.maxr_src, CUBE_NT));
+         check_lds(lds);
+         static std::once_flag attr;   // large n needs more than the default dynamic LDS
+         std::call_once(attr, [] {
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_source<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           163840));
+         });
+         const int nb = a.nsrc_big + (a.nsrc - a.nsrc_big + CUBE_NT / 64 - 1) / (CUBE_NT / 64);
+         if (nb)
+            hipLaunchKernelGGL((k_source<T>), dim3(nb), dim3(CUBE_NT), lds, s, a.src, a.nsrc_big, a.nsrc,
                                (const T*)a.tpart, (const T*)a.PV, (C2*)a.spec, wl);
-         }
          break;
       }
       case 2: {
@@ -986,20 +978,19 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          break;
       }
       case 3: {
-         if (a.ntgt_big) {
-            size_t lds = 0;
-            for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, target_cube_lds<T>(n));
-            check_lds(lds);
-            hipLaunchKernelGGL((k_target<T>), dim3(a.ntgt_big), dim3(TGT_BIG_NT), lds, s, a.tgt, (const T*)a.UTI,
-                               (const C2*)a.acc, (T*)a.w);
-         }
-         const int n4 = a.ntgt - a.ntgt_big;
-         if (n4) {
-            const int wl = (int)((target_cube_lds<T>(4) + 15) / 16 * 16);
-            check_lds(4 * (size_t)wl);
-            hipLaunchKernelGGL((k_target4<T>), dim3((n4 + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.tgt + a.ntgt_big, n4,
+         const int wl = (int)((target_cube_lds<T>(4) + 15) / 16 * 16);
+         size_t lds = (size_t)(CUBE_NT / 64) * wl;
+         for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, target_cube_lds<T>(n));
+         check_lds(lds);
+         static std::once_flag attr;   // large n needs more than the default dynamic LDS
+         std::call_once(attr, [] {
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_target<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           163840));
+         });
+         const int nb = a.ntgt_big + (a.ntgt - a.ntgt_big + CUBE_NT / 64 - 1) / (CUBE_NT / 64);
+         if (nb)
+            hipLaunchKernelGGL((k_target<T>), dim3(nb), dim3(CUBE_NT), lds, s, a.tgt, a.ntgt_big, a.ntgt,
                                (const T*)a.UTI, (const C2*)a.acc, (T*)a.w, wl);
-         }
          break;
       }
       case 4: {
