@@ -37,6 +37,13 @@ __device__ __forceinline__ T wave_sum(T v) {
    return v;
 }
 
+// Marks a loaded value as used here, so the compiler cannot sink its load
+// into a later conditional block (where it would wait for it alone).
+template <typename T>
+__device__ __forceinline__ void keep(const T& v) {
+   asm volatile("" ::"v"(v));
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs.  Launches whose blocks
 // share data across tree-order neighbours use a host-built block map
 // (xcd_block_map): block b runs logical block map[b], and each XCD gets a
@@ -641,7 +648,9 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
 #pragma unroll
       for (int u = 0; u < 4; u++) {
          const int b = b0 + u * nt;
-         if (b < PAD) v[u] = acc[d.acc_off + b];
+         v[u] = acc[d.acc_off + (b < PAD ? b : 0)];   // valid address, no branch
+         keep(v[u].x);
+         keep(v[u].y);
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -658,13 +667,17 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
    const int kr = tid >> 4, ql = tid & 15, RP = nt >> 4;
    for (int k = kr; k < d.r; k += 2 * RP) {
       const T* row0 = uti + (long long)k * GP3;
-      const T* row1 = row0 + (long long)RP * GP3;
       const bool two = k + RP < d.r;
+      const T* row1 = two ? row0 + (long long)RP * GP3 : row0;   // a valid row either way
       T a = 0, b = 0;
 #pragma unroll
-      for (int q = ql; q < GP3; q += 16) {
-         a += row0[q] * h[q];
-         if (two) b += row1[q] * h[q];
+      for (int q0 = 0; q0 < GP3; q0 += 16) {
+         // clamped column and selects: no branch around the loads
+         const int q = q0 + ql < GP3 ? q0 + ql : 0;
+         const bool okq = q0 + ql < GP3;
+         const T x0 = row0[q], x1 = row1[q], hq = h[q];
+         a += okq ? x0 * hq : (T)0;
+         b += okq && two ? x1 * hq : (T)0;
       }
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) {

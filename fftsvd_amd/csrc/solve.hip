@@ -266,10 +266,13 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const dou
          // the last partial batch, all of its loads in flight together
          double v[7];
 #pragma unroll
-         for (int u = 0; u < 7; u++) v[u] = j + u < m ? (double)V[j + u][i] : 0.0;
+         for (int u = 0; u < 7; u++) {
+            const bool ok = j + u < m;
+            const double x = (double)V[ok ? j + u : j][i];   // valid vector, no branch
+            v[u] = ok ? x : 0.0;
+         }
 #pragma unroll
-         for (int u = 0; u < 7; u++)
-            if (j + u < m) s += h[j + u] * v[u];
+         for (int u = 0; u < 7; u++) s += h[j + u < m ? j + u : j] * v[u];   // v[u] = 0 past m
       }
       const double wn = w[i] - s;
       w[i] = wn;
@@ -291,10 +294,13 @@ __global__ void k_combine(long long n, int m, const double* __restrict__ y, cons
          // eight basis values in flight, summed in order of j
          double v[8];
 #pragma unroll
-         for (int u = 0; u < 8; u++) v[u] = j0 + u < m ? (double)V[j0 + u][i] : 0.0;
+         for (int u = 0; u < 8; u++) {
+            const bool ok = j0 + u < m;
+            const double x = (double)V[ok ? j0 + u : j0][i];   // valid vector, no branch
+            v[u] = ok ? x : 0.0;
+         }
 #pragma unroll
-         for (int u = 0; u < 8; u++)
-            if (j0 + u < m) s += y[j0 + u] * v[u];
+         for (int u = 0; u < 8; u++) s += y[j0 + u < m ? j0 + u : j0] * v[u];   // v[u] = 0 past m
       }
       x[i] = s;
    }
