@@ -30,6 +30,9 @@ _SIGS = {
     "fftsvd_problem_create_multi": (C.c_int, [C.c_int, _ip, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double,
                                               C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                               C.POINTER(_vp)]),
+    "fftsvd_problem_create_stern": (C.c_int, [C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip,
+                                              _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                              C.c_int, C.c_double, C.c_int, C.POINTER(_vp)]),
     "fftsvd_problem_destroy": (C.c_int, [_vp]),
     "fftsvd_problem_size": (C.c_int, [_vp, _ip, _ip]),
     "fftsvd_problem_matvec": (C.c_int, [_vp, C.c_int, _dp, _dp]),
@@ -170,13 +173,30 @@ class Problem:
     generateRHS -> solveSurfaceOperator -> collectPotentials."""
 
     def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, kappa=0.0, maxpanels=32, gridpoints=4,
-                 svdeps=1e-5, want_f32=True, cavities=None):
+                 svdeps=1e-5, want_f32=True, cavities=None, stern_verts9=None):
         v = _f64(verts9).reshape(-1, 9)
         self.qpos = _f64(qpos).reshape(-1, 3)
         self.q = _f64(q).reshape(-1)
         self.eps_in = eps_in
         h = _vp()
-        if cavities:
+        if stern_verts9 is not None:
+            # Stern layer: one salt (ion-exclusion) surface around the
+            # dielectric (and its cavities); unknowns start with the salt's
+            sv = _f64(stern_verts9).reshape(-1, 9)
+            cv = [_f64(c).reshape(-1, 9) for c in (cavities or [])]
+            allv = np.ascontiguousarray(np.vstack([sv, v] + cv))
+            ns = np.array([len(sv)], dtype=np.int32)
+            nd = np.array([len(v)], dtype=np.int32)
+            dp = np.zeros(1, dtype=np.int32)
+            nc = np.array([len(c) for c in cv] or [0], dtype=np.int32)
+            cp = np.zeros(max(1, len(cv)), dtype=np.int32)
+            z = np.zeros(1, dtype=np.int32)
+            _check(lib().fftsvd_problem_create_stern(1, _i(ns), 1, _i(nd), _i(dp), len(cv), _i(nc), _i(cp), 0, _i(z),
+                                                     _i(z), _d(allv), _d(self.qpos), _d(self.q), len(self.q), eps_in,
+                                                     eps_out, kappa, maxpanels, gridpoints, svdeps, int(want_f32),
+                                                     C.byref(h)))
+            self.n = len(allv)
+        elif cavities:
             cv = [_f64(c).reshape(-1, 9) for c in cavities]
             allv = np.ascontiguousarray(np.vstack([v] + cv))
             nd = np.array([len(v)], dtype=np.int32)

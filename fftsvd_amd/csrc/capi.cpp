@@ -162,6 +162,44 @@ int fftsvd_problem_create_multi(int ndiel, const int* npan_diel, int ncav, const
    API_CATCH
 }
 
+int fftsvd_problem_create_stern(int nsalt, const int* npan_salt, int ndiel, const int* npan_diel,
+                                const int* diel_parent, int ncav, const int* npan_cav, const int* cav_parent,
+                                int nscav, const int* npan_scav, const int* scav_parent, const double* verts9,
+                                const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
+                                double kappa, int maxpanels, int gridpoints, double svdeps, int want_f32,
+                                fftsvd_problem** out) {
+   API_TRY
+   if (gridpoints != 4) throw std::runtime_error("only 4 grid points per cube edge (49-point sphere) are supported");
+   if (nsalt < 1 || ndiel < 1) throw std::runtime_error("need at least one salt and one dielectric surface");
+   if (!(kappa > 0.0)) throw std::runtime_error("the Stern-layer formulation needs kappa > 0");
+   size_t off = 0;
+   SurfaceSpec sp;
+   for (int i = 0; i < nsalt; i++) {
+      sp.salt_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_salt[i]);
+      off += 9 * (size_t)npan_salt[i];
+   }
+   // dielectrics and dielectric cavities as in the salt-free layout
+   SurfaceSpec rest = make_spec(ndiel, npan_diel, ncav, npan_cav, cav_parent, verts9 + off, qpos, q, ncharge, eps_in,
+                                eps_out, kappa, maxpanels, gridpoints, svdeps, want_f32);
+   for (int i = 0; i < ndiel; i++) off += 9 * (size_t)npan_diel[i];
+   for (int j = 0; j < ncav; j++) off += 9 * (size_t)npan_cav[j];
+   rest.salt_verts = std::move(sp.salt_verts);
+   for (int i = 0; i < ndiel; i++) {
+      if (diel_parent[i] < 0 || diel_parent[i] >= nsalt) throw std::runtime_error("dielectric parent out of range");
+      rest.dielectric_parent.push_back(diel_parent[i]);
+   }
+   for (int l = 0; l < nscav; l++) {
+      if (scav_parent[l] < 0 || scav_parent[l] >= ncav) throw std::runtime_error("salt cavity parent out of range");
+      rest.salt_cavity_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_scav[l]);
+      off += 9 * (size_t)npan_scav[l];
+      rest.salt_cavity_parent.push_back(scav_parent[l]);
+   }
+   auto* h = new fftsvd_problem;
+   h->pr = build_problem(rest);
+   *out = h;
+   API_CATCH
+}
+
 int fftsvd_problem_create(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
                           double eps_in, double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
                           int want_f32, fftsvd_problem** out) {
@@ -675,18 +713,26 @@ void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBe
                              unsigned int* dielectriccavityparent, Panel** saltcavitypanels,
                              unsigned int* numsaltcavitypanels, unsigned int numsaltcavities,
                              unsigned int* saltcavityparent, unsigned int numtotalpanels) {
-   (void)saltpanels;
-   (void)numsaltpanels;
-   (void)numsalts;
-   (void)dielectricparent;
-   (void)saltcavitypanels;
-   (void)numsaltcavitypanels;
-   (void)numsaltcavities;
-   (void)saltcavityparent;
    const double kappa = std::sqrt(ionicstrength) / DEBYE_CONSTANT;
-   if (ionicstrength > 0.0 && ionexclusionradius > 1e-6)
-      error((char*)"the Stern-layer formulation is not supported; set ionexclusionradius to 0");
+   const bool stern = ionicstrength > 0.0 && ionexclusionradius > 1e-6;
    SurfaceSpec sp;
+   auto panel_verts = [](Panel* ps, unsigned np) {
+      std::vector<double> v;
+      for (unsigned k = 0; k < np; k++)
+         for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) v.push_back(ps[k]->vertex[a][d]);
+      return v;
+   };
+   if (stern) {
+      // Stern layer (FFTSVDpbeAPI.c:1064-1377): salt surfaces enclose the
+      // dielectrics; salt cavities sit inside dielectric cavities
+      if (numsalts == 0) error((char*)"the Stern-layer formulation needs a salt surface");
+      for (unsigned i = 0; i < numsalts; i++) sp.salt_verts.push_back(panel_verts(saltpanels[i], numsaltpanels[i]));
+      for (unsigned j = 0; j < numdielectrics; j++) {
+         if (dielectricparent[j] >= numsalts) error((char*)"dielectric %u has no enclosing salt surface", j);
+         sp.dielectric_parent.push_back((int)dielectricparent[j]);
+      }
+   }
    for (unsigned i = 0; i < numdielectrics; i++) {
       std::vector<double> v;
       for (unsigned k = 0; k < numdielectricpanels[i]; k++)
@@ -695,16 +741,20 @@ void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBe
       sp.dielectric_verts.push_back(v);
    }
    std::vector<unsigned> cav_used;
+   std::vector<int> cav_index(numdielectriccavities, -1);
    for (unsigned j = 0; j < numdielectriccavities; j++) {
       if (dielectriccavityparent[j] >= numdielectrics) continue;  // 999: removed by the mesher
-      std::vector<double> v;
-      for (unsigned k = 0; k < numdielectriccavitypanels[j]; k++)
-         for (int a = 0; a < 3; a++)
-            for (int d = 0; d < 3; d++) v.push_back(dielectriccavitypanels[j][k]->vertex[a][d]);
-      sp.cavity_verts.push_back(v);
+      cav_index[j] = (int)sp.cavity_verts.size();
+      sp.cavity_verts.push_back(panel_verts(dielectriccavitypanels[j], numdielectriccavitypanels[j]));
       sp.cavity_parent.push_back((int)dielectriccavityparent[j]);
       cav_used.push_back(j);
    }
+   if (stern)
+      for (unsigned l = 0; l < numsaltcavities; l++) {
+         if (saltcavityparent[l] >= numdielectriccavities || cav_index[saltcavityparent[l]] < 0) continue;
+         sp.salt_cavity_verts.push_back(panel_verts(saltcavitypanels[l], numsaltcavitypanels[l]));
+         sp.salt_cavity_parent.push_back(cav_index[saltcavityparent[l]]);
+      }
    for (unsigned a = 0; a < numPDBentries; a++) {
       sp.qpos.push_back(PDBentries[a].x);
       sp.qpos.push_back(PDBentries[a].y);

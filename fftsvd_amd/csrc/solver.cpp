@@ -67,11 +67,17 @@ Problem::~Problem() {
    if (stream) (void)hipStreamDestroy(stream);
 }
 
-// Region layout of the salt-without-Stern and no-salt formulations
-// (reference generateSurfaceOperator, FFTSVDpbeAPI.c:1381-1730): an
-// exterior region over every dielectric surface, one interior region per
-// dielectric (own panels followed by its cavities' panels) holding all the
-// charges, one region per cavity with the exterior kernel.
+// Region layouts (reference generateSurfaceOperator).
+// Salt without Stern and no salt (FFTSVDpbeAPI.c:1381-1730): an exterior
+// region over every dielectric surface, one interior region per dielectric
+// (own panels followed by its cavities' panels) holding all the charges,
+// one region per cavity with the exterior kernel.
+// Stern layer (FFTSVDpbeAPI.c:1064-1377): an exterior Yukawa region over
+// every salt surface; per salt surface a Laplace region (eps_out) over its
+// panels and those of the dielectrics it encloses; per dielectric a Laplace
+// region (eps_in) over its panels and its cavities', holding the charges;
+// per dielectric cavity a Laplace region (eps_out) over its panels and its
+// salt cavities'; per salt cavity a Yukawa region.
 std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    double t0 = now_ms();
    auto prp = std::make_unique<Problem>();
@@ -81,70 +87,89 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    pr.eps_in = spec.eps_in;
    pr.qpos = spec.qpos;
    pr.q = spec.q;
-   const int nd = (int)spec.dielectric_verts.size(), nc = (int)spec.cavity_verts.size();
-   std::vector<int> dstart(nd + 1, 0), cstart(nc + 1, 0);
-   for (int i = 0; i < nd; i++) {
-      int n = (int)(spec.dielectric_verts[i].size() / 9);
-      dstart[i + 1] = dstart[i] + n;
-      for (int k = 0; k < n; k++) {
-         const double* v = &spec.dielectric_verts[i][9 * (size_t)k];
-         PanelGeom g;
-         panel_geometry(v, v + 3, v + 6, &g);
-         pr.panels.push_back(g);
+   const bool stern = !spec.salt_verts.empty();
+   // global panel order: [salt surfaces,] dielectrics, dielectric cavities[, salt cavities]
+   auto add_surfaces = [&](const std::vector<std::vector<double>>& surf) {
+      std::vector<int> start(surf.size() + 1, (int)pr.panels.size());
+      for (size_t i = 0; i < surf.size(); i++) {
+         const int n = (int)(surf[i].size() / 9);
+         start[i + 1] = start[i] + n;
+         for (int k = 0; k < n; k++) {
+            const double* v = &surf[i][9 * (size_t)k];
+            PanelGeom g;
+            panel_geometry(v, v + 3, v + 6, &g);
+            pr.panels.push_back(g);
+         }
       }
-   }
-   for (int j = 0; j < nc; j++) {
-      int n = (int)(spec.cavity_verts[j].size() / 9);
-      cstart[j + 1] = cstart[j] + n;
-      for (int k = 0; k < n; k++) {
-         const double* v = &spec.cavity_verts[j][9 * (size_t)k];
-         PanelGeom g;
-         panel_geometry(v, v + 3, v + 6, &g);
-         pr.panels.push_back(g);
-      }
-   }
+      return start;
+   };
+   const std::vector<int> sstart = add_surfaces(spec.salt_verts);
+   const std::vector<int> dstart = add_surfaces(spec.dielectric_verts);
+   const std::vector<int> cstart = add_surfaces(spec.cavity_verts);
+   const std::vector<int> scstart = add_surfaces(spec.salt_cavity_verts);
    pr.ntotal = (int)pr.panels.size();
-   const int ndp = dstart[nd];
+   const int nsalt = (int)spec.salt_verts.size(), nd = (int)spec.dielectric_verts.size();
+   const int nc = (int)spec.cavity_verts.size(), nsc = (int)spec.salt_cavity_verts.size();
+   if (stern && ((int)spec.dielectric_parent.size() != nd || (int)spec.salt_cavity_parent.size() != nsc))
+      throw std::runtime_error("Stern layout: one parent per dielectric and per salt cavity");
+   if (!stern && nsc > 0) throw std::runtime_error("salt cavities need the Stern-layer formulation");
    const int outer_kernel = spec.kappa > 0.0 ? YUKAWA : LAPLACE;
+   const double outer_kappa = spec.kappa > 0.0 ? spec.kappa : 0.0;
 
-   // regions in pre-order: root, dielectric 0, its cavities, dielectric 1, ...
-   pr.regions.emplace_back();
-   {
-      Region& R = pr.regions[0];
-      R.kernel = outer_kernel;
-      R.eps = spec.eps_out;
-      R.kappa = spec.kappa > 0.0 ? spec.kappa : 0.0;
-      R.mynpan = 0;
-      for (int k = 0; k < ndp; k++) R.panel_ids.push_back(k);
-   }
-   for (int i = 0; i < nd; i++) {
-      int ri = (int)pr.regions.size();
+   auto new_region = [&](int parent, int kernel, double eps, double kappa, int p0, int p1) {
+      const int ri = (int)pr.regions.size();
       pr.regions.emplace_back();
-      pr.regions[0].child.push_back(ri);
-      {
-         Region& D = pr.regions[ri];
-         D.kernel = LAPLACE;
-         D.eps = spec.eps_in;
-         D.kappa = 0.0;
-         D.parent = 0;
-         D.mynpan = dstart[i + 1] - dstart[i];
-         D.has_charges = true;
-         for (int k = dstart[i]; k < dstart[i + 1]; k++) D.panel_ids.push_back(k);
+      Region& R = pr.regions[ri];
+      R.kernel = kernel;
+      R.eps = eps;
+      R.kappa = kappa;
+      R.parent = parent;
+      R.mynpan = p1 - p0;
+      for (int k = p0; k < p1; k++) R.panel_ids.push_back(k);
+      if (parent >= 0) pr.regions[parent].child.push_back(ri);
+      return ri;
+   };
+   // appends a region's own panels to every ancestor's tree panels up to
+   // (not including) `stop`
+   auto add_to_parent = [&](int ri) {
+      const int parent = pr.regions[ri].parent;
+      if (parent < 0) return;
+      for (int k = 0; k < pr.regions[ri].mynpan; k++)
+         pr.regions[parent].panel_ids.push_back(pr.regions[ri].panel_ids[k]);
+   };
+   // dielectric j under parent region `par`, with its cavities (and, in
+   // the Stern layout, their salt cavities)
+   auto add_dielectric = [&](int par, int j) {
+      const int rd = new_region(par, LAPLACE, spec.eps_in, 0.0, dstart[j], dstart[j + 1]);
+      pr.regions[rd].has_charges = true;
+      if (par >= 0 && stern) add_to_parent(rd);
+      for (int c = 0; c < nc; c++) {
+         if (spec.cavity_parent[c] != j) continue;
+         const int rc = stern ? new_region(rd, LAPLACE, spec.eps_out, 0.0, cstart[c], cstart[c + 1])
+                              : new_region(rd, outer_kernel, spec.eps_out, outer_kappa, cstart[c], cstart[c + 1]);
+         add_to_parent(rc);
+         if (!stern) continue;
+         for (int l = 0; l < nsc; l++) {
+            if (spec.salt_cavity_parent[l] != c) continue;
+            const int rl = new_region(rc, YUKAWA, spec.eps_out, outer_kappa, scstart[l], scstart[l + 1]);
+            add_to_parent(rl);
+         }
       }
-      for (int j = 0; j < nc; j++) {
-         if (spec.cavity_parent[j] != i) continue;
-         int rj = (int)pr.regions.size();
-         pr.regions.emplace_back();
-         pr.regions[ri].child.push_back(rj);
-         Region& C = pr.regions[rj];
-         C.kernel = outer_kernel;
-         C.eps = spec.eps_out;
-         C.kappa = spec.kappa > 0.0 ? spec.kappa : 0.0;
-         C.parent = ri;
-         C.mynpan = cstart[j + 1] - cstart[j];
-         for (int k = cstart[j]; k < cstart[j + 1]; k++) C.panel_ids.push_back(ndp + k);
-         for (int k : C.panel_ids) pr.regions[ri].panel_ids.push_back(k);
+   };
+   // regions in pre-order
+   if (stern) {
+      if (!(spec.kappa > 0.0)) throw std::runtime_error("the Stern-layer formulation needs a salt (kappa > 0)");
+      const int root = new_region(-1, YUKAWA, spec.eps_out, outer_kappa, 0, 0);
+      for (int k = sstart[0]; k < sstart[nsalt]; k++) pr.regions[root].panel_ids.push_back(k);
+      for (int i = 0; i < nsalt; i++) {
+         const int ri = new_region(root, LAPLACE, spec.eps_out, 0.0, sstart[i], sstart[i + 1]);
+         for (int j = 0; j < nd; j++)
+            if (spec.dielectric_parent[j] == i) add_dielectric(ri, j);
       }
+   } else {
+      const int root = new_region(-1, outer_kernel, spec.eps_out, outer_kappa, 0, 0);
+      for (int k = dstart[0]; k < dstart[nd]; k++) pr.regions[root].panel_ids.push_back(k);
+      for (int j = 0; j < nd; j++) add_dielectric(root, j);
    }
    // start indices (SurfaceOperator_initStartIndices, SurfaceOperator.c:39-47)
    {

@@ -17,6 +17,13 @@ struct SurfaceSpec {
    std::vector<std::vector<double>> dielectric_verts;   // per dielectric, npan*9
    std::vector<std::vector<double>> cavity_verts;       // per cavity, npan*9
    std::vector<int> cavity_parent;
+   // Stern-layer formulation when salt_verts is not empty: the salt
+   // (ion-exclusion) surfaces, the salt surface enclosing each dielectric,
+   // and the salt cavities with the dielectric cavity enclosing each
+   std::vector<std::vector<double>> salt_verts;
+   std::vector<int> dielectric_parent;
+   std::vector<std::vector<double>> salt_cavity_verts;
+   std::vector<int> salt_cavity_parent;
    std::vector<double> qpos, q;                         // all atoms
    double eps_in = 4.0, eps_out = 80.0, kappa = 0.0;
    int maxpanels = 32, gridpoints = 4;

@@ -197,8 +197,10 @@ void readSRF(const char* filename, Panel*** saltpanels, unsigned int** numsaltpa
              unsigned int* numsaltcavities, unsigned int** saltcavityparent, unsigned int* numtotalpanels);
 
 /* Operator (reference FFTSVDpbeAPI.h:125-131; FFTSVDpbeAPI.c:1053-1739).
- * Salt-without-Stern and no-salt formulations; the Stern-layer
- * formulation (ionexclusionradius > 0 with salt) is rejected with error(). */
+ * Stern-layer formulation when ionexclusionradius > 0 with salt (salt
+ * surfaces enclosing the dielectrics, dielectric cavities, salt cavities
+ * inside those), otherwise the salt-without-Stern and no-salt
+ * formulations. */
 void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBentries, unsigned int numPDBentries,
                              Panel** saltpanels, unsigned int* numsaltpanels, unsigned int numsalts,
                              Panel** dielectricpanels, unsigned int* numdielectricpanels, unsigned int numdielectrics,

@@ -41,6 +41,19 @@ int fftsvd_problem_create_multi(int ndiel, const int* npan_diel, int ncav, const
                                 const int* cav_parent, const double* verts9, const double* qpos, const double* q,
                                 int ncharge, double eps_in, double eps_out, double kappa, int maxpanels,
                                 int gridpoints, double svdeps, int want_f32, fftsvd_problem** out);
+/* Stern-layer formulation (mirrors generateSurfaceOperator's Stern branch,
+ * FFTSVDpbeAPI.c:1064-1377): nsalt salt (ion-exclusion) surfaces, ndiel
+ * dielectric surfaces each inside salt surface diel_parent[i], ncav
+ * dielectric cavities (parent: a dielectric), nscav salt cavities (parent:
+ * a dielectric cavity); vertices concatenated salt, dielectric, cavity,
+ * salt cavity.  Needs kappa > 0.  Unknowns follow the reference's pre-order
+ * start indices (salt surfaces' phi and dphi first). */
+int fftsvd_problem_create_stern(int nsalt, const int* npan_salt, int ndiel, const int* npan_diel,
+                                const int* diel_parent, int ncav, const int* npan_cav, const int* cav_parent,
+                                int nscav, const int* npan_scav, const int* scav_parent, const double* verts9,
+                                const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
+                                double kappa, int maxpanels, int gridpoints, double svdeps, int want_f32,
+                                fftsvd_problem** out);
 int fftsvd_problem_destroy(fftsvd_problem* p);
 int fftsvd_problem_size(fftsvd_problem* p, int* npanels, int* ncharges);
 

@@ -41,6 +41,8 @@ def _load():
         "orc_free": (None, [C.c_void_p]),
         "orc_problem_single": (C.c_void_p, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, C.c_int, C.c_int, C.c_double]),
+        "orc_problem_stern": (C.c_void_p, [_dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.c_int, C.c_int, C.c_double]),
         "orc_problem_free": (None, [C.c_void_p]),
         "orc_so_matvec": (None, [C.c_void_p, _dp, _dp]),
         "orc_rhs": (None, [C.c_void_p, _dp]),
@@ -262,17 +264,27 @@ class Tree:
 
 
 class Problem:
-    """One dielectric surface with its charges (two-region operator)."""
+    """One dielectric surface with its charges (two-region operator), or,
+    with `stern_verts9` (a salt / ion-exclusion surface enclosing it, kappa
+    > 0), the three-region Stern-layer operator (FFTSVDpbeAPI.c:1064-1377);
+    unknowns then run salt phi, salt dphi, dielectric phi, dielectric dphi."""
 
     def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, kappa=0.0, maxpanels=32, gridpoints=4,
-                 svdeps=1e-5):
+                 svdeps=1e-5, stern_verts9=None):
         self.verts9 = f64(verts9).reshape(-1, 9)
         self.qpos = f64(qpos).reshape(-1, 3)
         self.q = f64(q).reshape(-1)
-        self.n = len(self.verts9)
         self.eps_in = eps_in
-        self.h = lib().orc_problem_single(_d(self.verts9), self.n, _d(self.qpos), _d(self.q), len(self.q), eps_in,
-                                          eps_out, kappa, maxpanels, gridpoints, svdeps)
+        if stern_verts9 is None:
+            self.n = len(self.verts9)
+            self.h = lib().orc_problem_single(_d(self.verts9), self.n, _d(self.qpos), _d(self.q), len(self.q),
+                                              eps_in, eps_out, kappa, maxpanels, gridpoints, svdeps)
+        else:
+            self.salt9 = f64(stern_verts9).reshape(-1, 9)
+            self.n = len(self.salt9) + len(self.verts9)
+            self.h = lib().orc_problem_stern(_d(self.salt9), len(self.salt9), _d(self.verts9), len(self.verts9),
+                                             _d(self.qpos), _d(self.q), len(self.q), eps_in, eps_out, kappa,
+                                             maxpanels, gridpoints, svdeps)
         lib().orc_precond_build(self.h)
 
     def __del__(self):

@@ -142,6 +142,9 @@ typedef struct {
 orc_problem* orc_problem_single(const double* verts9, int npan, const double* qpos, const double* q,
                                 int ncharge, double eps_in, double eps_out, double kappa,
                                 int maxpanels, int gridpoints, double svdeps);
+orc_problem* orc_problem_stern(const double* salt9, int nsalt, const double* diel9, int ndiel, const double* qpos,
+                               const double* q, int ncharge, double eps_in, double eps_out, double kappa,
+                               int maxpanels, int gridpoints, double svdeps);
 void orc_problem_free(orc_problem* pr);
 void orc_so_matvec(orc_problem* pr, const double* x, double* y);
 void orc_rhs(orc_problem* pr, double* rhs);
@@ -165,7 +168,7 @@ void orc_tree_T(orc_tree* t, int level, int off, double* out);
 void orc_tree_set_cube_mats(orc_tree* t, int c, int rowrank, int colrank, const double* Ds, const double* Dd,
                             const double* VT, const double* U, const double* PVs, const double* PVd,
                             const double* UTI);
-orc_tree* orc_problem_tree(orc_problem* pr, int which /* 0 = exterior, 1 = interior */);
+orc_tree* orc_problem_tree(orc_problem* pr, int which /* depth along the first-child chain */);
 void orc_tree_dense(orc_tree* t, double* S, double* D);   /* npt x npan each, direct integrals */
 
 /* Unit-level helpers */

@@ -69,6 +69,67 @@ orc_problem* orc_problem_single(const double* verts9, int npan, const double* qp
    return pr;
 }
 
+/* Leaf region of the nested operator over panels[p0 .. p0+ntree) with the
+ * first `mynpan` of them its own. */
+static orc_region* make_region(orc_panel* panels, int p0, int ntree, int mynpan, int kernel, double eps,
+                               double kappa, int maxpanels, int gridpoints, double svdeps) {
+   orc_region* r = region_new();
+   r->kernel = kernel;
+   r->eps = eps;
+   r->kappa = kappa;
+   r->mynpan = mynpan;
+   double* cent = (double*)malloc(sizeof(double) * 3 * (size_t)(ntree > 0 ? ntree : 1));
+   for (int i = 0; i < ntree; i++) memcpy(&cent[3 * i], panels[p0 + i].c, 3 * sizeof(double));
+   r->tree = orc_tree_build(panels + p0, ntree, cent, ntree, kernel, kappa, maxpanels, gridpoints, svdeps);
+   orc_tree_fill(r->tree);
+   r->resint = (double*)calloc(mynpan > 0 ? mynpan : 1, sizeof(double));
+   r->resext = (double*)calloc(mynpan > 0 ? mynpan : 1, sizeof(double));
+   free(cent);
+   return r;
+}
+
+static void add_child(orc_region* parent, orc_region* child) {
+   parent->child = (orc_region**)realloc(parent->child, sizeof(orc_region*) * (size_t)(parent->nchild + 1));
+   parent->child[parent->nchild++] = child;
+   child->parent = parent;
+}
+
+/* Stern-layer formulation with one salt (ion-exclusion) surface enclosing
+ * one dielectric surface and its charges, no cavities
+ * (FFTSVDpbeAPI.c:1064-1377): the exterior region (Yukawa, eps_out, kappa)
+ * over the salt panels; inside the salt surface and outside the dielectric
+ * (Laplace, eps_out) over the salt panels followed by the dielectric
+ * panels; inside the dielectric (Laplace, eps_in) over the dielectric
+ * panels, holding the charges.  Unknowns follow the pre-order start
+ * indices: salt phi, salt dphi, dielectric phi, dielectric dphi. */
+orc_problem* orc_problem_stern(const double* salt9, int nsalt, const double* diel9, int ndiel, const double* qpos,
+                               const double* q, int ncharge, double eps_in, double eps_out, double kappa,
+                               int maxpanels, int gridpoints, double svdeps) {
+   orc_problem* pr = (orc_problem*)calloc(1, sizeof(orc_problem));
+   const int n = nsalt + ndiel;
+   pr->ntotal = n;
+   pr->panels = (orc_panel*)malloc(sizeof(orc_panel) * (size_t)n);
+   orc_panels_from_verts(salt9, nsalt, pr->panels);
+   orc_panels_from_verts(diel9, ndiel, pr->panels + nsalt);
+   orc_region* root = make_region(pr->panels, 0, nsalt, 0, ORC_YUKAWA, eps_out, kappa, maxpanels, gridpoints,
+                                  svdeps);
+   orc_region* salt = make_region(pr->panels, 0, n, nsalt, ORC_LAPLACE, eps_out, 0.0, maxpanels, gridpoints,
+                                  svdeps);
+   orc_region* in = make_region(pr->panels, nsalt, ndiel, ndiel, ORC_LAPLACE, eps_in, 0.0, maxpanels, gridpoints,
+                                svdeps);
+   in->ncharge = ncharge;
+   in->qpos = (double*)malloc(sizeof(double) * 3 * (ncharge > 0 ? ncharge : 1));
+   in->q = (double*)malloc(sizeof(double) * (ncharge > 0 ? ncharge : 1));
+   memcpy(in->qpos, qpos, sizeof(double) * 3 * ncharge);
+   memcpy(in->q, q, sizeof(double) * ncharge);
+   add_child(root, salt);
+   add_child(salt, in);
+   pr->root = root;
+   int index = 0;
+   init_start(root, &index);
+   return pr;
+}
+
 static void region_free(orc_region* r) {
    if (!r) return;
    for (int i = 0; i < r->nchild; i++) region_free(r->child[i]);
@@ -87,7 +148,12 @@ void orc_problem_free(orc_problem* pr) {
    free(pr);
 }
 
-orc_tree* orc_problem_tree(orc_problem* pr, int which) { return which == 0 ? pr->root->tree : pr->root->child[0]->tree; }
+/* which = depth along the first-child chain: 0 root, 1 its first child, ... */
+orc_tree* orc_problem_tree(orc_problem* pr, int which) {
+   orc_region* r = pr->root;
+   for (int k = 0; k < which && r->nchild > 0; k++) r = r->child[0];
+   return r->tree;
+}
 
 /* SurfaceOperator_multiply (SurfaceOperator.c:114-158) with
  * joinpotentials / splitpotentials (SurfaceOperator.c:51-87). */

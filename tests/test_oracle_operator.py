@@ -3,8 +3,8 @@ tree partition and list completeness (Tree.c:12-178, Cube.c:352-713);
 compressed apply vs the dense all-pairs operator to the SVD tolerance
 (Tree.c:597-683); exact transpose by the dot-product test (Tree.c:769-832);
 2x2 preconditioner vs its closed form (SurfaceOperator.c:170-197); GMRES
-vs a dense solve (GMRES.c:446-539); analytic Born and Kirkwood spheres
-end to end; the .vert/.face reader (VertFace.c:26-202)."""
+vs a dense solve (GMRES.c:446-539); analytic Born, Kirkwood and
+Stern-layer spheres end to end; the .vert/.face reader (VertFace.c:26-202)."""
 import numpy as np
 import pytest
 
@@ -146,6 +146,35 @@ def test_kirkwood_energy_salt():
     E = pr.energy(sol, 332.06)
     Ea = 332.06 * 0.5 * (1 / (80 * 2.0 * (1 + k * 2.0)) - 1 / (4 * 2.0))
     assert abs(E - Ea) < 0.01 * abs(Ea)
+
+
+def test_stern_sphere_energy():
+    """Stern layer (FFTSVDpbeAPI.c:1064-1377): dielectric sphere a inside an
+    ion-exclusion sphere b, central charge.  Potential A/r + B between the
+    spheres, C exp(-kr)/r outside, continuity of phi and eps dphi/dr give the
+    reaction potential q/(eps_out a) - q k/(eps_out (1+kb)) - q/(eps_in a).
+    The energy is pinned to the flat-panel discretisation error, and the
+    Stern shift (against the same mesh without the layer) to 2 %."""
+    a, b, k = 2.0, 3.0, 0.124
+    Vd, Fd = meshes.icosphere(a, 3)
+    Vs, Fs = meshes.icosphere(b, 3)
+    Pd, Ps = meshes.panels_from_mesh(Vd, Fd), meshes.panels_from_mesh(Vs, Fs)
+    pr = oracle.Problem(Pd, [[0, 0, 0]], [1.0], 4.0, 80.0, k, stern_verts9=Ps)
+    assert pr.n == len(Pd) + len(Ps)
+    sol, it, _ = pr.gmres(pr.rhs(), 1e-6)
+    E = pr.energy(sol, 332.06)
+    Ea = 0.5 * 332.06 * (1 / (80 * a) - k / (80 * (1 + k * b)) - 1 / (4 * a))
+    assert abs(E - Ea) < 0.01 * abs(Ea)
+    assert it < 40
+    # the unknowns of the salt surface come first: the RHS has no charge
+    # terms on the salt phi rows, only on the dielectric's
+    r = pr.rhs()
+    assert np.all(r[:len(Ps)] == 0.0) and np.all(r[2 * len(Ps):2 * len(Ps) + len(Pd)] > 0.0)
+    p0 = oracle.Problem(Pd, [[0, 0, 0]], [1.0], 4.0, 80.0, k)
+    s0, _, _ = p0.gmres(p0.rhs(), 1e-6)
+    shift = E - p0.energy(s0, 332.06)
+    shift_a = Ea - 0.5 * 332.06 * (1 / (80 * a * (1 + k * a)) - 1 / (4 * a))
+    assert abs(shift - shift_a) < 0.02 * abs(shift_a)
 
 
 def test_vertface_reader(tmp_path):
