@@ -409,8 +409,11 @@ void assignRadiiCharges(PDBentry* PDBentries, unsigned int numPDBentries, SIZent
 // angstrom.  Writes <srf>.d0.vert / .face next to the .srf and the .srf.
 // Salt (Stern) surfaces are not generated: that formulation is rejected.
 void generateFlatSRF(const char* pdbfilename, const char* sizfilename, const char* srffilename) {
-   if (ionicstrength > 0.0 && ionexclusionradius > 1e-6)
-      error((char*)"the Stern-layer formulation is not supported; set ionexclusionradius to 0");
+   // with salt and an ion-exclusion radius (the Stern layer) the salt
+   // surface is meshed too: the union of the atom spheres grown by
+   // ionexclusionradius (meshmaker.c:105-133, there MSMS with a vanishing
+   // probe), at the salt vertex density
+   const bool stern = ionicstrength > 0.0 && ionexclusionradius > 1e-6;
    const std::string in(pdbfilename);
    auto ends = [&](const char* e) {
       const size_t n = std::strlen(e);
@@ -439,28 +442,38 @@ void generateFlatSRF(const char* pdbfilename, const char* sizfilename, const cha
       }
    std::free(atoms);
    if (xyzr.empty()) error((char*)"no atom with a positive radius in %s", pdbfilename);
-   const double density = dielectricvertexdensity > 0.0 ? dielectricvertexdensity : 1.0;
    const double probe = proberadius > 0.0 ? proberadius : 1.4;
-   // the mesher's vertex density is about 4.5 / h^2 per square angstrom
-   const double h = std::sqrt(4.5 / density);
-   double* V = nullptr;
-   int* F = nullptr;
-   int nv = 0, nf = 0;
-   if (fftsvd_mesh_ses(xyzr.data(), (int)(xyzr.size() / 4), probe, h, &V, &nv, &F, &nf))
-      error((char*)"surface generation failed for %s", pdbfilename);
    const std::string srf(srffilename);
    const size_t slash = srf.rfind('/');
-   const std::string base = srf + ".d0", name = (slash == std::string::npos ? srf : srf.substr(slash + 1)) + ".d0";
-   FILE* fv = open_or_die("vert", (base + ".vert").c_str(), "w");
-   for (int i = 0; i < nv; i++) std::fprintf(fv, "%.9f %.9f %.9f\n", V[3 * i], V[3 * i + 1], V[3 * i + 2]);
-   std::fclose(fv);
-   FILE* ff = open_or_die("face", (base + ".face").c_str(), "w");
-   for (int i = 0; i < nf; i++) std::fprintf(ff, "%d %d %d 1\n", F[3 * i] + 1, F[3 * i + 1] + 1, F[3 * i + 2] + 1);
-   std::fclose(ff);
-   fftsvd_free(V);
-   fftsvd_free(F);
+   const std::string stem = slash == std::string::npos ? srf : srf.substr(slash + 1);
+   // meshes one surface into <srf><tag>.vert/.face; the mesher's vertex
+   // density is about 4.5 / h^2 per square angstrom
+   auto mesh_to = [&](const char* tag, double density, bool grown) {
+      const double h = std::sqrt(4.5 / (density > 0.0 ? density : 1.0));
+      double* V = nullptr;
+      int* F = nullptr;
+      int nv = 0, nf = 0;
+      const int rc = grown ? fftsvd_mesh_union(xyzr.data(), (int)(xyzr.size() / 4), ionexclusionradius, h, &V, &nv,
+                                               &F, &nf)
+                           : fftsvd_mesh_ses(xyzr.data(), (int)(xyzr.size() / 4), probe, h, &V, &nv, &F, &nf);
+      if (rc) error((char*)"surface generation failed for %s", pdbfilename);
+      const std::string base = srf + tag;
+      FILE* fv = open_or_die("vert", (base + ".vert").c_str(), "w");
+      for (int i = 0; i < nv; i++) std::fprintf(fv, "%.9f %.9f %.9f\n", V[3 * i], V[3 * i + 1], V[3 * i + 2]);
+      std::fclose(fv);
+      FILE* ff = open_or_die("face", (base + ".face").c_str(), "w");
+      for (int i = 0; i < nf; i++) std::fprintf(ff, "%d %d %d 1\n", F[3 * i] + 1, F[3 * i + 1] + 1, F[3 * i + 2] + 1);
+      std::fclose(ff);
+      fftsvd_free(V);
+      fftsvd_free(F);
+      return stem + tag;
+   };
+   const std::string dname = mesh_to(".d0", dielectricvertexdensity, false);
+   const std::string sname = stern ? mesh_to(".s0", saltvertexdensity, true) : std::string();
+   // .srf: panel types, salt surfaces, dielectrics, dielectric cavities,
+   // salt cavities, then the parents of dielectrics and of both cavity kinds
    FILE* fs = open_or_die("surface", srffilename, "w");
-   std::fprintf(fs, "f\nf\n\n%s\n\n\n0\n\n\n", name.c_str());
+   std::fprintf(fs, "f\nf\n%s\n%s\n\n\n0\n\n\n", sname.c_str(), dname.c_str());
    std::fclose(fs);
 }
 

@@ -101,13 +101,13 @@ struct UF {
    }
 };
 
-}  // namespace
-
-extern "C" {
-
-int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, double** verts_out, int* nverts,
-                    int** faces_out, int* nfaces) {
-   try {
+// SES of the atoms with a probe of radius `probe` (union_mode false), or
+// the surface of the union of the atom spheres grown by `probe` (union_mode
+// true: the ion-exclusion surface, which the reference's meshmaker asks of
+// MSMS with a vanishing probe, meshmaker.c:105-133).
+int mesh_impl(const double* xyzr, int natoms, double probe, double h, bool union_mode, double** verts_out,
+              int* nverts, int** faces_out, int* nfaces) {
+   {
       if (natoms < 1 || !(h > 0) || !(probe > 0)) throw std::runtime_error("bad mesher arguments");
       std::vector<double> ctr(3 * (size_t)natoms);
       std::vector<double> R(natoms);
@@ -131,7 +131,7 @@ int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, doub
       // accessible samples on the grown spheres (golden-angle spirals)
       const double delta = std::min(0.25, 0.5 * h);
       std::vector<double> samp;
-      {
+      if (!union_mode) {
          std::vector<std::vector<double>> part(natoms);
          parallel_for(natoms, [&](long long i) {
             int n = std::max(32, (int)std::ceil(4 * M_PI * R[i] * R[i] / (delta * delta)));
@@ -173,7 +173,9 @@ int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, doub
             dsas = std::min(dsas, d);
          });
          double val;
-         if (dsas >= 0) {
+         if (union_mode) {
+            val = -std::max(-10.0, std::min(dsas, 10.0));   // positive inside the grown spheres
+         } else if (dsas >= 0) {
             val = -probe - std::min(dsas, 10.0);
          } else {
             double best = cut * cut;
@@ -309,8 +311,29 @@ int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, doub
       std::memcpy(*verts_out, Vo.data(), sizeof(double) * Vo.size());
       std::memcpy(*faces_out, Fo.data(), sizeof(int) * Fo.size());
       return 0;
+   }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, double** verts_out, int* nverts,
+                    int** faces_out, int* nfaces) {
+   try {
+      return mesh_impl(xyzr, natoms, probe, h, false, verts_out, nverts, faces_out, nfaces);
    } catch (const std::exception& e) {
       std::fprintf(stderr, "fftsvd_mesh_ses: %s\n", e.what());
+      return -1;
+   }
+}
+
+int fftsvd_mesh_union(const double* xyzr, int natoms, double grow, double h, double** verts_out, int* nverts,
+                      int** faces_out, int* nfaces) {
+   try {
+      return mesh_impl(xyzr, natoms, grow, h, true, verts_out, nverts, faces_out, nfaces);
+   } catch (const std::exception& e) {
+      std::fprintf(stderr, "fftsvd_mesh_union: %s\n", e.what());
       return -1;
    }
 }

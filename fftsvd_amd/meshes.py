@@ -126,12 +126,23 @@ def synthetic_globule(natoms=2000, seed=20261202, step=1.5, minsep=2.4, radius_r
 def ses_mesh(xyz, radii, probe=1.4, spacing=0.6):
     """Solvent-excluded surface triangulated by the library's mesher
     (fftsvd_mesh_ses); returns (verts, faces) in the MSMS convention."""
+    return _mesh("fftsvd_mesh_ses", xyz, radii, probe, spacing)
+
+
+def union_mesh(xyz, radii, grow, spacing=0.6):
+    """Surface of the union of the atom spheres grown by `grow` (the
+    ion-exclusion surface of the Stern layer), by the library's mesher
+    (fftsvd_mesh_union); (verts, faces) in the MSMS convention."""
+    return _mesh("fftsvd_mesh_union", xyz, radii, grow, spacing)
+
+
+def _mesh(entry, xyz, radii, probe, spacing):
     import ctypes as C
 
     from fftsvd_amd import lib
 
     L = lib()
-    f = L.fftsvd_mesh_ses
+    f = getattr(L, entry)
     f.restype = C.c_int
     f.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double, C.POINTER(C.POINTER(C.c_double)),
                   C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
@@ -144,7 +155,7 @@ def ses_mesh(xyz, radii, probe=1.4, spacing=0.6):
     rc = f(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), probe, spacing, C.byref(pv), C.byref(nv), C.byref(pf),
            C.byref(nf))
     if rc != 0:
-        raise RuntimeError("SES mesher failed")
+        raise RuntimeError("mesher failed (%s)" % entry)
     V = np.ctypeslib.as_array(pv, shape=(nv.value * 3,)).reshape(-1, 3).copy()
     F = np.ctypeslib.as_array(pf, shape=(nf.value * 3,)).reshape(-1, 3).copy().astype(np.int64)
     L.fftsvd_free(C.cast(pv, C.c_void_p))

@@ -117,6 +117,12 @@ int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, c
  * counter-clockwise seen from outside; free both arrays with fftsvd_free. */
 int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, double** verts, int* nverts,
                     int** faces, int* nfaces);
+/* Surface of the union of the atom spheres grown by `grow` (x, y, z, r per
+ * atom), contoured like fftsvd_mesh_ses: the ion-exclusion (Stern) surface
+ * the reference's meshmaker asks MSMS for with radii + ionexclusionradius
+ * and a vanishing probe (meshmaker.c:105-133). */
+int fftsvd_mesh_union(const double* xyzr, int natoms, double grow, double h, double** verts, int* nverts,
+                      int** faces, int* nfaces);
 void fftsvd_free(void* p);
 
 /* Precision used by solveSurfaceOperator (0 FP64, 1 mixed); default 0. */

@@ -198,3 +198,71 @@ def test_generate_flat_srf(tmp_path):
     assert np.abs(n.sum(axis=0)).max() < 1e-8 * np.abs(n).sum()
     vol = np.einsum("ij,ij->", P.mean(axis=1), n) / 3
     assert vol > 20.0
+
+
+def _closed_outward(P):
+    """(area-vector sum relative to total area, enclosed volume) of panels
+    in the reference's order (v0, v2, v1 of the MSMS face)."""
+    n = np.cross(P[:, 2] - P[:, 0], P[:, 1] - P[:, 0]) / 2
+    return np.abs(n.sum(axis=0)).max() / np.abs(n).sum(), np.einsum("ij,ij->", P.mean(axis=1), n) / 3
+
+
+def test_union_mesh_grown_spheres():
+    """fftsvd_mesh_union: the surface of the union of spheres grown by g.
+    One atom: every vertex at r + g (to the grid's interpolation error) and
+    the enclosed volume that of the sphere; two overlapping atoms: closed,
+    outward, vertices on the grown spheres (away from their crease)."""
+    from fftsvd_amd import meshes
+    r, g, h = 1.7, 2.0, 0.35
+    V, F = meshes.union_mesh(np.zeros((1, 3)), [r], g, h)
+    d = np.linalg.norm(V, axis=1)
+    assert np.abs(d - (r + g)).max() < 0.05 * h + 1e-3
+    P = meshes.panels_from_mesh(V, F).reshape(-1, 3, 3)
+    closure, vol = _closed_outward(P)
+    assert closure < 1e-10
+    assert abs(vol - 4 / 3 * np.pi * (r + g) ** 3) < 0.01 * vol
+    c = np.array([[0, 0, 0], [2.5, 0, 0]])
+    V, F = meshes.union_mesh(c, [1.5, 1.2], 1.0, h)
+    R = np.array([2.5, 2.2])
+    dist = np.abs((np.linalg.norm(V[:, None, :] - c[None], axis=2) - R[None]).min(axis=1))
+    # linear interpolation cuts the crease where the spheres meet: there
+    # the contour lies inside by up to a quarter cell
+    assert np.quantile(dist, 0.95) < 0.05 * h + 1e-3 and dist.max() < 0.3 * h
+    closure, vol = _closed_outward(meshes.panels_from_mesh(V, F).reshape(-1, 3, 3))
+    assert closure < 1e-10 and vol > 0
+
+
+def test_generate_flat_srf_stern(tmp_path):
+    """With salt and ionexclusionradius > 0, generateFlatSRF also writes the
+    ion-exclusion surface (atom spheres grown by the radius) as the salt
+    surface of the .srf, enclosing the dielectric."""
+    siz, _ = write_tables(tmp_path)
+    f = tmp_path / "m.pdb"
+    f.write_text(pdb_line("ATOM", 1, "N", "ALA", "A", 1, 0, 0, 0) + pdb_line("ATOM", 2, "CA", "ALA", "A", 1, 1.45, 0, 0) +
+                 pdb_line("ATOM", 3, "O", "ALA", "A", 1, 2.2, 1.2, 0))
+    refapi.set_global("proberadius", 1.4)
+    refapi.set_global("dielectricvertexdensity", 4.0)
+    refapi.set_global("saltvertexdensity", 2.0)
+    refapi.set_global("ionicstrength", 0.145)
+    refapi.set_global("ionexclusionradius", 2.0)
+    srf = tmp_path / "op.srf"
+    try:
+        L.generateFlatSRF(str(f).encode(), str(siz).encode(), str(srf).encode())
+    finally:
+        refapi.set_global("ionicstrength", 0.0)
+        refapi.set_global("ionexclusionradius", 0.0)
+    assert srf.read_text().splitlines()[2] == "op.srf.s0"
+    s = refapi.read_srf(srf)
+    assert s.num[0].value == 1 and s.num[1].value == 1
+    assert s.parents[0][0] == 0
+    Ps = panels_of(s, 0, 0).reshape(-1, 3, 3)
+    Pd = panels_of(s, 1, 0).reshape(-1, 3, 3)
+    assert s.total.value == len(Ps) + len(Pd)
+    cs, vs = _closed_outward(Ps)
+    cd, vd = _closed_outward(Pd)
+    assert cs < 1e-8 and cd < 1e-8 and vs > vd > 20.0
+    # every salt vertex is outside the dielectric surface's bounding box grown by 1 A
+    lo, hi = Pd.reshape(-1, 3).min(axis=0), Pd.reshape(-1, 3).max(axis=0)
+    vs_pts = Ps.reshape(-1, 3)
+    inside = np.all((vs_pts > lo + 1.0) & (vs_pts < hi - 1.0), axis=1)
+    assert not inside.any()
