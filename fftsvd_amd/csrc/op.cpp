@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstring>
 #include <functional>
+#include <string>
 #include <iterator>
 #include <stdexcept>
 
@@ -51,7 +52,8 @@ static bool timing_on() {
 }
 static void timing_mark(const char* what, double& t) {
    const double n = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-   if (timing_on()) std::fprintf(stderr, "[fftsvd setup] %-28s %8.2f ms\n", what, n - t);
+   // duration of the step and its end on the process clock
+   if (timing_on()) std::fprintf(stderr, "[fftsvd setup] %-28s %8.2f ms  (ends %.2f)\n", what, n - t, n);
    t = n;
 }
 
@@ -448,6 +450,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    t.kappa = kappa;
    t.svdeps = svdeps;
    op.stats.phase_ms[0] += now_ms() - tp;
+   timing_mark("tree", tp);
    op.stats.depth = t.depth;
    op.stats.ncubes = (int)t.cubes.size();
 
@@ -510,6 +513,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    FSVD_CHECK(hipStreamSynchronize(s));
    op.stats.setup_ms_near = now_ms() - t0;
    op.stats.phase_ms[1] += op.stats.setup_ms_near;
+   timing_mark("near field", t0);
    t0 = now_ms();
 
    // ---------------- far field ----------------
@@ -564,7 +568,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       DBuf<double> d_pinv;
       d_pinv.upload(op.h_pinv[l], s);
       ph[2] += now_ms() - tp;
-      timing_mark("level: pinv + T", tl);
+      timing_mark((std::string("L") + std::to_string(l) + " " + "pinv + T").c_str(), tl);
       tp = now_ms();
 
       // cubes of this level with interactions
@@ -667,10 +671,10 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          FSVD_CHECK(hipStreamSynchronize(s));
          ph[3] += now_ms() - tp;
          tp = now_ms();
-         timing_mark("level: sampling", tl);
+         timing_mark((std::string("L") + std::to_string(l) + " " + "sampling").c_str(), tl);
          std::vector<int> ranks;
          run_mgs(mgs, Abuf->p, Qbuf->p, ranks, svdeps, s);
-         timing_mark("level: pivoted MGS", tl);
+         timing_mark((std::string("L") + std::to_string(l) + " " + "pivoted MGS").c_str(), tl);
          ph[4] += now_ms() - tp;
          tp = now_ms();
          std::vector<int> next;
@@ -763,7 +767,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       keep.clear();
 
-      timing_mark("level: compaction", tl);
+      timing_mark((std::string("L") + std::to_string(l) + " " + "compaction").c_str(), tl);
       // ---- equivalent densities: PV = P VT^T, UTI = U^T I (Cube.c:795-1002,1414-1522) ----
       {
          std::vector<IntBlock> blocks;
@@ -911,7 +915,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       }
       FSVD_CHECK(hipStreamSynchronize(s));
       ph[5] += now_ms() - tp;
-      timing_mark("level: PV, UTI", tl);
+      timing_mark((std::string("L") + std::to_string(l) + " " + "PV, UTI").c_str(), tl);
       lev_far[l - 2] = (long long)(lv.VT64.n + lv.PV64.n + lv.UT64.n + lv.UTI64.n);
       op.levels[l - 2] = std::move(lv);
    };

@@ -3,6 +3,8 @@
 #include "solver.h"
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -78,8 +80,31 @@ Problem::~Problem() {
 // region (eps_in) over its panels and its cavities', holding the charges;
 // per dielectric cavity a Laplace region (eps_out) over its panels and its
 // salt cavities'; per salt cavity a Yukawa region.
+// Runs fn(i) for i in [0, n) on up to 16 host threads (contiguous ranges).
+template <typename F>
+static void host_parallel_for(long long n, F&& fn) {
+   const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   if (n < 4096 || nth == 1) {
+      for (long long i = 0; i < n; i++) fn(i);
+      return;
+   }
+   std::vector<std::thread> th;
+   for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t] {
+         const long long a = n * t / nth, b = n * (t + 1) / nth;
+         for (long long i = a; i < b; i++) fn(i);
+      });
+   for (auto& x : th) x.join();
+}
+
+static void setup_event(const char* what) {
+   const char* e = std::getenv("FFTSVD_TIMING");
+   if (e && e[0] == '1') std::fprintf(stderr, "[fftsvd setup] %-28s at %.2f\n", what, now_ms());
+}
+
 std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    double t0 = now_ms();
+   setup_event("problem start");
    auto prp = std::make_unique<Problem>();
    Problem& pr = *prp;
    FSVD_CHECK(hipStreamCreate(&pr.stream));
@@ -94,12 +119,12 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
       for (size_t i = 0; i < surf.size(); i++) {
          const int n = (int)(surf[i].size() / 9);
          start[i + 1] = start[i] + n;
-         for (int k = 0; k < n; k++) {
+         const size_t base = pr.panels.size();
+         pr.panels.resize(base + n);
+         host_parallel_for(n, [&](long long k) {
             const double* v = &surf[i][9 * (size_t)k];
-            PanelGeom g;
-            panel_geometry(v, v + 3, v + 6, &g);
-            pr.panels.push_back(g);
-         }
+            panel_geometry(v, v + 3, v + 6, &pr.panels[base + k]);
+         });
       }
       return start;
    };
@@ -108,6 +133,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    const std::vector<int> cstart = add_surfaces(spec.cavity_verts);
    const std::vector<int> scstart = add_surfaces(spec.salt_cavity_verts);
    pr.ntotal = (int)pr.panels.size();
+   setup_event("panel geometry");
    const int nsalt = (int)spec.salt_verts.size(), nd = (int)spec.dielectric_verts.size();
    const int nc = (int)spec.cavity_verts.size(), nsc = (int)spec.salt_cavity_verts.size();
    if (stern && ((int)spec.dielectric_parent.size() != nd || (int)spec.salt_cavity_parent.size() != nsc))
@@ -205,29 +231,53 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
          if (R.shares_tree_with < 0) todo.push_back((int)r);
       }
       const size_t nt = todo.size();
+      // operators over the same panel set share its host and device
+      // geometry and its octree (tree_src: the first such operator)
       std::vector<std::vector<PanelGeom>> tp(nt);
       std::vector<std::vector<double>> cent(nt);
       std::vector<int> tree_src(nt, -1);
       std::vector<Tree> trees(nt);
+      std::vector<std::thread> uploads;
+      std::vector<std::exception_ptr> up_err(nt);
+      int dev = 0;
+      FSVD_CHECK(hipGetDevice(&dev));
       for (size_t k = 0; k < nt; k++) {
          Region& R = pr.regions[todo[k]];
-         tp[k].resize(R.panel_ids.size());
-         cent[k].resize(3 * R.panel_ids.size());
-         for (size_t i = 0; i < R.panel_ids.size(); i++) {
-            tp[k][i] = pr.panels[R.panel_ids[i]];
-            for (int d = 0; d < 3; d++) cent[k][3 * i + d] = tp[k][i].c[d];
-         }
-         R.d_panels.upload(tp[k], s);
          for (size_t o = 0; o < k; o++)
             if (tree_src[o] < 0 && pr.regions[todo[o]].panel_ids == R.panel_ids) {
                tree_src[k] = (int)o;
                break;
             }
-         if (tree_src[k] < 0)
-            build_tree(trees[k], tp[k].data(), (int)tp[k].size(), cent[k].data(), (int)tp[k].size(),
-                       spec.maxpanels, spec.gridpoints);
+         if (tree_src[k] >= 0) continue;
+         const size_t n = R.panel_ids.size();
+         tp[k].resize(n);
+         cent[k].resize(3 * n);
+         host_parallel_for((long long)n, [&](long long i) {
+            tp[k][i] = pr.panels[R.panel_ids[i]];
+            for (int d = 0; d < 3; d++) cent[k][3 * i + d] = tp[k][i].c[d];
+         });
+         // the pageable copy blocks its thread: overlap it with the host tree
+         uploads.emplace_back([&, k] {
+            try {
+               FSVD_CHECK(hipSetDevice(dev));
+               pr.regions[todo[k]].d_panels.upload(tp[k], s);
+            } catch (...) {
+               up_err[k] = std::current_exception();
+            }
+         });
+         try {
+            build_tree(trees[k], tp[k].data(), (int)n, cent[k].data(), (int)n, spec.maxpanels, spec.gridpoints);
+         } catch (...) {
+            for (auto& t : uploads) t.join();
+            throw;
+         }
+         setup_event("panels and tree");
       }
+      for (auto& t : uploads) t.join();
+      for (auto& e : up_err)
+         if (e) std::rethrow_exception(e);
       FSVD_CHECK(hipStreamSynchronize(s));
+      setup_event("trees ready");
       std::vector<hipStream_t> streams(nt);
       for (size_t k = 0; k < nt; k++) FSVD_CHECK(hipStreamCreate(&streams[k]));
       std::vector<std::exception_ptr> err(nt);
@@ -236,9 +286,11 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
          th.emplace_back([&, k]() {
             try {
                Region& R = pr.regions[todo[k]];
-               const Tree& pre = trees[tree_src[k] < 0 ? k : (size_t)tree_src[k]];
-               R.op = build_tree_op(tp[k], R.d_panels.p, cent[k].data(), (int)tp[k].size(), R.kernel, R.kappa,
-                                    spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32, streams[k], &pre);
+               const size_t src = tree_src[k] < 0 ? k : (size_t)tree_src[k];
+               const Tree& pre = trees[src];
+               R.op = build_tree_op(tp[src], pr.regions[todo[src]].d_panels.p, cent[src].data(), (int)tp[src].size(),
+                                    R.kernel, R.kappa, spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32,
+                                    streams[k], &pre);
                FSVD_CHECK(hipStreamSynchronize(streams[k]));
             } catch (...) {
                err[k] = std::current_exception();
@@ -371,6 +423,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned, sizeof(double) * 2048));
    FSVD_CHECK(hipStreamSynchronize(s));
    pr.setup_ms = now_ms() - t0;
+   setup_event("problem end");
    return prp;
 }
 

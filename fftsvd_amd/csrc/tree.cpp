@@ -4,6 +4,7 @@
 // integrals are near-field; the storage is this library's own: cubes in a
 // flat array (parents first) and panels/points renumbered into "tree order"
 // so that every cube owns contiguous index ranges.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -87,38 +88,253 @@ struct Builder {
 };
 
 void make_lists(Tree& t) {
-   for (size_t id = 0; id < t.cubes.size(); id++) {
+   // compact copies of the fields the lists read
+   const int nc = (int)t.cubes.size();
+   std::vector<int> idx(3 * (size_t)nc), child(8 * (size_t)nc);
+   for (int i = 0; i < nc; i++) {
+      for (int d = 0; d < 3; d++) idx[3 * (size_t)i + d] = t.cubes[i].idx[d];
+      for (int k = 0; k < 8; k++) child[8 * (size_t)i + k] = t.cubes[i].child[k];
+   }
+   int near[8 * 27], far[8 * 27];
+   for (int id = 0; id < nc; id++) {
       Cube& c = t.cubes[id];
       c.loc.clear();
       c.itr.clear();
       if (c.parent < 0) continue;
-      const Cube& P = t.cubes[c.parent];
+      const int* pch = &child[8 * (size_t)c.parent];
+      const int x = idx[3 * (size_t)id], y = idx[3 * (size_t)id + 1], z = idx[3 * (size_t)id + 2];
+      // siblings, then the parent's neighbours' children in list order:
+      // adjacent ones are local, the rest interact
+      int nn = 0, nf = 0;
       for (int k = 0; k < 8; k++)
-         if (P.child[k] >= 0 && P.child[k] != (int)id) c.loc.push_back(P.child[k]);
-      for (int l : P.loc) {
-         const Cube& L = t.cubes[l];
+         if (pch[k] >= 0 && pch[k] != id) near[nn++] = pch[k];
+      for (int l : t.cubes[c.parent].loc) {
+         const int* lch = &child[8 * (size_t)l];
          for (int k = 0; k < 8; k++) {
-            int o = L.child[k];
+            const int o = lch[k];
             if (o < 0) continue;
-            const Cube& O = t.cubes[o];
-            if (std::abs(O.idx[0] - c.idx[0]) <= 1 && std::abs(O.idx[1] - c.idx[1]) <= 1 &&
-                std::abs(O.idx[2] - c.idx[2]) <= 1)
-               c.loc.push_back(o);
+            const int* oi = &idx[3 * (size_t)o];
+            if (std::abs(oi[0] - x) <= 1 && std::abs(oi[1] - y) <= 1 && std::abs(oi[2] - z) <= 1)
+               near[nn++] = o;
+            else
+               far[nf++] = o;
          }
       }
-      for (int l : P.loc) {
-         const Cube& L = t.cubes[l];
-         for (int k = 0; k < 8; k++) {
-            int o = L.child[k];
-            if (o < 0) continue;
-            const Cube& O = t.cubes[o];
-            if (std::abs(O.idx[0] - c.idx[0]) > 1 || std::abs(O.idx[1] - c.idx[1]) > 1 ||
-                std::abs(O.idx[2] - c.idx[2]) > 1)
-               c.itr.push_back(o);
-         }
-      }
+      c.loc.assign(near, near + nn);
+      c.itr.assign(far, far + nf);
    }
 }
+
+// Level-synchronous construction of the same tree on flat arrays.  Every
+// panel and point carries its cell bounds and a cell code (3 bits per level,
+// child k = 4*cx + 2*cy + cz, the Builder's order); one level deeper is the
+// Builder's centroid comparison and bound update per entity, then a stable
+// 8-way regroup inside each cell, so each cell's entities stay in ascending
+// index order -- exactly the Builder's per-cube lists -- and cells in code
+// order are the Builder's depth-first leaf order.  The depth loop adds one
+// level per step instead of rebuilding.  Returns false (tree untouched) for
+// non-finite coordinates or a depth past MAXLEV; the Builder then decides.
+class LevelTree {
+ public:
+   static constexpr int MAXLEV = 20;
+
+   LevelTree(const PanelGeom* panels, int npan, const double* pts, int npt) : npan_(npan), npt_(npt) {
+      const int ne = npan + npt;
+      x_.resize(3 * (size_t)ne);
+      for (int i = 0; i < npan; i++)
+         for (int d = 0; d < 3; d++) x_[3 * (size_t)i + d] = panels[i].c[d];
+      if (npt) std::memcpy(&x_[3 * (size_t)npan], pts, sizeof(double) * 3 * (size_t)npt);
+   }
+
+   bool build(Tree& t, const double b0[3], const double b1[3], int maxpanels) {
+      const int ne = npan_ + npt_;
+      for (size_t i = 0; i < x_.size(); i++)
+         if (!std::isfinite(x_[i])) return false;
+      lo_.resize(3 * (size_t)ne);
+      hi_.resize(3 * (size_t)ne);
+      for (int e = 0; e < ne; e++)
+         for (int d = 0; d < 3; d++) {
+            lo_[3 * (size_t)e + d] = b0[d];
+            hi_[3 * (size_t)e + d] = b1[d];
+         }
+      code_.assign(ne, 0);
+      ord_.assign(1, std::vector<int>(ne));
+      for (int i = 0; i < npan_; i++) ord_[0][i] = i;
+      for (int i = 0; i < npt_; i++) ord_[0][npan_ + i] = npan_ + i;
+      cells_.assign(1, {});
+      cells_[0].push_back({0, 0, npan_, npan_, ne});
+      int depth = 2;
+      for (;; depth++) {
+         if (depth - 1 > MAXLEV) return false;
+         while ((int)ord_.size() < depth)
+            if (!deepen()) return false;
+         if (npan_ == 0 || npt_ == 0) break;
+         const std::vector<Cell>& leaves = cells_[depth - 1];
+         int within = 0;
+         for (const Cell& c : leaves)
+            if (c.pb - c.pa <= maxpanels) within++;
+         if ((double)within / (double)leaves.size() > 0.90) break;
+      }
+      emit(t, depth, b0, b1);
+      return true;
+   }
+
+ private:
+   struct Cell {
+      uint64_t code;
+      int pa, pb;   // panel range in this level's order
+      int qa, qb;   // point range (offset by npan)
+   };
+
+   // one level deeper for every entity, regroup, list the cells
+   bool deepen() {
+      const int ne = npan_ + npt_;
+      for (int e = 0; e < ne; e++) {
+         int k = 0;
+         for (int d = 0; d < 3; d++) {
+            double& lo = lo_[3 * (size_t)e + d];
+            double& hi = hi_[3 * (size_t)e + d];
+            const double s = lo + hi;
+            const double ctr = s * 0.5;
+            const double g = x_[3 * (size_t)e + d];
+            if (g >= ctr) {
+               const double diff = ctr - lo;
+               lo += diff;
+               hi = ctr + diff;
+               k |= 4 >> d;
+            } else if (g < ctr) {
+               hi = ctr;
+            } else {
+               return false;
+            }
+         }
+         code_[e] = code_[e] << 3 | (uint64_t)k;
+      }
+      const std::vector<int>& prev = ord_.back();
+      std::vector<int> next(ne);
+      // panels then points, each regrouped inside the previous cells
+      auto regroup = [&](int a, int b) {
+         int i = a;
+         while (i < b) {
+            const uint64_t parent = code_[prev[i]] >> 3;
+            int j = i + 1;
+            while (j < b && code_[prev[j]] >> 3 == parent) j++;
+            int cnt[9] = {0};
+            for (int r = i; r < j; r++) cnt[(code_[prev[r]] & 7) + 1]++;
+            for (int k = 0; k < 8; k++) cnt[k + 1] += cnt[k];
+            for (int r = i; r < j; r++) next[i + cnt[code_[prev[r]] & 7]++] = prev[r];
+            i = j;
+         }
+      };
+      regroup(0, npan_);
+      regroup(npan_, ne);
+      // cells: merge of the distinct panel and point codes
+      std::vector<Cell> cells;
+      int i = 0, j = npan_;
+      while (i < npan_ || j < ne) {
+         uint64_t c;
+         if (j >= ne) c = code_[next[i]];
+         else if (i >= npan_) c = code_[next[j]];
+         else c = std::min(code_[next[i]], code_[next[j]]);
+         Cell cell{c, i, i, j, j};
+         while (i < npan_ && code_[next[i]] == c) i++;
+         while (j < ne && code_[next[j]] == c) j++;
+         cell.pb = i;
+         cell.qb = j;
+         cells.push_back(cell);
+      }
+      ord_.push_back(std::move(next));
+      cells_.push_back(std::move(cells));
+      return true;
+   }
+
+   // cubes in the Builder's depth-first order, lists and tree order
+   void emit(Tree& t, int depth, const double b0[3], const double b1[3]) {
+      t.depth = depth;
+      t.cubes.clear();
+      const int L = depth - 1;
+      std::vector<size_t> cursor(L + 1, 0);
+      std::vector<std::pair<int, int>> where;   // per cube: level, cell
+      const int zero[3] = {0, 0, 0};
+      add_cube(t, 0, zero, b0, b1, -1, L, cursor, where);
+      const int nc = (int)t.cubes.size();
+      t.pan_perm.resize(npan_);
+      t.pt_perm.resize(npt_);
+      for (int i = 0; i < npan_; i++) t.pan_perm[i] = ord_[L][i];
+      for (int i = 0; i < npt_; i++) t.pt_perm[i] = ord_[L][npan_ + i] - npan_;
+      t.cube_pan_sorted_off.assign(nc + 1, 0);
+      t.cube_pt_sorted_off.assign(nc + 1, 0);
+      for (int id = 0; id < nc; id++) {
+         const Cell& c = cells_[where[id].first][where[id].second];
+         Cube& cb = t.cubes[id];
+         cb.np = c.pb - c.pa;
+         cb.nq = c.qb - c.qa;
+         if (cb.leaf) {
+            cb.p0 = c.pa;
+            cb.q0 = c.qa - npan_;
+         }
+         t.cube_pan_sorted_off[id + 1] = t.cube_pan_sorted_off[id] + cb.np;
+         t.cube_pt_sorted_off[id + 1] = t.cube_pt_sorted_off[id] + cb.nq;
+      }
+      t.cube_pan_sorted.resize(t.cube_pan_sorted_off[nc]);
+      t.cube_pt_sorted.resize(t.cube_pt_sorted_off[nc]);
+      for (int id = 0; id < nc; id++) {
+         const std::vector<int>& o = ord_[where[id].first];
+         const Cell& c = cells_[where[id].first][where[id].second];
+         std::copy(o.begin() + c.pa, o.begin() + c.pb, t.cube_pan_sorted.begin() + t.cube_pan_sorted_off[id]);
+         int* q = t.cube_pt_sorted.data() + t.cube_pt_sorted_off[id];
+         for (int r = c.qa; r < c.qb; r++) *q++ = o[r] - npan_;
+      }
+   }
+
+   // the cube of cell cursor[level] (the next unvisited cell of that level)
+   int add_cube(Tree& t, int level, const int idx[3], const double b0[3], const double b1[3], int parent, int L,
+                std::vector<size_t>& cursor, std::vector<std::pair<int, int>>& where) {
+      const int id = (int)t.cubes.size();
+      const int cell = (int)cursor[level]++;
+      const uint64_t code = cells_[level][cell].code;
+      t.cubes.emplace_back();
+      where.push_back({level, cell});
+      {
+         Cube& c = t.cubes[id];
+         c.level = level;
+         for (int d = 0; d < 3; d++) {
+            c.idx[d] = idx[d];
+            c.b0[d] = b0[d];
+            c.b1[d] = b1[d];
+            double s = b0[d] + b1[d];
+            c.ctr[d] = s * 0.5;
+         }
+         c.parent = parent;
+         for (int k = 0; k < 8; k++) c.child[k] = -1;
+         c.leaf = (level == L);
+      }
+      if (level == L) return id;
+      const double ctr[3] = {t.cubes[id].ctr[0], t.cubes[id].ctr[1], t.cubes[id].ctr[2]};
+      const std::vector<Cell>& next = cells_[level + 1];
+      while (cursor[level + 1] < next.size() && next[cursor[level + 1]].code >> 3 == code) {
+         const int k = (int)(next[cursor[level + 1]].code & 7);
+         const int sel[3] = {(k >> 2) & 1, (k >> 1) & 1, k & 1};
+         double cb0[3] = {b0[0], b0[1], b0[2]}, cb1[3] = {ctr[0], ctr[1], ctr[2]};
+         for (int d = 0; d < 3; d++)
+            if (sel[d]) {
+               double diff = ctr[d] - cb0[d];
+               cb0[d] += diff;
+               cb1[d] += diff;
+            }
+         const int cidx[3] = {2 * idx[0] + sel[0], 2 * idx[1] + sel[1], 2 * idx[2] + sel[2]};
+         const int ch = add_cube(t, level + 1, cidx, cb0, cb1, id, L, cursor, where);
+         t.cubes[id].child[k] = ch;
+      }
+      return id;
+   }
+
+   int npan_, npt_;
+   std::vector<double> x_, lo_, hi_;
+   std::vector<uint64_t> code_;
+   std::vector<std::vector<int>> ord_;    // per level: panels then points, cell order
+   std::vector<std::vector<Cell>> cells_;
+};
 
 }  // namespace
 
@@ -147,6 +363,27 @@ void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, i
    for (int d = 0; d < 3; d++) {
       b0[d] = ctr[d] - ext * 0.5;
       b1[d] = ctr[d] + ext * 0.5;
+   }
+   if (LevelTree(panels, npan, pts, npt).build(t, b0, b1, maxpanels)) {
+      t.gpl.assign(t.depth, 0);
+      t.gpl[t.depth - 1] = gridpoints;
+      if (t.depth >= 2) t.gpl[t.depth - 2] = gridpoints;
+      for (int i = t.depth - 3, p = 1; i >= 0; i--, p++) t.gpl[i] = gridpoints + p;
+      make_lists(t);
+      for (int i = (int)t.cubes.size() - 1; i >= 0; i--) {
+         Cube& c = t.cubes[i];
+         if (c.leaf) continue;
+         int p0 = -1, q0 = -1;
+         for (int k = 0; k < 8; k++) {
+            int ch = c.child[k];
+            if (ch < 0) continue;
+            if (p0 < 0) p0 = t.cubes[ch].p0;
+            if (q0 < 0) q0 = t.cubes[ch].q0;
+         }
+         c.p0 = p0 < 0 ? 0 : p0;
+         c.q0 = q0 < 0 ? 0 : q0;
+      }
+      return;
    }
    int zero[3] = {0, 0, 0};
    std::vector<int> all_pan(npan), all_pt(npt);
