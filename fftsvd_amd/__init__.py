@@ -35,6 +35,9 @@ _SIGS = {
                                               C.c_int, C.c_double, C.c_int, C.POINTER(_vp)]),
     "fftsvd_problem_destroy": (C.c_int, [_vp]),
     "fftsvd_problem_size": (C.c_int, [_vp, _ip, _ip]),
+    "fftsvd_problem_unknowns": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                     C.c_double, C.c_int, C.POINTER(_vp)]),
     "fftsvd_problem_matvec": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_precond": (C.c_int, [_vp, _dp, _dp]),
     "fftsvd_problem_rhs": (C.c_int, [_vp, _dp]),
@@ -211,6 +214,7 @@ class Problem:
                                                kappa, maxpanels, gridpoints, svdeps, int(want_f32), C.byref(h)))
             self.n = len(v)
         self.h = h
+        self.nvec = 2 * self.n
 
     def close(self):
         if getattr(self, "h", None):
@@ -227,23 +231,23 @@ class Problem:
         return Tree(self, which)
 
     def matvec(self, x, precision=F64):
-        y = np.zeros(2 * self.n)
+        y = np.zeros(self.nvec)
         _check(lib().fftsvd_problem_matvec(self.h, precision, _d(_f64(x)), _d(y)))
         return y
 
     def rhs(self):
-        r = np.zeros(2 * self.n)
+        r = np.zeros(self.nvec)
         _check(lib().fftsvd_problem_rhs(self.h, _d(r)))
         return r
 
     def precond_solve(self, b):
-        x = np.zeros(2 * self.n)
+        x = np.zeros(self.nvec)
         _check(lib().fftsvd_problem_precond(self.h, _d(_f64(b)), _d(x)))
         return x
 
     def solve(self, rhs, tol=1e-6, maxit=1000, precision=F64):
         """Returns (solution, iterations, residual history, info dict)."""
-        sol = np.zeros(2 * self.n)
+        sol = np.zeros(self.nvec)
         it = C.c_int()
         hist = np.zeros(maxit)
         info = np.zeros(6)
@@ -280,3 +284,37 @@ class Problem:
         parts = np.zeros(4)
         _check(lib().fftsvd_problem_bench_matvec(self.h, precision, nrep, C.byref(tot), _d(parts)))
         return tot.value / nrep, dict(zip(("leaf", "project_source", "translate", "target"), parts.tolist()))
+
+
+class Qual(Problem):
+    """Qualocation operator (generateQualocationOperator ->
+    generateQualocationOperatorPreconditioner -> generateQualocationRHS ->
+    solveQualocationOperator -> collectQualocationPotentials): Laplace only,
+    one unknown per panel.  The defaults are generateQualocationOperator's
+    (32 panels per leaf, 4 grid points, SVD 1e-5); solveBEMproblem uses 27,
+    3 and 1e-5."""
+
+    def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, maxpanels=32, gridpoints=4, svdeps=1e-5,
+                 want_f32=True):
+        v = _f64(verts9).reshape(-1, 9)
+        self.qpos = _f64(qpos).reshape(-1, 3)
+        self.q = _f64(q).reshape(-1)
+        self.eps_in = eps_in
+        h = _vp()
+        _check(lib().fftsvd_qual_create(_d(v), len(v), _d(self.qpos), _d(self.q), len(self.q), eps_in, eps_out,
+                                        maxpanels, gridpoints, svdeps, int(want_f32), C.byref(h)))
+        self.h = h
+        self.n = len(v)
+        nv = C.c_longlong()
+        _check(lib().fftsvd_problem_unknowns(self.h, C.byref(nv)))
+        self.nvec = nv.value
+
+    def tree(self, which=0):
+        return Tree(self, which)
+
+    def potentials(self, sol):
+        return self.reaction_potentials(sol)
+
+    def energy(self, sol, conversion=332.112):
+        """1/2 sum q phi * conversion / eps_in (FFTSVDsolvecfqual.c:153-167)."""
+        return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in

@@ -24,6 +24,7 @@ int main(void) {
    unsigned int nsalt = 0, ndiel = 0, ndcav = 0, nscav = 0, ntotal = 0;
    unsigned int *dielpar = NULL, *dcavpar = NULL, *scavpar = NULL;
    SurfaceOperator op = NULL;
+   QualocationOperator qop = NULL;
    Preconditioner pc = NULL;
    Vector rhs = NULL, sol = NULL;
    double energy = 0.0;
@@ -33,7 +34,6 @@ int main(void) {
 
    readParams(delphiparameterfilename);
    checkParams();
-   if (usequalocation) error("the qualocation formulation is not part of this build");
    readPDB(delphipdbinputfilename, &natoms, &atoms);
    readSIZ(delphisizfilename, &nsiz, &siz);
    readCRG(delphicrgfilename, &ncrg, &crg);
@@ -49,15 +49,27 @@ int main(void) {
            &scav, &nscavp, &nscav, &scavpar, &ntotal);
    printf("%u panels\n", ntotal);
 
-   generateSurfaceOperator(&op, atoms, natoms, salt, nsaltp, nsalt, diel, ndielp, ndiel, dielpar, dcav, ndcavp, ndcav,
-                           dcavpar, scav, nscavp, nscav, scavpar, ntotal);
-   generateSurfaceOperatorPreconditioner(&pc, op, ntotal);
-   generateRHS(&rhs, op, ntotal);
-   solveSurfaceOperator(&sol, op, pc, rhs, ntotal);
+   /* the qualocation or the surface (first-kind) formulation, as FFTSVDpbe.c:169-233 */
+   if (usequalocation) {
+      printf("qualocation operator\n");
+      generateQualocationOperator(&qop, atoms, natoms, diel, ndielp, ndiel, dcav, ndcavp, ndcav, dcavpar, ntotal);
+      generateQualocationOperatorPreconditioner(&pc, qop, ntotal);
+      generateQualocationRHS(&rhs, qop, ntotal);
+      solveQualocationOperator(&sol, qop, pc, rhs, ntotal);
+   } else {
+      generateSurfaceOperator(&op, atoms, natoms, salt, nsaltp, nsalt, diel, ndielp, ndiel, dielpar, dcav, ndcavp,
+                              ndcav, dcavpar, scav, nscavp, nscav, scavpar, ntotal);
+      generateSurfaceOperatorPreconditioner(&pc, op, ntotal);
+      generateRHS(&rhs, op, ntotal);
+      solveSurfaceOperator(&sol, op, pc, rhs, ntotal);
+   }
 
    readPDB(delphipdbcollectfilename, &ncollect, &collect);
    assignRadiiCharges(collect, ncollect, siz, nsiz, crg, ncrg);
-   collectPotentials(collect, ncollect, atoms, natoms, op, sol);
+   if (usequalocation)
+      collectQualocationPotentials(collect, ncollect, atoms, natoms, qop, sol);
+   else
+      collectPotentials(collect, ncollect, atoms, natoms, op, sol);
    if (outputfrc) writeFRC(delphifrcfilename, collect, ncollect);
 
    /* reaction-field energy, 1/2 sum q phi, in kcal/mol (potentials are kT/e) */

@@ -448,7 +448,7 @@ __host__ __device__ constexpr size_t source_cube_lds(int n, int maxr, int nt) {
           sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n + (nt > n * n * n ? nt : 0) + 2 * maxr) + 64;
 }
 
-// Source cubes, one launch: the nbig cubes with more than four grid points
+// Source cubes, one launch: the nbig cubes with other than four grid points
 // per side first, one block of CUBE_NT threads each (these cubes are few,
 // so each gets many loads in flight), then the cubes of four grid points
 // per side (n^3 = 64, one per lane), one per wavefront, CUBE_NT/64 per
@@ -464,6 +464,7 @@ __global__ void __launch_bounds__(CUBE_NT) k_source(const SrcPlan* __restrict__ 
       const SrcPlan d = src[blockIdx.x];
       const BlockGroup gr = block_group();
       switch (d.n) {
+         case 3: source_body<3, T>(d, tpart, PV, spec, smem, gr); break;
          case 5: source_body<5, T>(d, tpart, PV, spec, smem, gr); break;
          case 6: source_body<6, T>(d, tpart, PV, spec, smem, gr); break;
          case 7: source_body<7, T>(d, tpart, PV, spec, smem, gr); break;
@@ -700,7 +701,7 @@ __host__ __device__ constexpr size_t target_cube_lds(int n) {
           sizeof(T) * (size_t)(2 * (2 * n - 1) + n * n * n) + 64;
 }
 
-// Target cubes, one launch: the nbig cubes with more than four grid points
+// Target cubes, one launch: the nbig cubes with other than four grid points
 // per side first, one block of CUBE_NT threads each, then the cubes of four
 // grid points per side, one per wavefront in its own LDS slice.
 template <typename T>
@@ -713,6 +714,7 @@ __global__ void __launch_bounds__(CUBE_NT) k_target(const TgtPlan* __restrict__ 
       const TgtPlan d = tgt[blockIdx.x];
       const BlockGroup gr = block_group();
       switch (d.n) {
+         case 3: target_body<3, T>(d, UTI, acc, w, smem, gr); break;
          case 5: target_body<5, T>(d, UTI, acc, w, smem, gr); break;
          case 6: target_body<6, T>(d, UTI, acc, w, smem, gr); break;
          case 7: target_body<7, T>(d, UTI, acc, w, smem, gr); break;
@@ -918,6 +920,7 @@ size_t far_target_lds(int n, int maxr) {
 
 #define FSVD_DISPATCH_N(n, CALL) \
    switch (n) {                  \
+      case 3: CALL(3); break;    \
       case 4: CALL(4); break;    \
       case 5: CALL(5); break;    \
       case 6: CALL(6); break;    \
@@ -956,7 +959,8 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       case 1: {
          const int wl = (int)((source_cube_lds<T>(4, a.maxr_src, 64) + 15) / 16 * 16);
          size_t lds = (size_t)(CUBE_NT / 64) * wl;
-         for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, source_cube_lds<T>(n, a.maxr_src, CUBE_NT));
+         for (int n = 3; n <= a.max_n; n++)
+            if (n != 4) lds = std::max(lds, source_cube_lds<T>(n, a.maxr_src, CUBE_NT));
          check_lds(lds);
          static std::once_flag attr;   // large n needs more than the default dynamic LDS
          std::call_once(attr, [] {
@@ -993,7 +997,8 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       case 3: {
          const int wl = (int)((target_cube_lds<T>(4) + 15) / 16 * 16);
          size_t lds = (size_t)(CUBE_NT / 64) * wl;
-         for (int n = 5; n <= a.max_n; n++) lds = std::max(lds, target_cube_lds<T>(n));
+         for (int n = 3; n <= a.max_n; n++)
+            if (n != 4) lds = std::max(lds, target_cube_lds<T>(n));
          check_lds(lds);
          static std::once_flag attr;   // large n needs more than the default dynamic LDS
          std::call_once(attr, [] {

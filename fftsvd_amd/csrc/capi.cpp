@@ -152,7 +152,8 @@ int fftsvd_problem_create_multi(int ndiel, const int* npan_diel, int ncav, const
                                 double eps_out, double kappa, int maxpanels, int gridpoints, double svdeps,
                                 int want_f32, fftsvd_problem** out) {
    API_TRY
-   if (gridpoints != 4) throw std::runtime_error("only 4 grid points per cube edge (49-point sphere) are supported");
+   if (gridpoints != 3 && gridpoints != 4)
+      throw std::runtime_error("3 or 4 grid points per cube edge at the finest levels are supported");
    if (ndiel < 1) throw std::runtime_error("need at least one dielectric surface");
    auto sp = make_spec(ndiel, npan_diel, ncav, npan_cav, cav_parent, verts9, qpos, q, ncharge, eps_in, eps_out, kappa,
                        maxpanels, gridpoints, svdeps, want_f32);
@@ -169,7 +170,8 @@ int fftsvd_problem_create_stern(int nsalt, const int* npan_salt, int ndiel, cons
                                 double kappa, int maxpanels, int gridpoints, double svdeps, int want_f32,
                                 fftsvd_problem** out) {
    API_TRY
-   if (gridpoints != 4) throw std::runtime_error("only 4 grid points per cube edge (49-point sphere) are supported");
+   if (gridpoints != 3 && gridpoints != 4)
+      throw std::runtime_error("3 or 4 grid points per cube edge at the finest levels are supported");
    if (nsalt < 1 || ndiel < 1) throw std::runtime_error("need at least one salt and one dielectric surface");
    if (!(kappa > 0.0)) throw std::runtime_error("the Stern-layer formulation needs kappa > 0");
    size_t off = 0;
@@ -207,6 +209,44 @@ int fftsvd_problem_create(const double* verts9, int npan, const double* qpos, co
                                       maxpanels, gridpoints, svdeps, want_f32, out);
 }
 
+int fftsvd_qual_create(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
+                       double eps_in, double eps_out, int maxpanels, int gridpoints, double svdeps, int want_f32,
+                       fftsvd_problem** out) {
+   API_TRY
+   if (!out) throw std::runtime_error("null output handle");
+   *out = nullptr;
+   if (npan < 1 || ncharge < 0) throw std::runtime_error("bad sizes");
+   if (gridpoints != 3 && gridpoints != 4)
+      throw std::runtime_error("3 or 4 grid points per cube edge at the finest levels are supported");
+   QualSpec sp;
+   sp.verts.assign(verts9, verts9 + 9 * (size_t)npan);
+   if (ncharge) {
+      sp.qpos.assign(qpos, qpos + 3 * (size_t)ncharge);
+      sp.q.assign(q, q + ncharge);
+   }
+   sp.eps_in = eps_in;
+   sp.eps_out = eps_out;
+   sp.maxpanels = maxpanels;
+   sp.gridpoints = gridpoints;
+   sp.svdeps = svdeps;
+   sp.want32 = want_f32 != 0;
+   auto* h = new fftsvd_problem;
+   try {
+      h->pr = build_qual_problem(sp);
+   } catch (...) {
+      delete h;
+      throw;
+   }
+   *out = h;
+   API_CATCH
+}
+
+int fftsvd_problem_unknowns(fftsvd_problem* p, long long* n) {
+   API_TRY
+   if (n) *n = p->pr->nvec;
+   API_CATCH
+}
+
 int fftsvd_problem_destroy(fftsvd_problem* p) {
    API_TRY
    delete p;
@@ -223,7 +263,7 @@ int fftsvd_problem_size(fftsvd_problem* p, int* npanels, int* ncharges) {
 int fftsvd_problem_matvec(fftsvd_problem* p, int precision, const double* x, double* y) {
    API_TRY
    Problem& pr = *p->pr;
-   size_t n = 2 * (size_t)pr.ntotal;
+   size_t n = (size_t)pr.nvec;
    DBuf<double> dx, dy;
    dx.upload(x, n, pr.stream);
    dy.alloc(n);
@@ -236,7 +276,7 @@ int fftsvd_problem_matvec(fftsvd_problem* p, int precision, const double* x, dou
 int fftsvd_problem_precond(fftsvd_problem* p, const double* b, double* x) {
    API_TRY
    Problem& pr = *p->pr;
-   size_t n = 2 * (size_t)pr.ntotal;
+   size_t n = (size_t)pr.nvec;
    DBuf<double> db, dx;
    db.upload(b, n, pr.stream);
    dx.alloc(n);
@@ -249,7 +289,7 @@ int fftsvd_problem_precond(fftsvd_problem* p, const double* b, double* x) {
 int fftsvd_problem_rhs(fftsvd_problem* p, double* rhs) {
    API_TRY
    Problem& pr = *p->pr;
-   size_t n = 2 * (size_t)pr.ntotal;
+   size_t n = (size_t)pr.nvec;
    DBuf<double> d;
    d.alloc(n);
    problem_rhs(pr, d.p);
@@ -262,7 +302,7 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
                          int* iters, double* resid_hist, double* info) {
    API_TRY
    Problem& pr = *p->pr;
-   size_t n = 2 * (size_t)pr.ntotal;
+   size_t n = (size_t)pr.nvec;
    if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
    DBuf<double> db, dx;
    db.upload(rhs, n, pr.stream);
@@ -356,7 +396,7 @@ int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, doub
    API_TRY
    Problem& pr = *p->pr;
    hipStream_t s = pr.stream;
-   size_t n = 2 * (size_t)pr.ntotal;
+   size_t n = (size_t)pr.nvec;
    std::vector<double> hx(n);
    std::mt19937_64 rng(12345);
    std::uniform_real_distribution<double> U(-1.0, 1.0);
@@ -849,9 +889,11 @@ void solveSurfaceOperator(Vector* solution, SurfaceOperator so, Preconditioner p
       error((char*)"%s", g_err.c_str());
 }
 
-void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries, PDBentry* PDBinputentries,
-                       unsigned int numPDBinputentries, SurfaceOperator so, Vector solution) {
-   Problem& pr = so_problem(so);
+// potentials at the collect entries in kT/e, plus the Coulomb potential of
+// the input charges when addcoulombic (collectPotentials,
+// FFTSVDpbeAPI.c:1896-1922; collectQualocationPotentials, :1924-1950)
+static void collect_into(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries, PDBentry* PDBinputentries,
+                         unsigned int numPDBinputentries, Problem& pr, Vector solution) {
    std::vector<double> phi(pr.q.size() + 1, 0.0);
    problem_potentials(pr, solution, phi.data());
    for (unsigned a = 0; a < numPDBcollectentries; a++)
@@ -869,6 +911,146 @@ void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollecten
                       KT_CONVERSION * PDBinputentries[a].charge / (innerdielectric * distance);
             }
    }
+}
+
+void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries, PDBentry* PDBinputentries,
+                       unsigned int numPDBinputentries, SurfaceOperator so, Vector solution) {
+   collect_into(PDBcollectentries, numPDBcollectentries, PDBinputentries, numPDBinputentries, so_problem(so),
+                solution);
+}
+
+// ---- qualocation formulation (FFTSVDpbeAPI.c:1741-1869,1878-1894,1924-1950) ----
+
+void generateQualocationOperatorMinimal(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
+                                        unsigned int numPDBentries, Panel** dielectricpanels,
+                                        unsigned int* numdielectricpanels, unsigned int numdielectrics,
+                                        Panel** dielectriccavitypanels, unsigned int* numdielectriccavitypanels,
+                                        unsigned int numdielectriccavities, unsigned int* dielectriccavityparent,
+                                        unsigned int numtotalpanels) {
+   if (ionicstrength != 0.0) error((char*)"Qualocation method only works without salt!");
+   QualSpec sp;
+   // all dielectric panels, then the cavities' (parent 999: removed)
+   for (unsigned i = 0; i < numdielectrics; i++)
+      for (unsigned k = 0; k < numdielectricpanels[i]; k++)
+         for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) sp.verts.push_back(dielectricpanels[i][k]->vertex[a][d]);
+   for (unsigned i = 0; i < numdielectriccavities; i++) {
+      if (dielectriccavityparent[i] == 999) continue;
+      for (unsigned k = 0; k < numdielectriccavitypanels[i]; k++)
+         for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) sp.verts.push_back(dielectriccavitypanels[i][k]->vertex[a][d]);
+   }
+   if (sp.verts.size() / 9 != numtotalpanels)
+      error((char*)"qualocation panels (%u) differ from numtotalpanels (%u)", (unsigned)(sp.verts.size() / 9),
+            numtotalpanels);
+   for (unsigned a = 0; a < numPDBentries; a++) {
+      sp.qpos.push_back(PDBentries[a].x);
+      sp.qpos.push_back(PDBentries[a].y);
+      sp.qpos.push_back(PDBentries[a].z);
+      sp.q.push_back(toupper(PDBentries[a].chain) != 'X' ? PDBentries[a].charge : 0.0);
+   }
+   sp.eps_in = innerdielectric;
+   sp.eps_out = outerdielectric;
+   sp.maxpanels = MAX_PANELS_PER_FINEST_CUBE;
+   sp.gridpoints = GRID_SIZE;
+   sp.svdeps = SVD_ERROR;
+   sp.want32 = true;
+   auto* h = new fftsvd_problem;
+   try {
+      h->pr = build_qual_problem(sp);
+   } catch (const std::exception& e) {
+      delete h;
+      error((char*)"%s", e.what());
+   }
+   QualocationOperator o = (QualocationOperator)calloc(1, sizeof(_QualocationOperator));
+   o->tree = h;
+   o->innerdielectric = innerdielectric;
+   o->outerdielectric = outerdielectric;
+   Charge ch = (Charge)calloc(1, sizeof(_Charge));
+   ch->numcharges = numPDBentries;
+   ch->points = (Vector3D*)calloc(numPDBentries + 1, sizeof(Vector3D));
+   ch->charges = (Vector)calloc(numPDBentries + 1, sizeof(real));
+   for (unsigned a = 0; a < numPDBentries; a++) {
+      ch->points[a] = (Vector3D)calloc(1, sizeof(_Vector3D));
+      ch->points[a]->x = sp.qpos[3 * a];
+      ch->points[a]->y = sp.qpos[3 * a + 1];
+      ch->points[a]->z = sp.qpos[3 * a + 2];
+      ch->charges[a] = sp.q[a];
+   }
+   o->charges = ch;
+   *qualocationoperator = o;
+}
+
+void generateQualocationOperator(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
+                                 unsigned int numPDBentries, Panel** dielectricpanels,
+                                 unsigned int* numdielectricpanels, unsigned int numdielectrics,
+                                 Panel** dielectriccavitypanels, unsigned int* numdielectriccavitypanels,
+                                 unsigned int numdielectriccavities, unsigned int* dielectriccavityparent,
+                                 unsigned int numtotalpanels) {
+   generateQualocationOperatorMinimal(qualocationoperator, PDBentries, numPDBentries, dielectricpanels,
+                                      numdielectricpanels, numdielectrics, dielectriccavitypanels,
+                                      numdielectriccavitypanels, numdielectriccavities, dielectriccavityparent,
+                                      numtotalpanels);
+}
+
+static Problem& qo_problem(QualocationOperator qo) {
+   if (!qo || !qo->tree) error((char*)"qualocation operator was not generated by this library");
+   return *((fftsvd_problem*)qo->tree)->pr;
+}
+
+void generateQualocationOperatorPreconditioner(Preconditioner* preconditioner, QualocationOperator qo,
+                                               unsigned int numtotalpanels) {
+   qo_problem(qo);
+   Preconditioner P = (Preconditioner)calloc(1, sizeof(_Preconditioner));
+   P->rows = P->columns = numtotalpanels;
+   P->nnz = numtotalpanels;
+   P->device = qo->tree;
+   *preconditioner = P;
+}
+
+void generateQualocationRHS(Vector* rhs, QualocationOperator qo, unsigned int numtotalpanels) {
+   qo_problem(qo);
+   *rhs = (Vector)calloc((size_t)numtotalpanels + 1, sizeof(real));
+   if (fftsvd_problem_rhs((fftsvd_problem*)qo->tree, *rhs)) error((char*)"%s", g_err.c_str());
+}
+
+void solveQualocationOperator(Vector* solution, QualocationOperator qo, Preconditioner preconditioner, Vector rhs,
+                              unsigned int numtotalpanels) {
+   (void)preconditioner;
+   qo_problem(qo);
+   *solution = (Vector)calloc((size_t)numtotalpanels + 1, sizeof(real));
+   if (fftsvd_problem_solve((fftsvd_problem*)qo->tree, g_precision, rhs, *solution, errortolerance, 1000, nullptr,
+                            nullptr, nullptr))
+      error((char*)"%s", g_err.c_str());
+}
+
+void collectQualocationPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries,
+                                  PDBentry* PDBinputentries, unsigned int numPDBinputentries, QualocationOperator qo,
+                                  Vector solution) {
+   collect_into(PDBcollectentries, numPDBcollectentries, PDBinputentries, numPDBinputentries, qo_problem(qo),
+                solution);
+}
+
+void solveBEMproblem(unsigned int* numpanels, real* paneldata, real* innerdiel, real* outerdiel, real* rhs,
+                     real* sol) {
+   const unsigned n = *numpanels;
+   QualSpec sp;
+   sp.verts.assign(paneldata, paneldata + 9 * (size_t)n);
+   sp.eps_in = *innerdiel;
+   sp.eps_out = *outerdiel;
+   sp.maxpanels = 27;
+   sp.gridpoints = 3;
+   sp.svdeps = 1e-5;
+   sp.want32 = g_precision != PREC_F64;
+   fftsvd_problem h;
+   try {
+      h.pr = build_qual_problem(sp);
+   } catch (const std::exception& e) {
+      error((char*)"%s", e.what());
+   }
+   h.pr->gmres_zero_test = false;   // GMRES_solv_ecf_qual has no zero-RHS test
+   if (fftsvd_problem_solve(&h, g_precision, rhs, sol, 1e-5, 1000, nullptr, nullptr, nullptr))
+      error((char*)"%s", g_err.c_str());
 }
 
 }  // extern "C"

@@ -16,7 +16,7 @@ struct IntBlock {
    int pan_off;       // into the block-panel index array (caller panel indices)
    long long out_off;
    int ld_pt, ld_pan;
-   long long layer_stride;
+   long long layer_stride;   // double layer at +layer_stride; < 0: write the double layer only, at out_off
 };
 
 // Point-kernel block: out[off + i*ld_pt + j*ld_src] = G(dest_i, src_j)

@@ -31,8 +31,12 @@ __global__ void __launch_bounds__(256) k_integral_blocks(const IntBlock* __restr
       double s, d;
       panel_integrals(kernel, kappa, x, P, s, d);
       long long o = b.out_off + (long long)i * b.ld_pt + (long long)j * b.ld_pan;
-      out[o] = s;
-      out[o + b.layer_stride] = d;
+      if (b.layer_stride >= 0) {
+         out[o] = s;
+         out[o + b.layer_stride] = d;
+      } else {
+         out[o] = d;   // double layer only
+      }
    }
 }
 

@@ -399,15 +399,15 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    }
    {
       // cubes of four grid points per side (one per wavefront) after the
-      // larger ones (one per block): each class is one contiguous launch
+      // others (one per block): each class is one contiguous launch
       auto big_first = [](auto v) {
-         std::stable_partition(v.begin(), v.end(), [](const auto& e) { return e.n > 4; });
+         std::stable_partition(v.begin(), v.end(), [](const auto& e) { return e.n != 4; });
          return v;
       };
       const std::vector<SrcPlan> src_o = big_first(src);
       const std::vector<TgtPlan> tgt_o = big_first(tgt);
-      P.nsrc_big = (int)std::count_if(src.begin(), src.end(), [](const SrcPlan& e) { return e.n > 4; });
-      P.ntgt_big = (int)std::count_if(tgt.begin(), tgt.end(), [](const TgtPlan& e) { return e.n > 4; });
+      P.nsrc_big = (int)std::count_if(src.begin(), src.end(), [](const SrcPlan& e) { return e.n != 4; });
+      P.ntgt_big = (int)std::count_if(tgt.begin(), tgt.end(), [](const TgtPlan& e) { return e.n != 4; });
       P.src.upload(src_o, s);
       P.tgt.upload(tgt_o, s);
       FSVD_CHECK(hipStreamSynchronize(s));
@@ -430,7 +430,8 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s,
-                                      const Tree* pre) {
+                                      const Tree* pre, int layers) {
+   if (layers != 1 && layers != 2) throw std::runtime_error("layers must be 1 (double) or 2 (single and double)");
    auto opp = std::make_unique<TreeOp>();
    TreeOp& op = *opp;
    op.kernel = kernel;
@@ -609,7 +610,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          for (size_t pi = 0; pi < pending.size(); pi++) {
             Prob& P = probs[pending[pi]];
             const Cube& c = t.cubes[cl[P.cube]];
-            const int K = 2 * P.nsamp * P.pass;
+            const int K = layers * P.nsamp * P.pass;
             const int half = P.nsamp * P.pass;
             MgsDesc m;
             m.rank_slot = (int)pi;
@@ -636,7 +637,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
                b.pan_off = c.p0;
                b.ld_pt = c.np;
                b.ld_pan = 1;
-               b.layer_stride = (long long)half * c.np;
+               b.layer_stride = layers == 2 ? (long long)half * c.np : -1;
                m.rows = c.np;
                m.cols = K;
             } else {
@@ -654,7 +655,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
                b.pan_off = base;
                b.ld_pt = 1;
                b.ld_pan = c.nq;
-               b.layer_stride = (long long)half * c.nq;
+               b.layer_stride = layers == 2 ? (long long)half * c.nq : -1;
                m.rows = c.nq;
                m.cols = K;
             }
@@ -681,7 +682,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
          for (size_t pi = 0; pi < pending.size(); pi++) {
             Prob& P = probs[pending[pi]];
             int r = ranks[pi];
-            if (r < (2 * P.pass * P.nsamp) / 2) {
+            if (r < (layers * P.pass * P.nsamp) / 2) {
                P.rank = r;
                qsrc[pending[pi]] = Qbuf->p + mgs[pi].q_off;
             } else {

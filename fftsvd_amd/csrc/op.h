@@ -136,10 +136,12 @@ struct TreeOp {
 // geometry already on the device in `d_panels`) and `npt` points.
 // `pre`, if given, is the octree of exactly these panels and points (it
 // depends on neither the kernel nor kappa) and is copied instead of rebuilt.
+// `layers` = 2 samples the bases from both layers (SINGLE_AND_DOUBLE_LAYER_INT),
+// 1 from the double layer only (DOUBLE_LAYER_INT, Cube.c:1171-1412).
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s,
-                                      const Tree* pre = nullptr);
+                                      const Tree* pre = nullptr, int layers = 2);
 
 // y_t = S xs_t + D xd_t; all vectors in tree order, device pointers.
 void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s);

@@ -355,6 +355,55 @@ __global__ void __launch_bounds__(256) k_potentials(int ne, const PanelGeom* __r
    if (threadIdx.x == 0) phi[j] += s;
 }
 
+// Qualocation operator (Solv_ecf_multiply_qual, GMRES.c:159-181), around
+// the transposed double-layer tree apply in tree order:
+//   gather: xa[k] = area_t[k] * x[perm[k]]
+//   finish: b = (y_t[k] - diag_t[k] xa[k]) * (1/(4 pi eps_in))
+//               + (c0 + diag_t[k] / (4 pi eps_in)) xa[k],  out[perm[k]] = b
+// with c0 = -eps_out / ((eps_out - eps_in) eps_in).
+template <typename XT>
+__global__ void k_qual_gather(int n, const int* __restrict__ perm, const double* __restrict__ area_t,
+                              const XT* __restrict__ x, double* __restrict__ xa) {
+   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+      xa[k] = area_t[k] * (double)x[perm[k]];
+}
+
+__global__ void k_qual_finish(int n, const int* __restrict__ perm, const double* __restrict__ diag_t,
+                              const double* __restrict__ xa, const double* __restrict__ yt, double inv4pe,
+                              double fourpe, double c0, double* __restrict__ out) {
+   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+      const double d = diag_t[k], a = xa[k];
+      double b = yt[k];
+      b -= d * a;
+      b *= inv4pe;
+      b += (c0 + (d) / fourpe) * a;
+      out[perm[k]] = b;
+   }
+}
+
+// Diagonal preconditioner solve, x = b / p (x may alias b).
+__global__ void k_diag_div(int n, const double* __restrict__ p, const double* b, double* x) {
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = b[i] / p[i];
+}
+
+// Qualocation right-hand side (Charge_makerhs_ecf_qual, Charge.c:129-142):
+// rhs[i] = sum_j -dlp(charge j, panel i) q_j / (4 pi eps_in), in charge order.
+__global__ void __launch_bounds__(256) k_qual_rhs(int n, const PanelGeom* __restrict__ panels, int nq,
+                                                  const double* __restrict__ qpos, const double* __restrict__ q,
+                                                  double fourpe, double* __restrict__ rhs) {
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const PanelGeom& P = panels[i];
+      double r = 0.0;
+      for (int j = 0; j < nq; j++) {
+         const double x[3] = {qpos[3 * j], qpos[3 * j + 1], qpos[3 * j + 2]};
+         double sl, dl;
+         laplace_integrals(x, P, sl, dl);
+         r += -dl * q[j] / fourpe;
+      }
+      rhs[i] = r;
+   }
+}
+
 // diagonal of the near-field blocks: diag[p0 + i] from leaf rows i < p
 __global__ void k_leaf_diag(int nleaf, const LeafDesc* __restrict__ leaves, const double* __restrict__ D,
                             double* __restrict__ ds, double* __restrict__ dd) {
@@ -451,6 +500,26 @@ template <typename VT>
 void launch_combine(long long n, int m, const double* y, const VT* const* V, double* x, hipStream_t s) {
    hipLaunchKernelGGL((k_combine<VT>), dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, y, V, x);
 }
+template <typename XT>
+void launch_qual_gather(int n, const int* perm, const double* area_t, const XT* x, double* xa, hipStream_t s) {
+   if (n > 0) hipLaunchKernelGGL((k_qual_gather<XT>), dim3(nblocks(n)), dim3(256), 0, s, n, perm, area_t, x, xa);
+}
+template void launch_qual_gather<double>(int, const int*, const double*, const double*, double*, hipStream_t);
+template void launch_qual_gather<float>(int, const int*, const double*, const float*, double*, hipStream_t);
+void launch_qual_finish(int n, const int* perm, const double* diag_t, const double* xa, const double* yt,
+                        double inv4pe, double fourpe, double c0, double* out, hipStream_t s) {
+   if (n > 0)
+      hipLaunchKernelGGL(k_qual_finish, dim3(nblocks(n)), dim3(256), 0, s, n, perm, diag_t, xa, yt, inv4pe, fourpe, c0,
+                         out);
+}
+void launch_diag_div(int n, const double* p, const double* b, double* x, hipStream_t s) {
+   if (n > 0) hipLaunchKernelGGL(k_diag_div, dim3(nblocks(n)), dim3(256), 0, s, n, p, b, x);
+}
+void launch_qual_rhs(int n, const PanelGeom* panels, int nq, const double* qpos, const double* q, double fourpe,
+                     double* rhs, hipStream_t s) {
+   if (n > 0) hipLaunchKernelGGL(k_qual_rhs, dim3(nblocks(n)), dim3(256), 0, s, n, panels, nq, qpos, q, fourpe, rhs);
+}
+
 #define FSVD_KRYLOV_INST(VT)                                                                                        \
    template void launch_multidot<VT>(long long, int, int, const VT* const*, const double*, double*, hipStream_t); \
    template void launch_multiaxpy<VT>(long long, int, const double*, const VT* const*, double*, double*,          \

@@ -43,6 +43,13 @@ void launch_rhs(int ne, const double* cent, const int* out, int nq, const double
 void launch_potentials(int nq, int ne, const PanelGeom* panels, const int* pidx, const double* coef,
                        const int* self_ok, const double* qpos, double* phi, hipStream_t s);
 void launch_leaf_diag(int nleaf, const LeafDesc* leaves, const double* D, double* ds, double* dd, hipStream_t s);
+template <typename XT>
+void launch_qual_gather(int n, const int* perm, const double* area_t, const XT* x, double* xa, hipStream_t s);
+void launch_qual_finish(int n, const int* perm, const double* diag_t, const double* xa, const double* yt,
+                        double inv4pe, double fourpe, double c0, double* out, hipStream_t s);
+void launch_diag_div(int n, const double* p, const double* b, double* x, hipStream_t s);
+void launch_qual_rhs(int n, const PanelGeom* panels, int nq, const double* qpos, const double* q, double fourpe,
+                     double* rhs, hipStream_t s);
 
 namespace {
 double now_ms() {
@@ -133,6 +140,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    const std::vector<int> cstart = add_surfaces(spec.cavity_verts);
    const std::vector<int> scstart = add_surfaces(spec.salt_cavity_verts);
    pr.ntotal = (int)pr.panels.size();
+   pr.nvec = 2LL * pr.ntotal;
    setup_event("panel geometry");
    const int nsalt = (int)spec.salt_verts.size(), nd = (int)spec.dielectric_verts.size();
    const int nc = (int)spec.cavity_verts.size(), nsc = (int)spec.salt_cavity_verts.size();
@@ -414,7 +422,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    }
    pr.d_qpos.upload(pr.qpos, s);
    pr.d_q.upload(pr.q, s);
-   long long n2 = 2LL * pr.ntotal;
+   long long n2 = pr.nvec;
    pr.w.alloc(n2);
    pr.r.alloc(n2);
    pr.tmp.alloc(n2);
@@ -427,11 +435,105 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    return prp;
 }
 
+// Qualocation operator (generateQualocationOperator(Minimal),
+// FFTSVDpbeAPI.c:1741-1839; QualocationOperator.c): one Laplace tree with
+// double-layer-only bases over the panels in caller order, the qualocation
+// diagonal, and the diagonal preconditioner
+// (Preconditioner_fill_diagonal_solv_ecf_qual, Preconditioner.c:49-60).
+std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec) {
+   double t0 = now_ms();
+   if (!(spec.eps_out != spec.eps_in)) throw std::runtime_error("qualocation needs eps_out != eps_in");
+   auto prp = std::make_unique<Problem>();
+   Problem& pr = *prp;
+   FSVD_CHECK(hipStreamCreate(&pr.stream));
+   hipStream_t s = pr.stream;
+   pr.qual = true;
+   pr.eps_in = spec.eps_in;
+   pr.eps_out = spec.eps_out;
+   pr.qpos = spec.qpos;
+   pr.q = spec.q;
+   const int n = (int)(spec.verts.size() / 9);
+   if (n < 1) throw std::runtime_error("qualocation operator without panels");
+   pr.panels.resize(n);
+   host_parallel_for(n, [&](long long k) {
+      const double* v = &spec.verts[9 * (size_t)k];
+      panel_geometry(v, v + 3, v + 6, &pr.panels[k]);
+   });
+   pr.ntotal = n;
+   pr.nvec = n;
+   pr.regions.emplace_back();
+   Region& R = pr.regions[0];
+   R.kernel = LAPLACE;
+   R.eps = spec.eps_in;
+   R.mynpan = n;
+   R.has_charges = !spec.q.empty();
+   R.panel_ids.resize(n);
+   for (int i = 0; i < n; i++) R.panel_ids[i] = i;
+   std::vector<double> cent(3 * (size_t)n);
+   for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) cent[3 * (size_t)i + d] = pr.panels[i].c[d];
+   R.d_panels.upload(pr.panels, s);
+   R.op = build_tree_op(pr.panels, R.d_panels.p, cent.data(), n, LAPLACE, 0.0, spec.maxpanels, spec.gridpoints,
+                        spec.svdeps, spec.want32, s, nullptr, 1);
+   TreeOp& op = *R.op;
+   if (op.tree.pt_perm != op.tree.pan_perm) throw std::runtime_error("point/panel partitions differ");
+   // Tree_extractdiagonal(DOUBLE_LAYER_INT) (Tree.c:861-875): leaf D_double[i][i]
+   DBuf<double> dds;
+   dds.alloc(n);
+   pr.qual_diag_t.alloc(n);
+   launch_leaf_diag(op.nleaf, op.leaves.p, op.D64.p, dds.p, pr.qual_diag_t.p, s);
+   std::vector<double> diag_t(n), area_t(n), pdiag(n);
+   FSVD_CHECK(hipMemcpyAsync(diag_t.data(), pr.qual_diag_t.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   const double c0 = -spec.eps_out / ((spec.eps_out - spec.eps_in) * spec.eps_in);
+   for (int k = 0; k < n; k++) {
+      const int i = op.tree.pan_perm[k];
+      area_t[k] = pr.panels[i].area;
+      pdiag[i] = (c0 + (diag_t[k]) / (4.0 * M_PI * spec.eps_in)) * pr.panels[i].area;
+   }
+   for (int i = 0; i < n; i++)
+      if (!(std::fabs(pdiag[i]) > 0.0) || !std::isfinite(pdiag[i]))
+         throw std::runtime_error("singular qualocation preconditioner entry");
+   pr.qual_perm.upload(op.tree.pan_perm, s);
+   pr.qual_area_t.upload(area_t, s);
+   pr.qual_pdiag.upload(pdiag, s);
+   R.xs.alloc(n + 1);
+   R.yt.alloc(n + 1);
+   pr.d_qpos.upload(pr.qpos, s);
+   pr.d_q.upload(pr.q, s);
+   pr.w.alloc(n);
+   pr.r.alloc(n);
+   pr.tmp.alloc(n);
+   pr.partial.alloc(2 * 512);
+   pr.hbuf.alloc(1024 + 8);
+   FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned, sizeof(double) * 2048));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   pr.setup_ms = now_ms() - t0;
+   return prp;
+}
+
+// Qualocation matvec: y = A x with A applied through the transposed
+// double-layer tree in tree order.
+template <typename XT>
+static void qual_matvec(Problem& pr, int prec, const XT* x, double* y) {
+   hipStream_t s = pr.stream;
+   Region& R = pr.regions[0];
+   TreeOp& op = *R.op;
+   const int n = pr.ntotal;
+   if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
+   launch_qual_gather<XT>(n, pr.qual_perm.p, pr.qual_area_t.p, x, R.xs.p, s);
+   tree_apply_transpose(op, prec, 1, R.xs.p, R.yt.p, s);
+   const double fourpe = 4.0 * M_PI * pr.eps_in;
+   const double c0 = -pr.eps_out / ((pr.eps_out - pr.eps_in) * pr.eps_in);
+   launch_qual_finish(n, pr.qual_perm.p, pr.qual_diag_t.p, R.xs.p, R.yt.p, 1.0 / fourpe, fourpe, c0, y, s);
+}
+
 // Every region writes a disjoint set of rows of y (the phi rows of its own
 // panels and the dphi rows of its children's panels), so regions on
 // different streams never touch the same output.
 template <typename XT>
 static void matvec_impl(Problem& pr, int prec, const XT* x, double* y) {
+   if (pr.qual) return qual_matvec(pr, prec, x, y);
    hipStream_t s = pr.stream;
    if (pr.naux > 0) {
       FSVD_CHECK(hipEventRecord(pr.ev_fork, s));
@@ -475,12 +577,19 @@ void problem_matvec(Problem& pr, int prec, const double* x, double* y) { matvec_
 void problem_matvec(Problem& pr, int prec, const float* x, double* y) { matvec_impl(pr, prec, x, y); }
 
 void problem_precond(Problem& pr, const double* b, double* x) {
+   if (pr.qual) return launch_diag_div(pr.ntotal, pr.qual_pdiag.p, b, x, pr.stream);
    launch_precond(pr.nblocks, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, b, x, pr.stream);
 }
 
 void problem_rhs(Problem& pr, double* rhs) {
    hipStream_t s = pr.stream;
-   long long n2 = 2LL * pr.ntotal;
+   if (pr.qual) {
+      launch_qual_rhs(pr.ntotal, pr.regions[0].d_panels.p, (int)pr.q.size(), pr.d_qpos.p, pr.d_q.p,
+                      4.0 * M_PI * pr.eps_in, rhs, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+      return;
+   }
+   long long n2 = pr.nvec;
    FSVD_CHECK(hipMemsetAsync(rhs, 0, sizeof(double) * n2, s));
    for (Region& R : pr.regions) {
       if (!R.has_charges) continue;
@@ -511,13 +620,13 @@ static VT* basis_vec(Problem& pr, int j) {
    if constexpr (std::is_same<VT, double>::value) {
       while ((int)pr.basis.size() <= j) {
          pr.basis.emplace_back(new DBuf<double>());
-         pr.basis.back()->alloc(2LL * pr.ntotal);
+         pr.basis.back()->alloc(pr.nvec);
       }
       return pr.basis[j]->p;
    } else {
       while ((int)pr.basis32.size() <= j) {
          pr.basis32.emplace_back(new DBuf<float>());
-         pr.basis32.back()->alloc(2LL * pr.ntotal);
+         pr.basis32.back()->alloc(pr.nvec);
       }
       return pr.basis32[j]->p;
    }
@@ -554,7 +663,7 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
    double t0 = now_ms();
    SolveInfo info;
    hipStream_t s = pr.stream;
-   const long long n = 2LL * pr.ntotal;
+   const long long n = pr.nvec;
    const int np = partial_blocks(n);
    double* r = pr.r.p;
    double* w = pr.w.p;
@@ -567,7 +676,7 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
    double nr2 = 0.0;
    for (int k = 0; k < np; k++) nr2 += pr.h_pinned[k];
    double normr = std::sqrt(nr2);
-   if (normr < tol) {
+   if (pr.gmres_zero_test ? normr < tol : !(normr > 0.0)) {
       FSVD_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * n, s));
       FSVD_CHECK(hipStreamSynchronize(s));
       info.ms = now_ms() - t0;
@@ -672,7 +781,7 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
 }
 
 static double precond_norm(Problem& pr, const double* v) {
-   const long long n = 2LL * pr.ntotal;
+   const long long n = pr.nvec;
    const int np = partial_blocks(n);
    problem_precond(pr, v, pr.tmp.p);
    launch_dot_partial(n, np, pr.tmp.p, pr.tmp.p, pr.partial.p, pr.stream);
@@ -688,7 +797,7 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
    if (mode == PREC_F64) return problem_gmres(pr, PREC_F64, rhs, sol, tol, maxit, hist);
    double t0 = now_ms();
    hipStream_t s = pr.stream;
-   const long long n = 2LL * pr.ntotal;
+   const long long n = pr.nvec;
    SolveInfo tot = problem_gmres(pr, PREC_F32, rhs, sol, tol, maxit, hist);
    double bn = precond_norm(pr, rhs);
    DBuf<double> res, e;
@@ -720,12 +829,28 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out) {
    hipStream_t s = pr.stream;
    const int nq = (int)pr.q.size();
    DBuf<PanelGeom> dall;
-   dall.upload(pr.panels, s);
+   if (!pr.qual) dall.upload(pr.panels, s);
    DBuf<double> dphi;
    dphi.alloc(std::max(nq, 1));
    FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max(nq, 1), s));
+   if (pr.qual) {
+      // QualocationOperator_calculateReactionPotentials (QualocationOperator.c:55-70):
+      // phi[c] = sum_i sol[i] slp(c, panel i)
+      std::vector<int> pidx(pr.ntotal), self(pr.ntotal, 0);
+      std::vector<double> coef(2 * (size_t)pr.ntotal, 0.0);
+      for (int i = 0; i < pr.ntotal; i++) {
+         pidx[i] = i;
+         coef[2 * (size_t)i] = sol[i];
+      }
+      DBuf<int> dp, dsf;
+      DBuf<double> dc;
+      dp.upload(pidx, s);
+      dsf.upload(self, s);
+      dc.upload(coef, s);
+      launch_potentials(nq, pr.ntotal, pr.regions[0].d_panels.p, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
+   }
    for (Region& R : pr.regions) {
-      if (!R.has_charges) continue;
+      if (pr.qual || !R.has_charges) continue;
       std::vector<int> pidx, self;
       std::vector<double> coef;
       for (int i = 0; i < R.mynpan; i++) {

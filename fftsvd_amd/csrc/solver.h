@@ -31,6 +31,18 @@ struct SurfaceSpec {
    bool want32 = true;
 };
 
+// Qualocation operator (QualocationOperator.c): one Laplace tree over all
+// dielectric (and cavity) panels with double-layer-only bases, one unknown
+// per panel, Laplace only.
+struct QualSpec {
+   std::vector<double> verts;   // npan*9, caller order
+   std::vector<double> qpos, q;
+   double eps_in = 4.0, eps_out = 80.0;
+   int maxpanels = 32, gridpoints = 4;
+   double svdeps = 1e-5;
+   bool want32 = true;
+};
+
 struct Region {
    int kernel = LAPLACE;
    double eps = 1.0, kappa = 0.0;
@@ -66,6 +78,16 @@ struct Problem {
    hipEvent_t ev_fork = nullptr, ev_join[MAX_AUX] = {nullptr, nullptr, nullptr};
    int naux = 0;
    int ntotal = 0;                       // total panels
+   long long nvec = 0;                   // unknowns: 2N (surface operator) or N (qualocation)
+   // qualocation operator (regions[0] holds its tree): per tree position the
+   // panel, its area and its D_double diagonal; the diagonal preconditioner
+   // per panel; GMRES tests ||P^-1 b|| < tol before iterating unless
+   // gmres_zero_test is off (GMRES_solv_ecf_qual has no such test)
+   bool qual = false;
+   double eps_out = 80.0;
+   DBuf<int> qual_perm;
+   DBuf<double> qual_area_t, qual_diag_t, qual_pdiag;
+   bool gmres_zero_test = true;
    std::vector<PanelGeom> panels;        // global panel geometry
    std::vector<Region> regions;          // regions[0] = exterior (root)
    std::vector<double> qpos, q;
@@ -93,6 +115,7 @@ struct Problem {
 };
 
 std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec);
+std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec);
 
 // y = A x over the 2N unknowns (reference SurfaceOperator_topmultiply,
 // SurfaceOperator.c:90-97); device pointers.

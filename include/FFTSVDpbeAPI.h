@@ -79,6 +79,16 @@ typedef struct _SurfaceOperator {
 } _SurfaceOperator;
 typedef _SurfaceOperator* SurfaceOperator;
 
+/* reference QualocationOperator.h:3-10; `tree` points at the library's
+ * device operator, M1M3 is always NULL. */
+typedef struct _QualocationOperator {
+   void* tree;
+   void* M1M3;
+   real innerdielectric, outerdielectric;
+   Charge charges;
+} _QualocationOperator;
+typedef _QualocationOperator* QualocationOperator;
+
 /* reference Preconditioner.h:3-17 (factors live on the device) */
 typedef struct _Preconditioner {
    unsigned int rows, columns, nnz;
@@ -221,6 +231,47 @@ void solveSurfaceOperator(Vector* solution, SurfaceOperator pbesurfaceoperator, 
 /* reference FFTSVDpbeAPI.h:156; FFTSVDpbeAPI.c:1896-1922 */
 void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries, PDBentry* PDBinputentries,
                        unsigned int numPDBinputentries, SurfaceOperator pbesurfaceoperator, Vector solution);
+
+/* Qualocation (second-kind, apparent-surface-charge) formulation, Laplace
+ * only (reference FFTSVDpbeAPI.h:133-158; FFTSVDpbeAPI.c:1741-1869,1878-1882,
+ * 1890-1894,1924-1950).  One tree over the dielectric and cavity panels with
+ * double-layer-only bases; one unknown per panel.  error() when ionicstrength
+ * is not 0.  accelerateM3 is not honoured: potentials and the right-hand side
+ * are always the direct sums (the reference's default). */
+void generateQualocationOperator(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
+                                 unsigned int numPDBentries, Panel** dielectricpanels,
+                                 unsigned int* numdielectricpanels, unsigned int numdielectrics,
+                                 Panel** dielectriccavitypanels, unsigned int* numdielectriccavitypanels,
+                                 unsigned int numdielectriccavities, unsigned int* dielectriccavityparent,
+                                 unsigned int numtotalpanels);
+/* The same operator; in the reference the tree is filled only by
+ * generateQualocationOperator (FFTSVDpbeAPI.c:1781-1839), here both build it. */
+void generateQualocationOperatorMinimal(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
+                                        unsigned int numPDBentries, Panel** dielectricpanels,
+                                        unsigned int* numdielectricpanels, unsigned int numdielectrics,
+                                        Panel** dielectriccavitypanels, unsigned int* numdielectriccavitypanels,
+                                        unsigned int numdielectriccavities, unsigned int* dielectriccavityparent,
+                                        unsigned int numtotalpanels);
+/* reference FFTSVDpbeAPI.c:1863-1869: diagonal preconditioner */
+void generateQualocationOperatorPreconditioner(Preconditioner* preconditioner,
+                                               QualocationOperator qualocationoperator, unsigned int numtotalpanels);
+/* reference FFTSVDpbeAPI.c:1878-1882 (Charge_makerhs_ecf_qual) */
+void generateQualocationRHS(Vector* rhs, QualocationOperator qualocationoperator, unsigned int numtotalpanels);
+/* reference FFTSVDpbeAPI.c:1890-1894 (GMRES_QualocationOperator to errortolerance;
+ * the iteration cap is MAXITERATIONS = 1000) */
+void solveQualocationOperator(Vector* solution, QualocationOperator qualocationoperator,
+                              Preconditioner preconditioner, Vector rhs, unsigned int numtotalpanels);
+/* reference FFTSVDpbeAPI.c:1924-1950 */
+void collectQualocationPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollectentries,
+                                  PDBentry* PDBinputentries, unsigned int numPDBinputentries,
+                                  QualocationOperator qualocationoperator, Vector solution);
+/* GAMESS entry (reference FFTSVDpbeAPI.h:159-163; FFTSVDpbeAPI.c:1974-2026):
+ * panels as 9 doubles each (v1, v2, v3, used in that order), the qualocation
+ * operator with 27 panels per leaf, 3 grid points, SVD 1e-5, GMRES to 1e-5;
+ * sol receives one value per panel.  A zero right-hand side gives a zero
+ * solution (the reference divides by its zero norm). */
+void solveBEMproblem(unsigned int* numpanels, real* paneldata, real* innerdielectric, real* outerdielectric,
+                     real* rhs, real* sol);
 
 #ifdef __cplusplus
 }

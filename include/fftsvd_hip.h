@@ -54,8 +54,22 @@ int fftsvd_problem_create_stern(int nsalt, const int* npan_salt, int ndiel, cons
                                 const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
                                 double kappa, int maxpanels, int gridpoints, double svdeps, int want_f32,
                                 fftsvd_problem** out);
+/* Qualocation operator (mirrors generateQualocationOperator(Minimal) and
+ * generateQualocationOperatorPreconditioner, FFTSVDpbeAPI.c:1741-1839,1863-1869;
+ * QualocationOperator.c): Laplace only, one tree over the npan panels whose
+ * bases are sampled from the double layer alone (DOUBLE_LAYER_INT), one
+ * unknown per panel, diagonal preconditioner.  The problem_* calls below
+ * then take N-vectors: matvec is Solv_ecf_multiply_qual (GMRES.c:159-181),
+ * rhs Charge_makerhs_ecf_qual (Charge.c:129-142), solve
+ * GMRES_QualocationOperator (GMRES.c:541-640) and potentials
+ * QualocationOperator_calculateReactionPotentials (QualocationOperator.c:55-70). */
+int fftsvd_qual_create(const double* verts9, int npan, const double* qpos, const double* q, int ncharge,
+                       double eps_in, double eps_out, int maxpanels, int gridpoints, double svdeps, int want_f32,
+                       fftsvd_problem** out);
 int fftsvd_problem_destroy(fftsvd_problem* p);
 int fftsvd_problem_size(fftsvd_problem* p, int* npanels, int* ncharges);
+/* Length of the problem's vectors: 2N for a surface operator, N for qualocation. */
+int fftsvd_problem_unknowns(fftsvd_problem* p, long long* n);
 
 /* y = A x (SurfaceOperator_topmultiply, SurfaceOperator.c:90-97) */
 int fftsvd_problem_matvec(fftsvd_problem* p, int precision, const double* x, double* y);

@@ -64,6 +64,15 @@ def _load():
         "orc_tree_apply_both": (None, [C.c_void_p, _dp, _dp, _dp]),
         "orc_tree_apply_transpose": (None, [C.c_void_p, C.c_int, _dp, _dp]),
         "orc_tree_dense": (None, [C.c_void_p, _dp, _dp]),
+        "orc_qual_create": (C.c_void_p, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                         C.c_int, C.c_double]),
+        "orc_qual_free": (None, [C.c_void_p]),
+        "orc_qual_matvec": (None, [C.c_void_p, _dp, _dp]),
+        "orc_qual_precond_diag": (None, [C.c_void_p, _dp]),
+        "orc_qual_rhs": (None, [C.c_void_p, _dp]),
+        "orc_qual_gmres": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp]),
+        "orc_qual_potentials": (None, [C.c_void_p, _dp, _dp]),
+        "orc_qual_tree": (C.c_void_p, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -326,3 +335,57 @@ class Problem:
         """Delta G = 1/2 sum q phi * conversion / eps_in (FFTSVDsalt.c:166-172)."""
         phi = self.reaction_potentials(sol)
         return 0.5 * float(np.dot(self.q, phi)) * conversion / self.eps_in
+
+
+class Qual:
+    """Qualocation operator (QualocationOperator.c): one Laplace tree over
+    the panels with double-layer-only bases, one unknown per panel.  The
+    defaults are generateQualocationOperator's (32 panels per leaf, 4 grid
+    points, SVD 1e-5); solveBEMproblem uses 27, 3 and 1e-5."""
+
+    def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, maxpanels=32, gridpoints=4, svdeps=1e-5):
+        self.verts9 = f64(verts9).reshape(-1, 9)
+        self.qpos = f64(qpos).reshape(-1, 3)
+        self.q = f64(q).reshape(-1)
+        self.eps_in = eps_in
+        self.n = len(self.verts9)
+        self.h = lib().orc_qual_create(_d(self.verts9), self.n, _d(self.qpos), _d(self.q), len(self.q), eps_in,
+                                       eps_out, maxpanels, gridpoints, svdeps)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_qual_free(self.h)
+            self.h = None
+
+    def tree(self, which=0):
+        return Tree(lib().orc_qual_tree(self.h), self)
+
+    def matvec(self, x):
+        y = np.zeros(self.n)
+        lib().orc_qual_matvec(self.h, _d(f64(x)), _d(y))
+        return y
+
+    def precond_diag(self):
+        p = np.zeros(self.n)
+        lib().orc_qual_precond_diag(self.h, _d(p))
+        return p
+
+    def rhs(self):
+        r = np.zeros(self.n)
+        lib().orc_qual_rhs(self.h, _d(r))
+        return r
+
+    def gmres(self, rhs, tol=1e-6, maxit=1000, early_exit=True):
+        sol = np.zeros(self.n)
+        hist = np.zeros(maxit)
+        it = lib().orc_qual_gmres(self.h, _d(f64(rhs)), _d(sol), tol, maxit, int(early_exit), _d(hist))
+        return sol, it, hist[:it]
+
+    def potentials(self, sol):
+        phi = np.zeros(len(self.q))
+        lib().orc_qual_potentials(self.h, _d(f64(sol)), _d(phi))
+        return phi
+
+    def energy(self, sol, conversion=332.112):
+        """1/2 sum q phi * conversion / eps_in (FFTSVDsolvecfqual.c:153-167)."""
+        return 0.5 * float(np.dot(self.q, self.potentials(sol))) * conversion / self.eps_in

@@ -98,11 +98,15 @@ typedef struct {
    orc_cube* cubes;           /* cubes[0] = root */
    double** pinv;             /* per level, gp3 x 49 row-major */
    double** T;                /* per level, 343 offsets x pad complex (NULL for inner offsets) */
+   int double_only;           /* DOUBLE_LAYER_INT tree: bases sampled from the double layer only */
 } orc_tree;
 
 orc_tree* orc_tree_build(const orc_panel* panels, int npan, const double* pts, int npt,
                          int kernel, double kappa, int maxpanels, int gridpoints, double svdeps);
 void orc_tree_fill(orc_tree* t);
+/* Sample the bases from the double layer only (a DOUBLE_LAYER_INT tree,
+ * Cube.c:1171-1412); call before orc_tree_fill. */
+void orc_tree_set_double_only(orc_tree* t);
 void orc_tree_free(orc_tree* t);
 /* y = S*xs + D*xd over all (point, panel) pairs, compressed (Tree.c:597-683) */
 void orc_tree_apply_both(orc_tree* t, const double* xs, const double* xd, double* y);
@@ -185,6 +189,35 @@ int orc_colbasis_mgs(const double* A, int rows, int cols, double eps, double* Q 
 /* Reading MSMS-style .vert/.face (VertFace.c:26-202): returns number of panels kept. */
 int orc_read_vertface(const char* vertfile, const char* facefile, int flip, int accessible, double** verts9);
 void orc_free(void* p);
+
+/* ------------------------------------------------------------------ */
+/* Qualocation (second-kind, apparent-surface-charge) operator on one
+ * Laplace tree of double-layer integrals (QualocationOperator.c;
+ * Solv_ecf_multiply_qual, GMRES.c:159-181; GMRES_QualocationOperator,
+ * GMRES.c:541-640; Preconditioner_fill_diagonal_solv_ecf_qual,
+ * Preconditioner.c:49-60; Charge_makerhs_ecf_qual, Charge.c:129-142). */
+typedef struct {
+   int npan, nq;
+   orc_panel* panels;
+   double* cent;
+   orc_tree* tree;
+   double* diag;              /* D_double self entries (Tree_extractdiagonal) */
+   double* qpos;
+   double* q;
+   double idiel, odiel;
+} orc_qual;
+
+orc_qual* orc_qual_create(const double* verts9, int npan, const double* qpos, const double* q, int nq,
+                          double idiel, double odiel, int maxpanels, int gridpoints, double svdeps);
+void orc_qual_free(orc_qual* o);
+void orc_qual_matvec(orc_qual* o, const double* x, double* b);
+void orc_qual_precond_diag(orc_qual* o, double* p);
+void orc_qual_rhs(orc_qual* o, double* rhs);
+/* early_exit: return zeros when ||P^-1 b|| < tol (GMRES_QualocationOperator);
+ * 0 for GMRES_solv_ecf_qual, which has no such test.  Returns the iterations. */
+int orc_qual_gmres(orc_qual* o, const double* rhs, double* sol, double tol, int maxit, int early_exit, double* hist);
+void orc_qual_potentials(orc_qual* o, const double* sol, double* phi);
+orc_tree* orc_qual_tree(orc_qual* o);
 
 #ifdef __cplusplus
 }

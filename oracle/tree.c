@@ -323,24 +323,33 @@ static void fill_UVT(orc_tree* t, orc_cube* c) {
       if (t->cubes[c->itr[i]].npt > 0) wpt[npts++] = c->itr[i];
       if (t->cubes[c->itr[i]].npan > 0) wpn[npns++] = c->itr[i];
    }
+   /* a DOUBLE_LAYER_INT tree samples one layer: npts*pass rows of
+      double-layer integrals and the rank test against half of those */
+   const int nl = t->double_only ? 1 : 2;
    if (c->npan > 0 && npts > 0) {
       for (int pass = 1;; pass++) {
-         int rows = 2 * npts * pass;
+         int rows = nl * npts * pass;
          double* R = (double*)malloc(sizeof(double) * (size_t)rows * c->npan);
          for (int p = 0; p < pass; p++)
             for (int r = 0; r < npts; r++) {
                orc_cube* P = &t->cubes[wpt[r]];
                int pi = P->pt[(unsigned)p * (unsigned)P->npt / (unsigned)pass];
-               for (int j = 0; j < c->npan; j++)
-                  orc_integrate(t->kernel, t->kappa, &t->pts[3 * pi], &t->panels[c->pan[j]],
-                                &R[(size_t)(p * npts + r) * c->npan + j],
-                                &R[(size_t)(p * npts + r + npts * pass) * c->npan + j]);
+               for (int j = 0; j < c->npan; j++) {
+                  double sl, dl;
+                  orc_integrate(t->kernel, t->kappa, &t->pts[3 * pi], &t->panels[c->pan[j]], &sl, &dl);
+                  if (t->double_only) {
+                     R[(size_t)(p * npts + r) * c->npan + j] = dl;
+                  } else {
+                     R[(size_t)(p * npts + r) * c->npan + j] = sl;
+                     R[(size_t)(p * npts + r + npts * pass) * c->npan + j] = dl;
+                  }
+               }
             }
          double* VT;
          int rank;
          rowbasis(R, rows, c->npan, t->svdeps, &VT, &rank);
          free(R);
-         if (rank < (2 * pass * npts) / 2) {
+         if (rank < (nl * pass * npts) / 2) {
             c->VT = VT;
             c->rowrank = rank;
             break;
@@ -350,21 +359,28 @@ static void fill_UVT(orc_tree* t, orc_cube* c) {
    }
    if (c->npt > 0 && npns > 0) {
       for (int pass = 1;; pass++) {
-         int cols = 2 * npns * pass;
+         int cols = nl * npns * pass;
          double* C = (double*)malloc(sizeof(double) * (size_t)c->npt * cols);
          for (int p = 0; p < pass; p++)
             for (int q = 0; q < npns; q++) {
                orc_cube* P = &t->cubes[wpn[q]];
                int pj = P->pan[(unsigned)p * (unsigned)P->npan / (unsigned)pass];
-               for (int r = 0; r < c->npt; r++)
-                  orc_integrate(t->kernel, t->kappa, &t->pts[3 * c->pt[r]], &t->panels[pj],
-                                &C[(size_t)r * cols + p * npns + q], &C[(size_t)r * cols + p * npns + q + npns * pass]);
+               for (int r = 0; r < c->npt; r++) {
+                  double sl, dl;
+                  orc_integrate(t->kernel, t->kappa, &t->pts[3 * c->pt[r]], &t->panels[pj], &sl, &dl);
+                  if (t->double_only) {
+                     C[(size_t)r * cols + p * npns + q] = dl;
+                  } else {
+                     C[(size_t)r * cols + p * npns + q] = sl;
+                     C[(size_t)r * cols + p * npns + q + npns * pass] = dl;
+                  }
+               }
             }
          double* U;
          int rank;
          colbasis(C, c->npt, cols, t->svdeps, &U, &rank);
          free(C);
-         if (rank < (2 * pass * npns) / 2) {
+         if (rank < (nl * pass * npns) / 2) {
             c->U = U;
             c->colrank = rank;
             break;
@@ -440,6 +456,8 @@ static void fill_PV_UTI(orc_tree* t, orc_cube* c) {
       free(I);
    }
 }
+
+void orc_tree_set_double_only(orc_tree* t) { t->double_only = 1; }
 
 /* Tree_fill (Tree.c:513-565): D, pinv, T, U/VT, PV/UTI. */
 void orc_tree_fill(orc_tree* t) {

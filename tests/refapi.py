@@ -2,6 +2,8 @@
 used by the tests: struct layouts, globals and the solve-path entry points."""
 import ctypes as C
 
+import numpy as np
+
 import fftsvd_amd as fa
 
 
@@ -96,6 +98,43 @@ def solve_path(s, pdb):
     L.solveSurfaceOperator(C.byref(sol), op, pc, rhs, s.total)
     L.collectPotentials(pdb, C.c_uint(n), pdb, C.c_uint(n), op, sol)
     return op, rhs, sol
+
+
+def qual_path(s, pdb):
+    """generateQualocationOperator -> preconditioner -> RHS -> solve ->
+    collectQualocationPotentials, as the reference driver calls them with
+    usequalocation (FFTSVDpbe.c:169-231).  Returns (rhs, sol) pointers."""
+    L = lib()
+    for f in ("generateQualocationOperator", "generateQualocationOperatorPreconditioner", "generateQualocationRHS",
+              "solveQualocationOperator", "collectQualocationPotentials"):
+        getattr(L, f).restype = None
+    qo = C.c_void_p()
+    n = len(pdb)
+    L.generateQualocationOperator(C.byref(qo), pdb, C.c_uint(n), s.ptrs[1], s.counts[1], s.num[1], s.ptrs[2],
+                                  s.counts[2], s.num[2], s.parents[1], s.total)
+    pc = C.c_void_p()
+    L.generateQualocationOperatorPreconditioner(C.byref(pc), qo, s.total)
+    rhs = C.POINTER(C.c_double)()
+    L.generateQualocationRHS(C.byref(rhs), qo, s.total)
+    sol = C.POINTER(C.c_double)()
+    L.solveQualocationOperator(C.byref(sol), qo, pc, rhs, s.total)
+    L.collectQualocationPotentials(pdb, C.c_uint(n), pdb, C.c_uint(n), qo, sol)
+    return rhs, sol
+
+
+def solve_bem_problem(verts9, eps_in, eps_out, rhs):
+    """solveBEMproblem (FFTSVDpbeAPI.c:1974-2026), the GAMESS entry."""
+    L = lib()
+    L.solveBEMproblem.restype = None
+    v = np.ascontiguousarray(verts9, dtype=np.float64).reshape(-1)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    sol = np.zeros(len(r))
+    n = C.c_uint(len(r))
+    ei, eo = C.c_double(eps_in), C.c_double(eps_out)
+    dp = C.POINTER(C.c_double)
+    L.solveBEMproblem(C.byref(n), v.ctypes.data_as(dp), C.byref(ei), C.byref(eo), r.ctypes.data_as(dp),
+                      sol.ctypes.data_as(dp))
+    return sol
 
 
 def free(p):
