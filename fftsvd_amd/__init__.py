@@ -36,6 +36,9 @@ _SIGS = {
     "fftsvd_problem_destroy": (C.c_int, [_vp]),
     "fftsvd_problem_size": (C.c_int, [_vp, _ip, _ip]),
     "fftsvd_problem_unknowns": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    "fftsvd_problem_rhs_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_problem_solve_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
+    "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                      C.c_double, C.c_int, C.POINTER(_vp)]),
     "fftsvd_problem_matvec": (C.c_int, [_vp, C.c_int, _dp, _dp]),
@@ -268,6 +271,33 @@ class Problem:
     def energy(self, sol, conversion=332.112):
         """Delta G = 1/2 sum q phi * conversion / eps_in (reference FFTSVDsalt.c:166-172)."""
         return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in
+
+    # many charge vectors at once: arrays are (rows, ncol), one column per vector
+    def rhs_batch(self, Q):
+        Q = np.asfortranarray(_f64(Q).reshape(len(self.q), -1))
+        ncol = Q.shape[1]
+        R = np.zeros((self.nvec, ncol), order="F")
+        _check(lib().fftsvd_problem_rhs_batch(self.h, ncol, _d(Q), _d(R)))
+        return R
+
+    def solve_batch(self, RHS, tol=1e-6, maxit=1000, precision=F64):
+        """Returns (solutions, iterations per column, info per column)."""
+        RHS = np.asfortranarray(_f64(RHS).reshape(self.nvec, -1))
+        ncol = RHS.shape[1]
+        X = np.zeros((self.nvec, ncol), order="F")
+        its = np.zeros(ncol, dtype=np.int32)
+        info = np.zeros(6 * ncol)
+        _check(lib().fftsvd_problem_solve_batch(self.h, precision, ncol, _d(RHS), _d(X), tol, maxit, _i(its),
+                                                _d(info)))
+        keys = ("iterations", "refinements", "final_residual", "ms", "matvecs64", "matvecs32")
+        return X, its, [dict(zip(keys, info[6 * c:6 * c + 6].tolist())) for c in range(ncol)]
+
+    def potentials_batch(self, X):
+        X = np.asfortranarray(_f64(X).reshape(self.nvec, -1))
+        ncol = X.shape[1]
+        Phi = np.zeros((max(len(self.q), 1), ncol), order="F")
+        _check(lib().fftsvd_problem_potentials_batch(self.h, ncol, _d(X), _d(Phi)))
+        return Phi[:len(self.q)]
 
     def stats(self):
         out = np.zeros(32)

@@ -325,6 +325,55 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
    API_CATCH
 }
 
+int fftsvd_problem_rhs_batch(fftsvd_problem* p, int ncol, const double* q, double* rhs) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (ncol < 1) throw std::runtime_error("ncol must be positive");
+   const size_t nq = pr.q.size(), nv = (size_t)pr.nvec;
+   DBuf<double> dq, dr;
+   dq.upload(q, std::max<size_t>(nq * ncol, 1), pr.stream);
+   dr.alloc(nv * ncol);
+   problem_rhs_batch(pr, ncol, dq.p, dr.p);
+   FSVD_CHECK(hipMemcpyAsync(rhs, dr.p, sizeof(double) * nv * ncol, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const double* rhs, double* sol,
+                               double tol, int maxit, int* iters, double* info) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (ncol < 1) throw std::runtime_error("ncol must be positive");
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   const size_t nv = (size_t)pr.nvec;
+   DBuf<double> db, dx;
+   db.upload(rhs, nv * ncol, pr.stream);
+   dx.alloc(nv * ncol);
+   for (int c = 0; c < ncol; c++) {
+      SolveInfo si = problem_solve(pr, precision, db.p + nv * c, dx.p + nv * c, tol, maxit, nullptr);
+      if (iters) iters[c] = si.iterations;
+      if (info) {
+         double* o = info + 6 * (size_t)c;
+         o[0] = si.iterations;
+         o[1] = si.refinements;
+         o[2] = si.final_residual;
+         o[3] = si.ms;
+         o[4] = si.matvecs64;
+         o[5] = si.matvecs32;
+      }
+   }
+   FSVD_CHECK(hipMemcpyAsync(sol, dx.p, sizeof(double) * nv * ncol, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi) {
+   API_TRY
+   if (ncol < 1) throw std::runtime_error("ncol must be positive");
+   problem_potentials_batch(*p->pr, ncol, sol, phi);
+   API_CATCH
+}
+
 int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi) {
    API_TRY
    problem_potentials(*p->pr, sol, phi);

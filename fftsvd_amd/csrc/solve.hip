@@ -329,6 +329,37 @@ __global__ void k_rhs(int ne, const double* __restrict__ cent, const int* __rest
    rhs[out[e]] += s;
 }
 
+// Many charge vectors (columns of q, nq x ncol) at once: the distances are
+// formed once per (element, charge); each column sums in charge order with
+// zero charges skipped, exactly as k_rhs, so column c equals k_rhs on q_c.
+constexpr int RHS_NB = 8;
+__global__ void k_rhs_batch(int ne, const double* __restrict__ cent, const int* __restrict__ out, int nq, int ncol,
+                            const double* __restrict__ qpos, const double* __restrict__ q, long long ldr,
+                            double* __restrict__ rhs) {
+   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+      const double c0 = cent[3 * e], c1 = cent[3 * e + 1], c2 = cent[3 * e + 2];
+      for (int col0 = 0; col0 < ncol; col0 += RHS_NB) {
+         double s[RHS_NB];
+#pragma unroll
+         for (int c = 0; c < RHS_NB; c++) s[c] = 0.0;
+         for (int i = 0; i < nq; i++) {
+            const double d0 = qpos[3 * i] - c0, d1 = qpos[3 * i + 1] - c1, d2 = qpos[3 * i + 2] - c2;
+            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+#pragma unroll
+            for (int c = 0; c < RHS_NB; c++) {
+               if (col0 + c < ncol) {
+                  const double qi = q[(long long)(col0 + c) * nq + i];
+                  if (qi != 0.0) s[c] += qi / r;
+               }
+            }
+         }
+#pragma unroll
+         for (int c = 0; c < RHS_NB; c++)
+            if (col0 + c < ncol) rhs[(long long)(col0 + c) * ldr + out[e]] += s[c];
+      }
+   }
+}
+
 // phi[j] += sum over panels of (a_e * slp - b_e * dlp) - (self ? 4 pi b_e : 0)
 // with Laplace integrals at the charge (direct branch of
 // SurfaceOperator_calculateReactionPotentials, SurfaceOperator.c:364-396).
@@ -401,6 +432,74 @@ __global__ void __launch_bounds__(256) k_qual_rhs(int n, const PanelGeom* __rest
          r += -dl * q[j] / fourpe;
       }
       rhs[i] = r;
+   }
+}
+
+// k_potentials for ncol solution columns: coef holds (a, b) per element and
+// column (column-major, 2*ne per column); the integrals are formed once per
+// (charge, panel) and each column is summed and reduced as k_potentials.
+constexpr int POT_NB = 8;
+__global__ void __launch_bounds__(256) k_potentials_batch(int ne, int ncol, const PanelGeom* __restrict__ panels,
+                                                          const int* __restrict__ pidx,
+                                                          const double* __restrict__ coef,
+                                                          const int* __restrict__ self_ok,
+                                                          const double* __restrict__ qpos, int nq,
+                                                          double* __restrict__ phi) {
+   __shared__ double red[8];
+   const int j = blockIdx.x;
+   const double x[3] = {qpos[3 * j], qpos[3 * j + 1], qpos[3 * j + 2]};
+   for (int col0 = 0; col0 < ncol; col0 += POT_NB) {
+      double s[POT_NB];
+#pragma unroll
+      for (int c = 0; c < POT_NB; c++) s[c] = 0.0;
+      for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+         const PanelGeom& P = panels[pidx[e]];
+         double sl, dl;
+         laplace_integrals(x, P, sl, dl);
+         const bool self = self_ok[e] && x[0] == P.c[0] && x[1] == P.c[1] && x[2] == P.c[2];
+#pragma unroll
+         for (int c = 0; c < POT_NB; c++) {
+            if (col0 + c < ncol) {
+               const double* cf = coef + (long long)(col0 + c) * 2 * ne;
+               const double a = cf[2 * e], b = cf[2 * e + 1];
+               double v = a * sl - b * dl;
+               if (self) v -= 4.0 * M_PI * b;
+               s[c] += v;
+            }
+         }
+      }
+#pragma unroll
+      for (int c = 0; c < POT_NB; c++) {
+         if (col0 + c < ncol) {   // uniform over the block
+            const double t = block_sum(s[c], red);
+            if (threadIdx.x == 0) phi[(long long)(col0 + c) * nq + j] += t;
+         }
+      }
+   }
+}
+
+// k_qual_rhs for ncol charge columns (q nq x ncol), columns of rhs ldr apart
+__global__ void __launch_bounds__(256) k_qual_rhs_batch(int n, const PanelGeom* __restrict__ panels, int nq, int ncol,
+                                                        const double* __restrict__ qpos, const double* __restrict__ q,
+                                                        double fourpe, long long ldr, double* __restrict__ rhs) {
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const PanelGeom& P = panels[i];
+      for (int col0 = 0; col0 < ncol; col0 += RHS_NB) {
+         double r[RHS_NB];
+#pragma unroll
+         for (int c = 0; c < RHS_NB; c++) r[c] = 0.0;
+         for (int j = 0; j < nq; j++) {
+            const double x[3] = {qpos[3 * j], qpos[3 * j + 1], qpos[3 * j + 2]};
+            double sl, dl;
+            laplace_integrals(x, P, sl, dl);
+#pragma unroll
+            for (int c = 0; c < RHS_NB; c++)
+               if (col0 + c < ncol) r[c] += -dl * q[(long long)(col0 + c) * nq + j] / fourpe;
+         }
+#pragma unroll
+         for (int c = 0; c < RHS_NB; c++)
+            if (col0 + c < ncol) rhs[(long long)(col0 + c) * ldr + i] = r[c];
+      }
    }
 }
 
@@ -518,6 +617,24 @@ void launch_diag_div(int n, const double* p, const double* b, double* x, hipStre
 void launch_qual_rhs(int n, const PanelGeom* panels, int nq, const double* qpos, const double* q, double fourpe,
                      double* rhs, hipStream_t s) {
    if (n > 0) hipLaunchKernelGGL(k_qual_rhs, dim3(nblocks(n)), dim3(256), 0, s, n, panels, nq, qpos, q, fourpe, rhs);
+}
+
+void launch_rhs_batch(int ne, const double* cent, const int* out, int nq, int ncol, const double* qpos,
+                      const double* q, long long ldr, double* rhs, hipStream_t s) {
+   if (ne > 0 && ncol > 0)
+      hipLaunchKernelGGL(k_rhs_batch, dim3(nblocks(ne)), dim3(256), 0, s, ne, cent, out, nq, ncol, qpos, q, ldr, rhs);
+}
+void launch_potentials_batch(int nq, int ne, int ncol, const PanelGeom* panels, const int* pidx, const double* coef,
+                             const int* self_ok, const double* qpos, double* phi, hipStream_t s) {
+   if (nq > 0 && ne > 0 && ncol > 0)
+      hipLaunchKernelGGL(k_potentials_batch, dim3(nq), dim3(256), 0, s, ne, ncol, panels, pidx, coef, self_ok, qpos,
+                         nq, phi);
+}
+void launch_qual_rhs_batch(int n, const PanelGeom* panels, int nq, int ncol, const double* qpos, const double* q,
+                           double fourpe, long long ldr, double* rhs, hipStream_t s) {
+   if (n > 0 && ncol > 0)
+      hipLaunchKernelGGL(k_qual_rhs_batch, dim3(nblocks(n)), dim3(256), 0, s, n, panels, nq, ncol, qpos, q, fourpe,
+                         ldr, rhs);
 }
 
 #define FSVD_KRYLOV_INST(VT)                                                                                        \

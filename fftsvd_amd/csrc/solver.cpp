@@ -48,6 +48,12 @@ void launch_qual_gather(int n, const int* perm, const double* area_t, const XT* 
 void launch_qual_finish(int n, const int* perm, const double* diag_t, const double* xa, const double* yt,
                         double inv4pe, double fourpe, double c0, double* out, hipStream_t s);
 void launch_diag_div(int n, const double* p, const double* b, double* x, hipStream_t s);
+void launch_rhs_batch(int ne, const double* cent, const int* out, int nq, int ncol, const double* qpos,
+                      const double* q, long long ldr, double* rhs, hipStream_t s);
+void launch_potentials_batch(int nq, int ne, int ncol, const PanelGeom* panels, const int* pidx, const double* coef,
+                             const int* self_ok, const double* qpos, double* phi, hipStream_t s);
+void launch_qual_rhs_batch(int n, const PanelGeom* panels, int nq, int ncol, const double* qpos, const double* q,
+                           double fourpe, long long ldr, double* rhs, hipStream_t s);
 void launch_qual_rhs(int n, const PanelGeom* panels, int nq, const double* qpos, const double* q, double fourpe,
                      double* rhs, hipStream_t s);
 
@@ -615,6 +621,43 @@ void problem_rhs(Problem& pr, double* rhs) {
    }
 }
 
+// Right-hand sides of ncol charge vectors at once (q: device, ncharge x ncol
+// column-major; rhs: device, nvec x ncol); column c equals problem_rhs with
+// the problem's charges replaced by q_c.
+void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs) {
+   hipStream_t s = pr.stream;
+   const int nq = (int)pr.q.size();
+   if (pr.qual) {
+      launch_qual_rhs_batch(pr.ntotal, pr.regions[0].d_panels.p, nq, ncol, pr.d_qpos.p, q, 4.0 * M_PI * pr.eps_in,
+                            pr.nvec, rhs, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+      return;
+   }
+   FSVD_CHECK(hipMemsetAsync(rhs, 0, sizeof(double) * pr.nvec * ncol, s));
+   for (Region& R : pr.regions) {
+      if (!R.has_charges) continue;
+      std::vector<double> cent;
+      std::vector<int> out;
+      for (int j = 0; j < R.mynpan; j++) {
+         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         out.push_back(R.startphi + j);
+      }
+      for (int c : R.child) {
+         const Region& C = pr.regions[c];
+         for (int j = 0; j < C.mynpan; j++) {
+            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            out.push_back(C.startdphi + j);
+         }
+      }
+      DBuf<double> dc;
+      DBuf<int> dout;
+      dc.upload(cent, s);
+      dout.upload(out, s);
+      launch_rhs_batch((int)out.size(), dc.p, dout.p, nq, ncol, pr.d_qpos.p, q, pr.nvec, rhs, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+}
+
 template <typename VT>
 static VT* basis_vec(Problem& pr, int j) {
    if constexpr (std::is_same<VT, double>::value) {
@@ -877,6 +920,73 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out) {
       FSVD_CHECK(hipStreamSynchronize(s));
    }
    if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+// Reaction potentials of ncol solutions at once (sol: host, nvec x ncol;
+// phi: host, ncharge x ncol); column c equals problem_potentials on sol_c.
+void problem_potentials_batch(Problem& pr, int ncol, const double* sol, double* phi_out) {
+   hipStream_t s = pr.stream;
+   const int nq = (int)pr.q.size();
+   const long long nv = pr.nvec;
+   DBuf<PanelGeom> dall;
+   if (!pr.qual) dall.upload(pr.panels, s);
+   DBuf<double> dphi;
+   dphi.alloc(std::max<long long>((long long)nq * ncol, 1));
+   FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max<long long>((long long)nq * ncol, 1), s));
+   auto run = [&](const std::vector<int>& pidx, const std::vector<int>& self, const std::vector<double>& coef,
+                  const PanelGeom* panels) {
+      DBuf<int> dp, dsf;
+      DBuf<double> dc;
+      dp.upload(pidx, s);
+      dsf.upload(self, s);
+      dc.upload(coef, s);
+      launch_potentials_batch(nq, (int)pidx.size(), ncol, panels, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   };
+   if (pr.qual) {
+      const int n = pr.ntotal;
+      std::vector<int> pidx(n), self(n, 0);
+      std::vector<double> coef(2 * (size_t)n * ncol, 0.0);
+      for (int i = 0; i < n; i++) pidx[i] = i;
+      for (int c = 0; c < ncol; c++)
+         for (int i = 0; i < n; i++) coef[2 * ((size_t)c * n + i)] = sol[(size_t)c * nv + i];
+      run(pidx, self, coef, pr.regions[0].d_panels.p);
+   } else {
+      for (Region& R : pr.regions) {
+         if (!R.has_charges) continue;
+         std::vector<int> pidx, self;
+         for (int i = 0; i < R.mynpan; i++) {
+            pidx.push_back(R.panel_ids[i]);
+            self.push_back(1);
+         }
+         for (int c : R.child)
+            for (int i = 0; i < pr.regions[c].mynpan; i++) {
+               pidx.push_back(pr.regions[c].panel_ids[i]);
+               self.push_back(0);
+            }
+         const size_t ne = pidx.size();
+         std::vector<double> coef(2 * ne * ncol);
+         for (int col = 0; col < ncol; col++) {
+            const double* x = sol + (size_t)col * nv;
+            double* cf = coef.data() + 2 * ne * col;
+            size_t e = 0;
+            for (int i = 0; i < R.mynpan; i++, e++) {
+               cf[2 * e] = x[R.startdphi + i];
+               cf[2 * e + 1] = x[R.startphi + i];
+            }
+            for (int c : R.child) {
+               const Region& C = pr.regions[c];
+               for (int i = 0; i < C.mynpan; i++, e++) {
+                  cf[2 * e] = -C.eps / R.eps * x[C.startdphi + i];
+                  cf[2 * e + 1] = -x[C.startphi + i];
+               }
+            }
+         }
+         run(pidx, self, coef, dall.p);
+      }
+   }
+   if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq * ncol, hipMemcpyDeviceToHost, s));
    FSVD_CHECK(hipStreamSynchronize(s));
 }
 

@@ -130,5 +130,9 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
 SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit,
                         std::vector<double>* hist);
 void problem_potentials(Problem& pr, const double* sol_host, double* phi_host);
+// Many charge vectors / solutions at once (column-major; q and rhs on the
+// device, sol and phi on the host).
+void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs);
+void problem_potentials_batch(Problem& pr, int ncol, const double* sol_host, double* phi_host);
 
 }  // namespace fsvd

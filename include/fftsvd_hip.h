@@ -86,6 +86,18 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
                          int* iters, double* resid_hist, double* info);
 /* reaction potentials at the charges (SurfaceOperator.c:323-401 direct branch) */
 int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi);
+/* Many charge vectors at once (the multi-right-hand-side pattern of the
+ * reference's hessian(), FFTSVDpbe.c:52-99, which loops generateRHS ->
+ * solveSurfaceOperator -> collectPotentials per charge vector).  Arrays are
+ * column-major: q ncharge x ncol, rhs and sol nvec x ncol, phi ncharge x
+ * ncol, info 6 per column as fftsvd_problem_solve.  Column c equals the
+ * single-vector call with the problem's charges replaced by q_c; the
+ * integrals of the right-hand sides and potentials are formed once for all
+ * columns. */
+int fftsvd_problem_rhs_batch(fftsvd_problem* p, int ncol, const double* q, double* rhs);
+int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const double* rhs, double* sol,
+                               double tol, int maxit, int* iters, double* info);
+int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi);
 /* Operator statistics, up to 32 doubles: 0 panels, 1 regions, 2 setup ms,
  * 3 near entries, 4 far entries, 5 spectrum bins, 6 translation MACs,
  * 7 depth, 8 leaves, 9 tree applications per matvec, 10/11 algorithmic

@@ -20,7 +20,7 @@ RADII = {"N": 1.55, "CA": 1.90, "C": 1.70, "O": 1.52}
 CHARGES = {"N": -0.40, "CA": 0.10, "C": 0.55, "O": -0.55}
 
 
-def write_inputs(d, salt, ionrad=0.0):
+def write_inputs(d, salt, ionrad=0.0, extra=""):
     xyz, _, _ = meshes.synthetic_globule(natoms=24, seed=5, confine=6.0)
     xyz = np.round(xyz, 3)   # the precision PDB columns carry
     lines = []
@@ -33,7 +33,7 @@ def write_inputs(d, salt, ionrad=0.0):
     (d / "fort.12").write_text("atom__resnumbc_charge_\n" +
                                "".join("%-6s%-3s%-4s%1s%8.3f\n" % (k, "", "", "", v) for k, v in CHARGES.items()))
     (d / "fort.10").write_text("indi=4.0\nexdi=80.0\nsalt=%g\nionrad=%g\nprbrad=1.4\nde=1e-6\ndvert=2.0\n"
-                               "svert=1.0\nenergy(g)\nout(frc)\n" % (salt, ionrad))
+                               "svert=1.0\nenergy(g)\nout(frc)\n%s" % (salt, ionrad, extra))
     return xyz
 
 
@@ -94,3 +94,30 @@ def test_driver_stern_layer(tmp_path):
     ref0 = 561.0 * plain.reaction_potentials(x0) / 4.0
     E, E0 = np.dot(q, ref), np.dot(q, ref0)
     assert E < 0 and E0 < 0 and abs(E - E0) > 1e-4 * abs(E0)
+
+
+def test_driver_qualocation(tmp_path):
+    """usequal=true: the driver runs the qualocation formulation; the FRC
+    potentials equal the extended API's qualocation problem on the same
+    surface, and the energy agrees with the first-kind formulation to the
+    discretisation error (a few percent on this 1060-panel surface, with
+    atoms close to it)."""
+    xyz = write_inputs(tmp_path, 0.0, extra="usequal=true\n")
+    r = subprocess.run([build.PBE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and "qualocation operator" in r.stdout
+    frc = (tmp_path / "fort.16").read_text().splitlines()
+    rows = [l for l in frc[12:] if not l.startswith("total energy")]
+    pot = np.array([float(l[35:50]) for l in rows])
+    q = np.array([CHARGES[NAMES[i % 4]] for i in range(len(xyz))])
+    V = np.loadtxt(tmp_path / "operator.srf.d0.vert")
+    F = np.loadtxt(tmp_path / "operator.srf.d0.face", dtype=np.int64)[:, :3] - 1
+    P = meshes.panels_from_mesh(V, F)
+    qual = fa.Qual(P, xyz, q, 4.0, 80.0)
+    x, it, hist, info = qual.solve(qual.rhs(), 1e-6, precision=fa.F64)
+    ref = 561.0 * qual.potentials(x) / 4.0
+    np.testing.assert_allclose(pot, ref, rtol=0, atol=2e-6 * np.abs(ref).max() + 1e-6)
+    first = fa.Problem(P, xyz, q, 4.0, 80.0, 0.0)
+    x1, _, _, _ = first.solve(first.rhs(), 1e-6, precision=fa.F64)
+    E, E1 = np.dot(q, ref), np.dot(q, 561.0 * first.reaction_potentials(x1) / 4.0)
+    assert E < 0 and abs(E - E1) < 0.05 * abs(E1)
