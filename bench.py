@@ -11,8 +11,10 @@ layers, all regions) in mixed precision (FP32 compressed operator, FP64
 vectors), on vectors already resident in HBM.  The JSON line also carries
 the FP64 matvec rate, the solve wall-times (setup, GMRES, potentials) in
 both modes, the solvation energies and their relative difference, the
-roofline of the dominant kernel and a CPU baseline (the FP64 oracle on a
-bounded sample of the same recipe).
+roofline of the dominant kernel, a CPU baseline (the FP64 oracle on a
+bounded sample of the same recipe) and the batched reaction-potential solve
+of 64 seeded charge vectors on the same operator (per-column mixed-vs-FP64
+energy error, aggregate matvec rate).
 """
 import argparse
 import json
@@ -124,6 +126,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--multi-rhs", type=int, default=64, help="charge vectors of the batched solve (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus)
@@ -137,6 +140,11 @@ def main():
     print("# workload: %d atoms, %d panels (grid %.3f A), net charge %+.3f, mesh %.1fs" %
           (len(xyz), len(P), h, q.sum(), t_mesh), flush=True)
 
+    # HIP runtime and device initialisation (once per process, ~0.15 s) is
+    # timed apart from the operator setup
+    t0 = time.time()
+    dev_name, dev_cu = fa.device_info()
+    t_init = time.time() - t0
     t0 = time.time()
     prob = fa.Problem(P, xyz, q, 4.0, 80.0, 0.124, want_f32=True)
     t_setup = time.time() - t0
@@ -167,6 +175,46 @@ def main():
             solve[name]["wall_s"] = t_setup + t_rhs + solve[name]["gmres_s"] + solve[name]["potentials_s"]
         solve["rhs_s"] = t_rhs
         solve["gmres_speedup_mixed"] = solve["fp64"]["gmres_s"] / solve["mixed"]["gmres_s"]
+
+    # batched reaction-potential solve: seeded i.i.d. U[-0.8, 0.8] charge
+    # vectors on the same atoms, one operator; per-column energy error of
+    # the mixed mode against FP64 and the aggregate matvec rate of the solves
+    multi = None
+    if not args.no_solve and args.multi_rhs > 0:
+        ncol = args.multi_rhs
+        Q = np.column_stack([np.random.default_rng(args.seed + 1 + c).uniform(-0.8, 0.8, len(xyz))
+                             for c in range(ncol)])
+        t0 = time.time()
+        R = prob.rhs_batch(Q)
+        t_rhs_b = time.time() - t0
+        multi = {"columns": ncol, "rhs_s": t_rhs_b}
+        E = {}
+        for name, prec in (("fp64", fa.F64), ("mixed", fa.MIXED)):
+            t0 = time.time()
+            X, its, infos = prob.solve_batch(R, 1e-6, precision=prec)
+            t_solve = time.time() - t0
+            t0 = time.time()
+            Phi = prob.potentials_batch(X)
+            t_pot = time.time() - t0
+            E[name] = 0.5 * np.sum(Q * Phi, axis=0) * 332.112 / 4.0
+            nmv = sum(i["matvecs64"] + i["matvecs32"] for i in infos)
+            multi[name] = {"solve_s": t_solve, "potentials_s": t_pot, "iterations_mean": float(np.mean(its)),
+                           "matvecs": int(nmv), "matvecs_per_s": nmv / t_solve}
+            print("# batched %s: %d columns, %.2fs solve (%.0f matvecs/s aggregate), potentials %.2fs" %
+                  (name, ncol, t_solve, nmv / t_solve, t_pot), flush=True)
+        rel = np.abs(E["mixed"] - E["fp64"]) / np.abs(E["fp64"])
+        multi["energy_rel_err_mixed_vs_fp64_max"] = float(rel.max())
+        multi["energy_rel_err_mixed_vs_fp64_median"] = float(np.median(rel))
+        multi["gmres_speedup_mixed"] = multi["fp64"]["solve_s"] / multi["mixed"]["solve_s"]
+        # both modes stop at a 1e-6 residual, so part of their difference is
+        # the tolerance itself: against FP64 converged to 1e-10 (untimed)
+        Xc, _, _ = prob.solve_batch(R, 1e-10, precision=fa.F64)
+        Ec = 0.5 * np.sum(Q * prob.potentials_batch(Xc), axis=0) * 332.112 / 4.0
+        for name in ("fp64", "mixed"):
+            multi[name]["energy_rel_err_vs_fp64_tol1e-10_max"] = float((np.abs(E[name] - Ec) / np.abs(Ec)).max())
+        print("# batched energies: mixed vs fp64 max %.2e (median %.2e); vs fp64 at 1e-10: fp64 %.2e, mixed %.2e" %
+              (rel.max(), np.median(rel), multi["fp64"]["energy_rel_err_vs_fp64_tol1e-10_max"],
+               multi["mixed"]["energy_rel_err_vs_fp64_tol1e-10_max"]), flush=True)
 
     # matvec throughput, device-resident vectors, HIP-event timed on the library stream
     prob.bench_matvec(fa.MIXED, max(1, args.warmup))
@@ -235,7 +283,10 @@ def main():
         "wall_step_check_ms": wall32 * 1e3 / args.steps,
         "matvec_parts_ms": {"mixed": parts32, "fp64": parts64},
         "setup_s": t_setup,
+        "device_init_s": t_init,
+        "device": {"name": dev_name, "cus": dev_cu},
         "solve": solve,
+        "multi_rhs": multi,
         "operator": st,
         "roofline": roofline,
         "roofline_stages": rf,

@@ -39,6 +39,9 @@ _SIGS = {
     "fftsvd_problem_rhs_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_solve_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
+    "fftsvd_problem_grad_charges": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
     "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                      C.c_double, C.c_int, C.POINTER(_vp)]),
     "fftsvd_problem_matvec": (C.c_int, [_vp, C.c_int, _dp, _dp]),
@@ -271,6 +274,28 @@ class Problem:
     def energy(self, sol, conversion=332.112):
         """Delta G = 1/2 sum q phi * conversion / eps_in (reference FFTSVDsalt.c:166-172)."""
         return 0.5 * float(np.dot(self.q, self.reaction_potentials(sol))) * conversion / self.eps_in
+
+    # adjoint
+    def matvec_transpose(self, y, precision=F64):
+        x = np.zeros(self.nvec)
+        _check(lib().fftsvd_problem_matvec_transpose(self.h, precision, _d(_f64(y)), _d(x)))
+        return x
+
+    def solve_adjoint(self, rhs, tol=1e-6, maxit=1000, precision=F64):
+        sol = np.zeros(self.nvec)
+        it = C.c_int()
+        info = np.zeros(6)
+        _check(lib().fftsvd_problem_solve_adjoint(self.h, precision, _d(_f64(rhs)), _d(sol), tol, maxit,
+                                                  C.byref(it), _d(info)))
+        return sol, it.value
+
+    def grad_charges(self, sol, tol=1e-6, maxit=1000, precision=F64, conversion=332.112):
+        """d(energy)/d(charges) at the forward solution sol, in the units of
+        energy(): conversion/eps_in * 1/2 (phi + B^T lambda)."""
+        g = np.zeros(max(len(self.q), 1))
+        it = C.c_int()
+        _check(lib().fftsvd_problem_grad_charges(self.h, precision, _d(_f64(sol)), tol, maxit, _d(g), C.byref(it)))
+        return g[:len(self.q)] * conversion / self.eps_in
 
     # many charge vectors at once: arrays are (rows, ncol), one column per vector
     def rhs_batch(self, Q):

@@ -367,6 +367,60 @@ int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const
    API_CATCH
 }
 
+int fftsvd_problem_matvec_transpose(fftsvd_problem* p, int precision, const double* y, double* x) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (pr.qual) throw std::runtime_error("the adjoint of the qualocation operator is not provided");
+   const size_t n = (size_t)pr.nvec;
+   DBuf<double> dy, dx;
+   dy.upload(y, n, pr.stream);
+   dx.alloc(n);
+   pr.adjoint = true;
+   try {
+      problem_matvec(pr, precision, dy.p, dx.p);
+   } catch (...) {
+      pr.adjoint = false;
+      throw;
+   }
+   pr.adjoint = false;
+   FSVD_CHECK(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_solve_adjoint(fftsvd_problem* p, int precision, const double* rhs, double* sol, double tol,
+                                 int maxit, int* iters, double* info) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   const size_t n = (size_t)pr.nvec;
+   DBuf<double> db, dx;
+   db.upload(rhs, n, pr.stream);
+   dx.alloc(n);
+   SolveInfo si = problem_solve_adjoint(pr, precision, db.p, dx.p, tol, maxit);
+   FSVD_CHECK(hipMemcpyAsync(sol, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   if (iters) *iters = si.iterations;
+   if (info) {
+      info[0] = si.iterations;
+      info[1] = si.refinements;
+      info[2] = si.final_residual;
+      info[3] = si.ms;
+      info[4] = si.matvecs64;
+      info[5] = si.matvecs32;
+   }
+   API_CATCH
+}
+
+int fftsvd_problem_grad_charges(fftsvd_problem* p, int precision, const double* sol, double tol, int maxit,
+                                double* grad, int* iters) {
+   API_TRY
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   SolveInfo si = problem_grad_charges(*p->pr, precision, sol, tol, maxit, grad);
+   if (iters) *iters = si.iterations;
+   API_CATCH
+}
+
 int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi) {
    API_TRY
    if (ncol < 1) throw std::runtime_error("ncol must be positive");

@@ -30,6 +30,41 @@ __global__ void k_split(int n, const int* __restrict__ out_idx, const double* __
    if (k < n) y[out_idx[k]] = yt[k] + corr[k] * xd[k];
 }
 
+// Transposed region operator, per region on one stream: gather
+// yh[k] = y[out_idx[k]], then (after the transposed tree applies into
+// vs = S^T yh and vd = D^T yh) scatter-add x[dphi_idx[k]] += s_dphi vs and
+// x[phi_idx[k]] += s_phi (vd + corr yh): the transpose of k_join / tree
+// apply / k_split.  Within a region the targets are distinct.
+template <typename XT>
+__global__ void k_gather_out(int n, const int* __restrict__ out_idx, const XT* __restrict__ y,
+                             double* __restrict__ yh) {
+   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+      yh[k] = (double)y[out_idx[k]];
+}
+
+__global__ void k_scatter_adj(int n, const int* __restrict__ phi_idx, const int* __restrict__ dphi_idx,
+                              const double* __restrict__ s_phi, const double* __restrict__ s_dphi,
+                              const double* __restrict__ corr, const double* __restrict__ yh,
+                              const double* __restrict__ vs, const double* __restrict__ vd, double* __restrict__ x) {
+   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+      x[dphi_idx[k]] += s_dphi[k] * vs[k];
+      x[phi_idx[k]] += s_phi[k] * (vd[k] + corr[k] * yh[k]);
+   }
+}
+
+// x = P^{-T} b for the transposed 2x2 blocks; in place safe.
+__global__ void k_precond_t(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
+                            const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x) {
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k < nb) {
+      int i = ia[k], j = ib[k];
+      double bi = b[i], bj = b[j];
+      const double* m = M + 4 * (size_t)k;
+      x[i] = m[0] * bi + m[2] * bj;
+      x[j] = m[1] * bi + m[3] * bj;
+   }
+}
+
 // x = P^{-1} b for disjoint 2x2 blocks (factored on the host); in place safe.
 __global__ void k_precond(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
                           const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x) {
@@ -503,6 +538,46 @@ __global__ void __launch_bounds__(256) k_qual_rhs_batch(int n, const PanelGeom* 
    }
 }
 
+// Transposed collection: per element e (a panel of a charged region) the
+// sums over charges of q_j slp(j, e) and q_j dlp(j, e), with the 4 pi
+// self term of k_potentials counted when the charge sits at the centroid.
+__global__ void __launch_bounds__(256) k_collect_t(int ne, const PanelGeom* __restrict__ panels,
+                                                   const int* __restrict__ pidx, const int* __restrict__ self_ok,
+                                                   int nq, const double* __restrict__ qpos,
+                                                   const double* __restrict__ q, double* __restrict__ ssl,
+                                                   double* __restrict__ sdl) {
+   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+      const PanelGeom& P = panels[pidx[e]];
+      double a = 0.0, b = 0.0;
+      for (int j = 0; j < nq; j++) {
+         const double x[3] = {qpos[3 * j], qpos[3 * j + 1], qpos[3 * j + 2]};
+         double sl, dl;
+         laplace_integrals(x, P, sl, dl);
+         a += q[j] * sl;
+         b += q[j] * dl;
+         if (self_ok[e] && x[0] == P.c[0] && x[1] == P.c[1] && x[2] == P.c[2]) b += 4.0 * M_PI * q[j];
+      }
+      ssl[e] = a;
+      sdl[e] = b;
+   }
+}
+
+// Transposed right-hand side: g[j] += sum_e lam[out[e]] / |c_e - x_j|.
+__global__ void __launch_bounds__(256) k_rhs_t(int ne, const double* __restrict__ cent, const int* __restrict__ out,
+                                               const double* __restrict__ lam, const double* __restrict__ qpos,
+                                               double* __restrict__ g) {
+   __shared__ double red[8];
+   const int j = blockIdx.x;
+   const double x0 = qpos[3 * j], x1 = qpos[3 * j + 1], x2 = qpos[3 * j + 2];
+   double s = 0.0;
+   for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const double d0 = x0 - cent[3 * e], d1 = x1 - cent[3 * e + 1], d2 = x2 - cent[3 * e + 2];
+      s += lam[out[e]] / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+   }
+   s = block_sum(s, red);
+   if (threadIdx.x == 0) g[j] += s;
+}
+
 // diagonal of the near-field blocks: diag[p0 + i] from leaf rows i < p
 __global__ void k_leaf_diag(int nleaf, const LeafDesc* __restrict__ leaves, const double* __restrict__ D,
                             double* __restrict__ ds, double* __restrict__ dd) {
@@ -635,6 +710,32 @@ void launch_qual_rhs_batch(int n, const PanelGeom* panels, int nq, int ncol, con
    if (n > 0 && ncol > 0)
       hipLaunchKernelGGL(k_qual_rhs_batch, dim3(nblocks(n)), dim3(256), 0, s, n, panels, nq, ncol, qpos, q, fourpe,
                          ldr, rhs);
+}
+
+template <typename XT>
+void launch_gather_out(int n, const int* out_idx, const XT* y, double* yh, hipStream_t s) {
+   if (n > 0) hipLaunchKernelGGL((k_gather_out<XT>), dim3(nblocks(n)), dim3(256), 0, s, n, out_idx, y, yh);
+}
+template void launch_gather_out<double>(int, const int*, const double*, double*, hipStream_t);
+template void launch_gather_out<float>(int, const int*, const float*, double*, hipStream_t);
+void launch_scatter_adj(int n, const int* pi, const int* di, const double* sp, const double* sd, const double* corr,
+                        const double* yh, const double* vs, const double* vd, double* x, hipStream_t s) {
+   if (n > 0)
+      hipLaunchKernelGGL(k_scatter_adj, dim3(nblocks(n)), dim3(256), 0, s, n, pi, di, sp, sd, corr, yh, vs, vd, x);
+}
+void launch_precond_t(int nb, const int* ia, const int* ib, const double* M, const double* b, double* x,
+                      hipStream_t s) {
+   if (nb > 0) hipLaunchKernelGGL(k_precond_t, dim3(nblocks(nb)), dim3(256), 0, s, nb, ia, ib, M, b, x);
+}
+void launch_collect_t(int ne, const PanelGeom* panels, const int* pidx, const int* self_ok, int nq, const double* qpos,
+                      const double* q, double* ssl, double* sdl, hipStream_t s) {
+   if (ne > 0)
+      hipLaunchKernelGGL(k_collect_t, dim3(nblocks(ne)), dim3(256), 0, s, ne, panels, pidx, self_ok, nq, qpos, q, ssl,
+                         sdl);
+}
+void launch_rhs_t(int nq, int ne, const double* cent, const int* out, const double* lam, const double* qpos, double* g,
+                  hipStream_t s) {
+   if (nq > 0 && ne > 0) hipLaunchKernelGGL(k_rhs_t, dim3(nq), dim3(256), 0, s, ne, cent, out, lam, qpos, g);
 }
 
 #define FSVD_KRYLOV_INST(VT)                                                                                        \

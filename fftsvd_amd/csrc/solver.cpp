@@ -48,6 +48,16 @@ void launch_qual_gather(int n, const int* perm, const double* area_t, const XT* 
 void launch_qual_finish(int n, const int* perm, const double* diag_t, const double* xa, const double* yt,
                         double inv4pe, double fourpe, double c0, double* out, hipStream_t s);
 void launch_diag_div(int n, const double* p, const double* b, double* x, hipStream_t s);
+template <typename XT>
+void launch_gather_out(int n, const int* out_idx, const XT* y, double* yh, hipStream_t s);
+void launch_scatter_adj(int n, const int* pi, const int* di, const double* sp, const double* sd, const double* corr,
+                        const double* yh, const double* vs, const double* vd, double* x, hipStream_t s);
+void launch_precond_t(int nb, const int* ia, const int* ib, const double* M, const double* b, double* x,
+                      hipStream_t s);
+void launch_collect_t(int ne, const PanelGeom* panels, const int* pidx, const int* self_ok, int nq, const double* qpos,
+                      const double* q, double* ssl, double* sdl, hipStream_t s);
+void launch_rhs_t(int nq, int ne, const double* cent, const int* out, const double* lam, const double* qpos, double* g,
+                  hipStream_t s);
 void launch_rhs_batch(int ne, const double* cent, const int* out, int nq, int ncol, const double* qpos,
                       const double* q, long long ldr, double* rhs, hipStream_t s);
 void launch_potentials_batch(int nq, int ne, int ncol, const PanelGeom* panels, const int* pidx, const double* coef,
@@ -534,11 +544,33 @@ static void qual_matvec(Problem& pr, int prec, const XT* x, double* y) {
    launch_qual_finish(n, pr.qual_perm.p, pr.qual_diag_t.p, R.xs.p, R.yt.p, 1.0 / fourpe, fourpe, c0, y, s);
 }
 
+// x = A^T y: per region (in order, on the library stream) the transposes
+// of k_split, the tree apply (both layers) and k_join, scatter-added into x
+// (SurfaceOperator_topmultiply transposed; Tree_multiply_transpose,
+// Tree.c:769-832, per layer).
+template <typename XT>
+static void adjoint_matvec(Problem& pr, int prec, const XT* y, double* x) {
+   if (pr.qual) throw std::runtime_error("the adjoint of the qualocation operator is not provided");
+   hipStream_t s = pr.stream;
+   FSVD_CHECK(hipMemsetAsync(x, 0, sizeof(double) * pr.nvec, s));
+   for (Region& R : pr.regions) {
+      if (R.panel_ids.empty()) continue;
+      const int n = (int)R.panel_ids.size();
+      TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+      if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
+      launch_gather_out<XT>(n, R.out_idx.p, y, R.yt.p, s);
+      tree_apply_transpose(op, prec, 0, R.yt.p, R.xs.p, s);
+      tree_apply_transpose(op, prec, 1, R.yt.p, R.xd.p, s);
+      launch_scatter_adj(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, R.corr.p, R.yt.p, R.xs.p, R.xd.p, x, s);
+   }
+}
+
 // Every region writes a disjoint set of rows of y (the phi rows of its own
 // panels and the dphi rows of its children's panels), so regions on
 // different streams never touch the same output.
 template <typename XT>
 static void matvec_impl(Problem& pr, int prec, const XT* x, double* y) {
+   if (pr.adjoint) return adjoint_matvec(pr, prec, x, y);
    if (pr.qual) return qual_matvec(pr, prec, x, y);
    hipStream_t s = pr.stream;
    if (pr.naux > 0) {
@@ -584,6 +616,7 @@ void problem_matvec(Problem& pr, int prec, const float* x, double* y) { matvec_i
 
 void problem_precond(Problem& pr, const double* b, double* x) {
    if (pr.qual) return launch_diag_div(pr.ntotal, pr.qual_pdiag.p, b, x, pr.stream);
+   if (pr.adjoint) return launch_precond_t(pr.nblocks, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, b, x, pr.stream);
    launch_precond(pr.nblocks, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, b, x, pr.stream);
 }
 
@@ -988,6 +1021,105 @@ void problem_potentials_batch(Problem& pr, int ncol, const double* sol, double* 
    }
    if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq * ncol, hipMemcpyDeviceToHost, s));
    FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+// A^T x = b by the same GMRES (left preconditioner P^T), FP64 or mixed.
+SolveInfo problem_solve_adjoint(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit) {
+   if (pr.qual) throw std::runtime_error("the adjoint of the qualocation operator is not provided");
+   struct Flag {
+      Problem& p;
+      explicit Flag(Problem& q) : p(q) { p.adjoint = true; }
+      ~Flag() { p.adjoint = false; }
+   } flag(pr);
+   return problem_solve(pr, mode, rhs, sol, tol, maxit, nullptr);
+}
+
+SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
+                               double* grad_host) {
+   if (pr.qual) throw std::runtime_error("charge gradients of the qualocation operator are not provided");
+   hipStream_t s = pr.stream;
+   const int nq = (int)pr.q.size();
+   const long long nv = pr.nvec;
+   // phi = C x
+   std::vector<double> phi(std::max(nq, 1), 0.0);
+   problem_potentials(pr, sol_host, phi.data());
+   // g = C^T q
+   std::vector<double> g(nv, 0.0);
+   DBuf<PanelGeom> dall;
+   dall.upload(pr.panels, s);
+   for (Region& R : pr.regions) {
+      if (!R.has_charges) continue;
+      std::vector<int> pidx, self;
+      for (int i = 0; i < R.mynpan; i++) {
+         pidx.push_back(R.panel_ids[i]);
+         self.push_back(1);
+      }
+      for (int c : R.child)
+         for (int i = 0; i < pr.regions[c].mynpan; i++) {
+            pidx.push_back(pr.regions[c].panel_ids[i]);
+            self.push_back(0);
+         }
+      const int ne = (int)pidx.size();
+      DBuf<int> dp, dsf;
+      DBuf<double> dsl, ddl;
+      dp.upload(pidx, s);
+      dsf.upload(self, s);
+      dsl.alloc(std::max(ne, 1));
+      ddl.alloc(std::max(ne, 1));
+      launch_collect_t(ne, dall.p, dp.p, dsf.p, nq, pr.d_qpos.p, pr.d_q.p, dsl.p, ddl.p, s);
+      std::vector<double> ssl(ne), sdl(ne);
+      FSVD_CHECK(hipMemcpyAsync(ssl.data(), dsl.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipMemcpyAsync(sdl.data(), ddl.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      int e = 0;
+      for (int i = 0; i < R.mynpan; i++, e++) {
+         g[R.startdphi + i] += ssl[e];
+         g[R.startphi + i] += -sdl[e];
+      }
+      for (int c : R.child) {
+         const Region& C = pr.regions[c];
+         for (int i = 0; i < C.mynpan; i++, e++) {
+            g[C.startdphi + i] += -C.eps / R.eps * ssl[e];
+            g[C.startphi + i] += sdl[e];
+         }
+      }
+   }
+   // lambda = A^-T g
+   DBuf<double> dg, dlam;
+   dg.upload(g, s);
+   dlam.alloc(nv);
+   SolveInfo si = problem_solve_adjoint(pr, mode, dg.p, dlam.p, tol, maxit);
+   // B^T lambda over the right-hand-side rows of each charged region
+   DBuf<double> dgq;
+   dgq.alloc(std::max(nq, 1));
+   FSVD_CHECK(hipMemsetAsync(dgq.p, 0, sizeof(double) * std::max(nq, 1), s));
+   for (Region& R : pr.regions) {
+      if (!R.has_charges) continue;
+      std::vector<double> cent;
+      std::vector<int> out;
+      for (int j = 0; j < R.mynpan; j++) {
+         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         out.push_back(R.startphi + j);
+      }
+      for (int c : R.child) {
+         const Region& C = pr.regions[c];
+         for (int j = 0; j < C.mynpan; j++) {
+            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            out.push_back(C.startdphi + j);
+         }
+      }
+      DBuf<double> dc;
+      DBuf<int> dout;
+      dc.upload(cent, s);
+      dout.upload(out, s);
+      launch_rhs_t(nq, (int)out.size(), dc.p, dout.p, dlam.p, pr.d_qpos.p, dgq.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   std::vector<double> bt(std::max(nq, 1));
+   if (nq) FSVD_CHECK(hipMemcpyAsync(bt.data(), dgq.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   for (int j = 0; j < nq; j++) grad_host[j] = 0.5 * (phi[j] + bt[j]);
+   return si;
 }
 
 }  // namespace fsvd

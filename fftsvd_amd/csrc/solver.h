@@ -88,6 +88,9 @@ struct Problem {
    DBuf<int> qual_perm;
    DBuf<double> qual_area_t, qual_diag_t, qual_pdiag;
    bool gmres_zero_test = true;
+   // while set, problem_matvec applies A^T and problem_precond P^-T (the
+   // adjoint solves of problem_solve_adjoint)
+   bool adjoint = false;
    std::vector<PanelGeom> panels;        // global panel geometry
    std::vector<Region> regions;          // regions[0] = exterior (root)
    std::vector<double> qpos, q;
@@ -133,6 +136,13 @@ void problem_potentials(Problem& pr, const double* sol_host, double* phi_host);
 // Many charge vectors / solutions at once (column-major; q and rhs on the
 // device, sol and phi on the host).
 void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs);
+// Adjoint: A^T solve (device vectors) and the gradient of
+// E = 1/2 sum_j q_j phi_j with respect to the charges,
+// dE/dq = 1/2 (phi + B^T lambda), A^T lambda = C^T q (B: charges -> RHS,
+// C: solution -> reaction potentials); sol and grad on the host.
+SolveInfo problem_solve_adjoint(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit);
+SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
+                               double* grad_host);
 void problem_potentials_batch(Problem& pr, int ncol, const double* sol_host, double* phi_host);
 
 }  // namespace fsvd

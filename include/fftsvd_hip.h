@@ -98,6 +98,20 @@ int fftsvd_problem_rhs_batch(fftsvd_problem* p, int ncol, const double* q, doubl
 int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const double* rhs, double* sol,
                                double tol, int maxit, int* iters, double* info);
 int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi);
+/* Adjoint (surface operators): x = A^T y, the transposes of the region
+ * join / tree apply / split (Tree_multiply_transpose, Tree.c:769-832, per
+ * layer); A^T x = b by the same GMRES with the transposed 2x2 blocks as
+ * preconditioner; and the gradient of E = 1/2 sum_j q_j phi_j with respect
+ * to the problem's charges at the forward solution sol (2N, host):
+ * dE/dq = 1/2 (phi + B^T lambda) with A^T lambda = C^T q, B the charges ->
+ * right-hand-side map and C the solution -> reaction-potential map.  The
+ * reference has no adjoint path; FFTSVDpbe.c's hessian() forms the same
+ * derivatives column by column from forward solves. */
+int fftsvd_problem_matvec_transpose(fftsvd_problem* p, int precision, const double* y, double* x);
+int fftsvd_problem_solve_adjoint(fftsvd_problem* p, int precision, const double* rhs, double* sol, double tol,
+                                 int maxit, int* iters, double* info);
+int fftsvd_problem_grad_charges(fftsvd_problem* p, int precision, const double* sol, double tol, int maxit,
+                                double* grad, int* iters);
 /* Operator statistics, up to 32 doubles: 0 panels, 1 regions, 2 setup ms,
  * 3 near entries, 4 far entries, 5 spectrum bins, 6 translation MACs,
  * 7 depth, 8 leaves, 9 tree applications per matvec, 10/11 algorithmic
