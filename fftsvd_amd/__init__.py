@@ -40,6 +40,7 @@ _SIGS = {
     "fftsvd_problem_solve_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_grad_charges": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
     "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -133,6 +134,9 @@ class Tree:
         depth, nc, npan = C.c_int(), C.c_int(), C.c_int()
         _check(lib().fftsvd_tree_info(prob.h, which, C.byref(depth), C.byref(nc), C.byref(npan)))
         self.depth, self.ncubes, self.n = depth.value, nc.value, npan.value
+        info = np.zeros(13, dtype=np.int32)   # the root cube holds every point
+        _check(lib().fftsvd_tree_cube_info(prob.h, which, 0, _i(info)))
+        self.npt = int(info[6])
 
     def cubes(self):
         out = []
@@ -148,29 +152,29 @@ class Tree:
         return out
 
     def cube_mats(self, cube, gp3):
-        np_, r, rr = cube["npan"], cube["rowrank"], cube["colrank"]
+        np_, nq, r, rr = cube["npan"], cube["npt"], cube["rowrank"], cube["colrank"]
         m = len(cube["dcols"])
         VT = np.zeros(max(r * np_, 1))
-        U = np.zeros(max(np_ * rr, 1))
+        U = np.zeros(max(nq * rr, 1))
         PVs = np.zeros(max(gp3 * r, 1))
         PVd = np.zeros(max(gp3 * r, 1))
         UTI = np.zeros(max(gp3 * rr, 1))
-        Ds = np.zeros(max(np_ * m, 1))
-        Dd = np.zeros(max(np_ * m, 1))
+        Ds = np.zeros(max(nq * m, 1))
+        Dd = np.zeros(max(nq * m, 1))
         _check(lib().fftsvd_tree_cube_mats(self.p.h, self.which, cube["id"], _d(VT), _d(U), _d(PVs), _d(PVd), _d(UTI),
                                            _d(Ds), _d(Dd)))
-        return dict(VT=VT[:r * np_].reshape(r, np_), U=U[:np_ * rr].reshape(np_, rr),
+        return dict(VT=VT[:r * np_].reshape(r, np_), U=U[:nq * rr].reshape(nq, rr),
                     PVs=PVs[:gp3 * r].reshape(gp3, r), PVd=PVd[:gp3 * r].reshape(gp3, r),
-                    UTI=UTI[:gp3 * rr].reshape(rr, gp3), Ds=Ds[:np_ * m].reshape(np_, m),
-                    Dd=Dd[:np_ * m].reshape(np_, m))
+                    UTI=UTI[:gp3 * rr].reshape(rr, gp3), Ds=Ds[:nq * m].reshape(nq, m),
+                    Dd=Dd[:nq * m].reshape(nq, m))
 
     def apply_both(self, xs, xd, precision=F64):
-        y = np.zeros(self.n)
+        y = np.zeros(self.npt)
         _check(lib().fftsvd_tree_apply(self.p.h, self.which, precision, _d(_f64(xs)), _d(_f64(xd)), _d(y)))
         return y
 
     def apply_transpose(self, layer, x, precision=F64):
-        y = np.zeros(self.n)
+        y = np.zeros(self.n)   # x has npt entries, y one per panel
         _check(lib().fftsvd_tree_apply_transpose(self.p.h, self.which, precision, layer, _d(_f64(x)), _d(y)))
         return y
 

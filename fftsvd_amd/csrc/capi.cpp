@@ -144,6 +144,7 @@ static SurfaceSpec make_spec(int ndiel, const int* npan_diel, int ncav, const in
    sp.gridpoints = gridpoints;
    sp.svdeps = svdeps;
    sp.want32 = want_f32 != 0;
+   sp.m3 = accelerateM3 != 0;
    return sp;
 }
 
@@ -434,8 +435,14 @@ int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi)
    API_CATCH
 }
 
+// which >= 0: the tree of region `which`; which < 0: the M3 tree of region -which-1
 static TreeOp& region_op(Problem& pr, int which) {
-   if (which < 0 || which >= (int)pr.regions.size()) throw std::runtime_error("no such region");
+   if (which < 0) {
+      const int r = -which - 1;
+      if (r >= (int)pr.regions.size() || !pr.regions[r].m3) throw std::runtime_error("no M3 tree for that region");
+      return *pr.regions[r].m3;
+   }
+   if (which >= (int)pr.regions.size()) throw std::runtime_error("no such region");
    Region& R = pr.regions[which];
    return R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
 }
@@ -571,6 +578,14 @@ int fftsvd_tree_cube_panels(fftsvd_problem* p, int which, int cube, int* pan, in
       const LeafDesc& L = op.h_leaves[op.cube_leaf[cube]];
       for (int k = 0; k < L.m; k++) dcols[k] = op.tree.pan_perm[op.h_dcol[L.col_off + k]];
    }
+   API_CATCH
+}
+
+int fftsvd_tree_cube_points(fftsvd_problem* p, int which, int cube, int* pt) {
+   API_TRY
+   TreeOp& op = region_op(*p->pr, which);
+   const Cube& c = op.tree.cubes.at(cube);
+   for (int k = 0; k < c.nq; k++) pt[k] = op.tree.pt_perm[c.q0 + k];
    API_CATCH
 }
 
@@ -911,6 +926,7 @@ void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBe
    sp.gridpoints = GRID_SIZE;
    sp.svdeps = SVD_ERROR;
    sp.want32 = true;
+   sp.m3 = accelerateM3 != 0;
    std::unique_ptr<Problem> pr;
    try {
       pr = build_problem(sp);
@@ -935,6 +951,7 @@ void generateSurfaceOperator(SurfaceOperator* pbesurfaceoperator, PDBentry* PDBe
       o->startphi = R.startphi;
       o->startdphi = R.startdphi;
       o->tree = h;
+      o->M3 = R.m3 ? (void*)h : nullptr;
       o->parent = R.parent >= 0 ? so[R.parent] : nullptr;
       o->numchildren = (unsigned)R.child.size();
       o->children = (SurfaceOperator*)calloc(R.child.size() + 1, sizeof(SurfaceOperator));

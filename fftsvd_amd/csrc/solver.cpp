@@ -394,6 +394,55 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
       R.xd.alloc(n + 1);
       R.yt.alloc(n + 1);
    }
+   // M3 trees (accelerateM3; FFTSVDpbeAPI.c:1256-1266,1503-1513): Laplace,
+   // both layers, from the region's tree panels to the charges; the region
+   // join (joinpotentials, SurfaceOperator.c:51-66) in the M3 tree's order
+   if (spec.m3 && !pr.q.empty()) {
+      for (Region& R : pr.regions) {
+         if (!R.has_charges) continue;
+         const int n = (int)R.panel_ids.size();
+         std::vector<PanelGeom> tp(n);
+         for (int i = 0; i < n; i++) tp[i] = pr.panels[R.panel_ids[i]];
+         R.m3_panels.upload(tp, s);
+         R.m3 = build_tree_op(tp, R.m3_panels.p, pr.qpos.data(), (int)pr.q.size(), LAPLACE, 0.0, spec.maxpanels,
+                              spec.gridpoints, spec.svdeps, false, s);
+         std::vector<int> lpi(n), ldi(n);
+         std::vector<double> lsp(n), lsd(n);
+         for (int l = 0; l < R.mynpan; l++) {
+            lpi[l] = R.startphi + l;
+            ldi[l] = R.startdphi + l;
+            lsp[l] = 1.0;
+            lsd[l] = -1.0;
+         }
+         int cur = R.mynpan;
+         for (int c : R.child) {
+            const Region& C = pr.regions[c];
+            for (int j = 0; j < C.mynpan; j++) {
+               lpi[cur + j] = C.startphi + j;
+               ldi[cur + j] = C.startdphi + j;
+               lsp[cur + j] = -1.0;
+               lsd[cur + j] = C.eps / R.eps;
+            }
+            cur += C.mynpan;
+         }
+         std::vector<int> pi(n), di(n);
+         std::vector<double> sp(n), sd(n);
+         for (int k = 0; k < n; k++) {
+            const int l = R.m3->tree.pan_perm[k];
+            pi[k] = lpi[l];
+            di[k] = ldi[l];
+            sp[k] = lsp[l];
+            sd[k] = lsd[l];
+         }
+         R.m3_phi_idx.upload(pi, s);
+         R.m3_dphi_idx.upload(di, s);
+         R.m3_s_phi.upload(sp, s);
+         R.m3_s_dphi.upload(sd, s);
+         R.m3_xs.alloc(n + 1);
+         R.m3_xd.alloc(n + 1);
+         R.m3_y.alloc(pr.q.size() + 1);
+      }
+   }
    // preconditioner: 2x2 blocks per owned panel from the region's own near-field diagonal
    // (SurfaceOperator_generatePreconditioner, SurfaceOperator.c:170-197)
    {
@@ -901,7 +950,7 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
    return tot;
 }
 
-void problem_potentials(Problem& pr, const double* sol, double* phi_out) {
+void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool use_m3) {
    hipStream_t s = pr.stream;
    const int nq = (int)pr.q.size();
    DBuf<PanelGeom> dall;
@@ -926,7 +975,24 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out) {
       launch_potentials(nq, pr.ntotal, pr.regions[0].d_panels.p, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
    }
    for (Region& R : pr.regions) {
-      if (pr.qual || !R.has_charges) continue;
+      if (!R.m3 || !use_m3) continue;
+      // SurfaceOperator_calculateReactionPotentials, M3 branch (SurfaceOperator.c:347-363):
+      // phi_j = -(S dphi + D phi)_j through the M3 tree
+      const int n = (int)R.panel_ids.size();
+      DBuf<double> dsol;
+      dsol.upload(sol, pr.nvec, s);
+      launch_join(n, R.m3_phi_idx.p, R.m3_dphi_idx.p, R.m3_s_phi.p, R.m3_s_dphi.p, dsol.p, R.m3_xs.p, R.m3_xd.p, s);
+      tree_apply(*R.m3, PREC_F64, R.m3_xs.p, R.m3_xd.p, R.m3_y.p, s);
+      std::vector<double> react(nq), acc(std::max(nq, 1));
+      FSVD_CHECK(hipMemcpyAsync(react.data(), R.m3_y.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipMemcpyAsync(acc.data(), dphi.p, sizeof(double) * std::max(nq, 1), hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      for (int k = 0; k < nq; k++) acc[R.m3->tree.pt_perm[k]] += -react[k];
+      FSVD_CHECK(hipMemcpyAsync(dphi.p, acc.data(), sizeof(double) * std::max(nq, 1), hipMemcpyHostToDevice, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   for (Region& R : pr.regions) {
+      if (pr.qual || !R.has_charges || (R.m3 && use_m3)) continue;
       std::vector<int> pidx, self;
       std::vector<double> coef;
       for (int i = 0; i < R.mynpan; i++) {
@@ -1042,7 +1108,7 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
    const long long nv = pr.nvec;
    // phi = C x
    std::vector<double> phi(std::max(nq, 1), 0.0);
-   problem_potentials(pr, sol_host, phi.data());
+   problem_potentials(pr, sol_host, phi.data(), false);
    // g = C^T q
    std::vector<double> g(nv, 0.0);
    DBuf<PanelGeom> dall;

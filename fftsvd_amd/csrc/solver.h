@@ -29,6 +29,9 @@ struct SurfaceSpec {
    int maxpanels = 32, gridpoints = 4;
    double svdeps = 1e-5;
    bool want32 = true;
+   // accelerateM3: reaction potentials through a panels -> charges tree per
+   // region with charges (FFTSVDpbeAPI.c:1503-1513) instead of direct sums
+   bool m3 = false;
 };
 
 // Qualocation operator (QualocationOperator.c): one Laplace tree over all
@@ -58,6 +61,11 @@ struct Region {
    DBuf<double> xs, xd, yt;
    bool has_charges = false;
    int shares_tree_with = -1;        // reuse another region's operator (same panels and kernel)
+   // M3 tree (region panels -> charges) and the region join in its panel order
+   std::unique_ptr<TreeOp> m3;
+   DBuf<PanelGeom> m3_panels;
+   DBuf<int> m3_phi_idx, m3_dphi_idx;
+   DBuf<double> m3_s_phi, m3_s_dphi, m3_xs, m3_xd, m3_y;
    int lane = 0;                     // matvec stream: 0 = main, k > 0 = aux[k-1]
 };
 
@@ -132,7 +140,10 @@ SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, d
 // Mixed precision: FP32-operator GMRES with FP64 residual refinement.
 SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit,
                         std::vector<double>* hist);
-void problem_potentials(Problem& pr, const double* sol_host, double* phi_host);
+// Reaction potentials at the charges: direct sums, or through the M3 trees
+// where built (use_m3).  The batched potentials and the charge gradient use
+// the direct sums.
+void problem_potentials(Problem& pr, const double* sol_host, double* phi_host, bool use_m3 = true);
 // Many charge vectors / solutions at once (column-major; q and rhs on the
 // device, sol and phi on the host).
 void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs);

@@ -60,7 +60,9 @@ typedef struct _Charge {
 typedef _Charge* Charge;
 
 /* reference SurfaceOperator.h:5-23; `tree` points at the library's device
- * operator for this region, M3 is always NULL. */
+ * operator for this region; M3 is non-NULL where an M3 tree was built
+ * (accelerateM3 set before generateSurfaceOperator; the potentials of
+ * collectPotentials then go through it, SurfaceOperator.c:347-363). */
 typedef struct _SurfaceOperator {
    BEMKernelType kernel;
    real epsilon;
@@ -236,8 +238,8 @@ void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollecten
  * only (reference FFTSVDpbeAPI.h:133-158; FFTSVDpbeAPI.c:1741-1869,1878-1882,
  * 1890-1894,1924-1950).  One tree over the dielectric and cavity panels with
  * double-layer-only bases; one unknown per panel.  error() when ionicstrength
- * is not 0.  accelerateM3 is not honoured: potentials and the right-hand side
- * are always the direct sums (the reference's default). */
+ * is not 0.  accelerateM3 (the M1M3 tree) is not honoured here: potentials
+ * and the right-hand side are the direct sums (the reference's default). */
 void generateQualocationOperator(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
                                  unsigned int numPDBentries, Panel** dielectricpanels,
                                  unsigned int* numdielectricpanels, unsigned int numdielectrics,

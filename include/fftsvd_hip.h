@@ -136,6 +136,11 @@ int fftsvd_tree_cube_info(fftsvd_problem* p, int which, int cube, int* info);
 /* caller (tree-local) panel indices of the cube in tree order, and of the
  * leaf's near-field columns */
 int fftsvd_tree_cube_panels(fftsvd_problem* p, int which, int cube, int* pan, int* dcols);
+/* Caller indices of a cube's points (its nq entries).  For every
+ * fftsvd_tree_* call, which >= 0 selects the tree of region `which` and
+ * which < 0 the M3 tree (panels -> charges, built when accelerateM3 is set
+ * at creation) of region -which-1. */
+int fftsvd_tree_cube_points(fftsvd_problem* p, int which, int cube, int* pt);
 /* FP64 blocks of one cube: VT (r x np), U (np x r'), PVs/PVd (gp3 x r),
  * UTI (r' x gp3), Ds/Dd (p x m); rows/columns in tree order. */
 int fftsvd_tree_cube_mats(fftsvd_problem* p, int which, int cube, double* VT, double* U, double* PVs, double* PVd,

@@ -64,6 +64,9 @@ def _load():
         "orc_tree_apply_both": (None, [C.c_void_p, _dp, _dp, _dp]),
         "orc_tree_apply_transpose": (None, [C.c_void_p, C.c_int, _dp, _dp]),
         "orc_tree_dense": (None, [C.c_void_p, _dp, _dp]),
+        "orc_m3_build": (None, [C.c_void_p]),
+        "orc_m3_potentials": (None, [C.c_void_p, _dp, _dp]),
+        "orc_problem_m3_tree": (C.c_void_p, [C.c_void_p, C.c_int]),
         "orc_qual_create": (C.c_void_p, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
                                          C.c_int, C.c_double]),
         "orc_qual_free": (None, [C.c_void_p]),
@@ -335,6 +338,18 @@ class Problem:
         """Delta G = 1/2 sum q phi * conversion / eps_in (FFTSVDsalt.c:166-172)."""
         phi = self.reaction_potentials(sol)
         return 0.5 * float(np.dot(self.q, phi)) * conversion / self.eps_in
+
+    def m3_potentials(self, sol):
+        """Reaction potentials through the panels -> charges M3 tree
+        (accelerateM3), built on first use."""
+        lib().orc_m3_build(self.h)
+        phi = np.zeros(len(self.q))
+        lib().orc_m3_potentials(self.h, _d(f64(sol)), _d(phi))
+        return phi
+
+    def m3_tree(self, which):
+        lib().orc_m3_build(self.h)
+        return Tree(lib().orc_problem_m3_tree(self.h, which), self)
 
 
 class Qual:

@@ -129,6 +129,7 @@ typedef struct orc_region {
    double* q;
    double* resint;
    double* resext;
+   orc_tree* m3;              /* panels -> charges tree (accelerateM3), or NULL */
 } orc_region;
 
 typedef struct {
@@ -173,6 +174,12 @@ void orc_tree_set_cube_mats(orc_tree* t, int c, int rowrank, int colrank, const 
                             const double* VT, const double* U, const double* PVs, const double* PVd,
                             const double* UTI);
 orc_tree* orc_problem_tree(orc_problem* pr, int which /* depth along the first-child chain */);
+/* M3 trees (accelerateM3, FFTSVDpbeAPI.c:1503-1513): per region with charges,
+ * a Laplace tree from the region's panels to its charges; the reaction
+ * potentials through it (SurfaceOperator.c:347-363). */
+void orc_m3_build(orc_problem* pr);
+void orc_m3_potentials(orc_problem* pr, const double* sol, double* phi);
+orc_tree* orc_problem_m3_tree(orc_problem* pr, int which);
 void orc_tree_dense(orc_tree* t, double* S, double* D);   /* npt x npan each, direct integrals */
 
 /* Unit-level helpers */

@@ -135,6 +135,7 @@ static void region_free(orc_region* r) {
    for (int i = 0; i < r->nchild; i++) region_free(r->child[i]);
    free(r->child);
    orc_tree_free(r->tree);
+   orc_tree_free(r->m3);
    free(r->qpos); free(r->q); free(r->resint); free(r->resext);
    free(r);
 }
@@ -413,4 +414,63 @@ void orc_reaction_potentials(orc_problem* pr, const double* sol, double* phi) {
    int nq = pr->root->child[0]->ncharge;
    memset(phi, 0, sizeof(double) * nq);
    region_react(pr->root, sol, phi);
+}
+
+/* M3 trees: Tree_allocate(region panels, charges, MAX_PANELS_PER_FINEST_CUBE,
+ * POISSON_KERNEL, GRID_SIZE, SVD_ERROR, SINGLE_AND_DOUBLE_LAYER_INT, 0)
+ * for every region with charges (FFTSVDpbeAPI.c:1503-1513; the Stern
+ * branch, :1256-1266). */
+static void region_m3(orc_region* r) {
+   if (r->ncharge > 0 && !r->m3) {
+      orc_tree* t = r->tree;
+      r->m3 = orc_tree_build(t->panels, t->npan, r->qpos, r->ncharge, ORC_LAPLACE, 0.0, t->maxpanels, t->gridpoints,
+                             t->svdeps);
+      orc_tree_fill(r->m3);
+   }
+   for (int c = 0; c < r->nchild; c++) region_m3(r->child[c]);
+}
+
+void orc_m3_build(orc_problem* pr) { region_m3(pr->root); }
+
+/* SurfaceOperator_calculateReactionPotentials, M3 branch (SurfaceOperator.c:347-363):
+ * joinpotentials, Tree_multiplyboth(react, M3, dphi, phi), phi_j = -react_j. */
+static void region_m3_react(orc_region* r, const double* src, double* out) {
+   if (r->m3) {
+      int n = r->tree->npan;
+      double* phi = (double*)calloc(n > 0 ? n : 1, sizeof(double));
+      double* dphi = (double*)calloc(n > 0 ? n : 1, sizeof(double));
+      double* react = (double*)calloc(r->ncharge, sizeof(double));
+      int cur = 0;
+      for (int i = 0; i < r->mynpan; i++) {
+         phi[cur + i] = src[r->startphi + i];
+         dphi[cur + i] = -1.0 * src[r->startdphi + i];
+      }
+      cur += r->mynpan;
+      for (int c = 0; c < r->nchild; c++) {
+         orc_region* ch = r->child[c];
+         for (int i = 0; i < ch->mynpan; i++) {
+            phi[cur + i] = -1.0 * src[ch->startphi + i];
+            dphi[cur + i] = (ch->eps / r->eps) * src[ch->startdphi + i];
+         }
+         cur += ch->mynpan;
+      }
+      orc_tree_apply_both(r->m3, dphi, phi, react);
+      for (int j = 0; j < r->ncharge; j++) out[j] = -react[j];
+      free(phi); free(dphi); free(react);
+   }
+   for (int c = 0; c < r->nchild; c++) region_m3_react(r->child[c], src, out);
+}
+
+void orc_m3_potentials(orc_problem* pr, const double* sol, double* phi) {
+   int nq = 0;
+   for (orc_region* r = pr->root; r; r = r->nchild ? r->child[0] : NULL)
+      if (r->ncharge > nq) nq = r->ncharge;
+   memset(phi, 0, sizeof(double) * nq);
+   region_m3_react(pr->root, sol, phi);
+}
+
+orc_tree* orc_problem_m3_tree(orc_problem* pr, int which) {
+   orc_region* r = pr->root;
+   for (int k = 0; k < which && r->nchild > 0; k++) r = r->child[0];
+   return r->m3;
 }
