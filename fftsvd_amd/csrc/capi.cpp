@@ -231,6 +231,7 @@ int fftsvd_qual_create(const double* verts9, int npan, const double* qpos, const
    sp.gridpoints = gridpoints;
    sp.svdeps = svdeps;
    sp.want32 = want_f32 != 0;
+   sp.m3 = accelerateM3 != 0;
    auto* h = new fftsvd_problem;
    try {
       h->pr = build_qual_problem(sp);
@@ -1075,6 +1076,7 @@ void generateQualocationOperatorMinimal(QualocationOperator* qualocationoperator
    sp.gridpoints = GRID_SIZE;
    sp.svdeps = SVD_ERROR;
    sp.want32 = true;
+   sp.m3 = accelerateM3 != 0;   // generateQualocationOperatorA1A3 (FFTSVDpbeAPI.c:1836-1838)
    auto* h = new fftsvd_problem;
    try {
       h->pr = build_qual_problem(sp);
@@ -1084,6 +1086,7 @@ void generateQualocationOperatorMinimal(QualocationOperator* qualocationoperator
    }
    QualocationOperator o = (QualocationOperator)calloc(1, sizeof(_QualocationOperator));
    o->tree = h;
+   o->M1M3 = h->pr->regions[0].m3 ? (void*)h : nullptr;
    o->innerdielectric = innerdielectric;
    o->outerdielectric = outerdielectric;
    Charge ch = (Charge)calloc(1, sizeof(_Charge));

@@ -564,6 +564,16 @@ std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec) {
    pr.qual_pdiag.upload(pdiag, s);
    R.xs.alloc(n + 1);
    R.yt.alloc(n + 1);
+   // M1M3 (generateQualocationOperatorA1A3, FFTSVDpbeAPI.c:1841-1851):
+   // Laplace, both layers, panels -> charges
+   if (spec.m3 && !spec.q.empty()) {
+      const int nq = (int)spec.q.size();
+      R.m3 = build_tree_op(pr.panels, R.d_panels.p, spec.qpos.data(), nq, LAPLACE, 0.0, spec.maxpanels,
+                           spec.gridpoints, spec.svdeps, false, s);
+      R.m3_xs.alloc(std::max(n, nq) + 1);
+      R.m3_xd.alloc(std::max(n, nq) + 1);
+      R.m3_y.alloc(std::max(n, nq) + 1);
+   }
    pr.d_qpos.upload(pr.qpos, s);
    pr.d_q.upload(pr.q, s);
    pr.w.alloc(n);
@@ -671,6 +681,23 @@ void problem_precond(Problem& pr, const double* b, double* x) {
 
 void problem_rhs(Problem& pr, double* rhs) {
    hipStream_t s = pr.stream;
+   if (pr.qual && pr.regions[0].m3) {
+      // QualocationOperator_makeRHS with M1M3 (QualocationOperator.c:33-41):
+      // RHS = A_d^T q * (-1 / (4 pi eps_in)) through the tree
+      Region& R = pr.regions[0];
+      TreeOp& m = *R.m3;
+      const int nq = (int)pr.q.size(), n = pr.ntotal;
+      std::vector<double> qt(nq), out(n), rh(n);
+      for (int k = 0; k < nq; k++) qt[k] = pr.q[m.tree.pt_perm[k]];
+      FSVD_CHECK(hipMemcpyAsync(R.m3_xs.p, qt.data(), sizeof(double) * nq, hipMemcpyHostToDevice, s));
+      tree_apply_transpose(m, PREC_F64, 1, R.m3_xs.p, R.m3_y.p, s);
+      FSVD_CHECK(hipMemcpyAsync(out.data(), R.m3_y.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      for (int k = 0; k < n; k++) rh[m.tree.pan_perm[k]] = out[k] * (-1.0 / (4.0 * M_PI * pr.eps_in));
+      FSVD_CHECK(hipMemcpyAsync(rhs, rh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      return;
+   }
    if (pr.qual) {
       launch_qual_rhs(pr.ntotal, pr.regions[0].d_panels.p, (int)pr.q.size(), pr.d_qpos.p, pr.d_q.p,
                       4.0 * M_PI * pr.eps_in, rhs, s);
@@ -958,6 +985,21 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool us
    DBuf<double> dphi;
    dphi.alloc(std::max(nq, 1));
    FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max(nq, 1), s));
+   if (pr.qual && pr.regions[0].m3 && use_m3) {
+      // with M1M3: Tree_multiply(phi, M1M3, sol, SINGLE_LAYER_INT) (QualocationOperator.c:60-61)
+      Region& R = pr.regions[0];
+      TreeOp& m = *R.m3;
+      const int n = pr.ntotal;
+      std::vector<double> xt(n), zero(n, 0.0), yt(std::max(nq, 1));
+      for (int k = 0; k < n; k++) xt[k] = sol[m.tree.pan_perm[k]];
+      FSVD_CHECK(hipMemcpyAsync(R.m3_xs.p, xt.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+      FSVD_CHECK(hipMemcpyAsync(R.m3_xd.p, zero.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+      tree_apply(m, PREC_F64, R.m3_xs.p, R.m3_xd.p, R.m3_y.p, s);
+      FSVD_CHECK(hipMemcpyAsync(yt.data(), R.m3_y.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      for (int k = 0; k < nq; k++) phi_out[m.tree.pt_perm[k]] = yt[k];
+      return;
+   }
    if (pr.qual) {
       // QualocationOperator_calculateReactionPotentials (QualocationOperator.c:55-70):
       // phi[c] = sum_i sol[i] slp(c, panel i)
@@ -975,7 +1017,7 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool us
       launch_potentials(nq, pr.ntotal, pr.regions[0].d_panels.p, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
    }
    for (Region& R : pr.regions) {
-      if (!R.m3 || !use_m3) continue;
+      if (pr.qual || !R.m3 || !use_m3) continue;
       // SurfaceOperator_calculateReactionPotentials, M3 branch (SurfaceOperator.c:347-363):
       // phi_j = -(S dphi + D phi)_j through the M3 tree
       const int n = (int)R.panel_ids.size();

@@ -44,6 +44,7 @@ struct QualSpec {
    int maxpanels = 32, gridpoints = 4;
    double svdeps = 1e-5;
    bool want32 = true;
+   bool m3 = false;             // accelerateM3: the M1M3 panels -> charges tree
 };
 
 struct Region {

@@ -82,7 +82,7 @@ typedef struct _SurfaceOperator {
 typedef _SurfaceOperator* SurfaceOperator;
 
 /* reference QualocationOperator.h:3-10; `tree` points at the library's
- * device operator, M1M3 is always NULL. */
+ * device operator; M1M3 is non-NULL when accelerateM3 built that tree. */
 typedef struct _QualocationOperator {
    void* tree;
    void* M1M3;
@@ -238,8 +238,9 @@ void collectPotentials(PDBentry* PDBcollectentries, unsigned int numPDBcollecten
  * only (reference FFTSVDpbeAPI.h:133-158; FFTSVDpbeAPI.c:1741-1869,1878-1882,
  * 1890-1894,1924-1950).  One tree over the dielectric and cavity panels with
  * double-layer-only bases; one unknown per panel.  error() when ionicstrength
- * is not 0.  accelerateM3 (the M1M3 tree) is not honoured here: potentials
- * and the right-hand side are the direct sums (the reference's default). */
+ * is not 0.  With accelerateM3 the M1M3 panels -> charges tree carries the
+ * right-hand side and the potentials (QualocationOperator.c:33-70);
+ * otherwise they are direct sums (the reference's default). */
 void generateQualocationOperator(QualocationOperator* qualocationoperator, PDBentry* PDBentries,
                                  unsigned int numPDBentries, Panel** dielectricpanels,
                                  unsigned int* numdielectricpanels, unsigned int numdielectrics,

@@ -76,6 +76,8 @@ def _load():
         "orc_qual_gmres": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp]),
         "orc_qual_potentials": (None, [C.c_void_p, _dp, _dp]),
         "orc_qual_tree": (C.c_void_p, [C.c_void_p]),
+        "orc_qual_m1m3_build": (None, [C.c_void_p]),
+        "orc_qual_m1m3_tree": (C.c_void_p, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -374,6 +376,11 @@ class Qual:
 
     def tree(self, which=0):
         return Tree(lib().orc_qual_tree(self.h), self)
+
+    def use_m1m3(self):
+        """Build the M1M3 tree; rhs() and potentials() then go through it."""
+        lib().orc_qual_m1m3_build(self.h)
+        return Tree(lib().orc_qual_m1m3_tree(self.h), self)
 
     def matvec(self, x):
         y = np.zeros(self.n)

@@ -212,6 +212,7 @@ typedef struct {
    double* qpos;
    double* q;
    double idiel, odiel;
+   orc_tree* m1m3;            /* panels -> charges tree (accelerateM3), or NULL */
 } orc_qual;
 
 orc_qual* orc_qual_create(const double* verts9, int npan, const double* qpos, const double* q, int nq,
@@ -225,6 +226,11 @@ void orc_qual_rhs(orc_qual* o, double* rhs);
 int orc_qual_gmres(orc_qual* o, const double* rhs, double* sol, double tol, int maxit, int early_exit, double* hist);
 void orc_qual_potentials(orc_qual* o, const double* sol, double* phi);
 orc_tree* orc_qual_tree(orc_qual* o);
+/* The M1M3 tree (generateQualocationOperatorA1A3, FFTSVDpbeAPI.c:1841-1851):
+ * once built, orc_qual_rhs and orc_qual_potentials go through it
+ * (QualocationOperator.c:33-70). */
+void orc_qual_m1m3_build(orc_qual* o);
+orc_tree* orc_qual_m1m3_tree(orc_qual* o);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ orc_qual* orc_qual_create(const double* verts9, int npan, const double* qpos, co
 void orc_qual_free(orc_qual* o) {
    if (!o) return;
    orc_tree_free(o->tree);
+   orc_tree_free(o->m1m3);
    free(o->panels);
    free(o->cent);
    free(o->diag);
@@ -80,8 +81,15 @@ void orc_qual_precond_diag(orc_qual* o, double* p) {
              o->panels[i].area;
 }
 
-/* Charge_makerhs_ecf_qual (Charge.c:129-142). */
+/* QualocationOperator_makeRHS (QualocationOperator.c:33-53): through the
+ * M1M3 tree, RHS = -A_d^T q / (4 pi eps_in); otherwise
+ * Charge_makerhs_ecf_qual (Charge.c:129-142). */
 void orc_qual_rhs(orc_qual* o, double* rhs) {
+   if (o->m1m3) {
+      orc_tree_apply_transpose(o->m1m3, 1, o->q, rhs);
+      for (int i = 0; i < o->npan; i++) rhs[i] *= -1.0 / (4.0 * M_PI * o->idiel);
+      return;
+   }
    for (int i = 0; i < o->npan; i++) {
       rhs[i] = 0.0;
       for (int j = 0; j < o->nq; j++) {
@@ -184,6 +192,13 @@ int orc_qual_gmres(orc_qual* o, const double* rhs, double* sol, double tol, int 
  * QualocationOperator.c:55-70): phi[c] = sum_i sol[i] * slp(c, panel i). */
 void orc_qual_potentials(orc_qual* o, const double* sol, double* phi) {
    memset(phi, 0, sizeof(double) * o->nq);
+   if (o->m1m3) {
+      /* Tree_multiply(phi, M1M3, sol, SINGLE_LAYER_INT) */
+      double* z = (double*)calloc(o->npan > 0 ? o->npan : 1, sizeof(double));
+      orc_tree_apply_both(o->m1m3, sol, z, phi);
+      free(z);
+      return;
+   }
    for (int i = 0; i < o->npan; i++)
       for (int c = 0; c < o->nq; c++) {
          double sl, dl;
@@ -193,3 +208,12 @@ void orc_qual_potentials(orc_qual* o, const double* sol, double* phi) {
 }
 
 orc_tree* orc_qual_tree(orc_qual* o) { return o->tree; }
+
+void orc_qual_m1m3_build(orc_qual* o) {
+   if (o->m1m3 || o->nq == 0) return;
+   o->m1m3 = orc_tree_build(o->panels, o->npan, o->qpos, o->nq, ORC_LAPLACE, 0.0, o->tree->maxpanels,
+                            o->tree->gridpoints, o->tree->svdeps);
+   orc_tree_fill(o->m1m3);
+}
+
+orc_tree* orc_qual_m1m3_tree(orc_qual* o) { return o->m1m3; }

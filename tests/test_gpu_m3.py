@@ -63,9 +63,10 @@ def test_m3_tree_matches_oracle(m3_pair):
         assert d["rowrank"] == c["rowrank"] and d["colrank"] == c["colrank"]
 
 
-def test_m3_potentials_identical_operator(m3_pair):
-    P, qpos, q, g, o = m3_pair
-    gt, ot = g.tree(-2), o.m3_tree(1)
+def _import_m3(g, which, ot):
+    """Load the GPU-built blocks of an M3 tree into the oracle's (panel and
+    point orders mapped per cube)."""
+    gt = g.tree(which)
     gpl = ot.gpl()
     gc = {_key(c): c for c in gt.cubes()}
     for c in ot.cubes():
@@ -73,7 +74,7 @@ def test_m3_potentials_identical_operator(m3_pair):
         gm = gt.cube_mats(d, int(gpl[c["level"]]) ** 3)
         ppos = {int(p): k for k, p in enumerate(d["pan"])}
         pperm = np.array([ppos[int(p)] for p in c["pan"]], dtype=int)
-        gpts = _points(g, -2, d["id"], len(c["pt"]))
+        gpts = _points(g, which, d["id"], len(c["pt"]))
         qpos_ = {int(p): k for k, p in enumerate(gpts)}
         qperm = np.array([qpos_[int(p)] for p in c["pt"]], dtype=int)
         leafD = c["leaf"] and len(c["pt"])
@@ -81,6 +82,11 @@ def test_m3_potentials_identical_operator(m3_pair):
                          gm["Dd"] if leafD else None, gm["VT"][:, pperm] if d["rowrank"] else None,
                          gm["U"][qperm, :] if d["colrank"] else None, gm["PVs"] if d["rowrank"] else None,
                          gm["PVd"] if d["rowrank"] else None, gm["UTI"] if d["colrank"] else None)
+
+
+def test_m3_potentials_identical_operator(m3_pair):
+    P, qpos, q, g, o = m3_pair
+    _import_m3(g, -2, o.m3_tree(1))
     sol = np.random.default_rng(6).normal(size=2 * len(P))
     po, pg = o.m3_potentials(sol), g.reaction_potentials(sol)
     assert np.abs(pg - po).max() <= 1e-10 * np.abs(po).max()
@@ -131,3 +137,30 @@ def test_m3_reference_sequence(tmp_path):
                                atol=1e-9 * np.abs(pot).max())
     refapi.free(rhs)
     refapi.free(sol)
+
+
+def test_qual_m1m3():
+    """Qualocation with accelerateM3: the M1M3 tree carries the right-hand
+    side (transposed double layer) and the potentials (single layer)
+    (QualocationOperator.c:33-70); identical operator against the oracle,
+    and against the direct sums to the tree's accuracy (the reference
+    itself calls the trees inaccurate here, SurfaceOperator.c:364)."""
+    V, F, qpos, q = blob()
+    P = meshes.panels_from_mesh(V, F)
+    refapi.set_global("accelerateM3", 1, C.c_uint)
+    try:
+        g = fa.Qual(P, qpos, q, 4.0, 80.0)
+    finally:
+        refapi.set_global("accelerateM3", 0, C.c_uint)
+    o = oracle.Qual(P, qpos, q, 4.0, 80.0)
+    ot = o.use_m1m3()
+    _import_m3(g, -1, ot)
+    ro, rg = o.rhs(), g.rhs()
+    assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
+    sol = np.random.default_rng(3).normal(size=len(P))
+    po, pg = o.potentials(sol), g.potentials(sol)
+    assert np.abs(pg - po).max() <= 1e-10 * np.abs(po).max()
+    direct = fa.Qual(P, qpos, q, 4.0, 80.0)
+    rd, pd = direct.rhs(), direct.potentials(sol)
+    assert np.linalg.norm(rg - rd) <= 3e-2 * np.linalg.norm(rd)
+    assert np.linalg.norm(pg - pd) <= 2e-3 * np.linalg.norm(pd)
