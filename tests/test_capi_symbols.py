@@ -36,3 +36,28 @@ def test_library_exports_every_declared_function():
     lib = ctypes.CDLL(fa.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_headers_declare_extended_paths():
+    names = declared_functions()
+    for required in ("generateQualocationOperator", "solveQualocationOperator", "collectQualocationPotentials",
+                     "solveBEMproblem", "fftsvd_qual_create", "fftsvd_problem_solve_batch",
+                     "fftsvd_problem_solve_adjoint", "fftsvd_problem_grad_charges", "fftsvd_tree_cube_points"):
+        assert required in names
+
+
+def test_argument_errors_before_any_device_call():
+    """Invalid sizes are rejected with a message before the library touches
+    a device (so this runs on a CPU-only host)."""
+    if not os.path.exists(fa.LIB_PATH):
+        pytest.skip("library not built")
+    import numpy as np
+    v = np.zeros((4, 9))
+    q = np.zeros((1, 3))
+    c = np.zeros(1)
+    with pytest.raises(fa.FFTSVDError, match="grid points"):
+        fa.Problem(v, q, c, gridpoints=5)
+    with pytest.raises(fa.FFTSVDError, match="grid points"):
+        fa.Qual(v, q, c, gridpoints=2)
+    with pytest.raises(fa.FFTSVDError, match="sizes"):
+        fa.Qual(np.zeros((0, 9)), q, c)
