@@ -998,10 +998,6 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
       lv.itoff.upload(lv.h_itoff, s);
       lv.isrcT.upload(lv.h_isrcT, s);
       lv.itoffT.upload(lv.h_itoffT, s);
-      lv.spec.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
-      lv.acc.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
-      lv.specT.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
-      lv.accT.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
    }
    timing_mark("interaction lists", tm);
    if (want32) {
@@ -1114,6 +1110,11 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
 template <typename T>
 static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipStream_t s) {
    const bool f32 = std::is_same<T, float>::value;
+   for (FarLevel& lv : op.levels)
+      if (!lv.specT.p) {
+         lv.specT.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);
+         lv.accT.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
+      }
    launch_zero(y, op.tree.npan, s);
    for (FarLevel& lv : op.levels)
       if (lv.ntgt)

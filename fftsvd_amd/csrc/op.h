@@ -65,8 +65,7 @@ struct FarLevel {
    DBuf<int> isrc, itoff, isrcT, itoffT;
    DBuf<double> T64;     // 343 * pad complex
    DBuf<float> T32;
-   DBuf<double> spec, acc;       // nsrc*pad, ntgt*pad complex (sized for FP64)
-   DBuf<double> specT, accT;     // ntgt*pad, nsrc*pad complex
+   DBuf<double> specT, accT;     // ntgt*pad, nsrc*pad complex: transposed apply only, allocated on first use
    // compressed blocks of this level (offsets in SrcDesc / TgtDesc)
    DBuf<double> VT64, PV64, UT64, UTI64;
    DBuf<float> VT32, PV32, UT32, UTI32;
