@@ -135,7 +135,8 @@ constexpr int TR_MAX_SRC = 1023;
 // level, GTgt entries at tgt0, their union of nsrc source spectra
 // (absolute offsets at src_off).  Each target's interactions are packed as
 // byte offsets of LDS rows, (T code * TR_ROW) | (local source * TR_ROW) << 16,
-// in interaction-list order, from a 16-byte aligned int_off; every list is
+// in a fixed order chosen per half-wave of four targets (their rows of one
+// step spread over the LDS bank quarters), from a 16-byte aligned int_off; every list is
 // followed by at least four readable words (the next list or padding).
 struct TrTile {
    int nt, tgt0, nsrc, src_off, pad, pad_;

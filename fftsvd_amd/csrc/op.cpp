@@ -144,6 +144,89 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
    return map;
 }
 
+// LDS bank quarter (64 banks x 4 bytes = 256 bytes per bank row) of a
+// 64-byte translation row at byte offset off.
+static inline int tr_quarter(unsigned off) { return (int)((off / TR_ROW) & 3u); }
+
+// Modelled LDS cycles of one step of up to four lists read together: the
+// largest number of T rows, plus the largest number of source rows, that
+// fall in one bank quarter.
+static int tr_step_cost(const std::vector<unsigned>* L, int nq, size_t e) {
+   int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0}, any = 0;
+   for (int j = 0; j < nq; j++)
+      if (e < L[j].size()) {
+         uT[tr_quarter(L[j][e] & 0xffffu)]++;
+         uS[tr_quarter(L[j][e] >> 16)]++;
+         any = 1;
+      }
+   if (!any) return 0;
+   return *std::max_element(uT, uT + 4) + *std::max_element(uS, uS + 4);
+}
+
+// Orders the interaction lists of the (up to) four targets whose lanes form
+// one half-wave (4 targets x TR_ROW / 8 lanes) in k_translate.  Each 64-byte
+// row read by a half-wave occupies one quarter of the LDS banks, so at every
+// step the four targets' T rows, and separately their source rows, are made
+// to fall in distinct quarters where the lists allow: greedily, the target
+// with the most words left picks first, each picking the word class (T
+// quarter, source quarter) that collides least with the picks made so far,
+// ties going to the class it holds most of.  Only the order of each list
+// changes (its sum is still taken in one fixed order); with cost set, the
+// modelled cycles before and after are added to cost[0], cost[1].
+static void schedule_half_wave(std::vector<unsigned>* L, int nq, std::vector<unsigned>& buf, long long* cost) {
+   size_t longest = 0, total = 0;
+   for (int j = 0; j < nq; j++) {
+      longest = std::max(longest, L[j].size());
+      total += L[j].size();
+   }
+   if (cost)
+      for (size_t e = 0; e < longest; e++) cost[0] += tr_step_cost(L, nq, e);
+   // words of each list grouped by class (T quarter * 4 + source quarter),
+   // in list order within a class: head[j][c] .. end[j][c] in buf
+   buf.resize(total);
+   int head[4][16], end[4][16], rem[4] = {0, 0, 0, 0};
+   int base = 0;
+   for (int j = 0; j < nq; j++) {
+      int cnt[16] = {};
+      for (unsigned w : L[j]) cnt[tr_quarter(w & 0xffffu) * 4 + tr_quarter(w >> 16)]++;
+      for (int c = 0; c < 16; c++) {
+         head[j][c] = end[j][c] = base;
+         base += cnt[c];
+      }
+      for (unsigned w : L[j]) buf[end[j][tr_quarter(w & 0xffffu) * 4 + tr_quarter(w >> 16)]++] = w;
+      rem[j] = (int)L[j].size();
+      L[j].clear();
+   }
+   for (;;) {
+      int order[4] = {0, 1, 2, 3}, live = 0;
+      for (int j = 0; j < nq; j++) live += rem[j] > 0;
+      if (!live) break;
+      std::stable_sort(order, order + nq, [&](int a, int b) { return rem[a] > rem[b]; });
+      int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0};
+      for (int oi = 0; oi < nq; oi++) {
+         const int j = order[oi];
+         if (!rem[j]) continue;
+         int best = -1, best_cost = 1 << 30, best_n = 0;
+         for (int c = 0; c < 16; c++) {
+            const int n = end[j][c] - head[j][c];
+            if (!n) continue;
+            const int cst = uT[c >> 2] + uS[c & 3];
+            if (cst < best_cost || (cst == best_cost && n > best_n)) {
+               best = c;
+               best_cost = cst;
+               best_n = n;
+            }
+         }
+         L[j].push_back(buf[head[j][best]++]);
+         uT[best >> 2]++;
+         uS[best & 3]++;
+         rem[j]--;
+      }
+   }
+   if (cost)
+      for (size_t e = 0; e < longest; e++) cost[1] += tr_step_cost(L, nq, e);
+}
+
 // Concatenates every level's far-field blocks into single buffers with
 // absolute offsets (row bases re-laid panel-major for the projection),
 // splits source cubes into projection chunks, and attaches to every leaf
@@ -377,6 +460,37 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       }
       // words the translation kernel may read ahead past the last list
       for (int k = 0; k < 8; k++) gint.push_back(0);
+      // each half-wave's lists reordered over the LDS bank quarters; the
+      // half-waves are independent, so threads take them in turn
+      {
+         std::vector<int2> quads;   // (first GTgt, targets)
+         for (const TrTile& tl : tiles)
+            for (int q0 = 0; q0 < tl.nt; q0 += 4) quads.push_back(make_int2(tl.tgt0 + q0, std::min(4, tl.nt - q0)));
+         const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+         std::vector<std::array<long long, 2>> cost(nth, std::array<long long, 2>{0, 0});
+         const bool model = timing_on();
+         std::vector<std::thread> pool;
+         for (int t = 0; t < nth; t++)
+            pool.emplace_back([&, t] {
+               std::vector<unsigned> lists[4], buf;
+               for (size_t qi = t; qi < quads.size(); qi += nth) {
+                  const int g0 = quads[qi].x, nq = quads[qi].y;
+                  for (int j = 0; j < nq; j++) {
+                     const GTgt& g = gtgt[g0 + j];
+                     lists[j].assign(gint.begin() + g.int_off, gint.begin() + g.int_off + g.nint);
+                  }
+                  schedule_half_wave(lists, nq, buf, model ? cost[t].data() : nullptr);
+                  for (int j = 0; j < nq; j++) std::copy(lists[j].begin(), lists[j].end(), gint.begin() + gtgt[g0 + j].int_off);
+               }
+            });
+         for (auto& th : pool) th.join();
+         if (model) {
+            long long c0 = 0, c1 = 0;
+            for (auto& c : cost) c0 += c[0], c1 += c[1];
+            std::fprintf(stderr, "[fftsvd setup] translation LDS steps (modelled cycles): list order %lld, scheduled %lld\n",
+                         c0, c1);
+         }
+      }
       P.ntask_f = (int)tf.size();
       P.ntask_d = (int)td.size();
       P.task_f.upload(tf, s);
