@@ -233,7 +233,7 @@ class Tree:
         UTI = np.zeros(max(gp3 * cube["colrank"], 1))
         lib().orc_tree_cube_mats(self.h, cube["id"], _d(Ds), _d(Dd), _d(VT), _d(U), _d(PVs), _d(PVd), _d(UTI))
         return dict(Ds=Ds[:drows * dcols].reshape(drows, dcols), Dd=Dd[:drows * dcols].reshape(drows, dcols),
-                    VT=VT[:cube["rowrank"] * len(cube["pan"])].reshape(cube["rowrank"], -1),
+                    VT=VT[:cube["rowrank"] * len(cube["pan"])].reshape(cube["rowrank"], len(cube["pan"])),
                     U=U[:cube["colrank"] * len(cube["pt"])].reshape(len(cube["pt"]), cube["colrank"]),
                     PVs=PVs[:gp3 * cube["rowrank"]].reshape(gp3, cube["rowrank"]),
                     PVd=PVd[:gp3 * cube["rowrank"]].reshape(gp3, cube["rowrank"]),
