@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["geometry.cpp", "tree.cpp", "host_linalg.cpp", "op.cpp", "solver.cpp", "capi.cpp", "mesh.cpp", "frontend.cpp",
-           "fill.hip", "apply.hip", "solve.hip"]
+           "fill.hip", "apply.hip", "solve.hip", "noalloc.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-x", "hip",
           # no packed FP32 instructions (v_pk_*_f32) anywhere in device code

@@ -44,6 +44,8 @@ static int g_precision = 0;
 
 namespace fsvd {
 void launch_copy(long long n4, const void* a, void* b, hipStream_t s);
+// the message fftsvd_last_error() returns, for entry points in other files
+void set_last_error(const char* msg) { g_err = msg; }
 }
 
 extern "C" {

@@ -155,6 +155,21 @@ int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int
 int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
                      double* slp, double* dlp);
 
+/* Allocation-free variant of fftsvd_integrate.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  verts9 (npan x 9), pts (npt x 3), slp and dlp (npt x npan,
+ * row-major) are DEVICE arrays of the caller; the call enqueues two kernels
+ * on `stream` (a hipStream_t, NULL for the default stream) and returns
+ * without waiting.  The per-panel geometry records live in a fixed device
+ * array of FFTSVD_NOALLOC_MAX_PANELS records (10.75 MB per GPU, held until
+ * the process ends): larger npan is rejected with an error.  Calls on one
+ * GPU share that array, so calls that overlap in time on one GPU race;
+ * npan * npt must stay below 2^31.  The results equal fftsvd_integrate's. */
+#define FFTSVD_NOALLOC_MAX_PANELS 16384
+int fftsvd_integrate_noalloc(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
+                             double* slp, double* dlp, void* stream);
+
 /* Solvent-excluded surface of atoms (x, y, z, r per atom) with a probe of
  * radius `probe`, contoured on a grid of spacing h; the library's own
  * replacement for the MSMS run behind the reference's generateFlatSRF
