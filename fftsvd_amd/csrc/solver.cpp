@@ -79,6 +79,10 @@ int partial_blocks(long long n) {
 }
 }  // namespace
 
+template <typename VT>
+static VT* basis_vec(Problem& pr, int j);
+static void reserve_solver_memory(Problem& pr, bool want32);
+
 Problem::~Problem() {
    regions.clear();
    basis.clear();
@@ -494,6 +498,7 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    pr.partial.alloc(2 * 512);
    pr.hbuf.alloc(1024 + 8);
    FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned, sizeof(double) * 2048));
+   reserve_solver_memory(pr, spec.want32);
    FSVD_CHECK(hipStreamSynchronize(s));
    pr.setup_ms = now_ms() - t0;
    setup_event("problem end");
@@ -582,6 +587,7 @@ std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec) {
    pr.partial.alloc(2 * 512);
    pr.hbuf.alloc(1024 + 8);
    FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned, sizeof(double) * 2048));
+   reserve_solver_memory(pr, spec.want32);
    FSVD_CHECK(hipStreamSynchronize(s));
    pr.setup_ms = now_ms() - t0;
    return prp;
@@ -767,21 +773,32 @@ void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs) {
    }
 }
 
+// Krylov basis vector j (slab j / BASIS_SLAB, allocated on first use).
 template <typename VT>
 static VT* basis_vec(Problem& pr, int j) {
+   constexpr int S = Problem::BASIS_SLAB;
+   const size_t off = (size_t)(j % S) * (size_t)pr.nvec;
    if constexpr (std::is_same<VT, double>::value) {
-      while ((int)pr.basis.size() <= j) {
+      while ((int)pr.basis.size() <= j / S) {
          pr.basis.emplace_back(new DBuf<double>());
-         pr.basis.back()->alloc(pr.nvec);
+         pr.basis.back()->alloc((size_t)S * pr.nvec);
       }
-      return pr.basis[j]->p;
+      return pr.basis[j / S]->p + off;
    } else {
-      while ((int)pr.basis32.size() <= j) {
+      while ((int)pr.basis32.size() <= j / S) {
          pr.basis32.emplace_back(new DBuf<float>());
-         pr.basis32.back()->alloc(pr.nvec);
+         pr.basis32.back()->alloc((size_t)S * pr.nvec);
       }
-      return pr.basis32[j]->p;
+      return pr.basis32[j / S]->p + off;
    }
+}
+
+// The first basis slab of each precision the problem can solve in, and the
+// device copy of the panels, made while the problem is built.
+static void reserve_solver_memory(Problem& pr, bool want32) {
+   basis_vec<double>(pr, 0);
+   if (want32) basis_vec<float>(pr, 0);
+   if (!pr.qual) pr.d_allpanels.upload(pr.panels, pr.stream);
 }
 
 // Device array of the basis pointers v_0..v_{m-1} (grown in steps of 64).
@@ -955,13 +972,16 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
    DBuf<double> res, e;
    res.alloc(n);
    e.alloc(n);
-   for (int it = 0; it < 4; it++) {
+   // up to MAX_REFINE corrections; the residual is evaluated once more
+   // after the last, so final_residual is always that of the returned sol
+   constexpr int MAX_REFINE = 4;
+   for (int it = 0;; it++) {
       problem_matvec(pr, PREC_F64, sol, pr.tmp.p);
       tot.matvecs64++;
       launch_axpby(n, 1.0, rhs, -1.0, pr.tmp.p, res.p, s);
       double rn = precond_norm(pr, res.p);
       tot.final_residual = bn > 0 ? rn / bn : rn;
-      if (!(tot.final_residual >= tol) || !(rn > 0.0)) break;
+      if (!(tot.final_residual >= tol) || !(rn > 0.0) || it == MAX_REFINE) break;
       // correction solve on the normalised residual (GMRES's zero-solution
       // test is absolute, GMRES.c:461-465), scaled back afterwards
       launch_axpby(n, 1.0 / rn, res.p, 0.0, res.p, res.p, s);
@@ -980,8 +1000,7 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
 void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool use_m3) {
    hipStream_t s = pr.stream;
    const int nq = (int)pr.q.size();
-   DBuf<PanelGeom> dall;
-   if (!pr.qual) dall.upload(pr.panels, s);
+   const PanelGeom* dall = pr.d_allpanels.p;
    DBuf<double> dphi;
    dphi.alloc(std::max(nq, 1));
    FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max(nq, 1), s));
@@ -1057,7 +1076,7 @@ void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool us
       dp.upload(pidx, s);
       dsf.upload(self, s);
       dc.upload(coef, s);
-      launch_potentials(nq, (int)pidx.size(), dall.p, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
+      launch_potentials(nq, (int)pidx.size(), dall, dp.p, dc.p, dsf.p, pr.d_qpos.p, dphi.p, s);
       FSVD_CHECK(hipStreamSynchronize(s));
    }
    if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
@@ -1070,8 +1089,7 @@ void problem_potentials_batch(Problem& pr, int ncol, const double* sol, double* 
    hipStream_t s = pr.stream;
    const int nq = (int)pr.q.size();
    const long long nv = pr.nvec;
-   DBuf<PanelGeom> dall;
-   if (!pr.qual) dall.upload(pr.panels, s);
+   const PanelGeom* dall = pr.d_allpanels.p;
    DBuf<double> dphi;
    dphi.alloc(std::max<long long>((long long)nq * ncol, 1));
    FSVD_CHECK(hipMemsetAsync(dphi.p, 0, sizeof(double) * std::max<long long>((long long)nq * ncol, 1), s));
@@ -1124,7 +1142,7 @@ void problem_potentials_batch(Problem& pr, int ncol, const double* sol, double* 
                }
             }
          }
-         run(pidx, self, coef, dall.p);
+         run(pidx, self, coef, dall);
       }
    }
    if (nq) FSVD_CHECK(hipMemcpyAsync(phi_out, dphi.p, sizeof(double) * nq * ncol, hipMemcpyDeviceToHost, s));
@@ -1153,8 +1171,7 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
    problem_potentials(pr, sol_host, phi.data(), false);
    // g = C^T q
    std::vector<double> g(nv, 0.0);
-   DBuf<PanelGeom> dall;
-   dall.upload(pr.panels, s);
+   const PanelGeom* dall = pr.d_allpanels.p;
    for (Region& R : pr.regions) {
       if (!R.has_charges) continue;
       std::vector<int> pidx, self;
@@ -1174,7 +1191,7 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
       dsf.upload(self, s);
       dsl.alloc(std::max(ne, 1));
       ddl.alloc(std::max(ne, 1));
-      launch_collect_t(ne, dall.p, dp.p, dsf.p, nq, pr.d_qpos.p, pr.d_q.p, dsl.p, ddl.p, s);
+      launch_collect_t(ne, dall, dp.p, dsf.p, nq, pr.d_qpos.p, pr.d_q.p, dsl.p, ddl.p, s);
       std::vector<double> ssl(ne), sdl(ne);
       FSVD_CHECK(hipMemcpyAsync(ssl.data(), dsl.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
       FSVD_CHECK(hipMemcpyAsync(sdl.data(), ddl.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));

@@ -101,6 +101,7 @@ struct Problem {
    // adjoint solves of problem_solve_adjoint)
    bool adjoint = false;
    std::vector<PanelGeom> panels;        // global panel geometry
+   DBuf<PanelGeom> d_allpanels;          // its device copy (direct potentials, charge gradient)
    std::vector<Region> regions;          // regions[0] = exterior (root)
    std::vector<double> qpos, q;
    double eps_in = 4.0;
@@ -113,7 +114,11 @@ struct Problem {
    // scratch
    DBuf<double> w, r, tmp, partial, hbuf, htmp, partial_m;
    // Krylov bases: FP64 for FP64 solves, FP32 for the FP32-operator solves
-   // of the mixed mode; device copies of their pointers
+   // of the mixed mode, in slabs of BASIS_SLAB vectors (the first slab of
+   // each is reserved when the problem is built, so a solve of up to
+   // BASIS_SLAB iterations allocates nothing); device copies of their
+   // pointers
+   static constexpr int BASIS_SLAB = 128;
    std::vector<std::unique_ptr<DBuf<double>>> basis;
    std::vector<std::unique_ptr<DBuf<float>>> basis32;
    DBuf<const double*> d_basis_ptr;

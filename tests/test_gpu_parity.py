@@ -211,3 +211,19 @@ def test_mixed_energy_blob(salt_pair):
     E64, E32 = g.energy(s64), g.energy(s32)
     assert abs(E32 - E64) <= 1e-6 * abs(E64)
     assert info["final_residual"] < 1e-6
+
+
+def test_mixed_refinement_runs_and_reports_final_residual(salt_pair):
+    """A tolerance below what the FP32 operator reaches on its own forces
+    FP64 residual refinement; the reported residual is that of the returned
+    solution (recomputed after the last correction)."""
+    P, g, o = salt_pair
+    rhs = g.rhs()
+    s32, it32, _, info = g.solve(rhs, 1e-10, precision=fa.MIXED)
+    s64, _, _, _ = g.solve(rhs, 1e-10, precision=fa.F64)
+    assert info["refinements"] >= 1
+    assert info["final_residual"] < 1e-10
+    r = rhs - g.matvec(s32, fa.F64)
+    bn = np.linalg.norm(g.precond_solve(rhs))
+    assert abs(np.linalg.norm(g.precond_solve(r)) / bn - info["final_residual"]) <= 1e-3 * info["final_residual"] + 1e-14
+    assert np.linalg.norm(s32 - s64) <= 1e-8 * np.linalg.norm(s64)
