@@ -66,7 +66,7 @@ def cpu_baseline(seed, budget_s=25.0):
 
 
 STAGE_KERNELS = {"leaf": "k_leaf<float>", "project_source": "k_project<float> + k_source<float>",
-                 "translate": "k_translate<float, 8, 8>", "target": "k_target<float>"}
+                 "translate": "k_translate_pipe", "target": "k_target<float>"}
 STAGE_LAUNCHES = {"leaf": 1, "project_source": 2, "translate": 1, "target": 1}   # per tree
 # LDS of MI355X: 256 CUs x 128 bytes per clock x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4

@@ -480,28 +480,15 @@ __global__ void __launch_bounds__(CUBE_NT) k_source(const SrcPlan* __restrict__ 
    source_body<4, T>(d, tpart, PV, spec, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
 }
 
-// Translation by tiles: a block owns BC frequency bins (one TR_ROW-byte row
-// per spectrum) of a run of tiles (each up to TR_TILE neighbouring target
-// cubes of one level).  It stages the 343 T rows of those bins in LDS once,
-// then per tile the rows of every source spectrum the tile's targets
-// interact with.  Four lanes share a target, each owning 16 bytes of the
-// row (two FP32 bins or one FP64 bin), and sum its interaction list from
-// LDS in interaction-list order; the packed interaction word holds the two
-// row byte offsets, so an address is one AND/OR or shift/OR away.
-// Lane share of a TR_ROW-byte row: LPT lanes per target, each reading
-// TR_ROW / LPT bytes (NB bins).
+// FP64 translation by tiles: a block owns BC frequency bins (one TR_ROW-byte
+// row per spectrum) of one tile (up to TR_TILE neighbouring target cubes of
+// one level).  It stages the 343 T rows of those bins and the rows of every
+// source spectrum the tile's targets interact with in LDS; LPT lanes share a
+// target, each owning TR_ROW / LPT bytes of the row, and sum its interaction
+// list from LDS in list order; the packed interaction word holds the two
+// row byte offsets, so an address is one AND or shift away.
 template <typename T, int LPT>
 struct TrVec;
-template <>
-struct TrVec<float, 8> {
-   using type = float2;   // bin l
-   static constexpr int NB = 1;
-};
-template <>
-struct TrVec<float, 4> {
-   using type = float4;   // bins 2l and 2l+1
-   static constexpr int NB = 2;
-};
 template <>
 struct TrVec<double, 4> {
    using type = double2;  // bin l
@@ -539,7 +526,7 @@ __global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restr
    extern __shared__ __align__(16) unsigned char smem[];
    const int4 tk = tasks[blockIdx.x];   // first tile, tile count, first bin
    const int b0 = tk.z, tid = threadIdx.x, nthr = TR_TILE * LPT;
-   const int pad = tiles[tk.x].pad;
+   const int pad = tiles[tk.x].pad, pst = tiles[tk.x].pst;
    const long long t_off = tiles[tk.x].t_off;
    C2* Ts = (C2*)smem;        // 343 rows
    C2* Ss = Ts + 343 * BC;    // source rows
@@ -552,7 +539,7 @@ __global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restr
          const int i = i0 + u * nthr, c = i / BC, b = i - c * BC;
          v[u].x = 0;
          v[u].y = 0;
-         if (i < 343 * BC && b0 + b < pad) v[u] = Tspec[t_off + (long long)c * pad + b0 + b];
+         if (i < 343 * BC && b0 + b < pad) v[u] = Tspec[t_off + (long long)c * pst + b0 + b];
       }
 #pragma unroll
       for (int u = 0; u < U; u++)
@@ -627,6 +614,304 @@ __global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restr
          }
       }
    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined FP32 translation.  See TpTile / TpOct in device.h for the layout.
+// Producer waves stage task tk (tile tl, nsrc >= 1) into the LDS buffer
+// buf: the tile's source rows, and the T rows of the task's level and chunk
+// when stage_t.  The rows are 64 bytes (8 bins), moved in 16-byte pieces:
+// producer lane pl takes pieces pl, pl + 64 TP_PW, ...  Loads are issued
+// in batches before their stores (addresses clamped to valid pieces,
+// stores only for real ones); the source row offsets are loaded first,
+// the T rows moved last.
+constexpr int TP_PL = 64 * TP_PW;                            // producer lanes
+constexpr int TP_SPIECES = (TP_MAX_SRC * 4 + TP_PL - 1) / TP_PL;   // source pieces per lane
+constexpr int TP_TPIECES = (343 * 4 + TP_PL - 1) / TP_PL;          // T pieces per lane
+constexpr int TP_BATCH = 10;
+__device__ __forceinline__ void tp_stage(const int4& tk, const TpTile& tl, bool stage_t, unsigned char* buf, int pl,
+                                         const int* __restrict__ gsrc, const float2* __restrict__ Tspec,
+                                         const float2* __restrict__ spec) {
+   const int b0 = tk.y;
+   const int npiece = tl.nsrc * 4;
+   // byte offsets of the pieces in the spectrum array (below 4 GiB, checked
+   // when the plan is built), so each load is one 32-bit register offset
+   unsigned so[TP_SPIECES];
+#pragma unroll
+   for (int u = 0; u < TP_SPIECES; u++) {
+      const int p = pl + TP_PL * u;
+      so[u] = 8u * (unsigned)(gsrc[tl.src_off + min(p >> 2, tl.nsrc - 1)] + b0 + 2 * (p & 3));
+   }
+   unsigned char* sbuf = buf + TP_TROWS * 64;
+#pragma unroll
+   for (int u0 = 0; u0 < TP_SPIECES; u0 += TP_BATCH) {
+      float4 v[TP_BATCH];
+#pragma unroll
+      for (int u = 0; u < TP_BATCH; u++)
+         if (u0 + u < TP_SPIECES) v[u] = *(const float4*)((const char*)spec + so[u0 + u]);
+#pragma unroll
+      for (int u = 0; u < TP_BATCH; u++) {
+         const int p = pl + TP_PL * (u0 + u);
+         if (u0 + u < TP_SPIECES && p < npiece) *(float4*)(sbuf + 16 * p) = v[u];
+      }
+      __builtin_amdgcn_sched_barrier(0);   // one batch of registers at a time
+   }
+   if (stage_t) {
+      float4 v[TP_TPIECES];
+#pragma unroll
+      for (int u = 0; u < TP_TPIECES; u++) {
+         const int p = min(pl + TP_PL * u, 343 * 4 - 1);
+         v[u] = *(const float4*)(Tspec + tl.t_off + (long long)(p >> 2) * tl.pst + b0 + 2 * (p & 3));
+      }
+#pragma unroll
+      for (int u = 0; u < TP_TPIECES; u++)
+         if (pl + TP_PL * u < 343 * 4) *(float4*)(buf + 16 * (pl + TP_PL * u)) = v[u];
+   }
+}
+
+// The same staging by direct global->LDS transfers (16 bytes per lane, lane
+// i of a transfer writes base + 16 i): no registers hold the rows.  The
+// transfers are issued unconditionally, in straight-line code, so the
+// compiler can keep them all in flight: source transfers cover all
+// TP_MAX_SRC rows (rows past the tile's sources repeat its last one and are
+// never read), T transfers 352 rows (rows past 342 repeat row 342), which
+// leaves the zero row TP_ZERO_ROW untouched.
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+__device__ __forceinline__ void tp_dma16(const void* g, unsigned char* base) {
+   __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)base, 16, 0, 0);
+}
+__device__ __forceinline__ void tp_stage_dma(const int4& tk, const TpTile& tl, bool stage_t, unsigned char* buf,
+                                             int pw, int lane, const int* __restrict__ gsrc,
+                                             const float2* __restrict__ Tspec, const float2* __restrict__ spec) {
+   constexpr int NS = TP_MAX_SRC * 4 / 64, NT = 352 * 4 / 64;   // transfers per buffer
+   constexpr int MS = (NS + TP_PW - 1) / TP_PW, MT = (NT + TP_PW - 1) / TP_PW;
+   const int b0 = tk.y, q2 = 2 * (lane & 3);
+   int so[MS];
+#pragma unroll
+   for (int u = 0; u < MS; u++) {
+      const int i = min(pw + u * TP_PW, NS - 1);
+      so[u] = gsrc[tl.src_off + min(i * 16 + (lane >> 2), tl.nsrc - 1)];
+   }
+   unsigned char* sbuf = buf + TP_TROWS * 64;
+#pragma unroll
+   for (int u = 0; u < MS; u++) {
+      const int i = min(pw + u * TP_PW, NS - 1);   // a repeated transfer rewrites the same rows
+      tp_dma16(spec + so[u] + b0 + q2, sbuf + i * 1024);
+   }
+   if (stage_t) {
+#pragma unroll
+      for (int u = 0; u < MT; u++) {
+         const int i = min(pw + u * TP_PW, NT - 1);
+         const int row = min(i * 16 + (lane >> 2), 342);
+         tp_dma16(Tspec + tl.t_off + (long long)row * tl.pst + b0 + q2, buf + i * 1024);
+      }
+   }
+   __builtin_amdgcn_s_waitcnt(0);
+}
+
+
+// Word group i (steps 4i..4i+3) of target t of octet o; reads past the
+// octet's last group are clamped to it.
+__device__ __forceinline__ int4 tp_group(const int* __restrict__ gint, const TpOct& o, int i, int t) {
+   const int last = max(o.nk / 4 - 1, 0);
+   return *((const int4*)(gint + o.int_off) + 8 * min(i, last) + t);
+}
+
+// Two word groups (eight steps) of a lane: all sixteen LDS reads are issued
+// before the first sum (a scheduling barrier keeps the compiler from
+// interleaving them), then the sums, steps alternating over four
+// accumulator pairs.
+__device__ __forceinline__ void tp_groups2(const int4& x, const int4& y, const unsigned char* Tl,
+                                           const unsigned char* Sl, float* re, float* im) {
+   const unsigned u[8] = {(unsigned)x.x, (unsigned)x.y, (unsigned)x.z, (unsigned)x.w,
+                          (unsigned)y.x, (unsigned)y.y, (unsigned)y.z, (unsigned)y.w};
+   float2 t[8], v[8];
+#pragma unroll
+   for (int q = 0; q < 8; q++) {
+      t[q] = *(const float2*)(Tl + (u[q] & 0xffffu));
+      v[q] = *(const float2*)(Sl + (u[q] >> 16));
+   }
+   __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+   for (int q = 0; q < 8; q++) {
+      float& r = re[q & 3];
+      float& m = im[q & 3];
+      r = fmaf(t[q].x, v[q].x, r);
+      r = fmaf(-t[q].y, v[q].y, r);
+      m = fmaf(t[q].x, v[q].y, m);
+      m = fmaf(t[q].y, v[q].x, m);
+   }
+}
+
+// Word-group pair p (steps 8p..8p+7) of target t of octet o (clamped to the
+// last pair).
+struct TpPair {
+   int4 x, y;
+};
+__device__ __forceinline__ TpPair tp_pair(const int* __restrict__ gint, const TpOct& o, int p, int t) {
+   return {tp_group(gint, o, 2 * p, t), tp_group(gint, o, 2 * p + 1, t)};
+}
+
+// The lane's steps of octet o (nk a multiple of 8, uniform over the wave),
+// one word-group pair (eight steps, sixteen LDS reads) at a time; the
+// words load two pairs ahead of their use, in three rotating register sets
+// (no copies, so no wait on a load in flight).  A: pair 0, loaded; B: pair
+// 1, issued by the caller.
+__device__ __forceinline__ void tp_sum(const TpOct& o, int t, TpPair A, TpPair B, const int* __restrict__ gint,
+                                       const unsigned char* Tl, const unsigned char* Sl, float* re, float* im) {
+   const int np = o.nk / 8;
+   for (int p = 0; p < np; p += 3) {
+      const TpPair C = tp_pair(gint, o, p + 2, t);
+      tp_groups2(A.x, A.y, Tl, Sl, re, im);
+      if (p + 1 >= np) break;
+      A = tp_pair(gint, o, p + 3, t);
+      tp_groups2(B.x, B.y, Tl, Sl, re, im);
+      if (p + 2 >= np) break;
+      B = tp_pair(gint, o, p + 4, t);
+      tp_groups2(C.x, C.y, Tl, Sl, re, im);
+   }
+}
+
+// Producer waves: stage task 0, then task k+1 while the consumers sum task
+// k; one barrier per task, as many as the consumers pass.
+template <int MODE>
+__device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
+                                            const int* __restrict__ gsrc, const float2* __restrict__ Tspec,
+                                            const float2* __restrict__ spec, unsigned char* smem, int* ctr, int pw,
+                                            int lane) {
+   long long key0, key1 = -1;   // the (T offset, chunk) each buffer holds
+   {
+      const int4 t0 = tasks[0];
+      const TpTile tl = tiles[t0.x];
+      key0 = tl.t_off * 4096 + t0.y;
+      if constexpr (MODE == 1)
+         tp_stage_dma(t0, tl, true, smem, pw, lane, gsrc, Tspec, spec);
+      else
+         tp_stage(t0, tl, true, smem, 64 * pw + lane, gsrc, Tspec, spec);
+   }
+   __syncthreads();
+   for (int k = 0; k < n; k++) {
+      // task k+2's counter was task k-1's, done at the last barrier; it is
+      // claimed from no earlier than the barrier that ends this iteration
+      if (pw == 0 && lane == 0) ctr[(k + 2) % 3] = 0;
+      if (k + 1 < n) {
+         const int4 tn = tasks[k + 1];
+         const TpTile tl = tiles[tn.x];
+         const long long key = tl.t_off * 4096 + tn.y;
+         const int nb = (k + 1) & 1;
+         const bool st = (nb ? key1 : key0) != key;
+         if (nb)
+            key1 = key;
+         else
+            key0 = key;
+         if constexpr (MODE == 1)
+            tp_stage_dma(tn, tl, st, smem + nb * TP_BUF, pw, lane, gsrc, Tspec, spec);
+         else
+            tp_stage(tn, tl, st, smem + nb * TP_BUF, 64 * pw + lane, gsrc, Tspec, spec);
+      }
+      __syncthreads();
+   }
+}
+
+// One claim on task k's octet counter (three counters in rotation, in LDS
+// after the two buffers): the next unclaimed octet index of the task.  One
+// lane of the wave adds, the wave reads its result.
+__device__ __forceinline__ int tp_claim(int* ctr, int k, int lane) {
+   int v = 0;
+   if (lane == 0) v = atomicAdd(ctr + k % 3, 1);
+   return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Consumer waves claim the octets of task k one at a time, longest first
+// (the plan sorts them), so the waves of a block finish a task together.
+// While an octet is summed, the next is claimed and its first words load:
+// the next octet of task k, or, when task k has none left, one of task k+1
+// (its counter was cleared before the previous barrier).  One barrier per
+// task.
+__device__ __forceinline__ void tp_consumer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
+                                            const TpOct* __restrict__ octs, const long long* __restrict__ oacc,
+                                            const int* __restrict__ gint, float2* __restrict__ acc,
+                                            const unsigned char* smem, int* ctr, int lane) {
+   const int t = lane >> 3, b = lane & 7;
+   int kc = -1, ic = 0;   // the claimed octet (task kc, index ic), none when kc < k
+   TpOct o = {0, 0, 0, 0};
+   TpPair A = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+   long long ao = -1;
+   __syncthreads();
+   for (int k = 0; k < n; k++) {
+      const int4 tk = tasks[k];
+      const int pad = tiles[tk.x].pad;
+      const unsigned char* Tl = smem + (k & 1) * TP_BUF + b * 8;
+      const unsigned char* Sl = Tl + TP_TROWS * 64;
+      if (kc < k) {
+         ic = tp_claim(ctr, k, lane);
+         if (ic < tk.w) {
+            kc = k;
+            o = octs[tk.z + ic];
+            A = tp_pair(gint, o, 0, t);
+            ao = oacc[o.acc_idx + t];
+         }
+      }
+      while (kc == k) {
+         int kn = k, jn = tp_claim(ctr, k, lane);
+         if (jn >= tk.w) {
+            kn = -1;
+            if (k + 1 < n) {
+               jn = tp_claim(ctr, k + 1, lane);
+               if (jn < tasks[k + 1].w) kn = k + 1;
+            }
+         }
+         // pair 1 of this octet, then the next octet's pair 0 (issued in
+         // this order, so waiting for the former never waits for the latter)
+         const TpPair B = tp_pair(gint, o, 1, t);
+         TpOct on = {0, 0, 0, 0};
+         TpPair An = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+         long long aon = -1;
+         if (kn >= 0) {
+            on = octs[tasks[kn].z + jn];
+            An = tp_pair(gint, on, 0, t);
+            aon = oacc[on.acc_idx + t];
+         }
+         {
+            float re[4] = {0.f, 0.f, 0.f, 0.f}, im[4] = {0.f, 0.f, 0.f, 0.f};
+            tp_sum(o, t, A, B, gint, Tl, Sl, re, im);
+            const float rs = (re[0] + re[1]) + (re[2] + re[3]), is = (im[0] + im[1]) + (im[2] + im[3]);
+            if (ao >= 0 && tk.y + b < pad) acc[ao + tk.y + b] = make_float2(rs, is);
+         }
+         kc = kn;
+         ic = jn;
+         o = on;
+         A = An;
+         ao = aon;
+      }
+      __syncthreads();
+   }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict__ ranges,
+                                                          const int4* __restrict__ tasks_all,
+                                                          const TpTile* __restrict__ tiles,
+                                                          const TpOct* __restrict__ octs,
+                                                          const long long* __restrict__ oacc,
+                                                          const int* __restrict__ gsrc, const int* __restrict__ gint,
+                                                          const float2* __restrict__ Tspec,
+                                                          const float2* __restrict__ spec, float2* __restrict__ acc) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int2 rg = ranges[blockIdx.x];
+   const int tid = threadIdx.x, lane = tid & 63;
+   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   // the zero T row (padding words) of both buffers, which no transfer
+   // writes, and the three octet counters
+   int* ctr = (int*)(smem + 2 * TP_BUF);
+   if (tid < 16) ((float2*)(smem + (tid >> 3) * TP_BUF + TP_ZERO_ROW * 64))[tid & 7] = make_float2(0.f, 0.f);
+   if (tid < 3) ctr[tid] = 0;
+   if (rg.y == 0) return;   // uniform over the block: no barrier is reached
+   // both roles pass rg.y + 1 barriers
+   if (wave >= TP_CW)
+      tp_producer<MODE>(tasks_all + rg.x, rg.y, tiles, gsrc, Tspec, spec, smem, ctr, wave - TP_CW, lane);
+   else
+      tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, acc, smem, ctr, lane);
 }
 
 template <int n, typename T, typename G>
@@ -974,24 +1259,43 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          break;
       }
       case 2: {
-         // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
-         constexpr bool f32 = std::is_same<T, float>::value;
-         constexpr int BC = f32 ? TR_BINS_F32 : TR_BINS_F64;
-         constexpr int LPT = f32 ? TR_LPT_F32 : TR_LPT_F64;
-         const size_t lds_max = sizeof(C2) * (size_t)(343 + TR_MAX_SRC) * BC;
-         const size_t lds = sizeof(C2) * (size_t)(343 + a.tr_max_src) * BC;
-         check_lds(lds_max);
-         if (a.tr_max_src > TR_MAX_SRC) throw std::runtime_error("translation tile above its LDS bound");
-         static std::once_flag attr;
-         std::call_once(attr, [lds_max] {
-            FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC, LPT>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-         });
-         const int nt = f32 ? a.ntask_f : a.ntask_d;
-         if (nt)
-            hipLaunchKernelGGL((k_translate<T, BC, LPT>), dim3(nt), dim3(TR_TILE * LPT), lds, s,
-                               f32 ? a.task_f : a.task_d,
-                               a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+         if constexpr (std::is_same<T, float>::value) {
+            // pipelined: two LDS buffers per persistent block, three counters
+            constexpr size_t lds = 2 * (size_t)TP_BUF + 16;
+            check_lds(lds);
+            static std::once_flag attr;
+            std::call_once(attr, [] {
+               FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<0>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+               FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<1>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            });
+            if (a.nblk_p) {
+               if (a.mode_p == 1)
+                  hipLaunchKernelGGL(k_translate_pipe<1>, dim3(a.nblk_p), dim3(TP_NT), lds, s, a.range_p, a.task_p,
+                                     a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p, a.gint_p, (const float2*)a.Tsp,
+                                     (const float2*)a.spec, (float2*)a.acc);
+               else
+                  hipLaunchKernelGGL(k_translate_pipe<0>, dim3(a.nblk_p), dim3(TP_NT), lds, s, a.range_p, a.task_p,
+                                     a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p, a.gint_p, (const float2*)a.Tsp,
+                                     (const float2*)a.spec, (float2*)a.acc);
+            }
+         } else {
+            // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
+            constexpr int BC = TR_BINS_F64, LPT = TR_LPT_F64;
+            const size_t lds_max = sizeof(C2) * (size_t)(343 + TR_MAX_SRC) * BC;
+            const size_t lds = sizeof(C2) * (size_t)(343 + a.tr_max_src) * BC;
+            check_lds(lds_max);
+            if (a.tr_max_src > TR_MAX_SRC) throw std::runtime_error("translation tile above its LDS bound");
+            static std::once_flag attr;
+            std::call_once(attr, [lds_max] {
+               FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate<T, BC, LPT>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+            });
+            if (a.ntask_d)
+               hipLaunchKernelGGL((k_translate<T, BC, LPT>), dim3(a.ntask_d), dim3(TR_TILE * LPT), lds, s, a.task_d,
+                                  a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+         }
          break;
       }
       case 3: {

@@ -111,40 +111,81 @@ struct TgtPlan {
    long long t_off, acc_off, uti_off;
 };
 
-// Translation tiles.  A block handles TR_BINS_F32 (FP32) or TR_BINS_F64
-// (FP64) frequency bins, one 64-byte row per spectrum, of up to TR_TILE
-// tree-order neighbouring targets; tiles grow while their source union
-// stays within TR_SRC_BUDGET, so the block's LDS ((343 + sources) rows)
-// stays under 72 KiB and two blocks (32 FP32 waves) share a CU.  Larger
-// tiles reuse each staged source row for more targets: on the 100k-panel
-// globule (both trees), 32-target tiles ran the FP32 translation in
-// 0.30 ms, 64 in 0.23 ms and 128 in 0.20 ms; one tile per block instead of
-// three (more blocks, shorter tail) then 0.16 ms.  TR_MAX_SRC bounds a
-// single target's sources (local source row offsets must fit 16 bits).
-constexpr int TR_BINS_F32 = 8, TR_BINS_F64 = 4;
-// lanes per target: FP32 one bin (8 bytes) per lane, FP64 one bin (16 bytes)
-constexpr int TR_LPT_F32 = 8, TR_LPT_F64 = 4;
+// FP64 translation tiles (k_translate<double>).  A block handles TR_BINS_F64
+// frequency bins, one 64-byte row per spectrum, of up to TR_TILE tree-order
+// neighbouring targets, four lanes per target; tiles grow while their
+// source union stays within TR_SRC_BUDGET, so the block's LDS ((343 +
+// sources) rows) stays under 85 KiB.  TR_MAX_SRC bounds a tile's sources
+// (local source row offsets must fit 16 bits).
+constexpr int TR_BINS_F64 = 4;
+constexpr int TR_LPT_F64 = 4;
 constexpr int TR_ROW = 64;
 constexpr int TR_TILE = 128;
 constexpr int TR_SRC_BUDGET = 800;
-// Tiles per translation block (they share the block's staged T spectra).
-constexpr int TR_RUN = 1;
 constexpr int TR_MAX_SRC = 1023;
 
 // Translation tile: nt <= TR_TILE consecutive (tree-order) target cubes of one
 // level, GTgt entries at tgt0, their union of nsrc source spectra
-// (absolute offsets at src_off).  Each target's interactions are packed as
-// byte offsets of LDS rows, (T code * TR_ROW) | (local source * TR_ROW) << 16,
-// in a fixed order chosen per half-wave of four targets (their rows of one
-// step spread over the LDS bank quarters), from a 16-byte aligned int_off; every list is
+// (absolute offsets at src_off), the level's pad and padded stride pst, its
+// T spectra at t_off.  Each target's interactions are packed as byte
+// offsets of LDS rows, (T code * TR_ROW) | (local source * TR_ROW) << 16, in
+// interaction-list order from a 16-byte aligned int_off; every list is
 // followed by at least four readable words (the next list or padding).
 struct TrTile {
-   int nt, tgt0, nsrc, src_off, pad, pad_;
+   int nt, tgt0, nsrc, src_off, pad, pst;
    long long t_off;
 };
 struct GTgt {
    long long acc_off;
    int int_off, nint;
+};
+
+// Spectrum rows in HBM start every TR_PST_ALIGN bins: a level's spectra
+// (forward, accumulated, T) are stored at stride pst = pad rounded up to
+// that multiple, so every 8-bin (64-byte FP32) row is 64-byte aligned.
+constexpr int TR_PST_ALIGN = 8;
+__host__ __device__ constexpr int tr_pst(int pad) { return (pad + TR_PST_ALIGN - 1) / TR_PST_ALIGN * TR_PST_ALIGN; }
+
+// Pipelined FP32 translation (k_translate_pipe).  Persistent blocks (one
+// per CU) of TP_CW consumer waves and TP_PW producer waves work through
+// their list of tasks (tile x 8-bin chunk).  While the consumers sum task k
+// from one LDS buffer, the producers stage task k+1 into the other (the T
+// rows only when the level or chunk changes); the consumers claim the
+// task's octets one at a time from an LDS counter.  A buffer holds TP_TROWS T
+// rows (the 343 offsets, rows the staging may fill with copies, and a zero
+// row for padding words) and up to TP_MAX_SRC source rows, 64 bytes each.
+// A consumer wave sums eight targets of similar list length at once (an
+// octet): lane (t, b) = (lane >> 3, lane & 7) takes target t of the octet
+// and bin b of the chunk, so every lane owns one output bin.
+constexpr int TP_CW = 13, TP_PW = 3, TP_NT = 64 * (TP_CW + TP_PW);
+constexpr int TP_TROWS = 356;        // 352 staged rows (343 used), the zero row 352, 3 spare
+constexpr int TP_ZERO_ROW = 352;
+constexpr int TP_MAX_SRC = 912;
+constexpr int TP_BUF = (TP_TROWS + TP_MAX_SRC) * 64;
+constexpr int TP_SRC_BUDGET = 880;   // tile growth stops above this union size
+constexpr int TP_TILE = 512;         // targets per tile at most
+// a tile of more than TP_QUANTUM targets is cut to a multiple of it (whole
+// octets)
+constexpr int TP_QUANTUM = 8;
+constexpr unsigned TP_NULL_WORD = TP_ZERO_ROW * 64u;   // zero T row x source row 0
+static_assert(2 * TP_BUF <= 163840, "two pipeline buffers must fit the LDS of a CU");
+static_assert(TP_MAX_SRC % 16 == 0 && (TP_MAX_SRC - 1) * 64 < 65536, "source rows and 16-bit offsets");
+static_assert(TP_TROWS % 4 == 0, "source rows start on an LDS bank-quarter boundary");
+
+// Tile of the pipelined translation: its level's T spectra at t_off (code c
+// at t_off + c * pst), the union of its nsrc source spectra (absolute
+// offsets at src_off in gsrc), the level's pad and stride.
+struct TpTile {
+   long long t_off;
+   int nsrc, src_off, pad, pst;
+};
+// An octet of the pipelined translation: nk steps (the longest list,
+// rounded up to 8); its words from int_off (16-byte aligned) in groups of
+// four steps, group i holding target t's words of steps 4i..4i+3 at
+// int_off + 4 * (8 i + t) (null words past the end of a list); target t's
+// accumulated spectrum at oacc[acc_idx + t] (-1: no target).
+struct TpOct {
+   int int_off, nk, acc_idx, pad_;
 };
 
 // Far-field output block of a leaf: rows of one target cube's U^T
@@ -170,11 +211,20 @@ struct ApplyArgs {
    const ProjChunk* chunks;
    const SrcPlan* src;
    const TgtPlan* tgt;
-   int ntask_f, ntask_d, tr_max_src;  // translation tasks (first tile, tiles, first bin), FP32 / FP64
-   const int4 *task_f, *task_d;
+   int ntask_d, tr_max_src;          // FP64 translation tasks (tile, 1, first bin)
+   const int4* task_d;
    const TrTile* tiles;
    const GTgt* gtgt;
    const int *gsrc, *gint;
+   // pipelined FP32 translation: nblk_p persistent blocks, block b takes the
+   // tasks (tile, first bin, first TpOct, octets) range_p[b]
+   int nblk_p, mode_p;   // mode: 0 staging through registers, 1 direct global->LDS transfers
+   const int2* range_p;
+   const int4* task_p;
+   const TpTile* tiles_p;
+   const TpOct* oct_p;
+   const long long* oacc_p;
+   const int *gsrc_p, *gint_p;
    int nmap_leaf;                    // XCD block map of the leaf launch
    const int* map_leaf;
    const LeafDesc* leaves;

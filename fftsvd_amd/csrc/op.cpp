@@ -13,8 +13,10 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <cstdlib>
 #include <functional>
 #include <string>
+#include <unordered_map>
 #include <iterator>
 #include <stdexcept>
 
@@ -144,87 +146,357 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
    return map;
 }
 
-// LDS bank quarter (64 banks x 4 bytes = 256 bytes per bank row) of a
-// 64-byte translation row at byte offset off.
-static inline int tr_quarter(unsigned off) { return (int)((off / TR_ROW) & 3u); }
+// Per-level data the translation plans need.
+struct TrLevel {
+   int t0, nt;         // the level's targets: tgt[t0 .. t0 + nt)
+   int pad, pst, nsrc;
+   long long t_off;    // its T spectra (complex index; code c at t_off + c * pst)
+   long long sbase;    // its forward spectra: source slot k at sbase + k * pst
+};
 
-// Modelled LDS cycles of one step of up to four lists read together: the
-// largest number of T rows, plus the largest number of source rows, that
-// fall in one bank quarter.
-static int tr_step_cost(const std::vector<unsigned>* L, int nq, size_t e) {
-   int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0}, any = 0;
-   for (int j = 0; j < nq; j++)
-      if (e < L[j].size()) {
-         uT[tr_quarter(L[j][e] & 0xffffu)]++;
-         uS[tr_quarter(L[j][e] >> 16)]++;
-         any = 1;
-      }
-   if (!any) return 0;
-   return *std::max_element(uT, uT + 4) + *std::max_element(uS, uS + 4);
-}
-
-// Orders the interaction lists of the (up to) four targets whose lanes form
-// one half-wave (4 targets x TR_ROW / 8 lanes) in k_translate.  Each 64-byte
-// row read by a half-wave occupies one quarter of the LDS banks, so at every
-// step the four targets' T rows, and separately their source rows, are made
-// to fall in distinct quarters where the lists allow: greedily, the target
-// with the most words left picks first, each picking the word class (T
-// quarter, source quarter) that collides least with the picks made so far,
-// ties going to the class it holds most of.  Only the order of each list
-// changes (its sum is still taken in one fixed order); with cost set, the
-// modelled cycles before and after are added to cost[0], cost[1].
-static void schedule_half_wave(std::vector<unsigned>* L, int nq, std::vector<unsigned>& buf, long long* cost) {
-   size_t longest = 0, total = 0;
-   for (int j = 0; j < nq; j++) {
-      longest = std::max(longest, L[j].size());
-      total += L[j].size();
-   }
-   if (cost)
-      for (size_t e = 0; e < longest; e++) cost[0] += tr_step_cost(L, nq, e);
-   // words of each list grouped by class (T quarter * 4 + source quarter),
-   // in list order within a class: head[j][c] .. end[j][c] in buf
-   buf.resize(total);
-   int head[4][16], end[4][16], rem[4] = {0, 0, 0, 0};
-   int base = 0;
-   for (int j = 0; j < nq; j++) {
-      int cnt[16] = {};
-      for (unsigned w : L[j]) cnt[tr_quarter(w & 0xffffu) * 4 + tr_quarter(w >> 16)]++;
-      for (int c = 0; c < 16; c++) {
-         head[j][c] = end[j][c] = base;
-         base += cnt[c];
-      }
-      for (unsigned w : L[j]) buf[end[j][tr_quarter(w & 0xffffu) * 4 + tr_quarter(w >> 16)]++] = w;
-      rem[j] = (int)L[j].size();
-      L[j].clear();
-   }
-   for (;;) {
-      int order[4] = {0, 1, 2, 3}, live = 0;
-      for (int j = 0; j < nq; j++) live += rem[j] > 0;
-      if (!live) break;
-      std::stable_sort(order, order + nq, [&](int a, int b) { return rem[a] > rem[b]; });
-      int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0};
-      for (int oi = 0; oi < nq; oi++) {
-         const int j = order[oi];
-         if (!rem[j]) continue;
-         int best = -1, best_cost = 1 << 30, best_n = 0;
-         for (int c = 0; c < 16; c++) {
-            const int n = end[j][c] - head[j][c];
-            if (!n) continue;
-            const int cst = uT[c >> 2] + uS[c & 3];
-            if (cst < best_cost || (cst == best_cost && n > best_n)) {
-               best = c;
-               best_cost = cst;
-               best_n = n;
+// Tiles of consecutive targets of one level: a tile grows while it has
+// fewer than max_tile targets and the union of their sources stays within
+// budget; a tile whose union exceeds max_src is halved until it fits.
+// emit(k0, k1, union) receives each tile (targets tgt[t0 + k0 .. t0 + k1)
+// and the absolute spectrum offsets of its sources in order of first
+// appearance).
+template <typename F>
+static void make_tiles(const TrLevel& L, const std::vector<TgtPlan>& tgt, const std::vector<int>& isrc, int max_tile,
+                       int budget, int max_src, F&& emit, int quantum = 1) {
+   std::vector<int> stamp(std::max(L.nsrc, 1), -1);
+   auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
+   int epoch = 0;
+   std::function<void(int, int)> make = [&](int k0, int k1) {
+      const int mark = -2 - (epoch++);
+      std::vector<int> so;
+      for (int k = k0; k < k1; k++) {
+         const TgtPlan& tp = tgt[L.t0 + k];
+         for (int e = 0; e < tp.nint; e++) {
+            const int sl = slot_of(isrc[tp.int_off + e]);
+            if (stamp[sl] != mark) {
+               stamp[sl] = mark;
+               so.push_back(isrc[tp.int_off + e]);
             }
          }
-         L[j].push_back(buf[head[j][best]++]);
-         uT[best >> 2]++;
-         uS[best & 3]++;
-         rem[j]--;
+      }
+      if ((int)so.size() > max_src) {
+         if (k1 - k0 < 2) throw std::runtime_error("translation tile exceeds its LDS bound");
+         const int km = (k0 + k1) / 2;
+         make(k0, km);
+         make(km, k1);
+         return;
+      }
+      emit(k0, k1, so);
+   };
+   for (int k0 = 0; k0 < L.nt;) {
+      int k1 = k0, un = 0;
+      while (k1 < L.nt && k1 - k0 < max_tile) {
+         const TgtPlan& tp = tgt[L.t0 + k1];
+         int add = 0;
+         for (int e = 0; e < tp.nint; e++) add += stamp[slot_of(isrc[tp.int_off + e])] != k0;   // lists hold distinct sources
+         if (k1 > k0 && un + add > budget) break;
+         for (int e = 0; e < tp.nint; e++) stamp[slot_of(isrc[tp.int_off + e])] = k0;
+         un += add;
+         k1++;
+      }
+      // a tile that stops short of the level's end is cut to a multiple of
+      // the quantum when it holds more than one
+      if (k1 < L.nt && k1 - k0 > quantum) k1 = k0 + (k1 - k0) / quantum * quantum;
+      make(k0, k1);
+      k0 = k1;
+   }
+}
+
+// FP64 translation (k_translate<double>): tiles of up to TR_TILE targets, one
+// task per tile and 4-bin chunk; each target's words hold the two LDS row
+// byte offsets (T code * TR_ROW | local source * TR_ROW << 16) in
+// interaction-list order, from a 16-byte aligned start, and every list is
+// followed by at least four readable words.
+static void build_translate_f64(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
+                                const std::vector<int>& isrc, const std::vector<int>& itoff, hipStream_t s) {
+   std::vector<TrTile> tiles;
+   std::vector<GTgt> gtgt;
+   std::vector<int> gsrc, gint;
+   std::vector<int4> td;
+   for (const TrLevel& L : lvls) {
+      const int first = (int)tiles.size();
+      make_tiles(L, tgt, isrc, TR_TILE, TR_SRC_BUDGET, TR_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
+         std::unordered_map<int, int> loc;
+         for (size_t i = 0; i < so.size(); i++) loc[so[i]] = (int)i;
+         TrTile t;
+         t.nt = k1 - k0;
+         t.tgt0 = (int)gtgt.size();
+         t.nsrc = (int)so.size();
+         t.src_off = (int)gsrc.size();
+         t.pad = L.pad;
+         t.pst = L.pst;
+         t.t_off = L.t_off;
+         gsrc.insert(gsrc.end(), so.begin(), so.end());
+         P.tr_max_src = std::max(P.tr_max_src, t.nsrc);
+         for (int k = k0; k < k1; k++) {
+            const TgtPlan& tp = tgt[L.t0 + k];
+            GTgt g;
+            g.acc_off = tp.acc_off;
+            while (gint.size() % 4) gint.push_back(0);
+            g.int_off = (int)gint.size();
+            g.nint = tp.nint;
+            for (int e = 0; e < tp.nint; e++)
+               gint.push_back((int)((unsigned)(itoff[tp.int_off + e] * TR_ROW) |
+                                    ((unsigned)(loc.at(isrc[tp.int_off + e]) * TR_ROW) << 16)));
+            gtgt.push_back(g);
+         }
+         tiles.push_back(t);
+      });
+      for (int ti = first; ti < (int)tiles.size(); ti++)
+         for (int b0 = 0; b0 < L.pad; b0 += TR_BINS_F64) td.push_back(make_int4(ti, 1, b0, 0));
+   }
+   for (int k = 0; k < 8; k++) gint.push_back(0);   // read-ahead past the last list
+   P.ntask_d = (int)td.size();
+   P.task_d.upload(td, s);
+   P.tiles.upload(tiles, s);
+   P.gtgt.upload(gtgt, s);
+   P.gsrc.upload(gsrc, s);
+   P.gint.upload(gint, s);
+   FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+// Words of an octet for k_translate_pipe: the interaction lists of up to
+// eight targets stepped together (target t's word at step k read by the
+// lanes 8t..8t+7), nk4 steps (the longest list rounded up to 8), stored in
+// groups of four steps: target t's word of step k at 32 (k / 4) + 4 t + k % 4.
+// A half-wave (targets 0-3 or 4-7) reads four 64-byte T rows and four
+// source rows per step, each row one quarter of the 256-byte LDS bank row;
+// each list is reordered so that at every step the half-wave's four T
+// rows, and separately its four source rows, fall in distinct quarters
+// where the lists allow (greedy: the target with most words left picks
+// first, each taking the word class (T quarter, source quarter) that
+// collides least with the picks made so far, ties to the class it holds
+// most of).  Finished lists take the null word (zero T row).  Quarters:
+// T code c -> c & 3; local source l -> l & 3 (both regions start on a
+// quarter boundary).
+static void octet_words(const std::vector<std::vector<unsigned>>& lists, int nk4, std::vector<unsigned>& out) {
+   out.assign(32 * (size_t)(nk4 / 4), TP_NULL_WORD);
+   for (int h = 0; h < 2; h++) {
+      std::vector<unsigned> cls[4][16];
+      int head[4][16] = {}, rem[4] = {0, 0, 0, 0};
+      for (int j = 0; j < 4; j++) {
+         const int t = 4 * h + j;
+         if (t >= (int)lists.size()) continue;
+         for (unsigned w : lists[t]) cls[j][((w & 0xffffu) / 64 & 3) * 4 + ((w >> 16) / 64 & 3)].push_back(w);
+         rem[j] = (int)lists[t].size();
+      }
+      for (int k = 0; k < nk4; k++) {
+         int order[4] = {0, 1, 2, 3};
+         std::stable_sort(order, order + 4, [&](int x, int y) { return rem[x] > rem[y]; });
+         int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0};
+         for (int oi = 0; oi < 4; oi++) {
+            const int j = order[oi];
+            unsigned w = TP_NULL_WORD;
+            if (rem[j] > 0) {
+               int best = -1, best_cost = 1 << 30, best_n = 0;
+               for (int c = 0; c < 16; c++) {
+                  const int n = (int)cls[j][c].size() - head[j][c];
+                  if (n <= 0) continue;
+                  const int cst = uT[c >> 2] + uS[c & 3];
+                  if (cst < best_cost || (cst == best_cost && n > best_n)) {
+                     best = c;
+                     best_cost = cst;
+                     best_n = n;
+                  }
+               }
+               w = cls[j][best][head[j][best]++];
+               rem[j]--;
+            }
+            uT[(w & 0xffffu) / 64 & 3]++;
+            uS[(w >> 16) / 64 & 3]++;
+            out[(size_t)32 * (k / 4) + 4 * (4 * h + j) + (k & 3)] = w;
+         }
       }
    }
-   if (cost)
-      for (size_t e = 0; e < longest; e++) cost[1] += tr_step_cost(L, nq, e);
+}
+
+// FP32 translation (k_translate_pipe): tiles of up to TP_TILE targets whose
+// source union fits an LDS buffer; per tile the targets sorted by list
+// length and cut into octets of eight, longest first (the consumer waves
+// claim them in this order); tasks (tile, 8-bin chunk, first octet, octets)
+// listed by level, then chunk, then tile, and dealt to the persistent
+// blocks as described below.
+static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
+                                 const std::vector<int>& isrc, const std::vector<int>& itoff, hipStream_t s) {
+   std::vector<TpTile> tiles;
+   std::vector<TpOct> octs;
+   std::vector<long long> oacc;
+   std::vector<int> gsrc, gint;
+   std::vector<int4> tasks;
+   std::vector<double> cost;
+   std::vector<unsigned> words;
+   for (const TrLevel& L : lvls) {
+      std::vector<int4> lt;   // per tile of the level: index, first slot, slots, modelled cost
+      make_tiles(
+          L, tgt, isrc, TP_TILE, TP_SRC_BUDGET, TP_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
+         std::unordered_map<int, int> loc;
+         for (size_t i = 0; i < so.size(); i++) loc[so[i]] = (int)i;
+         TpTile t;
+         t.t_off = L.t_off;
+         t.src_off = (int)gsrc.size();
+         t.pad = L.pad;
+         t.pst = L.pst;
+         if (so.empty())
+            gsrc.push_back(0);   // a readable row for the producers' clamped loads
+         else
+            gsrc.insert(gsrc.end(), so.begin(), so.end());
+         t.nsrc = std::max(1, (int)so.size());
+         // targets by list length, longest first, cut into octets
+         const int n = k1 - k0;
+         std::vector<int> order(n);
+         for (int k = 0; k < n; k++) order[k] = k;
+         std::stable_sort(order.begin(), order.end(),
+                          [&](int x, int y) { return tgt[L.t0 + k0 + x].nint > tgt[L.t0 + k0 + y].nint; });
+         std::vector<TpOct> to;
+         for (int o0 = 0; o0 < n; o0 += 8) {
+            std::vector<std::vector<unsigned>> lists;
+            TpOct o;
+            o.acc_idx = (int)oacc.size();
+            o.pad_ = 0;
+            int longest = 0;
+            for (int t8 = 0; t8 < 8; t8++) {
+               if (o0 + t8 >= n) {
+                  oacc.push_back(-1);
+                  continue;
+               }
+               const TgtPlan& tp = tgt[L.t0 + k0 + order[o0 + t8]];
+               std::vector<unsigned> lw;
+               for (int e = 0; e < tp.nint; e++)
+                  lw.push_back((unsigned)(itoff[tp.int_off + e] * 64) |
+                               ((unsigned)(loc.at(isrc[tp.int_off + e]) * 64) << 16));
+               longest = std::max(longest, tp.nint);
+               lists.push_back(lw);
+               oacc.push_back(tp.acc_off);
+            }
+            o.nk = std::max(8, (longest + 7) / 8 * 8);
+            octet_words(lists, o.nk, words);
+            while (gint.size() % 4) gint.push_back((int)TP_NULL_WORD);
+            o.int_off = (int)gint.size();
+            for (unsigned w : words) gint.push_back((int)w);
+            to.push_back(o);
+         }
+         // octets in the task's order (longest first: the consumers claim
+         // them in turn); modelled cost of a task in steps per consumer wave
+         const int oct0 = (int)octs.size();
+         double steps = 0;
+         for (const TpOct& o : to) {
+            octs.push_back(o);
+            steps += o.nk;
+         }
+         const int tcost = (int)(steps / TP_CW + (to.empty() ? 0 : to[0].nk));
+         lt.push_back(make_int4((int)tiles.size(), oct0, (int)to.size(), tcost));
+         tiles.push_back(t);
+          },
+          TP_QUANTUM);
+      for (int b0 = 0; b0 < L.pad; b0 += 8)
+         for (const int4& e : lt) {
+            tasks.push_back(make_int4(e.x, b0, e.y, e.z));
+            cost.push_back(e.w + 16.0);   // summing steps plus a fixed share per task
+         }
+   }
+   // the producers address the FP32 spectra by 32-bit byte offsets
+   for (size_t i = 0; i < tiles.size(); i++)
+      for (int k = 0; k < tiles[i].nsrc; k++)
+         if (8.0 * ((double)gsrc[tiles[i].src_off + k] + tiles[i].pst) >= 4294967296.0)
+            throw std::runtime_error("FP32 spectra above 4 GiB: the pipelined translation cannot address them");
+   // word reads are clamped to each octet, so no padding is needed past the end
+   if (gint.empty()) gint.push_back((int)TP_NULL_WORD);
+   int ncu = 256;
+   {
+      int dev = 0, v = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+         ncu = v;
+   }
+   // tasks to persistent blocks: the chunk-major list cut into 8 contiguous
+   // eighths of equal cost, eighth x on the blocks b = x + 8 j of XCD x;
+   // within an eighth, tasks longest first to the least loaded block; each
+   // block then takes its tasks in list order (shared T rows in a row)
+   const int ntask = (int)tasks.size();
+   const int nblk = std::min(ntask, ncu);
+   std::vector<int2> range(std::max(nblk, 1), make_int2(0, 0));
+   std::vector<int4> btasks;
+   std::vector<double> bcost;
+   if (nblk > 0) {
+      const int ngrp = (nblk % 8 == 0) ? 8 : 1;
+      double total = 0;
+      for (double c : cost) total += c;
+      std::vector<int> grp(ntask);
+      double acc = 0;
+      for (int i = 0; i < ntask; i++) {
+         grp[i] = std::min(ngrp - 1, (int)((acc + 0.5 * cost[i]) * ngrp / total));
+         acc += cost[i];
+      }
+      std::vector<std::vector<int>> mine(nblk);
+      std::vector<double> load(nblk, 0.0);
+      for (int x = 0; x < ngrp; x++) {
+         std::vector<int> ids;
+         for (int i = 0; i < ntask; i++)
+            if (grp[i] == x) ids.push_back(i);
+         std::stable_sort(ids.begin(), ids.end(), [&](int u, int v) { return cost[u] > cost[v]; });
+         for (int i : ids) {
+            int best = -1;
+            for (int b = x; b < nblk; b += ngrp)
+               if (best < 0 || load[b] < load[best]) best = b;
+            mine[best].push_back(i);
+            load[best] += cost[i];
+         }
+      }
+      for (int b = 0; b < nblk; b++) {
+         std::sort(mine[b].begin(), mine[b].end());
+         range[b] = make_int2((int)btasks.size(), (int)mine[b].size());
+         for (int i : mine[b]) {
+            btasks.push_back(tasks[i]);
+            bcost.push_back(cost[i]);
+         }
+      }
+      tasks.swap(btasks);
+      cost.swap(bcost);
+   }
+   if (timing_on() && nblk > 0) {
+      double total = 0, worst = 0;
+      std::vector<double> rc(nblk, 0.0);
+      for (int b = 0; b < nblk; b++) {
+         for (int i = range[b].x; i < range[b].x + range[b].y; i++) rc[b] += cost[i];
+         total += rc[b];
+         worst = std::max(worst, rc[b]);
+      }
+      long long slots = 0, used = 0, steps = 0, nsrc = 0, targets = 0;
+      for (const TpOct& o : octs) {
+         slots++;
+         used++;
+         steps += o.nk;
+      }
+      for (long long a : oacc) targets += a >= 0;
+      for (const TpTile& t : tiles) nsrc += t.nsrc;
+      std::fprintf(stderr,
+                   "[fftsvd setup] pipelined translation: %d tiles (%.1f targets, %.1f sources each), %d tasks on %d "
+                   "blocks, range cost max/mean %.3f, %.1f steps per octet, %.3f of target slots used\n",
+                   (int)tiles.size(), (double)targets / std::max<size_t>(tiles.size(), 1),
+                   (double)nsrc / std::max<size_t>(tiles.size(), 1), ntask, nblk, worst / (total / nblk),
+                   (double)steps / std::max<long long>(used, 1), (double)targets / std::max<long long>(8 * slots, 1));
+   }
+   P.nblk_p = nblk;
+   P.ntask_p = ntask;
+   {
+      // staging by direct global->LDS transfers unless FFTSVD_TP_MODE=0
+      const char* m = std::getenv("FFTSVD_TP_MODE");
+      P.mode_p = (m && m[0] == '0') ? 0 : 1;
+   }
+   P.range_p.upload(range, s);
+   P.task_p.upload(tasks, s);
+   P.tiles_p.upload(tiles, s);
+   P.oct_p.upload(octs, s);
+   if (oacc.empty()) oacc.push_back(-1);
+   P.oacc_p.upload(oacc, s);
+   P.gsrc_p.upload(gsrc, s);
+   P.gint_p.upload(gint, s);
+   FSVD_CHECK(hipStreamSynchronize(s));
 }
 
 // Concatenates every level's far-field blocks into single buffers with
@@ -243,13 +515,15 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    for (const FarLevel& lv : op.levels) {
       vt_n += (long long)lv.VT64.n;
       pv_n += (long long)lv.PV64.n;
-      t_n += (long long)lv.T64.n;
+      t_n += 2LL * 343 * tr_pst(lv.pad);
       uti_n += (long long)lv.UTI64.n;
       ut_n += (long long)lv.UT64.n;
    }
    P.VTc64.alloc(std::max<long long>(vt_n, 1));
    P.PV64.alloc(std::max<long long>(pv_n, 1));
    P.Tsp64.alloc(std::max<long long>(t_n, 2));
+   // the bins between pad and the padded stride stay zero
+   FSVD_CHECK(hipMemsetAsync(P.Tsp64.p, 0, sizeof(double) * P.Tsp64.n, s));
    P.UTI64.alloc(std::max<long long>(uti_n, 1));
    P.UT64.alloc(std::max<long long>(ut_n, 1));
 
@@ -285,7 +559,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
             P.proj_maxcols = std::max(P.proj_maxcols, c.ncol);
          }
          sp.pv_off = pv_b + sd.pv_off;
-         sp.spec_off = spec_b + (long long)sd.spec_slot * lv.pad;
+         sp.spec_off = spec_b + (long long)sd.spec_slot * tr_pst(lv.pad);
          src.push_back(sp);
          P.maxr_src = std::max(P.maxr_src, sd.r);
          tr.push_back({lv.VT64.p + sd.vt_off, P.VTc64.p + vt_b + sd.vt_off, sd.r, sd.np});
@@ -302,10 +576,10 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          tp.int_off = (int)isrc.size();
          tp.w_off = w_n;
          tp.t_off = t_b / 2;
-         tp.acc_off = acc_b + (long long)td.spec_slot * lv.pad;
+         tp.acc_off = acc_b + (long long)td.spec_slot * tr_pst(lv.pad);
          tp.uti_off = uti_b + td.uti_off;
          for (int e = 0; e < td.nint; e++) {
-            long long so = spec_b + (long long)lv.h_isrc[td.int_off + e] * lv.pad;
+            long long so = spec_b + (long long)lv.h_isrc[td.int_off + e] * tr_pst(lv.pad);
             if (so > 0x7fffffffLL) throw std::runtime_error("spectrum offsets exceed 32 bits");
             isrc.push_back((int)so);
             itoff.push_back(lv.h_itoff[td.int_off + e]);
@@ -318,16 +592,20 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          if (b.n) FSVD_CHECK(hipMemcpyAsync(dst, b.p, sizeof(double) * b.n, hipMemcpyDeviceToDevice, s));
       };
       cp(P.PV64.p + pv_b, lv.PV64);
-      cp(P.Tsp64.p + t_b, lv.T64);
+      // T spectra re-laid at the padded row stride (343 rows of pad bins)
+      if (lv.T64.n)
+         FSVD_CHECK(hipMemcpy2DAsync(P.Tsp64.p + t_b, sizeof(double) * 2 * tr_pst(lv.pad), lv.T64.p,
+                                     sizeof(double) * 2 * lv.pad, sizeof(double) * 2 * lv.pad, 343,
+                                     hipMemcpyDeviceToDevice, s));
       cp(P.UTI64.p + uti_b, lv.UTI64);
       cp(P.UT64.p + ut_b, lv.UT64);
       vt_b += (long long)lv.VT64.n;
       pv_b += (long long)lv.PV64.n;
-      t_b += (long long)lv.T64.n;
+      t_b += 2LL * 343 * tr_pst(lv.pad);
       uti_b += (long long)lv.UTI64.n;
       ut_b += (long long)lv.UT64.n;
-      spec_b += (long long)lv.nsrc * lv.pad;
-      acc_b += (long long)lv.ntgt * lv.pad;
+      spec_b += (long long)lv.nsrc * tr_pst(lv.pad);
+      acc_b += (long long)lv.ntgt * tr_pst(lv.pad);
    }
    {
       DBuf<TransDesc> dtr;
@@ -366,140 +644,23 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.nublk = (int)ublk.size();
    P.chunks.upload(chunks, s);
    timing_mark("plan: leaf U blocks", tm);
-   // translation tiles: runs of up to TR_TILE consecutive targets of a level
-   // (tree order, so spatially compact) with the union of their sources;
-   // a run whose union exceeds TR_MAX_SRC is halved until it fits
+   // translation plans over the targets of each level (tree order, so
+   // consecutive targets are spatially compact): FP64 tiles for k_translate,
+   // FP32 tiles and persistent-block task ranges for k_translate_pipe
    {
-      std::vector<TrTile> tiles;
-      std::vector<GTgt> gtgt;
-      std::vector<int> gsrc, gint;
-      std::vector<int4> tf, td;
+      std::vector<TrLevel> tl;
       size_t tbase = 0;
       long long tb = 0, sbase = 0;
       for (size_t li = 0; li < L; li++) {
          const FarLevel& lv = op.levels[li];
-         const int nl = (int)lv.h_tgt.size();
-         // sources of this level by slot: spec offset = sbase + slot * pad
-         std::vector<int> stamp(lv.nsrc, -1), loc(lv.nsrc, 0);
-         auto slot_of = [&](int so) { return (int)(((long long)so - sbase) / lv.pad); };
-         int epoch = 0;
-         std::function<void(int, int)> make = [&](int k0, int k1) {
-            // union of the tiles' sources in order of first appearance
-            const int mark = -2 - (epoch++);
-            std::vector<int> so;
-            for (int k = k0; k < k1; k++) {
-               const TgtPlan& tp = tgt[tbase + k];
-               for (int e = 0; e < tp.nint; e++) {
-                  const int sl = slot_of(isrc[tp.int_off + e]);
-                  if (stamp[sl] != mark) {
-                     stamp[sl] = mark;
-                     loc[sl] = (int)so.size();
-                     so.push_back(isrc[tp.int_off + e]);
-                  }
-               }
-            }
-            if ((int)so.size() > TR_MAX_SRC && k1 - k0 > 1) {
-               const int km = (k0 + k1) / 2;
-               make(k0, km);
-               make(km, k1);
-               return;
-            }
-            if ((int)so.size() > TR_MAX_SRC) throw std::runtime_error("translation tile exceeds its LDS bound");
-            TrTile tl;
-            tl.nt = k1 - k0;
-            tl.tgt0 = (int)gtgt.size();
-            tl.nsrc = (int)so.size();
-            tl.src_off = (int)gsrc.size();
-            tl.pad = lv.pad;
-            tl.pad_ = 0;
-            tl.t_off = tb / 2;
-            gsrc.insert(gsrc.end(), so.begin(), so.end());
-            P.tr_max_src = std::max(P.tr_max_src, tl.nsrc);
-            for (int k = k0; k < k1; k++) {
-               const TgtPlan& tp = tgt[tbase + k];
-               GTgt g;
-               g.acc_off = tp.acc_off;
-               while (gint.size() % 4) gint.push_back(0);
-               g.int_off = (int)gint.size();
-               g.nint = tp.nint;
-               for (int e = 0; e < tp.nint; e++)
-                  gint.push_back((int)((unsigned)(itoff[tp.int_off + e] * TR_ROW) |
-                                       ((unsigned)(loc[slot_of(isrc[tp.int_off + e])] * TR_ROW) << 16)));
-               gtgt.push_back(g);
-            }
-            tiles.push_back(tl);
-         };
-         // grow each tile while its source union stays within the budget
-         // (union size tracked with a stamp per source slot)
-         const int tile_begin = (int)tiles.size();
-         for (int k0 = 0; k0 < nl;) {
-            int k1 = k0, un = 0;
-            while (k1 < nl && k1 - k0 < TR_TILE) {
-               const TgtPlan& tp = tgt[tbase + k1];
-               int add = 0;
-               for (int e = 0; e < tp.nint; e++) add += stamp[slot_of(isrc[tp.int_off + e])] != k0;
-               // sources within one list are distinct
-               if (k1 > k0 && un + add > TR_SRC_BUDGET) break;
-               for (int e = 0; e < tp.nint; e++) stamp[slot_of(isrc[tp.int_off + e])] = k0;
-               un += add;
-               k1++;
-            }
-            make(k0, k1);
-            k0 = k1;
-         }
-         // tasks: runs of TR_RUN tiles x one bin chunk
-         const int tile_end = (int)tiles.size();
-         for (int r0 = tile_begin; r0 < tile_end; r0 += TR_RUN) {
-            const int nr = std::min(TR_RUN, tile_end - r0);
-            for (int b0 = 0; b0 < lv.pad; b0 += TR_BINS_F32) tf.push_back(make_int4(r0, nr, b0, 0));
-            for (int b0 = 0; b0 < lv.pad; b0 += TR_BINS_F64) td.push_back(make_int4(r0, nr, b0, 0));
-         }
+         tl.push_back({(int)tbase, (int)lv.h_tgt.size(), lv.pad, tr_pst(lv.pad), lv.nsrc, tb / 2, sbase});
          tbase += lv.h_tgt.size();
-         tb += (long long)lv.T64.n;
-         sbase += (long long)lv.nsrc * lv.pad;
+         tb += 2LL * 343 * tr_pst(lv.pad);
+         sbase += (long long)lv.nsrc * tr_pst(lv.pad);
       }
-      // words the translation kernel may read ahead past the last list
-      for (int k = 0; k < 8; k++) gint.push_back(0);
-      // each half-wave's lists reordered over the LDS bank quarters; the
-      // half-waves are independent, so threads take them in turn
-      {
-         std::vector<int2> quads;   // (first GTgt, targets)
-         for (const TrTile& tl : tiles)
-            for (int q0 = 0; q0 < tl.nt; q0 += 4) quads.push_back(make_int2(tl.tgt0 + q0, std::min(4, tl.nt - q0)));
-         const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-         std::vector<std::array<long long, 2>> cost(nth, std::array<long long, 2>{0, 0});
-         const bool model = timing_on();
-         std::vector<std::thread> pool;
-         for (int t = 0; t < nth; t++)
-            pool.emplace_back([&, t] {
-               std::vector<unsigned> lists[4], buf;
-               for (size_t qi = t; qi < quads.size(); qi += nth) {
-                  const int g0 = quads[qi].x, nq = quads[qi].y;
-                  for (int j = 0; j < nq; j++) {
-                     const GTgt& g = gtgt[g0 + j];
-                     lists[j].assign(gint.begin() + g.int_off, gint.begin() + g.int_off + g.nint);
-                  }
-                  schedule_half_wave(lists, nq, buf, model ? cost[t].data() : nullptr);
-                  for (int j = 0; j < nq; j++) std::copy(lists[j].begin(), lists[j].end(), gint.begin() + gtgt[g0 + j].int_off);
-               }
-            });
-         for (auto& th : pool) th.join();
-         if (model) {
-            long long c0 = 0, c1 = 0;
-            for (auto& c : cost) c0 += c[0], c1 += c[1];
-            std::fprintf(stderr, "[fftsvd setup] translation LDS steps (modelled cycles): list order %lld, scheduled %lld\n",
-                         c0, c1);
-         }
-      }
-      P.ntask_f = (int)tf.size();
-      P.ntask_d = (int)td.size();
-      P.task_f.upload(tf, s);
-      P.task_d.upload(td, s);
-      P.tiles.upload(tiles, s);
-      P.gtgt.upload(gtgt, s);
-      P.gsrc.upload(gsrc, s);
-      P.gint.upload(gint, s);
-      FSVD_CHECK(hipStreamSynchronize(s));
+      build_translate_f64(P, tl, tgt, isrc, itoff, s);
+      if (want32) build_translate_pipe(P, tl, tgt, isrc, itoff, s);
+      timing_mark("plan: translation", tm);
    }
    {
       std::vector<double> wl(op.h_leaves.size());
@@ -531,6 +692,8 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    P.tpart.alloc(std::max(tpart_n, 1));
    P.spec.alloc(2 * std::max<long long>(spec_b, 1));
    P.acc.alloc(2 * std::max<long long>(acc_b, 1));
+   FSVD_CHECK(hipMemsetAsync(P.spec.p, 0, sizeof(double) * P.spec.n, s));
+   FSVD_CHECK(hipMemsetAsync(P.acc.p, 0, sizeof(double) * P.acc.n, s));
    P.w.alloc(std::max(w_n, 1));
    if (want32) {
       to_f32(P.VTc64, P.VTc32, s);
@@ -1154,11 +1317,18 @@ static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, doubl
    a.chunks = P.chunks.p;
    a.src = P.src.p;
    a.tgt = P.tgt.p;
-   a.ntask_f = P.ntask_f;
    a.ntask_d = P.ntask_d;
    a.tr_max_src = P.tr_max_src;
-   a.task_f = P.task_f.p;
    a.task_d = P.task_d.p;
+   a.nblk_p = P.nblk_p;
+   a.mode_p = P.mode_p;
+   a.range_p = P.range_p.p;
+   a.task_p = P.task_p.p;
+   a.tiles_p = P.tiles_p.p;
+   a.oct_p = P.oct_p.p;
+   a.oacc_p = P.oacc_p.p;
+   a.gsrc_p = P.gsrc_p.p;
+   a.gint_p = P.gint_p.p;
    a.tiles = P.tiles.p;
    a.gtgt = P.gtgt.p;
    a.gsrc = P.gsrc.p;

@@ -93,11 +93,18 @@ struct ApplyPlan {
    DBuf<ProjChunk> chunks;
    DBuf<SrcPlan> src;
    DBuf<TgtPlan> tgt;
-   int ntask_f = 0, ntask_d = 0, tr_max_src = 0;
-   DBuf<int4> task_f, task_d;
+   int ntask_d = 0, tr_max_src = 0;   // FP64 translation (k_translate<double>)
+   DBuf<int4> task_d;
    DBuf<TrTile> tiles;
    DBuf<GTgt> gtgt;
    DBuf<int> gsrc, gint;
+   int nblk_p = 0, ntask_p = 0, mode_p = 0;   // FP32 translation (k_translate_pipe)
+   DBuf<int2> range_p;
+   DBuf<int4> task_p;
+   DBuf<TpTile> tiles_p;
+   DBuf<TpOct> oct_p;
+   DBuf<long long> oacc_p;
+   DBuf<int> gsrc_p, gint_p;
    DBuf<int> map_leaf;   // XCD block map of the leaf launch
    int nmap_leaf = 0;
    DBuf<LeafU> ublk;
