@@ -630,8 +630,7 @@ constexpr int TP_SPIECES = (TP_MAX_SRC * 4 + TP_PL - 1) / TP_PL;   // source pie
 constexpr int TP_TPIECES = (343 * 4 + TP_PL - 1) / TP_PL;          // T pieces per lane
 constexpr int TP_BATCH = 10;
 __device__ __forceinline__ void tp_stage(const int4& tk, const TpTile& tl, bool stage_t, unsigned char* buf, int pl,
-                                         const int* __restrict__ gsrc, const float2* __restrict__ Tspec,
-                                         const float2* __restrict__ spec) {
+                                         const int* __restrict__ gsrc) {
    const int b0 = tk.y;
    const int npiece = tl.nsrc * 4;
    // byte offsets of the pieces in the spectrum array (below 4 GiB, checked
@@ -648,7 +647,7 @@ __device__ __forceinline__ void tp_stage(const int4& tk, const TpTile& tl, bool 
       float4 v[TP_BATCH];
 #pragma unroll
       for (int u = 0; u < TP_BATCH; u++)
-         if (u0 + u < TP_SPIECES) v[u] = *(const float4*)((const char*)spec + so[u0 + u]);
+         if (u0 + u < TP_SPIECES) v[u] = *(const float4*)((const char*)tl.spec + so[u0 + u]);
 #pragma unroll
       for (int u = 0; u < TP_BATCH; u++) {
          const int p = pl + TP_PL * (u0 + u);
@@ -661,7 +660,7 @@ __device__ __forceinline__ void tp_stage(const int4& tk, const TpTile& tl, bool 
 #pragma unroll
       for (int u = 0; u < TP_TPIECES; u++) {
          const int p = min(pl + TP_PL * u, 343 * 4 - 1);
-         v[u] = *(const float4*)(Tspec + tl.t_off + (long long)(p >> 2) * tl.pst + b0 + 2 * (p & 3));
+         v[u] = *(const float4*)(tl.T + (long long)(p >> 2) * tl.pst + b0 + 2 * (p & 3));
       }
 #pragma unroll
       for (int u = 0; u < TP_TPIECES; u++)
@@ -681,8 +680,7 @@ __device__ __forceinline__ void tp_dma16(const void* g, unsigned char* base) {
    __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)base, 16, 0, 0);
 }
 __device__ __forceinline__ void tp_stage_dma(const int4& tk, const TpTile& tl, bool stage_t, unsigned char* buf,
-                                             int pw, int lane, const int* __restrict__ gsrc,
-                                             const float2* __restrict__ Tspec, const float2* __restrict__ spec) {
+                                             int pw, int lane, const int* __restrict__ gsrc) {
    constexpr int NS = TP_MAX_SRC * 4 / 64, NT = 352 * 4 / 64;   // transfers per buffer
    constexpr int MS = (NS + TP_PW - 1) / TP_PW, MT = (NT + TP_PW - 1) / TP_PW;
    const int b0 = tk.y, q2 = 2 * (lane & 3);
@@ -696,14 +694,14 @@ __device__ __forceinline__ void tp_stage_dma(const int4& tk, const TpTile& tl, b
 #pragma unroll
    for (int u = 0; u < MS; u++) {
       const int i = min(pw + u * TP_PW, NS - 1);   // a repeated transfer rewrites the same rows
-      tp_dma16(spec + so[u] + b0 + q2, sbuf + i * 1024);
+      tp_dma16(tl.spec + so[u] + b0 + q2, sbuf + i * 1024);
    }
    if (stage_t) {
 #pragma unroll
       for (int u = 0; u < MT; u++) {
          const int i = min(pw + u * TP_PW, NT - 1);
          const int row = min(i * 16 + (lane >> 2), 342);
-         tp_dma16(Tspec + tl.t_off + (long long)row * tl.pst + b0 + q2, buf + i * 1024);
+         tp_dma16(tl.T + (long long)row * tl.pst + b0 + q2, buf + i * 1024);
       }
    }
    __builtin_amdgcn_s_waitcnt(0);
@@ -776,18 +774,17 @@ __device__ __forceinline__ void tp_sum(const TpOct& o, int t, TpPair A, TpPair B
 // k; one barrier per task, as many as the consumers pass.
 template <int MODE>
 __device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
-                                            const int* __restrict__ gsrc, const float2* __restrict__ Tspec,
-                                            const float2* __restrict__ spec, unsigned char* smem, int* ctr, int pw,
+                                            const int* __restrict__ gsrc, unsigned char* smem, int* ctr, int pw,
                                             int lane) {
-   long long key0, key1 = -1;   // the (T offset, chunk) each buffer holds
+   long long key0, key1 = -1;   // the T rows (level and chunk) each buffer holds
    {
       const int4 t0 = tasks[0];
       const TpTile tl = tiles[t0.x];
-      key0 = tl.t_off * 4096 + t0.y;
+      key0 = (long long)(tl.T + t0.y);
       if constexpr (MODE == 1)
-         tp_stage_dma(t0, tl, true, smem, pw, lane, gsrc, Tspec, spec);
+         tp_stage_dma(t0, tl, true, smem, pw, lane, gsrc);
       else
-         tp_stage(t0, tl, true, smem, 64 * pw + lane, gsrc, Tspec, spec);
+         tp_stage(t0, tl, true, smem, 64 * pw + lane, gsrc);
    }
    __syncthreads();
    for (int k = 0; k < n; k++) {
@@ -797,7 +794,7 @@ __device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int 
       if (k + 1 < n) {
          const int4 tn = tasks[k + 1];
          const TpTile tl = tiles[tn.x];
-         const long long key = tl.t_off * 4096 + tn.y;
+         const long long key = (long long)(tl.T + tn.y);
          const int nb = (k + 1) & 1;
          const bool st = (nb ? key1 : key0) != key;
          if (nb)
@@ -805,9 +802,9 @@ __device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int 
          else
             key0 = key;
          if constexpr (MODE == 1)
-            tp_stage_dma(tn, tl, st, smem + nb * TP_BUF, pw, lane, gsrc, Tspec, spec);
+            tp_stage_dma(tn, tl, st, smem + nb * TP_BUF, pw, lane, gsrc);
          else
-            tp_stage(tn, tl, st, smem + nb * TP_BUF, 64 * pw + lane, gsrc, Tspec, spec);
+            tp_stage(tn, tl, st, smem + nb * TP_BUF, 64 * pw + lane, gsrc);
       }
       __syncthreads();
    }
@@ -830,8 +827,8 @@ __device__ __forceinline__ int tp_claim(int* ctr, int k, int lane) {
 // task.
 __device__ __forceinline__ void tp_consumer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
                                             const TpOct* __restrict__ octs, const long long* __restrict__ oacc,
-                                            const int* __restrict__ gint, float2* __restrict__ acc,
-                                            const unsigned char* smem, int* ctr, int lane) {
+                                            const int* __restrict__ gint, const unsigned char* smem, int* ctr,
+                                            int lane) {
    const int t = lane >> 3, b = lane & 7;
    int kc = -1, ic = 0;   // the claimed octet (task kc, index ic), none when kc < k
    TpOct o = {0, 0, 0, 0};
@@ -841,6 +838,7 @@ __device__ __forceinline__ void tp_consumer(const int4* __restrict__ tasks, int 
    for (int k = 0; k < n; k++) {
       const int4 tk = tasks[k];
       const int pad = tiles[tk.x].pad;
+      float2* __restrict__ acc = tiles[tk.x].acc;
       const unsigned char* Tl = smem + (k & 1) * TP_BUF + b * 8;
       const unsigned char* Sl = Tl + TP_TROWS * 64;
       if (kc < k) {
@@ -894,9 +892,7 @@ __global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict
                                                           const TpTile* __restrict__ tiles,
                                                           const TpOct* __restrict__ octs,
                                                           const long long* __restrict__ oacc,
-                                                          const int* __restrict__ gsrc, const int* __restrict__ gint,
-                                                          const float2* __restrict__ Tspec,
-                                                          const float2* __restrict__ spec, float2* __restrict__ acc) {
+                                                          const int* __restrict__ gsrc, const int* __restrict__ gint) {
    extern __shared__ __align__(16) unsigned char smem[];
    const int2 rg = ranges[blockIdx.x];
    const int tid = threadIdx.x, lane = tid & 63;
@@ -909,9 +905,9 @@ __global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict
    if (rg.y == 0) return;   // uniform over the block: no barrier is reached
    // both roles pass rg.y + 1 barriers
    if (wave >= TP_CW)
-      tp_producer<MODE>(tasks_all + rg.x, rg.y, tiles, gsrc, Tspec, spec, smem, ctr, wave - TP_CW, lane);
+      tp_producer<MODE>(tasks_all + rg.x, rg.y, tiles, gsrc, smem, ctr, wave - TP_CW, lane);
    else
-      tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, acc, smem, ctr, lane);
+      tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, smem, ctr, lane);
 }
 
 template <int n, typename T, typename G>
@@ -1228,6 +1224,27 @@ void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const i
                       (typename Cplx<T>::type*)acc, pad);
 }
 
+void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
+                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s) {
+   // two LDS buffers per persistent block, three counters
+   constexpr size_t lds = 2 * (size_t)TP_BUF + 16;
+   check_lds(lds);
+   static std::once_flag attr;
+   std::call_once(attr, [] {
+      FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+      FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+   });
+   if (nblk <= 0) return;
+   if (mode == 1)
+      hipLaunchKernelGGL(k_translate_pipe<1>, dim3(nblk), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
+                         gint);
+   else
+      hipLaunchKernelGGL(k_translate_pipe<0>, dim3(nblk), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
+                         gint);
+}
+
 template <typename T>
 void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
    using C2 = typename Cplx<T>::type;
@@ -1260,26 +1277,8 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       }
       case 2: {
          if constexpr (std::is_same<T, float>::value) {
-            // pipelined: two LDS buffers per persistent block, three counters
-            constexpr size_t lds = 2 * (size_t)TP_BUF + 16;
-            check_lds(lds);
-            static std::once_flag attr;
-            std::call_once(attr, [] {
-               FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<0>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-               FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe<1>,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            });
-            if (a.nblk_p) {
-               if (a.mode_p == 1)
-                  hipLaunchKernelGGL(k_translate_pipe<1>, dim3(a.nblk_p), dim3(TP_NT), lds, s, a.range_p, a.task_p,
-                                     a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p, a.gint_p, (const float2*)a.Tsp,
-                                     (const float2*)a.spec, (float2*)a.acc);
-               else
-                  hipLaunchKernelGGL(k_translate_pipe<0>, dim3(a.nblk_p), dim3(TP_NT), lds, s, a.range_p, a.task_p,
-                                     a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p, a.gint_p, (const float2*)a.Tsp,
-                                     (const float2*)a.spec, (float2*)a.acc);
-            }
+            launch_translate_pipe(a.nblk_p, a.mode_p, a.range_p, a.task_p, a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p,
+                                  a.gint_p, s);
          } else {
             // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
             constexpr int BC = TR_BINS_F64, LPT = TR_LPT_F64;

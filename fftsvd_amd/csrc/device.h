@@ -172,11 +172,16 @@ static_assert(2 * TP_BUF <= 163840, "two pipeline buffers must fit the LDS of a 
 static_assert(TP_MAX_SRC % 16 == 0 && (TP_MAX_SRC - 1) * 64 < 65536, "source rows and 16-bit offsets");
 static_assert(TP_TROWS % 4 == 0, "source rows start on an LDS bank-quarter boundary");
 
-// Tile of the pipelined translation: its level's T spectra at t_off (code c
-// at t_off + c * pst), the union of its nsrc source spectra (absolute
-// offsets at src_off in gsrc), the level's pad and stride.
+// Tile of the pipelined translation: its level's FP32 T spectra (code c at
+// T + c * pst), its tree's forward spectra (source k of the tile at
+// spec + gsrc[src_off + k]) and accumulated spectra (octet target
+// outputs at acc + oacc[...]), the union size nsrc, the level's pad and
+// stride.  Tiles carry their pointers, so one launch can cover the
+// translations of several trees.
 struct TpTile {
-   long long t_off;
+   const float2* T;
+   const float2* spec;
+   float2* acc;
    int nsrc, src_off, pad, pst;
 };
 // An octet of the pipelined translation: nk steps (the longest list,

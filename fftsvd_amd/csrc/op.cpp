@@ -322,29 +322,27 @@ static void octet_words(const std::vector<std::vector<unsigned>>& lists, int nk4
 // listed by level, then chunk, then tile, and dealt to the persistent
 // blocks as described below.
 static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
-                                 const std::vector<int>& isrc, const std::vector<int>& itoff, hipStream_t s) {
-   std::vector<TpTile> tiles;
-   std::vector<TpOct> octs;
-   std::vector<long long> oacc;
-   std::vector<int> gsrc, gint;
-   std::vector<int4> tasks;
-   std::vector<double> cost;
+                                 const std::vector<int>& isrc, const std::vector<int>& itoff) {
+   PipeHost& H = P.pipe_h;
+   H = PipeHost();
    std::vector<unsigned> words;
    for (const TrLevel& L : lvls) {
-      std::vector<int4> lt;   // per tile of the level: index, first slot, slots, modelled cost
+      std::vector<int4> lt;   // per tile of the level: index, first octet, octets, modelled cost
       make_tiles(
           L, tgt, isrc, TP_TILE, TP_SRC_BUDGET, TP_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
          std::unordered_map<int, int> loc;
          for (size_t i = 0; i < so.size(); i++) loc[so[i]] = (int)i;
          TpTile t;
-         t.t_off = L.t_off;
-         t.src_off = (int)gsrc.size();
+         t.T = nullptr;
+         t.spec = nullptr;
+         t.acc = nullptr;
+         t.src_off = (int)H.gsrc.size();
          t.pad = L.pad;
          t.pst = L.pst;
          if (so.empty())
-            gsrc.push_back(0);   // a readable row for the producers' clamped loads
+            H.gsrc.push_back(0);   // a readable row for the producers' clamped loads
          else
-            gsrc.insert(gsrc.end(), so.begin(), so.end());
+            H.gsrc.insert(H.gsrc.end(), so.begin(), so.end());
          t.nsrc = std::max(1, (int)so.size());
          // targets by list length, longest first, cut into octets
          const int n = k1 - k0;
@@ -356,12 +354,12 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
          for (int o0 = 0; o0 < n; o0 += 8) {
             std::vector<std::vector<unsigned>> lists;
             TpOct o;
-            o.acc_idx = (int)oacc.size();
+            o.acc_idx = (int)H.oacc.size();
             o.pad_ = 0;
             int longest = 0;
             for (int t8 = 0; t8 < 8; t8++) {
                if (o0 + t8 >= n) {
-                  oacc.push_back(-1);
+                  H.oacc.push_back(-1);
                   continue;
                }
                const TgtPlan& tp = tgt[L.t0 + k0 + order[o0 + t8]];
@@ -371,41 +369,127 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
                                ((unsigned)(loc.at(isrc[tp.int_off + e]) * 64) << 16));
                longest = std::max(longest, tp.nint);
                lists.push_back(lw);
-               oacc.push_back(tp.acc_off);
+               H.oacc.push_back(tp.acc_off);
             }
             o.nk = std::max(8, (longest + 7) / 8 * 8);
             octet_words(lists, o.nk, words);
-            while (gint.size() % 4) gint.push_back((int)TP_NULL_WORD);
-            o.int_off = (int)gint.size();
-            for (unsigned w : words) gint.push_back((int)w);
+            while (H.gint.size() % 4) H.gint.push_back((int)TP_NULL_WORD);
+            o.int_off = (int)H.gint.size();
+            for (unsigned w : words) H.gint.push_back((int)w);
             to.push_back(o);
          }
          // octets in the task's order (longest first: the consumers claim
          // them in turn); modelled cost of a task in steps per consumer wave
-         const int oct0 = (int)octs.size();
+         const int oct0 = (int)H.octs.size();
          double steps = 0;
          for (const TpOct& o : to) {
-            octs.push_back(o);
+            H.octs.push_back(o);
             steps += o.nk;
          }
          const int tcost = (int)(steps / TP_CW + (to.empty() ? 0 : to[0].nk));
-         lt.push_back(make_int4((int)tiles.size(), oct0, (int)to.size(), tcost));
-         tiles.push_back(t);
+         lt.push_back(make_int4((int)H.tiles.size(), oct0, (int)to.size(), tcost));
+         H.tiles.push_back(t);
+         H.t_off.push_back(L.t_off);
           },
           TP_QUANTUM);
       for (int b0 = 0; b0 < L.pad; b0 += 8)
          for (const int4& e : lt) {
-            tasks.push_back(make_int4(e.x, b0, e.y, e.z));
-            cost.push_back(e.w + 16.0);   // summing steps plus a fixed share per task
+            H.tasks.push_back(make_int4(e.x, b0, e.y, e.z));
+            H.cost.push_back(e.w + 16.0);   // summing steps plus a fixed share per task
          }
    }
    // the producers address the FP32 spectra by 32-bit byte offsets
-   for (size_t i = 0; i < tiles.size(); i++)
-      for (int k = 0; k < tiles[i].nsrc; k++)
-         if (8.0 * ((double)gsrc[tiles[i].src_off + k] + tiles[i].pst) >= 4294967296.0)
+   for (size_t i = 0; i < H.tiles.size(); i++)
+      for (int k = 0; k < H.tiles[i].nsrc; k++)
+         if (8.0 * ((double)H.gsrc[H.tiles[i].src_off + k] + H.tiles[i].pst) >= 4294967296.0)
             throw std::runtime_error("FP32 spectra above 4 GiB: the pipelined translation cannot address them");
-   // word reads are clamped to each octet, so no padding is needed past the end
+}
+
+// Tasks to persistent blocks: the list cut into 8 contiguous eighths of
+// equal cost, eighth x on the blocks b = x + 8 j of XCD x (blocks are dealt
+// to the 8 XCDs round-robin, so one L2 serves the rows that neighbouring
+// tiles of an eighth share); within an eighth, tasks longest first to the
+// least loaded block; each block then takes its tasks in list order
+// (shared T rows in a row).  Returns the tasks in block order.
+static std::vector<int4> assign_pipe_blocks(const std::vector<int4>& tasks, const std::vector<double>& cost, int nblk,
+                                            std::vector<int2>& range, double* imbalance) {
+   const int ntask = (int)tasks.size();
+   range.assign(std::max(nblk, 1), make_int2(0, 0));
+   std::vector<int4> out;
+   if (nblk <= 0) return out;
+   const int ngrp = (nblk % 8 == 0) ? 8 : 1;
+   double total = 0;
+   for (double c : cost) total += c;
+   std::vector<int> grp(ntask);
+   double acc = 0;
+   for (int i = 0; i < ntask; i++) {
+      grp[i] = std::min(ngrp - 1, (int)((acc + 0.5 * cost[i]) * ngrp / total));
+      acc += cost[i];
+   }
+   std::vector<std::vector<int>> mine(nblk);
+   std::vector<double> load(nblk, 0.0);
+   for (int x = 0; x < ngrp; x++) {
+      std::vector<int> ids;
+      for (int i = 0; i < ntask; i++)
+         if (grp[i] == x) ids.push_back(i);
+      std::stable_sort(ids.begin(), ids.end(), [&](int u, int v) { return cost[u] > cost[v]; });
+      for (int i : ids) {
+         int best = -1;
+         for (int b = x; b < nblk; b += ngrp)
+            if (best < 0 || load[b] < load[best]) best = b;
+         mine[best].push_back(i);
+         load[best] += cost[i];
+      }
+   }
+   for (int b = 0; b < nblk; b++) {
+      std::sort(mine[b].begin(), mine[b].end());
+      range[b] = make_int2((int)out.size(), (int)mine[b].size());
+      for (int i : mine[b]) out.push_back(tasks[i]);
+   }
+   if (imbalance) *imbalance = *std::max_element(load.begin(), load.end()) / (total / nblk);
+   return out;
+}
+
+void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s) {
+   std::vector<TpTile> tiles;
+   std::vector<TpOct> octs;
+   std::vector<long long> oacc;
+   std::vector<int> gsrc, gint;
+   std::vector<int4> tasks;
+   std::vector<double> cost;
+   for (const TreeOp* op : ops) {
+      const ApplyPlan& P = op->plan;
+      const PipeHost& H = P.pipe_h;
+      if (!P.Tsp32.p) throw std::runtime_error("pipelined translation needs the FP32 operator");
+      const int tb = (int)tiles.size(), ob = (int)octs.size(), ab = (int)oacc.size(), sb = (int)gsrc.size();
+      const int gb = (int)gint.size();
+      for (size_t i = 0; i < H.tiles.size(); i++) {
+         TpTile t = H.tiles[i];
+         t.T = (const float2*)P.Tsp32.p + H.t_off[i];
+         t.spec = (const float2*)P.spec.p;
+         t.acc = (float2*)P.acc.p;
+         t.src_off += sb;
+         tiles.push_back(t);
+      }
+      for (TpOct o : H.octs) {
+         o.int_off += gb;
+         o.acc_idx += ab;
+         octs.push_back(o);
+      }
+      oacc.insert(oacc.end(), H.oacc.begin(), H.oacc.end());
+      gsrc.insert(gsrc.end(), H.gsrc.begin(), H.gsrc.end());
+      gint.insert(gint.end(), H.gint.begin(), H.gint.end());
+      for (size_t i = 0; i < H.tasks.size(); i++) {
+         int4 t = H.tasks[i];
+         t.x += tb;
+         t.z += ob;
+         tasks.push_back(t);
+         cost.push_back(H.cost[i]);
+      }
+   }
    if (gint.empty()) gint.push_back((int)TP_NULL_WORD);
+   if (oacc.empty()) oacc.push_back(-1);
+   if (gsrc.empty()) gsrc.push_back(0);
    int ncu = 256;
    {
       int dev = 0, v = 0;
@@ -413,89 +497,38 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
           hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
          ncu = v;
    }
-   // tasks to persistent blocks: the chunk-major list cut into 8 contiguous
-   // eighths of equal cost, eighth x on the blocks b = x + 8 j of XCD x;
-   // within an eighth, tasks longest first to the least loaded block; each
-   // block then takes its tasks in list order (shared T rows in a row)
-   const int ntask = (int)tasks.size();
-   const int nblk = std::min(ntask, ncu);
-   std::vector<int2> range(std::max(nblk, 1), make_int2(0, 0));
-   std::vector<int4> btasks;
-   std::vector<double> bcost;
-   if (nblk > 0) {
-      const int ngrp = (nblk % 8 == 0) ? 8 : 1;
-      double total = 0;
-      for (double c : cost) total += c;
-      std::vector<int> grp(ntask);
-      double acc = 0;
-      for (int i = 0; i < ntask; i++) {
-         grp[i] = std::min(ngrp - 1, (int)((acc + 0.5 * cost[i]) * ngrp / total));
-         acc += cost[i];
-      }
-      std::vector<std::vector<int>> mine(nblk);
-      std::vector<double> load(nblk, 0.0);
-      for (int x = 0; x < ngrp; x++) {
-         std::vector<int> ids;
-         for (int i = 0; i < ntask; i++)
-            if (grp[i] == x) ids.push_back(i);
-         std::stable_sort(ids.begin(), ids.end(), [&](int u, int v) { return cost[u] > cost[v]; });
-         for (int i : ids) {
-            int best = -1;
-            for (int b = x; b < nblk; b += ngrp)
-               if (best < 0 || load[b] < load[best]) best = b;
-            mine[best].push_back(i);
-            load[best] += cost[i];
-         }
-      }
-      for (int b = 0; b < nblk; b++) {
-         std::sort(mine[b].begin(), mine[b].end());
-         range[b] = make_int2((int)btasks.size(), (int)mine[b].size());
-         for (int i : mine[b]) {
-            btasks.push_back(tasks[i]);
-            bcost.push_back(cost[i]);
-         }
-      }
-      tasks.swap(btasks);
-      cost.swap(bcost);
-   }
+   const int nblk = std::min((int)tasks.size(), ncu);
+   std::vector<int2> range;
+   double imb = 1.0;
+   std::vector<int4> btasks = assign_pipe_blocks(tasks, cost, nblk, range, &imb);
    if (timing_on() && nblk > 0) {
-      double total = 0, worst = 0;
-      std::vector<double> rc(nblk, 0.0);
-      for (int b = 0; b < nblk; b++) {
-         for (int i = range[b].x; i < range[b].x + range[b].y; i++) rc[b] += cost[i];
-         total += rc[b];
-         worst = std::max(worst, rc[b]);
-      }
-      long long slots = 0, used = 0, steps = 0, nsrc = 0, targets = 0;
-      for (const TpOct& o : octs) {
-         slots++;
-         used++;
-         steps += o.nk;
-      }
+      long long targets = 0, steps = 0, nsrc = 0;
+      for (const TpOct& o : octs) steps += o.nk;
       for (long long a : oacc) targets += a >= 0;
       for (const TpTile& t : tiles) nsrc += t.nsrc;
       std::fprintf(stderr,
-                   "[fftsvd setup] pipelined translation: %d tiles (%.1f targets, %.1f sources each), %d tasks on %d "
-                   "blocks, range cost max/mean %.3f, %.1f steps per octet, %.3f of target slots used\n",
-                   (int)tiles.size(), (double)targets / std::max<size_t>(tiles.size(), 1),
-                   (double)nsrc / std::max<size_t>(tiles.size(), 1), ntask, nblk, worst / (total / nblk),
-                   (double)steps / std::max<long long>(used, 1), (double)targets / std::max<long long>(8 * slots, 1));
+                   "[fftsvd setup] pipelined translation (%d tree(s)): %d tiles (%.1f targets, %.1f sources each), "
+                   "%d tasks on %d blocks, block cost max/mean %.3f, %.1f steps per octet, %.3f of target slots "
+                   "used\n",
+                   (int)ops.size(), (int)tiles.size(), (double)targets / std::max<size_t>(tiles.size(), 1),
+                   (double)nsrc / std::max<size_t>(tiles.size(), 1), (int)tasks.size(), nblk, imb,
+                   (double)steps / std::max<size_t>(octs.size(), 1),
+                   (double)targets / std::max<size_t>(8 * octs.size(), 1));
    }
-   P.nblk_p = nblk;
-   P.ntask_p = ntask;
+   d.nblk = nblk;
    {
       // staging by direct global->LDS transfers unless FFTSVD_TP_MODE=0
       const char* m = std::getenv("FFTSVD_TP_MODE");
-      P.mode_p = (m && m[0] == '0') ? 0 : 1;
+      d.mode = (m && m[0] == '0') ? 0 : 1;
    }
-   P.range_p.upload(range, s);
-   P.task_p.upload(tasks, s);
-   P.tiles_p.upload(tiles, s);
-   P.oct_p.upload(octs, s);
-   if (oacc.empty()) oacc.push_back(-1);
-   P.oacc_p.upload(oacc, s);
-   P.gsrc_p.upload(gsrc, s);
-   P.gint_p.upload(gint, s);
+   if (btasks.empty()) btasks.push_back(make_int4(0, 0, 0, 0));
+   d.range.upload(range, s);
+   d.tasks.upload(btasks, s);
+   d.tiles.upload(tiles.empty() ? std::vector<TpTile>(1) : tiles, s);
+   d.octs.upload(octs.empty() ? std::vector<TpOct>(1) : octs, s);
+   d.oacc.upload(oacc, s);
+   d.gsrc.upload(gsrc, s);
+   d.gint.upload(gint, s);
    FSVD_CHECK(hipStreamSynchronize(s));
 }
 
@@ -659,7 +692,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          sbase += (long long)lv.nsrc * tr_pst(lv.pad);
       }
       build_translate_f64(P, tl, tgt, isrc, itoff, s);
-      if (want32) build_translate_pipe(P, tl, tgt, isrc, itoff, s);
+      if (want32) build_translate_pipe(P, tl, tgt, isrc, itoff);
       timing_mark("plan: translation", tm);
    }
    {
@@ -701,6 +734,7 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       to_f32(P.Tsp64, P.Tsp32, s);
       to_f32(P.UTI64, P.UTI32, s);
       to_f32(P.UT64, P.UT32, s);
+      make_pipe(P.pipe, {&op}, s);
    }
 }
 
@@ -1320,15 +1354,15 @@ static ApplyArgs make_args(TreeOp& op, const double* xs, const double* xd, doubl
    a.ntask_d = P.ntask_d;
    a.tr_max_src = P.tr_max_src;
    a.task_d = P.task_d.p;
-   a.nblk_p = P.nblk_p;
-   a.mode_p = P.mode_p;
-   a.range_p = P.range_p.p;
-   a.task_p = P.task_p.p;
-   a.tiles_p = P.tiles_p.p;
-   a.oct_p = P.oct_p.p;
-   a.oacc_p = P.oacc_p.p;
-   a.gsrc_p = P.gsrc_p.p;
-   a.gint_p = P.gint_p.p;
+   a.nblk_p = P.pipe.nblk;
+   a.mode_p = P.pipe.mode;
+   a.range_p = P.pipe.range.p;
+   a.task_p = P.pipe.tasks.p;
+   a.tiles_p = P.pipe.tiles.p;
+   a.oct_p = P.pipe.octs.p;
+   a.oacc_p = P.pipe.oacc.p;
+   a.gsrc_p = P.pipe.gsrc.p;
+   a.gint_p = P.pipe.gint.p;
    a.tiles = P.tiles.p;
    a.gtgt = P.gtgt.p;
    a.gsrc = P.gsrc.p;

@@ -84,6 +84,32 @@ struct TreeOpStats {
    double phase_ms[8] = {0};
 };
 
+// Pipelined FP32 translation plan of one tree, host form: tiles whose
+// pointers are not set yet (t_off holds each tile's T offset in complex
+// elements), octets, octet outputs, source offsets, interaction words, and
+// the tasks (tile, first bin, first octet, octets) in list order with
+// their modelled cost.
+struct PipeHost {
+   std::vector<TpTile> tiles;
+   std::vector<long long> t_off;
+   std::vector<TpOct> octs;
+   std::vector<long long> oacc;
+   std::vector<int> gsrc, gint;
+   std::vector<int4> tasks;
+   std::vector<double> cost;
+};
+// The device form: nblk persistent blocks, block b takes the tasks
+// range[b]; one launch of k_translate_pipe.  It may cover several trees.
+struct PipeDev {
+   int nblk = 0, mode = 1;   // mode: 0 staging through registers, 1 direct global->LDS transfers
+   DBuf<int2> range;
+   DBuf<int4> tasks;
+   DBuf<TpTile> tiles;
+   DBuf<TpOct> octs;
+   DBuf<long long> oacc;
+   DBuf<int> gsrc, gint;
+};
+
 // Forward-apply plan of a tree: every level's blocks concatenated, with
 // absolute offsets, so each stage is one launch.
 struct ApplyPlan {
@@ -98,13 +124,8 @@ struct ApplyPlan {
    DBuf<TrTile> tiles;
    DBuf<GTgt> gtgt;
    DBuf<int> gsrc, gint;
-   int nblk_p = 0, ntask_p = 0, mode_p = 0;   // FP32 translation (k_translate_pipe)
-   DBuf<int2> range_p;
-   DBuf<int4> task_p;
-   DBuf<TpTile> tiles_p;
-   DBuf<TpOct> oct_p;
-   DBuf<long long> oacc_p;
-   DBuf<int> gsrc_p, gint_p;
+   PipeHost pipe_h;   // FP32 translation (k_translate_pipe) of this tree, host form
+   PipeDev pipe;      // ... on the device
    DBuf<int> map_leaf;   // XCD block map of the leaf launch
    int nmap_leaf = 0;
    DBuf<LeafU> ublk;
@@ -137,6 +158,15 @@ struct TreeOp {
    std::vector<int> cube_src_slot, cube_tgt_slot, cube_leaf;   // per cube: slot in its level / leaf index, -1
    ApplyPlan plan;
 };
+
+// The device plan of one launch covering the FP32 translations of the given
+// trees (each with its own spectra), tasks dealt to the persistent blocks.
+void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s);
+void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
+                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s);
+inline void launch_pipe(const PipeDev& d, hipStream_t s) {
+   launch_translate_pipe(d.nblk, d.mode, d.range.p, d.tasks.p, d.tiles.p, d.octs.p, d.oacc.p, d.gsrc.p, d.gint.p, s);
+}
 
 // Builds tree, near field and far field for `npan` panels (caller-order
 // geometry already on the device in `d_panels`) and `npt` points.
