@@ -186,45 +186,54 @@ class Problem:
     generateRHS -> solveSurfaceOperator -> collectPotentials."""
 
     def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, kappa=0.0, maxpanels=32, gridpoints=4,
-                 svdeps=1e-5, want_f32=True, cavities=None, stern_verts9=None):
+                 svdeps=1e-5, want_f32=True, cavities=None, stern_verts9=None, cavity_parent=None,
+                 salt_cavities=None, salt_cavity_parent=None):
+        """cavities: dielectric cavities inside the surface (panel arrays,
+        normals pointing out of the cavity); salt_cavities (Stern layout
+        only): cavities inside the dielectric cavities salt_cavity_parent.
+        A parent of 999 drops the cavity, as the reference does for surfaces
+        the mesher rejected."""
         v = _f64(verts9).reshape(-1, 9)
         self.qpos = _f64(qpos).reshape(-1, 3)
         self.q = _f64(q).reshape(-1)
         self.eps_in = eps_in
         h = _vp()
+        cv = [_f64(c).reshape(-1, 9) for c in (cavities or [])]
+        cp = np.array(cavity_parent if cavity_parent is not None else [0] * len(cv), dtype=np.int32)
+        nc = np.array([len(c) for c in cv] or [0], dtype=np.int32)
+        cp1 = cp if len(cp) else np.zeros(1, dtype=np.int32)
         if stern_verts9 is not None:
             # Stern layer: one salt (ion-exclusion) surface around the
             # dielectric (and its cavities); unknowns start with the salt's
             sv = _f64(stern_verts9).reshape(-1, 9)
-            cv = [_f64(c).reshape(-1, 9) for c in (cavities or [])]
-            allv = np.ascontiguousarray(np.vstack([sv, v] + cv))
+            lv = [_f64(c).reshape(-1, 9) for c in (salt_cavities or [])]
+            lp = np.array(salt_cavity_parent if salt_cavity_parent is not None else [0] * len(lv), dtype=np.int32)
+            nl = np.array([len(c) for c in lv] or [0], dtype=np.int32)
+            lp1 = lp if len(lp) else np.zeros(1, dtype=np.int32)
+            allv = np.ascontiguousarray(np.vstack([sv, v] + cv + lv))
             ns = np.array([len(sv)], dtype=np.int32)
             nd = np.array([len(v)], dtype=np.int32)
             dp = np.zeros(1, dtype=np.int32)
-            nc = np.array([len(c) for c in cv] or [0], dtype=np.int32)
-            cp = np.zeros(max(1, len(cv)), dtype=np.int32)
-            z = np.zeros(1, dtype=np.int32)
-            _check(lib().fftsvd_problem_create_stern(1, _i(ns), 1, _i(nd), _i(dp), len(cv), _i(nc), _i(cp), 0, _i(z),
-                                                     _i(z), _d(allv), _d(self.qpos), _d(self.q), len(self.q), eps_in,
-                                                     eps_out, kappa, maxpanels, gridpoints, svdeps, int(want_f32),
-                                                     C.byref(h)))
-            self.n = len(allv)
-        elif cavities:
-            cv = [_f64(c).reshape(-1, 9) for c in cavities]
+            _check(lib().fftsvd_problem_create_stern(1, _i(ns), 1, _i(nd), _i(dp), len(cv), _i(nc), _i(cp1), len(lv),
+                                                     _i(nl), _i(lp1), _d(allv), _d(self.qpos), _d(self.q), len(self.q),
+                                                     eps_in, eps_out, kappa, maxpanels, gridpoints, svdeps,
+                                                     int(want_f32), C.byref(h)))
+        elif cv:
+            if salt_cavities:
+                raise ValueError("salt cavities need the Stern layout (stern_verts9)")
             allv = np.ascontiguousarray(np.vstack([v] + cv))
             nd = np.array([len(v)], dtype=np.int32)
-            nc = np.array([len(c) for c in cv], dtype=np.int32)
-            par = np.zeros(len(cv), dtype=np.int32)
-            _check(lib().fftsvd_problem_create_multi(1, _i(nd), len(cv), _i(nc), _i(par), _d(allv), _d(self.qpos),
+            _check(lib().fftsvd_problem_create_multi(1, _i(nd), len(cv), _i(nc), _i(cp1), _d(allv), _d(self.qpos),
                                                      _d(self.q), len(self.q), eps_in, eps_out, kappa, maxpanels,
                                                      gridpoints, svdeps, int(want_f32), C.byref(h)))
-            self.n = len(allv)
         else:
             _check(lib().fftsvd_problem_create(_d(v), len(v), _d(self.qpos), _d(self.q), len(self.q), eps_in, eps_out,
                                                kappa, maxpanels, gridpoints, svdeps, int(want_f32), C.byref(h)))
-            self.n = len(v)
         self.h = h
-        self.nvec = 2 * self.n
+        nv = C.c_longlong()
+        _check(lib().fftsvd_problem_unknowns(h, C.byref(nv)))
+        self.nvec = nv.value
+        self.n = self.nvec // 2
 
     def close(self):
         if getattr(self, "h", None):

@@ -132,9 +132,18 @@ static SurfaceSpec make_spec(int ndiel, const int* npan_diel, int ncav, const in
       sp.dielectric_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_diel[i]);
       off += 9 * (size_t)npan_diel[i];
    }
+   // a cavity parent of 999 marks a surface the mesher rejected: readSRF
+   // takes its panels out of the count and generateSurfaceOperator attaches
+   // it to no region (FFTSVDpbeAPI.c:1006-1015,1441-1444)
    for (int j = 0; j < ncav; j++) {
-      sp.cavity_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_cav[j]);
-      off += 9 * (size_t)npan_cav[j];
+      const size_t n9 = 9 * (size_t)npan_cav[j];
+      if (cav_parent[j] == 999) {
+         off += n9;
+         continue;
+      }
+      if (cav_parent[j] < 0 || cav_parent[j] >= ndiel) throw std::runtime_error("cavity parent out of range");
+      sp.cavity_verts.emplace_back(verts9 + off, verts9 + off + n9);
+      off += n9;
       sp.cavity_parent.push_back(cav_parent[j]);
    }
    sp.qpos.assign(qpos, qpos + 3 * (size_t)ncharge);
@@ -193,11 +202,21 @@ int fftsvd_problem_create_stern(int nsalt, const int* npan_salt, int ndiel, cons
       if (diel_parent[i] < 0 || diel_parent[i] >= nsalt) throw std::runtime_error("dielectric parent out of range");
       rest.dielectric_parent.push_back(diel_parent[i]);
    }
+   // dielectric cavities kept by make_spec, renumbered without the dropped ones
+   std::vector<int> cav_index(ncav, -1);
+   for (int j = 0, kept = 0; j < ncav; j++)
+      if (cav_parent[j] != 999) cav_index[j] = kept++;
    for (int l = 0; l < nscav; l++) {
+      const size_t n9 = 9 * (size_t)npan_scav[l];
+      if (scav_parent[l] == 999) {   // rejected by the mesher (FFTSVDpbeAPI.c:1023-1026)
+         off += n9;
+         continue;
+      }
       if (scav_parent[l] < 0 || scav_parent[l] >= ncav) throw std::runtime_error("salt cavity parent out of range");
-      rest.salt_cavity_verts.emplace_back(verts9 + off, verts9 + off + 9 * (size_t)npan_scav[l]);
-      off += 9 * (size_t)npan_scav[l];
-      rest.salt_cavity_parent.push_back(scav_parent[l]);
+      if (cav_index[scav_parent[l]] < 0) throw std::runtime_error("salt cavity inside a dropped dielectric cavity");
+      rest.salt_cavity_verts.emplace_back(verts9 + off, verts9 + off + n9);
+      off += n9;
+      rest.salt_cavity_parent.push_back(cav_index[scav_parent[l]]);
    }
    auto* h = new fftsvd_problem;
    h->pr = build_problem(rest);

@@ -33,6 +33,8 @@ def _load():
         "orc_panel_geom": (None, [_dp, _dp]),
         "orc_gl4": (None, [_dp, _dp]),
         "orc_yukawa_rule_v9": (None, [_dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+        "orc_yukawa_line": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                   _dp, _dp]),
         "orc_dft_r2c": (None, [C.c_int, _dp, _dp]),
         "orc_dft_c2r": (None, [C.c_int, _dp, _dp]),
         "orc_pinv": (None, [_dp, C.c_int, C.c_int, _dp]),
@@ -43,6 +45,10 @@ def _load():
                                             C.c_double, C.c_int, C.c_int, C.c_double]),
         "orc_problem_stern": (C.c_void_p, [_dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_int, C.c_int, C.c_double]),
+        "orc_problem_layout": (C.c_void_p, [_dp, _ip, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                            C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                            C.c_double]),
+        "orc_problem_ntotal": (C.c_int, [C.c_void_p]),
         "orc_problem_free": (None, [C.c_void_p]),
         "orc_so_matvec": (None, [C.c_void_p, _dp, _dp]),
         "orc_rhs": (None, [C.c_void_p, _dp]),
@@ -131,6 +137,20 @@ def yukawa_rule(x, v9, kappa, nodes, weights):
     return s.value, d.value
 
 
+def yukawa_line(z, x, y1, y2, kappa, nodes=None, weights=None):
+    """calcp_LogIntDoLine's radial line integral (slp, dlp part) with the
+    4-point Gauss-Legendre rule or a given angular rule."""
+    if nodes is None:
+        nodes, weights = np.zeros(4), np.zeros(4)
+        lib().orc_gl4(_d(nodes), _d(weights))
+    s = C.c_double()
+    d = C.c_double()
+    xq = f64(nodes)
+    wq = f64(weights)
+    lib().orc_yukawa_line(z, x, y1, y2, kappa, len(xq), _d(xq), _d(wq), C.byref(s), C.byref(d))
+    return s.value, d.value
+
+
 def panel_geom(v9):
     out = np.zeros(39)
     lib().orc_panel_geom(_d(f64(v9)), _d(out))
@@ -197,6 +217,13 @@ class Tree:
     @property
     def depth(self):
         return lib().orc_tree_depth(self.h)
+
+    @property
+    def n(self):
+        """Panels of the tree (those of its root cube)."""
+        info = np.zeros(13, dtype=np.int32)
+        lib().orc_tree_cube_info(self.h, 0, _i(info))
+        return int(info[5])
 
     def gpl(self):
         g = np.zeros(self.depth, dtype=np.int32)
@@ -281,15 +308,40 @@ class Problem:
     """One dielectric surface with its charges (two-region operator), or,
     with `stern_verts9` (a salt / ion-exclusion surface enclosing it, kappa
     > 0), the three-region Stern-layer operator (FFTSVDpbeAPI.c:1064-1377);
-    unknowns then run salt phi, salt dphi, dielectric phi, dielectric dphi."""
+    unknowns then run salt phi, salt dphi, dielectric phi, dielectric dphi.
+
+    `cavities` (panel arrays) are dielectric cavities inside the dielectric
+    surface; `salt_cavities` (Stern layout only) sit inside the dielectric
+    cavities given by `salt_cavity_parent`.  A parent of 999 drops the
+    cavity, as readSRF does for surfaces the mesher marked bad
+    (FFTSVDpbeAPI.c:1006-1029).  Regions and unknowns follow
+    generateSurfaceOperator's layout for every branch."""
 
     def __init__(self, verts9, qpos, q, eps_in=4.0, eps_out=80.0, kappa=0.0, maxpanels=32, gridpoints=4,
-                 svdeps=1e-5, stern_verts9=None):
+                 svdeps=1e-5, stern_verts9=None, cavities=None, cavity_parent=None, salt_cavities=None,
+                 salt_cavity_parent=None):
         self.verts9 = f64(verts9).reshape(-1, 9)
         self.qpos = f64(qpos).reshape(-1, 3)
         self.q = f64(q).reshape(-1)
         self.eps_in = eps_in
-        if stern_verts9 is None:
+        if cavities or salt_cavities:
+            salts = [] if stern_verts9 is None else [f64(stern_verts9).reshape(-1, 9)]
+            cav = [f64(c).reshape(-1, 9) for c in (cavities or [])]
+            scav = [f64(c).reshape(-1, 9) for c in (salt_cavities or [])]
+            surf = salts + [self.verts9] + cav + scav
+            allv = np.ascontiguousarray(np.vstack(surf))
+            counts = np.array([len(x) for x in surf], dtype=np.int32)
+            dpar = np.zeros(1, dtype=np.int32)
+            cpar = np.array(cavity_parent if cavity_parent is not None else [0] * len(cav), dtype=np.int32)
+            spar = np.array(salt_cavity_parent if salt_cavity_parent is not None else [0] * len(scav),
+                            dtype=np.int32)
+            cpar = cpar if len(cpar) else np.zeros(1, dtype=np.int32)
+            spar = spar if len(spar) else np.zeros(1, dtype=np.int32)
+            self.h = lib().orc_problem_layout(_d(allv), _i(counts), len(salts), 1, len(cav), len(scav), _i(dpar),
+                                              _i(cpar), _i(spar), _d(self.qpos), _d(self.q), len(self.q), eps_in,
+                                              eps_out, kappa, int(bool(salts)), maxpanels, gridpoints, svdeps)
+            self.n = lib().orc_problem_ntotal(self.h)
+        elif stern_verts9 is None:
             self.n = len(self.verts9)
             self.h = lib().orc_problem_single(_d(self.verts9), self.n, _d(self.qpos), _d(self.q), len(self.q),
                                               eps_in, eps_out, kappa, maxpanels, gridpoints, svdeps)

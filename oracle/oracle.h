@@ -49,6 +49,9 @@ void orc_laplace(const double* x, const orc_panel* p, double* slp, double* dlp);
 void orc_yukawa(const double* x, const orc_panel* p, double kappa, double* slp, double* dlp);
 void orc_yukawa_rule(const double* x, const orc_panel* p, double kappa, int nq, const double* gx, const double* gw,
                      double* slp, double* dlp);
+/* calcp_LogIntDoLine's radial line integral (FlatIntegration.c:473-635) alone */
+void orc_yukawa_line(double z, double x, double y1, double y2, double kappa, int nq, const double* gx,
+                     const double* gw, double* sl, double* dl);
 void orc_yukawa_rule_v9(const double* x, const double* v9, double kappa, int nq, const double* gx, const double* gw,
                         double* slp, double* dlp);
 void orc_integrate(int kernel, double kappa, const double* x, const orc_panel* p, double* slp, double* dlp);
@@ -121,6 +124,7 @@ typedef struct orc_region {
    int mynpan;
    int startphi, startdphi;
    orc_tree* tree;            /* over own panels followed by children's panels */
+   orc_panel* pan_store;      /* the tree's panels when gathered from several surfaces, or NULL */
    int nchild;
    struct orc_region** child;
    struct orc_region* parent;
@@ -150,6 +154,18 @@ orc_problem* orc_problem_single(const double* verts9, int npan, const double* qp
 orc_problem* orc_problem_stern(const double* salt9, int nsalt, const double* diel9, int ndiel, const double* qpos,
                                const double* q, int ncharge, double eps_in, double eps_out, double kappa,
                                int maxpanels, int gridpoints, double svdeps);
+/* Every surface layout generateSurfaceOperator builds (FFTSVDpbeAPI.c:1064-1730):
+ * surfaces concatenated salt, dielectric, dielectric cavity, salt cavity in
+ * verts9 with counts[nsalt + ndiel + ncav + nscav]; diel_parent indexes the
+ * salt surfaces (Stern layout only), cav_parent the dielectrics,
+ * scav_parent the dielectric cavities; a parent of 999 drops the cavity
+ * (readSRF, FFTSVDpbeAPI.c:1006-1029).  stern != 0 selects the Stern branch
+ * (needs nsalt > 0 and kappa > 0); otherwise salt surfaces and salt
+ * cavities are ignored and kappa > 0 selects the salt branch. */
+orc_problem* orc_problem_layout(const double* verts9, const int* counts, int nsalt, int ndiel, int ncav, int nscav,
+                                const int* diel_parent, const int* cav_parent, const int* scav_parent,
+                                const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
+                                double kappa, int stern, int maxpanels, int gridpoints, double svdeps);
 void orc_problem_free(orc_problem* pr);
 void orc_so_matvec(orc_problem* pr, const double* x, double* y);
 void orc_rhs(orc_problem* pr, double* rhs);
@@ -173,7 +189,8 @@ void orc_tree_T(orc_tree* t, int level, int off, double* out);
 void orc_tree_set_cube_mats(orc_tree* t, int c, int rowrank, int colrank, const double* Ds, const double* Dd,
                             const double* VT, const double* U, const double* PVs, const double* PVd,
                             const double* UTI);
-orc_tree* orc_problem_tree(orc_problem* pr, int which /* depth along the first-child chain */);
+orc_tree* orc_problem_tree(orc_problem* pr, int which /* region number in pre-order, 0 = root */);
+int orc_problem_ntotal(orc_problem* pr);   /* panels in the operator; unknowns = twice that */
 /* M3 trees (accelerateM3, FFTSVDpbeAPI.c:1503-1513): per region with charges,
  * a Laplace tree from the region's panels to its charges; the reaction
  * potentials through it (SurfaceOperator.c:347-363). */

@@ -285,6 +285,12 @@ static void yuk_line(double z, double x, double y1, double y2, double kappa, int
    *dl = d;
 }
 
+/* The radial line integral alone, for the schedule tests. */
+void orc_yukawa_line(double z, double x, double y1, double y2, double kappa, int nq, const double* gx,
+                     const double* gw, double* sl, double* dl) {
+   yuk_line(z, x, y1, y2, kappa, nq, gx, gw, sl, dl);
+}
+
 /* One edge's contribution, observation point projected to the origin
  * (calcp_LogInt, FlatIntegration.c:637-799).  The in-plane rotation the
  * reference applies when the edge-origin plane is not already the xy plane

@@ -130,12 +130,137 @@ orc_problem* orc_problem_stern(const double* salt9, int nsalt, const double* die
    return pr;
 }
 
+/* Region whose tree runs over the panels ids[0 .. nids) of `all` (own
+ * surface first, then the surfaces of its children), gathered into a
+ * panel array the region owns (the concatenated Panel arrays
+ * generateSurfaceOperator passes to Tree_allocate). */
+static orc_region* gather_region(const orc_panel* all, const int* ids, int nids, int mynpan, int kernel, double eps,
+                                 double kappa, int maxpanels, int gridpoints, double svdeps) {
+   orc_panel* store = (orc_panel*)malloc(sizeof(orc_panel) * (size_t)(nids > 0 ? nids : 1));
+   for (int i = 0; i < nids; i++) store[i] = all[ids[i]];
+   orc_region* r = make_region(store, 0, nids, mynpan, kernel, eps, kappa, maxpanels, gridpoints, svdeps);
+   r->pan_store = store;
+   return r;
+}
+
+static void set_charges(orc_region* r, const double* qpos, const double* q, int ncharge) {
+   r->ncharge = ncharge;
+   r->qpos = (double*)malloc(sizeof(double) * 3 * (ncharge > 0 ? ncharge : 1));
+   r->q = (double*)malloc(sizeof(double) * (ncharge > 0 ? ncharge : 1));
+   memcpy(r->qpos, qpos, sizeof(double) * 3 * ncharge);
+   memcpy(r->q, q, sizeof(double) * ncharge);
+}
+
+/* Appends the global indices of surface k's panels to ids. */
+static int push_surface(int* ids, int n, const int* start, int k) {
+   for (int p = start[k]; p < start[k + 1]; p++) ids[n++] = p;
+   return n;
+}
+
+/* generateSurfaceOperator (FFTSVDpbeAPI.c:1064-1730), every branch:
+ * Stern (1064-1377): exterior Yukawa region over all salt panels; per salt
+ *   surface a Laplace region (eps_out) over its panels and its
+ *   dielectrics'; per dielectric a Laplace region (eps_in, the charges)
+ *   over its panels and its cavities'; per dielectric cavity a Laplace
+ *   region (eps_out) over its panels and its salt cavities'; per salt
+ *   cavity a Yukawa region (eps_out) over its own panels.
+ * Salt without Stern (1381-1554) and no salt (1557-1730): the exterior
+ *   region (Yukawa with salt, else Laplace; eps_out) over all dielectric
+ *   panels; per dielectric a Laplace region (eps_in, the charges) over its
+ *   panels and its cavities'; per cavity a region of the exterior kernel
+ *   over its own panels.
+ * Children are taken in surface order, so the pre-order start indices
+ * (SurfaceOperator_initStartIndices) follow the reference's. */
+orc_problem* orc_problem_layout(const double* verts9, const int* counts, int nsalt, int ndiel, int ncav, int nscav,
+                                const int* diel_parent, const int* cav_parent, const int* scav_parent,
+                                const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
+                                double kappa, int stern, int maxpanels, int gridpoints, double svdeps) {
+   const int nsurf = nsalt + ndiel + ncav + nscav;
+   int* start = (int*)calloc((size_t)nsurf + 1, sizeof(int));
+   for (int k = 0; k < nsurf; k++) start[k + 1] = start[k] + counts[k];
+   const int nall = start[nsurf];
+   orc_problem* pr = (orc_problem*)calloc(1, sizeof(orc_problem));
+   pr->panels = (orc_panel*)malloc(sizeof(orc_panel) * (size_t)(nall > 0 ? nall : 1));
+   orc_panels_from_verts(verts9, nall, pr->panels);
+   int* ids = (int*)malloc(sizeof(int) * (size_t)(nall > 0 ? nall : 1));
+   const int S0 = 0, D0 = nsalt, C0 = nsalt + ndiel, L0 = nsalt + ndiel + ncav;   /* surface index bases */
+   const int yuk = kappa > 0.0;
+   orc_region* root;
+   if (stern) {
+      int n = 0;
+      for (int i = 0; i < nsalt; i++) n = push_surface(ids, n, start, S0 + i);
+      root = gather_region(pr->panels, ids, n, 0, ORC_YUKAWA, eps_out, kappa, maxpanels, gridpoints, svdeps);
+      for (int i = 0; i < nsalt; i++) {
+         n = push_surface(ids, 0, start, S0 + i);
+         for (int j = 0; j < ndiel; j++)
+            if (diel_parent[j] == i) n = push_surface(ids, n, start, D0 + j);
+         orc_region* rs = gather_region(pr->panels, ids, n, counts[S0 + i], ORC_LAPLACE, eps_out, 0.0, maxpanels,
+                                        gridpoints, svdeps);
+         add_child(root, rs);
+         for (int j = 0; j < ndiel; j++) {
+            if (diel_parent[j] != i) continue;
+            n = push_surface(ids, 0, start, D0 + j);
+            for (int k = 0; k < ncav; k++)
+               if (cav_parent[k] == j) n = push_surface(ids, n, start, C0 + k);
+            orc_region* rd = gather_region(pr->panels, ids, n, counts[D0 + j], ORC_LAPLACE, eps_in, 0.0, maxpanels,
+                                           gridpoints, svdeps);
+            set_charges(rd, qpos, q, ncharge);
+            add_child(rs, rd);
+            for (int k = 0; k < ncav; k++) {
+               if (cav_parent[k] != j) continue;
+               n = push_surface(ids, 0, start, C0 + k);
+               for (int l = 0; l < nscav; l++)
+                  if (scav_parent[l] == k) n = push_surface(ids, n, start, L0 + l);
+               orc_region* rc = gather_region(pr->panels, ids, n, counts[C0 + k], ORC_LAPLACE, eps_out, 0.0,
+                                              maxpanels, gridpoints, svdeps);
+               add_child(rd, rc);
+               for (int l = 0; l < nscav; l++) {
+                  if (scav_parent[l] != k) continue;
+                  n = push_surface(ids, 0, start, L0 + l);
+                  add_child(rc, gather_region(pr->panels, ids, n, counts[L0 + l], ORC_YUKAWA, eps_out, kappa,
+                                              maxpanels, gridpoints, svdeps));
+               }
+            }
+         }
+      }
+   } else {
+      const int outer = yuk ? ORC_YUKAWA : ORC_LAPLACE;
+      const double ok = yuk ? kappa : 0.0;
+      int n = 0;
+      for (int j = 0; j < ndiel; j++) n = push_surface(ids, n, start, D0 + j);
+      root = gather_region(pr->panels, ids, n, 0, outer, eps_out, ok, maxpanels, gridpoints, svdeps);
+      for (int j = 0; j < ndiel; j++) {
+         n = push_surface(ids, 0, start, D0 + j);
+         for (int k = 0; k < ncav; k++)
+            if (cav_parent[k] == j) n = push_surface(ids, n, start, C0 + k);
+         orc_region* rd = gather_region(pr->panels, ids, n, counts[D0 + j], ORC_LAPLACE, eps_in, 0.0, maxpanels,
+                                        gridpoints, svdeps);
+         set_charges(rd, qpos, q, ncharge);
+         add_child(root, rd);
+         for (int k = 0; k < ncav; k++) {
+            if (cav_parent[k] != j) continue;
+            n = push_surface(ids, 0, start, C0 + k);
+            add_child(rd, gather_region(pr->panels, ids, n, counts[C0 + k], outer, eps_out, ok, maxpanels,
+                                        gridpoints, svdeps));
+         }
+      }
+   }
+   pr->root = root;
+   int index = 0;
+   init_start(root, &index);
+   pr->ntotal = index / 2;
+   free(ids);
+   free(start);
+   return pr;
+}
+
 static void region_free(orc_region* r) {
    if (!r) return;
    for (int i = 0; i < r->nchild; i++) region_free(r->child[i]);
    free(r->child);
    orc_tree_free(r->tree);
    orc_tree_free(r->m3);
+   free(r->pan_store);
    free(r->qpos); free(r->q); free(r->resint); free(r->resext);
    free(r);
 }
@@ -149,11 +274,23 @@ void orc_problem_free(orc_problem* pr) {
    free(pr);
 }
 
-/* which = depth along the first-child chain: 0 root, 1 its first child, ... */
+/* Region number `which` in pre-order (0 the root, then each child before
+ * its siblings' subtrees: the order generateSurfaceOperator creates them),
+ * or NULL. */
+static orc_region* region_preorder(orc_region* r, int* which) {
+   if ((*which)-- == 0) return r;
+   for (int c = 0; c < r->nchild; c++) {
+      orc_region* f = region_preorder(r->child[c], which);
+      if (f) return f;
+   }
+   return NULL;
+}
+
+int orc_problem_ntotal(orc_problem* pr) { return pr->ntotal; }
+
 orc_tree* orc_problem_tree(orc_problem* pr, int which) {
-   orc_region* r = pr->root;
-   for (int k = 0; k < which && r->nchild > 0; k++) r = r->child[0];
-   return r->tree;
+   orc_region* r = region_preorder(pr->root, &which);
+   return r ? r->tree : NULL;
 }
 
 /* SurfaceOperator_multiply (SurfaceOperator.c:114-158) with
@@ -410,9 +547,17 @@ static void region_react(orc_region* r, const double* sol, double* phi) {
    for (int k = 0; k < r->nchild; k++) region_react(r->child[k], sol, phi);
 }
 
+static int region_max_charges(const orc_region* r) {
+   int n = r->ncharge;
+   for (int c = 0; c < r->nchild; c++) {
+      int m = region_max_charges(r->child[c]);
+      if (m > n) n = m;
+   }
+   return n;
+}
+
 void orc_reaction_potentials(orc_problem* pr, const double* sol, double* phi) {
-   int nq = pr->root->child[0]->ncharge;
-   memset(phi, 0, sizeof(double) * nq);
+   memset(phi, 0, sizeof(double) * region_max_charges(pr->root));
    region_react(pr->root, sol, phi);
 }
 
@@ -462,15 +607,11 @@ static void region_m3_react(orc_region* r, const double* src, double* out) {
 }
 
 void orc_m3_potentials(orc_problem* pr, const double* sol, double* phi) {
-   int nq = 0;
-   for (orc_region* r = pr->root; r; r = r->nchild ? r->child[0] : NULL)
-      if (r->ncharge > nq) nq = r->ncharge;
-   memset(phi, 0, sizeof(double) * nq);
+   memset(phi, 0, sizeof(double) * region_max_charges(pr->root));
    region_m3_react(pr->root, sol, phi);
 }
 
 orc_tree* orc_problem_m3_tree(orc_problem* pr, int which) {
-   orc_region* r = pr->root;
-   for (int k = 0; k < which && r->nchild > 0; k++) r = r->child[0];
-   return r->m3;
+   orc_region* r = region_preorder(pr->root, &which);
+   return r ? r->m3 : NULL;
 }

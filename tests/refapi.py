@@ -141,15 +141,18 @@ def free(p):
     C.CDLL(None).free(C.cast(p, C.c_void_p))
 
 
-def write_srf(path, dielectric_names, salt_names=(), parents=None):
-    """.srf with flat surfaces: salt line, dielectric line, (empty) cavity
-    lines, dielectric parents and (empty) cavity parents
+def write_srf(path, dielectric_names, salt_names=(), parents=None, cavity_names=(), cavity_parents=(),
+              salt_cavity_names=(), salt_cavity_parents=()):
+    """.srf with flat surfaces: salt, dielectric, dielectric cavity and
+    salt cavity lines, then the dielectric, cavity and salt cavity parents
     (FFTSVDpbeAPI.c:776-1032 grammar)."""
     parents = parents if parents is not None else [0] * len(dielectric_names)
     with open(path, "w") as f:
         f.write("f\nf\n")
         f.write(" ".join(salt_names) + "\n")
         f.write(" ".join(dielectric_names) + "\n")
-        f.write("\n\n")
+        f.write(" ".join(cavity_names) + "\n")
+        f.write(" ".join(salt_cavity_names) + "\n")
         f.write(" ".join(str(p) for p in parents) + "\n")
-        f.write("\n\n")
+        f.write(" ".join(str(p) for p in cavity_parents) + "\n")
+        f.write(" ".join(str(p) for p in salt_cavity_parents) + "\n")

@@ -10,8 +10,12 @@ golden vectors for (the reference ships no tests):
   signs and integrands); with the reference's 4-point rule it stays within
   that rule's quadrature error; kappa -> 0 continuity to Laplace.
 * The 4-point rule equals Golub-Welsch (QuadratureRule.c:79-125).
-Bitwise subinterval boundaries of the 4-point scheme are restated, not
-pinned (parity unpinned at the 1e-4 level of that rule's own error).
+* The radial line's angular pieces (calcp_LogIntDoLine,
+  FlatIntegration.c:489-501,615-622): the 4-point line integral equals the
+  4-point rule on piece lists worked out by hand from that text, per case
+  to 1e-13, for every branch of the schedule.
+The split of an edge at the foot of the perpendicular (calcp_LogInt,
+FlatIntegration.c:637-799) is pinned only through the panel totals above.
 """
 import numpy as np
 import pytest
@@ -175,3 +179,66 @@ def test_yukawa_on_panel_and_kappa_continuity():
         assert abs(sy[0, 0] - sl[0, 0]) <= 5e-3 * abs(sl[0, 0])
         assert abs(dy[0, 0] - dlap[0, 0]) <= 5e-3 * max(abs(dlap[0, 0]), 1e-2)
         assert np.sign(dy[0, 0]) == np.sign(dlap[0, 0])
+
+
+# calcp_LogIntDoLine (FlatIntegration.c:489-501, 615-622) splits the angular
+# range of a radial line at distance x, from y1 to y2, into pieces: the first
+# ends at min(10 y1, y2) when x < y1, else at min(x, y2); each next piece
+# ends at ten times the previous end, or at y2 once that passes y2.  Piece
+# lists worked out by hand from that text, one per branch of the schedule:
+LINE_SCHEDULES = [
+    # (x, y1, y2, pieces)
+    (0.05, 0.02, 3.0, [(0.02, 0.05), (0.05, 0.5), (0.5, 3.0)]),       # x >= y1, growth then clip
+    (0.01, 0.02, 3.0, [(0.02, 0.2), (0.2, 2.0), (2.0, 3.0)]),         # x < y1, y2 > 10 y1
+    (0.01, 0.5, 3.0, [(0.5, 3.0)]),                                  # x < y1, y2 <= 10 y1
+    (5.0, 0.1, 3.0, [(0.1, 3.0)]),                                   # x beyond y2
+    (0.3, 0.0, 0.2, [(0.0, 0.2)]),                                   # from the foot, x beyond y2
+    (0.001, 0.0, 2.0, [(0.0, 0.001), (0.001, 0.01), (0.01, 0.1), (0.1, 1.0), (1.0, 2.0)]),   # four growth steps
+]
+
+
+def _line_by_pieces(z, x, pieces, kappa):
+    """The line integrand of calcp_LogIntDoLine, HELMHOLTZ kernel, integrated
+    with the 4-point Gauss-Legendre rule on each given piece."""
+    g, w = np.polynomial.legendre.leggauss(4)
+    g, w = 0.5 * (g + 1), 0.5 * w
+    az, sz = abs(z), np.sign(z)
+    s = d = 0.0
+    for a, b in pieces:
+        p1, p2 = np.arctan2(a, x), np.arctan2(b, x)
+        th = p1 + (p2 - p1) * g
+        R = np.sqrt((x / np.cos(th)) ** 2 + z * z)
+        s -= np.sum(w * (p2 - p1) * (np.exp(-kappa * R) - np.exp(-kappa * az)) / kappa)
+        d -= np.sum(w * (p2 - p1) * (sz * np.exp(-kappa * az) - z / R * np.exp(-kappa * R)))
+    return s, d
+
+
+@pytest.mark.parametrize("z", [0.3, -0.05])
+@pytest.mark.parametrize("case", range(len(LINE_SCHEDULES)))
+def test_yukawa_line_schedule(case, z):
+    """The 4-point line integral equals the 4-point rule on the hand-derived
+    pieces to rounding (1e-13), per case; shifting any piece boundary --
+    growth by 5 instead of 10, the first piece ending at x instead of
+    10 y1, one piece over the whole range -- moves it by far more, so a
+    changed split fails.  The integrand itself is pinned by the 16-point
+    brute-force check above."""
+    x, y1, y2, pieces = LINE_SCHEDULES[case]
+    kappa = 0.5
+    s, d = oracle.yukawa_line(z, x, y1, y2, kappa)
+    es, ed = _line_by_pieces(z, x, pieces, kappa)
+    assert abs(s - es) <= 1e-13 * abs(es)
+    assert abs(d - ed) <= 1e-13 * max(abs(ed), 1e-3)
+    # the same rule on other splits of [y1, y2]: one piece, and every
+    # interior boundary moved by a tenth of its value
+    alts = [[(y1, y2)]]
+    if len(pieces) > 1:
+        b = [p[0] for p in pieces] + [y2]
+        moved = [b[0]] + [v * 1.1 for v in b[1:-1]] + [b[-1]]
+        alts.append(list(zip(moved[:-1], moved[1:])))
+        for k in range(1, len(b) - 1):
+            one = list(b)
+            one[k] = 0.5 * (b[k - 1] + b[k])
+            alts.append(list(zip(one[:-1], one[1:])))
+    for alt in alts[1 if len(pieces) == 1 else 0:]:
+        als, ald = _line_by_pieces(z, x, alt, kappa)
+        assert abs(als - s) > 1e-10 * abs(s) or abs(ald - d) > 1e-10 * max(abs(d), 1e-3)

@@ -60,3 +60,23 @@ def test_parameter_globals_are_exported():
         refapi.set_global(name, 1.25)
         assert refapi.get_global(name) == 1.25
         refapi.set_global(name, 0.0)
+
+
+def test_readsrf_cavities_and_999(tmp_path):
+    """Cavity surfaces are read without the face flip (getFlatPanels,
+    FFTSVDpbeAPI.c:696-699); a cavity parent of 999 takes that cavity's
+    panels out of the total (FFTSVDpbeAPI.c:1006-1029)."""
+    Vd, Fd = meshes.icosphere(3.0, 2)
+    Vc, Fc = meshes.icosphere(1.0, 1)
+    Vb, Fb = meshes.icosphere(0.5, 0, center=(1.5, 0.0, 0.0))
+    for name, V, F in (("diel", Vd, Fd), ("cav", Vc, Fc[:, [0, 2, 1]]), ("bad", Vb, Fb[:, [0, 2, 1]])):
+        meshes.write_vertface(str(tmp_path / name), V, F)
+    refapi.write_srf(tmp_path / "op.srf", ["diel"], cavity_names=["cav", "bad"], cavity_parents=[0, 999])
+    s = refapi.read_srf(tmp_path / "op.srf")
+    assert s.num[1].value == 1 and s.num[2].value == 2 and s.num[3].value == 0
+    assert s.total.value == len(Fd) + len(Fc)
+    assert s.parents[1][0] == 0 and s.parents[1][1] == 999
+    # cavity faces as written (inward CCW) become panels whose normals
+    # point out of the cavity: the same panels as the flipped outward mesh
+    Vw = np.loadtxt(tmp_path / "cav.vert")
+    np.testing.assert_allclose(panels_of(s, 2, 0), meshes.panels_from_mesh(Vw, Fc), rtol=0, atol=1e-12)
