@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libfftsvd_hip.so")
+# FFTSVD_LIBRARY selects another build of the same library (the
+# host-sanitized one in _lib/asan/, for the sanitizer runs)
+LIB_PATH = os.environ.get("FFTSVD_LIBRARY") or os.path.join(HERE, "_lib", "libfftsvd_hip.so")
 HEADERS = [os.path.join(os.path.dirname(HERE), "include", h) for h in ("FFTSVDpbeAPI.h", "fftsvd_hip.h")]
 
 _dp = C.POINTER(C.c_double)
