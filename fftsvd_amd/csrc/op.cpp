@@ -6,6 +6,8 @@
 // that fold them into PV / UTI.  The host decides the tree, the sampling
 // pattern and the per-level pseudoinverses and translation spectra.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <exception>
 #include <thread>
 #include <array>
@@ -320,18 +322,116 @@ static void octet_words(const std::vector<std::vector<unsigned>>& lists, int nk4
 // length and cut into octets of eight, longest first (the consumer waves
 // claim them in this order); tasks (tile, 8-bin chunk, first octet, octets)
 // listed by level, then chunk, then tile, and dealt to the persistent
-// blocks as described below.
+// blocks as described below.  The tiles are cut level by level, then
+// worked out on up to 16 host threads (each tile's octets and words are
+// independent) and concatenated in tile order, so the plan does not
+// depend on the thread count.
+namespace {
+struct PipeTileJob {
+   int level, k0, k1;
+   std::vector<int> so;        // the tile's sources (absolute spectrum offsets)
+   // results
+   std::vector<TpOct> octs;    // int_off relative to words, acc_idx relative to oacc
+   std::vector<long long> oacc;
+   std::vector<unsigned> words;
+   int cost = 0;
+};
+}  // namespace
+
+static void pipe_tile_job(PipeTileJob& J, const TrLevel& L, const std::vector<TgtPlan>& tgt,
+                          const std::vector<int>& isrc, const std::vector<int>& itoff, std::vector<int>& loc) {
+   auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
+   for (size_t i = 0; i < J.so.size(); i++) loc[slot_of(J.so[i])] = (int)i;
+   const int n = J.k1 - J.k0;
+   std::vector<int> order(n);
+   for (int k = 0; k < n; k++) order[k] = k;
+   std::stable_sort(order.begin(), order.end(),
+                    [&](int x, int y) { return tgt[L.t0 + J.k0 + x].nint > tgt[L.t0 + J.k0 + y].nint; });
+   std::vector<unsigned> ow;
+   std::vector<std::vector<unsigned>> lists;
+   for (int o0 = 0; o0 < n; o0 += 8) {
+      lists.clear();
+      TpOct o;
+      o.acc_idx = (int)J.oacc.size();
+      o.pad_ = 0;
+      int longest = 0;
+      for (int t8 = 0; t8 < 8; t8++) {
+         if (o0 + t8 >= n) {
+            J.oacc.push_back(-1);
+            continue;
+         }
+         const TgtPlan& tp = tgt[L.t0 + J.k0 + order[o0 + t8]];
+         std::vector<unsigned> lw(tp.nint);
+         for (int e = 0; e < tp.nint; e++)
+            lw[e] = (unsigned)(itoff[tp.int_off + e] * 64) |
+                    ((unsigned)(loc[slot_of(isrc[tp.int_off + e])] * 64) << 16);
+         longest = std::max(longest, tp.nint);
+         lists.push_back(std::move(lw));
+         J.oacc.push_back(tp.acc_off);
+      }
+      o.nk = std::max(8, (longest + 7) / 8 * 8);
+      octet_words(lists, o.nk, ow);
+      while (J.words.size() % 4) J.words.push_back(TP_NULL_WORD);
+      o.int_off = (int)J.words.size();
+      J.words.insert(J.words.end(), ow.begin(), ow.end());
+      J.octs.push_back(o);
+   }
+   // modelled cost of a task of this tile in steps per consumer wave
+   double steps = 0;
+   for (const TpOct& o : J.octs) steps += o.nk;
+   J.cost = (int)(steps / TP_CW + (J.octs.empty() ? 0 : J.octs[0].nk));
+}
+
 static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
                                  const std::vector<int>& isrc, const std::vector<int>& itoff) {
    PipeHost& H = P.pipe_h;
    H = PipeHost();
-   std::vector<unsigned> words;
-   for (const TrLevel& L : lvls) {
-      std::vector<int4> lt;   // per tile of the level: index, first octet, octets, modelled cost
+   std::vector<PipeTileJob> jobs;
+   int maxsrc = 1;
+   for (int li = 0; li < (int)lvls.size(); li++) {
+      maxsrc = std::max(maxsrc, lvls[li].nsrc);
       make_tiles(
-          L, tgt, isrc, TP_TILE, TP_SRC_BUDGET, TP_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
-         std::unordered_map<int, int> loc;
-         for (size_t i = 0; i < so.size(); i++) loc[so[i]] = (int)i;
+          lvls[li], tgt, isrc, TP_TILE, TP_SRC_BUDGET, TP_MAX_SRC,
+          [&](int k0, int k1, const std::vector<int>& so) {
+             PipeTileJob J;
+             J.level = li;
+             J.k0 = k0;
+             J.k1 = k1;
+             J.so = so;
+             jobs.push_back(std::move(J));
+          },
+          TP_QUANTUM);
+   }
+   {
+      const int nth = (int)std::max<size_t>(1, std::min<size_t>({16u, std::thread::hardware_concurrency(),
+                                                                   (jobs.size() + 1) / 2}));
+      std::atomic<int> next{0};
+      std::exception_ptr err;
+      std::mutex em;
+      auto work = [&] {
+         std::vector<int> loc(maxsrc, 0);
+         for (int j; (j = next.fetch_add(1)) < (int)jobs.size();) {
+            try {
+               pipe_tile_job(jobs[j], lvls[jobs[j].level], tgt, isrc, itoff, loc);
+            } catch (...) {
+               std::lock_guard<std::mutex> g(em);
+               if (!err) err = std::current_exception();
+            }
+         }
+      };
+      std::vector<std::thread> th;
+      for (int t = 1; t < nth; t++) th.emplace_back(work);
+      work();
+      for (auto& x : th) x.join();
+      if (err) std::rethrow_exception(err);
+   }
+   // concatenation in tile order; tasks by level, then chunk, then tile
+   size_t j0 = 0;
+   for (int li = 0; li < (int)lvls.size(); li++) {
+      const TrLevel& L = lvls[li];
+      std::vector<int4> lt;   // per tile of the level: index, first octet, octets, modelled cost
+      for (; j0 < jobs.size() && jobs[j0].level == li; j0++) {
+         PipeTileJob& J = jobs[j0];
          TpTile t;
          t.T = nullptr;
          t.spec = nullptr;
@@ -339,59 +439,25 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
          t.src_off = (int)H.gsrc.size();
          t.pad = L.pad;
          t.pst = L.pst;
-         if (so.empty())
+         if (J.so.empty())
             H.gsrc.push_back(0);   // a readable row for the producers' clamped loads
          else
-            H.gsrc.insert(H.gsrc.end(), so.begin(), so.end());
-         t.nsrc = std::max(1, (int)so.size());
-         // targets by list length, longest first, cut into octets
-         const int n = k1 - k0;
-         std::vector<int> order(n);
-         for (int k = 0; k < n; k++) order[k] = k;
-         std::stable_sort(order.begin(), order.end(),
-                          [&](int x, int y) { return tgt[L.t0 + k0 + x].nint > tgt[L.t0 + k0 + y].nint; });
-         std::vector<TpOct> to;
-         for (int o0 = 0; o0 < n; o0 += 8) {
-            std::vector<std::vector<unsigned>> lists;
-            TpOct o;
-            o.acc_idx = (int)H.oacc.size();
-            o.pad_ = 0;
-            int longest = 0;
-            for (int t8 = 0; t8 < 8; t8++) {
-               if (o0 + t8 >= n) {
-                  H.oacc.push_back(-1);
-                  continue;
-               }
-               const TgtPlan& tp = tgt[L.t0 + k0 + order[o0 + t8]];
-               std::vector<unsigned> lw;
-               for (int e = 0; e < tp.nint; e++)
-                  lw.push_back((unsigned)(itoff[tp.int_off + e] * 64) |
-                               ((unsigned)(loc.at(isrc[tp.int_off + e]) * 64) << 16));
-               longest = std::max(longest, tp.nint);
-               lists.push_back(lw);
-               H.oacc.push_back(tp.acc_off);
-            }
-            o.nk = std::max(8, (longest + 7) / 8 * 8);
-            octet_words(lists, o.nk, words);
-            while (H.gint.size() % 4) H.gint.push_back((int)TP_NULL_WORD);
-            o.int_off = (int)H.gint.size();
-            for (unsigned w : words) H.gint.push_back((int)w);
-            to.push_back(o);
-         }
-         // octets in the task's order (longest first: the consumers claim
-         // them in turn); modelled cost of a task in steps per consumer wave
-         const int oct0 = (int)H.octs.size();
-         double steps = 0;
-         for (const TpOct& o : to) {
+            H.gsrc.insert(H.gsrc.end(), J.so.begin(), J.so.end());
+         t.nsrc = std::max(1, (int)J.so.size());
+         while (H.gint.size() % 4) H.gint.push_back((int)TP_NULL_WORD);
+         const int wbase = (int)H.gint.size(), abase = (int)H.oacc.size(), oct0 = (int)H.octs.size();
+         H.gint.insert(H.gint.end(), J.words.begin(), J.words.end());
+         H.oacc.insert(H.oacc.end(), J.oacc.begin(), J.oacc.end());
+         for (TpOct o : J.octs) {
+            o.int_off += wbase;
+            o.acc_idx += abase;
             H.octs.push_back(o);
-            steps += o.nk;
          }
-         const int tcost = (int)(steps / TP_CW + (to.empty() ? 0 : to[0].nk));
-         lt.push_back(make_int4((int)H.tiles.size(), oct0, (int)to.size(), tcost));
+         lt.push_back(make_int4((int)H.tiles.size(), oct0, (int)J.octs.size(), J.cost));
          H.tiles.push_back(t);
          H.t_off.push_back(L.t_off);
-          },
-          TP_QUANTUM);
+         std::vector<unsigned>().swap(J.words);
+      }
       for (int b0 = 0; b0 < L.pad; b0 += 8)
          for (const int4& e : lt) {
             H.tasks.push_back(make_int4(e.x, b0, e.y, e.z));
