@@ -21,7 +21,7 @@ _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
 F64, MIXED = 0, 1
-LAPLACE, YUKAWA = 0, 1
+LAPLACE, YUKAWA, YUKAWA_DKAPPA = 0, 1, 2
 
 _SIGS = {
     "fftsvd_last_error": (C.c_char_p, []),
@@ -44,6 +44,7 @@ _SIGS = {
     "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
+    "fftsvd_problem_grad_kappa": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
     "fftsvd_problem_grad_charges": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
     "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                      C.c_double, C.c_int, C.POINTER(_vp)]),
@@ -311,6 +312,15 @@ class Problem:
         it = C.c_int()
         _check(lib().fftsvd_problem_grad_charges(self.h, precision, _d(_f64(sol)), tol, maxit, _d(g), C.byref(it)))
         return g[:len(self.q)] * conversion / self.eps_in
+
+    def grad_kappa(self, sol, tol=1e-6, maxit=1000, precision=F64, conversion=332.112):
+        """d(energy)/d(kappa) at the forward solution sol, in the units of
+        energy(): conversion/eps_in * (-1/2 lambda^T dA/dkappa x)."""
+        g = C.c_double()
+        it = C.c_int()
+        _check(lib().fftsvd_problem_grad_kappa(self.h, precision, _d(_f64(sol)), tol, maxit, C.byref(g),
+                                               C.byref(it)))
+        return g.value * conversion / self.eps_in
 
     # many charge vectors at once: arrays are (rows, ncol), one column per vector
     def rhs_batch(self, Q):

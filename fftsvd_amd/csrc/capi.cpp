@@ -444,6 +444,16 @@ int fftsvd_problem_grad_charges(fftsvd_problem* p, int precision, const double* 
    API_CATCH
 }
 
+int fftsvd_problem_grad_kappa(fftsvd_problem* p, int precision, const double* sol, double tol, int maxit,
+                              double* dedk, int* iters) {
+   API_TRY
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   if (!dedk) throw std::runtime_error("null output");
+   SolveInfo si = problem_grad_kappa(*p->pr, precision, sol, tol, maxit, dedk);
+   if (iters) *iters = si.iterations;
+   API_CATCH
+}
+
 int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi) {
    API_TRY
    if (ncol < 1) throw std::runtime_error("ncol must be positive");
@@ -701,6 +711,7 @@ int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int
 int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
                      double* slp, double* dlp) {
    API_TRY
+   if (kernel < 0 || kernel > 2) throw std::runtime_error("kernel must be 0 (Laplace), 1 (Yukawa) or 2 (d/dkappa)");
    std::vector<PanelGeom> g(npan);
    for (int i = 0; i < npan; i++) panel_geometry(verts9 + 9 * (size_t)i, verts9 + 9 * (size_t)i + 3, verts9 + 9 * (size_t)i + 6, &g[i]);
    hipStream_t s;

@@ -129,7 +129,9 @@ void translation_spectra(int n, double h, int kernel, double kappa, std::complex
                      int lk = (k > N / 2) ? N - k : -k;
                      double p0 = h * (sh[0] - li), p1 = h * (sh[1] - lj), p2 = h * (sh[2] - lk);
                      double r = std::sqrt(p0 * p0 + p1 * p1 + p2 * p2);
-                     double g = (kernel == 1) ? std::exp(-kappa * r) / r : (r == 0.0 ? 0.0 : 1.0 / r);
+                     double g = (kernel == 1)   ? std::exp(-kappa * r) / r
+                                : (kernel == 2) ? -std::exp(-kappa * r)   // d/dkappa of e^{-kr}/r
+                                                : (r == 0.0 ? 0.0 : 1.0 / r);
                      data[((size_t)i * N + j) * N + k] = g;
                   }
                }

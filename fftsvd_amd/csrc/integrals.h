@@ -228,19 +228,126 @@ __device__ __forceinline__ void yukawa_integrals(const double* __restrict__ x, c
    dlp = D;
 }
 
+// kappa-derivative of yukawa_line's two sums: the same pieces and nodes,
+// the integrands differentiated:
+//   -(e^{-kR} - e^{-k|z|})/k          -> (R e^{-kR} - |z| e^{-k|z|})/k + (e^{-kR} - e^{-k|z|})/k^2
+//   -(sgn z e^{-k|z|} - (z/R) e^{-kR}) -> z (e^{-k|z|} - e^{-kR})
+__device__ __forceinline__ void yukawa_line_dk(double z, double x, double y1, double y2, double kappa, double& sl,
+                                               double& dl) {
+#pragma clang fp contract(off)
+   double gx[4], gw[4];
+   gl4_rule(gx, gw);
+   double az = z > 0.0 ? z : -z;
+   if (x < 0) x = -x;
+   double start = y1, end;
+   if (x < y1)
+      end = (y2 > 10.0 * y1) ? 10.0 * y1 : y2;
+   else
+      end = (x > y2) ? y2 : x;
+   double ekz = exp(-kappa * az);
+   double ik = 1.0 / kappa;
+   double ik2 = 1.0 / (kappa * kappa);
+   double s = 0.0, d = 0.0;
+   int guard = 0;
+   while (start < y2 && guard < 64) {
+      guard++;
+      double ph1 = atan2(start, x), ph2 = atan2(end, x);
+      double dph = ph2 - ph1;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+         double th = ph1 + (ph2 - ph1) * gx[i];
+         double q = x * (1.0 / cos(th));
+         double R = sqrt(q * q + z * z);
+         double ekr = exp(-kappa * R);
+         double wd = gw[i] * dph;
+         s = s + wd * ((R * ekr - az * ekz) * ik + (ekr - ekz) * ik2);
+         d = d + wd * z * (ekz - ekr);
+      }
+      start = end;
+      if (end == y2)
+         end = 10.0 * y2;
+      else if (end * 10.0 > y2)
+         end = y2;
+      else
+         end = 10.0 * end;
+   }
+   sl = s;
+   dl = d;
+}
+
+// kappa-derivative of yukawa_integrals (the 4-point scheme differentiated
+// term by term; the on-plane double layer, a constant, has derivative 0).
+__device__ __forceinline__ void yukawa_dk_integrals(const double* __restrict__ x, const PanelGeom& p, double kappa,
+                                                    double& dslp, double& ddlp) {
+#pragma clang fp contract(off)
+   double r0 = x[0] - p.c[0], r1 = x[1] - p.c[1], r2 = x[2] - p.c[2];
+   double pn[3];
+   pn[0] = p.axn[0] * r0 + p.axn[1] * r1 + p.axn[2] * r2;
+   pn[1] = p.axn[3] * r0 + p.axn[4] * r1 + p.axn[5] * r2;
+   pn[2] = p.axn[6] * r0 + p.axn[7] * r1 + p.axn[8] * r2;
+   double S = 0.0, D = 0.0;
+   for (int j = 0; j < 3; j++) {
+      int nx = (j == 2) ? 0 : j + 1;
+      const double* en = p.en + 3 * j;
+      double dd = (pn[0] * en[0] + pn[1] * en[1] + pn[2] * en[2]) - p.erhs[j];
+      double sg = ((dd > 0.0 && p.eov[j] > 0.0) || (dd < 0.0 && p.eov[j] < 0.0)) ? 1.0 : -1.0;
+      const double* a = p.pvn + 3 * j;
+      const double* b = p.pvn + 3 * nx;
+      double u0 = a[0] - pn[0], u1 = a[1] - pn[1];
+      double w0 = b[0] - pn[0], w1 = b[1] - pn[1];
+      double z = pn[2] - a[2];
+      double dv0 = u0 - w0, dv1 = u1 - w1;
+      double L = sqrt(dv0 * dv0 + dv1 * dv1);
+      double il = 1.0 / L;
+      double c = dv0 * il, s = dv1 * il;
+      double ux = s * u0 - c * u1, uy = c * u0 + s * u1;
+      double wy = c * w0 + s * w1;
+      double sl = 0.0, dl = 0.0;
+      if (!(fabs(ux) < 10 * 2.220446049250313e-16)) {
+         double y1 = wy, y2 = uy;
+         if (y2 < 0 && y1 < 0) {
+            y1 = -y1;
+            y2 = -y2;
+         }
+         if (y2 < y1) {
+            double t = y1;
+            y1 = y2;
+            y2 = t;
+         }
+         double st = (y1 >= 0) ? y1 : 0;
+         yukawa_line_dk(z, ux, st, y2, kappa, sl, dl);
+         if (y1 < 0.0) {
+            double s2, d2;
+            yukawa_line_dk(z, ux, 0.0, -y1, kappa, s2, d2);
+            sl += s2;
+            dl += d2;
+         }
+      }
+      S += sg * sl;
+      D += sg * dl;
+   }
+   if (fabs(pn[2]) < 10 * 1.1920928955078125e-07) D = 0.0;
+   dslp = S;
+   ddlp = D;
+}
+
 __device__ __forceinline__ void panel_integrals(int kernel, double kappa, const double* x, const PanelGeom& p,
                                                 double& slp, double& dlp) {
    if (kernel == YUKAWA)
       yukawa_integrals(x, p, kappa, slp, dlp);
+   else if (kernel == YUKAWA_DKAPPA)
+      yukawa_dk_integrals(x, p, kappa, slp, dlp);
    else
       laplace_integrals(x, p, slp, dlp);
 }
 
-// Point kernels (GreensFunction.c:74-80,115-118): 1/r (0 at r = 0), e^{-kr}/r.
+// Point kernels (GreensFunction.c:74-80,115-118): 1/r (0 at r = 0), e^{-kr}/r,
+// and the kappa-derivative of the latter, -e^{-kr}.
 __device__ __forceinline__ double green(int kernel, double kappa, const double* a, const double* b) {
    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
    double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
    if (kernel == YUKAWA) return exp(-kappa * r) / r;
+   if (kernel == YUKAWA_DKAPPA) return -exp(-kappa * r);   // d/dkappa of e^{-kr}/r
    if (r == 0.0) return 0.0;
    return 1.0 / r;
 }

@@ -15,7 +15,9 @@
 
 namespace fsvd {
 
-enum Kernel { LAPLACE = 0, YUKAWA = 1 };
+// YUKAWA_DKAPPA: d/dkappa of the Yukawa panel integrals (gradients only;
+// no operator is built with it)
+enum Kernel { LAPLACE = 0, YUKAWA = 1, YUKAWA_DKAPPA = 2 };
 enum Precision { PREC_F64 = 0, PREC_F32 = 1 };
 
 #define FSVD_CHECK(x)                                                                                  \

@@ -804,11 +804,165 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    }
 }
 
+// Equivalent densities of one level (Cube_fill_P_I + Cube_fill_PV_UTI,
+// Cube.c:795-1002,1414-1522): P = pinv Q2S(panels -> sphere) (both layers),
+// PV^T = VT P^T; I = (pinv Q2S(sphere -> points))^T, UTI = U^T I, for the
+// level's source and target cubes cl, with the sphere integrals and point
+// kernel of `kernel`; results in the level's PV and UTI layouts.
+static void level_pv_uti(const Tree& t, const std::vector<int>& cl, const FarLevel& lv,
+                         const std::vector<double>& ptsT, const PanelGeom* d_panels, const double* pinvp, int kernel,
+                         double kappa, const double* VT, const double* UT, double* PV, double* UTI, hipStream_t s) {
+   const int gp3 = lv.gp3;
+   std::vector<IntBlock> blocks;
+   std::vector<double> bpts(ptsT);
+   std::vector<double> sph;   // sphere points per cube (target side, Green blocks)
+   std::vector<GreenBlock> gblocks;
+   long long qs = 0, ps = 0, qp = 0, ip = 0;
+   struct Aux {
+      long long q_off, p_off;
+   };
+   std::vector<Aux> saux(lv.h_src.size()), taux(lv.h_tgt.size());
+   size_t si = 0, ti = 0;
+   for (size_t ci = 0; ci < cl.size(); ci++) {
+      const Cube& c = t.cubes[cl[ci]];
+      double box = c.b1[0] - c.b0[0];
+      double sc = box * 1.5 * 1;
+      bool is_src = c.np > 0 && c.rowrank > 0, is_tgt = c.nq > 0 && c.colrank > 0;
+      if (!is_src && !is_tgt) continue;
+      int sbase = (int)(bpts.size() / 3);
+      for (int q = 0; q < NQ; q++)
+         for (int a = 0; a < 3; a++) bpts.push_back(SPHERE49[q][a] * sc + c.ctr[a]);
+      if (is_src) {
+         IntBlock b;
+         b.npt = NQ;
+         b.npan = c.np;
+         b.pt_off = sbase;
+         b.pan_off = c.p0;
+         b.out_off = qs;
+         b.ld_pt = c.np;
+         b.ld_pan = 1;
+         b.layer_stride = (long long)NQ * c.np;
+         push_blocks(blocks, b);
+         saux[si] = {qs, ps};
+         qs += 2LL * NQ * c.np;
+         ps += 2LL * gp3 * c.np;
+         si++;
+      }
+      if (is_tgt) {
+         GreenBlock g;
+         g.ndest = c.nq;
+         g.nsrc = NQ;
+         g.dest_off = c.q0;
+         g.src_off = sbase;
+         g.out_off = qp;
+         g.ld_dest = 1;
+         g.ld_src = c.nq;
+         gblocks.push_back(g);
+         taux[ti] = {qp, ip};
+         qp += (long long)NQ * c.nq;
+         ip += (long long)gp3 * c.nq;
+         ti++;
+      }
+   }
+   DBuf<double> Qs, Pb, Qpb, Ib;
+   Qs.alloc(std::max<long long>(qs, 1));
+   Pb.alloc(std::max<long long>(ps, 1));
+   Qpb.alloc(std::max<long long>(qp, 1));
+   Ib.alloc(std::max<long long>(ip, 1));
+   run_integrals(blocks, bpts, t.pan_perm, d_panels, kernel, kappa, Qs.p, s);
+   if (!gblocks.empty()) {
+      DBuf<GreenBlock> dg;
+      DBuf<double> dp;
+      dg.upload(gblocks, s);
+      dp.upload(bpts, s);
+      launch_green_blocks((int)gblocks.size(), dg.p, dp.p, dp.p, kernel, kappa, Qpb.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   // P = pinv * Q2S (both layers); I^T = (pinv * Q2S_points)^T
+   std::vector<GemmDesc> g1, g2;
+   for (size_t k = 0; k < lv.h_src.size(); k++) {
+      const SrcDesc& sd = lv.h_src[k];
+      for (int layer = 0; layer < 2; layer++) {
+         GemmDesc d;
+         d.M = gp3;
+         d.N = sd.np;
+         d.K = NQ;
+         d.A = pinvp;
+         d.a_rs = NQ;
+         d.a_cs = 1;
+         d.B = Qs.p + saux[k].q_off + (long long)layer * NQ * sd.np;
+         d.b_rs = sd.np;
+         d.b_cs = 1;
+         d.C = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
+         d.c_rs = sd.np;
+         d.c_cs = 1;
+         g1.push_back(d);
+      }
+   }
+   for (size_t k = 0; k < lv.h_tgt.size(); k++) {
+      const TgtDesc& td = lv.h_tgt[k];
+      GemmDesc d;
+      d.M = td.nq;
+      d.N = gp3;
+      d.K = NQ;
+      d.A = Qpb.p + taux[k].q_off;
+      d.a_rs = 1;
+      d.a_cs = td.nq;
+      d.B = pinvp;
+      d.b_rs = 1;
+      d.b_cs = NQ;
+      d.C = Ib.p + taux[k].p_off;
+      d.c_rs = gp3;
+      d.c_cs = 1;
+      g1.push_back(d);
+   }
+   run_gemms(g1, s);
+   // PV^T (r x gp3) = VT (r x np) * P^T ; UTI (r x gp3) = U^T (r x nq) * I (nq x gp3)
+   for (size_t k = 0; k < lv.h_src.size(); k++) {
+      const SrcDesc& sd = lv.h_src[k];
+      for (int layer = 0; layer < 2; layer++) {
+         GemmDesc d;
+         d.M = sd.r;
+         d.N = gp3;
+         d.K = sd.np;
+         d.A = VT + sd.vt_off;
+         d.a_rs = sd.np;
+         d.a_cs = 1;
+         d.B = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
+         d.b_rs = 1;
+         d.b_cs = sd.np;
+         d.C = PV + sd.pv_off + (long long)layer * sd.r * gp3;
+         d.c_rs = gp3;
+         d.c_cs = 1;
+         g2.push_back(d);
+      }
+   }
+   for (size_t k = 0; k < lv.h_tgt.size(); k++) {
+      const TgtDesc& td = lv.h_tgt[k];
+      GemmDesc d;
+      d.M = td.r;
+      d.N = gp3;
+      d.K = td.nq;
+      d.A = UT + td.u_off;
+      d.a_rs = td.nq;
+      d.a_cs = 1;
+      d.B = Ib.p + taux[k].p_off;
+      d.b_rs = gp3;
+      d.b_cs = 1;
+      d.C = UTI + td.uti_off;
+      d.c_rs = gp3;
+      d.c_cs = 1;
+      g2.push_back(d);
+   }
+   run_gemms(g2, s);
+}
+
 std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s,
                                       const Tree* pre, int layers) {
    if (layers != 1 && layers != 2) throw std::runtime_error("layers must be 1 (double) or 2 (single and double)");
+   if (kernel != LAPLACE && kernel != YUKAWA) throw std::runtime_error("operators are built for the Laplace and Yukawa kernels");
    auto opp = std::make_unique<TreeOp>();
    TreeOp& op = *opp;
    op.kernel = kernel;
@@ -1147,150 +1301,7 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
 
       timing_mark((std::string("L") + std::to_string(l) + " " + "compaction").c_str(), tl);
       // ---- equivalent densities: PV = P VT^T, UTI = U^T I (Cube.c:795-1002,1414-1522) ----
-      {
-         std::vector<IntBlock> blocks;
-         std::vector<double> bpts(ptsT);
-         std::vector<double> sph;   // sphere points per cube (target side, Green blocks)
-         std::vector<GreenBlock> gblocks;
-         long long qs = 0, ps = 0, qp = 0, ip = 0;
-         struct Aux {
-            long long q_off, p_off;
-         };
-         std::vector<Aux> saux(lv.h_src.size()), taux(lv.h_tgt.size());
-         size_t si = 0, ti = 0;
-         for (size_t ci = 0; ci < cl.size(); ci++) {
-            const Cube& c = t.cubes[cl[ci]];
-            double box = c.b1[0] - c.b0[0];
-            double sc = box * 1.5 * 1;
-            bool is_src = c.np > 0 && c.rowrank > 0, is_tgt = c.nq > 0 && c.colrank > 0;
-            if (!is_src && !is_tgt) continue;
-            int sbase = (int)(bpts.size() / 3);
-            for (int q = 0; q < NQ; q++)
-               for (int a = 0; a < 3; a++) bpts.push_back(SPHERE49[q][a] * sc + c.ctr[a]);
-            if (is_src) {
-               IntBlock b;
-               b.npt = NQ;
-               b.npan = c.np;
-               b.pt_off = sbase;
-               b.pan_off = c.p0;
-               b.out_off = qs;
-               b.ld_pt = c.np;
-               b.ld_pan = 1;
-               b.layer_stride = (long long)NQ * c.np;
-               push_blocks(blocks, b);
-               saux[si] = {qs, ps};
-               qs += 2LL * NQ * c.np;
-               ps += 2LL * gp3 * c.np;
-               si++;
-            }
-            if (is_tgt) {
-               GreenBlock g;
-               g.ndest = c.nq;
-               g.nsrc = NQ;
-               g.dest_off = c.q0;
-               g.src_off = sbase;
-               g.out_off = qp;
-               g.ld_dest = 1;
-               g.ld_src = c.nq;
-               gblocks.push_back(g);
-               taux[ti] = {qp, ip};
-               qp += (long long)NQ * c.nq;
-               ip += (long long)gp3 * c.nq;
-               ti++;
-            }
-         }
-         DBuf<double> Qs, Pb, Qpb, Ib;
-         Qs.alloc(std::max<long long>(qs, 1));
-         Pb.alloc(std::max<long long>(ps, 1));
-         Qpb.alloc(std::max<long long>(qp, 1));
-         Ib.alloc(std::max<long long>(ip, 1));
-         run_integrals(blocks, bpts, t.pan_perm, d_panels, kernel, kappa, Qs.p, s);
-         if (!gblocks.empty()) {
-            DBuf<GreenBlock> dg;
-            DBuf<double> dp;
-            dg.upload(gblocks, s);
-            dp.upload(bpts, s);
-            launch_green_blocks((int)gblocks.size(), dg.p, dp.p, dp.p, kernel, kappa, Qpb.p, s);
-            FSVD_CHECK(hipStreamSynchronize(s));
-         }
-         // P = pinv * Q2S (both layers); I^T = (pinv * Q2S_points)^T
-         std::vector<GemmDesc> g1, g2;
-         for (size_t k = 0; k < lv.h_src.size(); k++) {
-            const SrcDesc& sd = lv.h_src[k];
-            for (int layer = 0; layer < 2; layer++) {
-               GemmDesc d;
-               d.M = gp3;
-               d.N = sd.np;
-               d.K = NQ;
-               d.A = d_pinv.p;
-               d.a_rs = NQ;
-               d.a_cs = 1;
-               d.B = Qs.p + saux[k].q_off + (long long)layer * NQ * sd.np;
-               d.b_rs = sd.np;
-               d.b_cs = 1;
-               d.C = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
-               d.c_rs = sd.np;
-               d.c_cs = 1;
-               g1.push_back(d);
-            }
-         }
-         for (size_t k = 0; k < lv.h_tgt.size(); k++) {
-            const TgtDesc& td = lv.h_tgt[k];
-            GemmDesc d;
-            d.M = td.nq;
-            d.N = gp3;
-            d.K = NQ;
-            d.A = Qpb.p + taux[k].q_off;
-            d.a_rs = 1;
-            d.a_cs = td.nq;
-            d.B = d_pinv.p;
-            d.b_rs = 1;
-            d.b_cs = NQ;
-            d.C = Ib.p + taux[k].p_off;
-            d.c_rs = gp3;
-            d.c_cs = 1;
-            g1.push_back(d);
-         }
-         run_gemms(g1, s);
-         // PV^T (r x gp3) = VT (r x np) * P^T ; UTI (r x gp3) = U^T (r x nq) * I (nq x gp3)
-         for (size_t k = 0; k < lv.h_src.size(); k++) {
-            const SrcDesc& sd = lv.h_src[k];
-            for (int layer = 0; layer < 2; layer++) {
-               GemmDesc d;
-               d.M = sd.r;
-               d.N = gp3;
-               d.K = sd.np;
-               d.A = lv.VT64.p + sd.vt_off;
-               d.a_rs = sd.np;
-               d.a_cs = 1;
-               d.B = Pb.p + saux[k].p_off + (long long)layer * gp3 * sd.np;
-               d.b_rs = 1;
-               d.b_cs = sd.np;
-               d.C = lv.PV64.p + sd.pv_off + (long long)layer * sd.r * gp3;
-               d.c_rs = gp3;
-               d.c_cs = 1;
-               g2.push_back(d);
-            }
-         }
-         for (size_t k = 0; k < lv.h_tgt.size(); k++) {
-            const TgtDesc& td = lv.h_tgt[k];
-            GemmDesc d;
-            d.M = td.r;
-            d.N = gp3;
-            d.K = td.nq;
-            d.A = lv.UT64.p + td.u_off;
-            d.a_rs = td.nq;
-            d.a_cs = 1;
-            d.B = Ib.p + taux[k].p_off;
-            d.b_rs = gp3;
-            d.b_cs = 1;
-            d.C = lv.UTI64.p + td.uti_off;
-            d.c_rs = gp3;
-            d.c_cs = 1;
-            g2.push_back(d);
-         }
-         run_gemms(g2, s);
-      }
+      level_pv_uti(t, cl, lv, ptsT, d_panels, d_pinv.p, kernel, kappa, lv.VT64.p, lv.UT64.p, lv.PV64.p, lv.UTI64.p, s);
       FSVD_CHECK(hipStreamSynchronize(s));
       ph[5] += now_ms() - tp;
       timing_mark((std::string("L") + std::to_string(l) + " " + "PV, UTI").c_str(), tl);
@@ -1524,6 +1535,178 @@ void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, doub
    } else {
       applyT_impl<double>(op, layer, x, y, s);
    }
+}
+
+}  // namespace fsvd
+
+namespace fsvd {
+
+// Derivative of the Moore-Penrose pseudoinverse Ap (n x m) of A (m x n,
+// both row-major) along dA, at constant rank (Golub and Pereyra 1973):
+// dAp = -Ap dA Ap + Ap Ap^T dA^T (I - A Ap) + (I - Ap A) dA^T Ap^T Ap.
+static void pinv_derivative(const double* A, const double* dA, const double* Ap, int m, int n, double* dAp) {
+   auto mul = [](const std::vector<double>& X, const std::vector<double>& Y, int r, int k, int c) {
+      std::vector<double> Z((size_t)r * c, 0.0);
+      for (int i = 0; i < r; i++)
+         for (int l = 0; l < k; l++) {
+            const double x = X[(size_t)i * k + l];
+            if (x == 0.0) continue;
+            for (int j = 0; j < c; j++) Z[(size_t)i * c + j] += x * Y[(size_t)l * c + j];
+         }
+      return Z;
+   };
+   auto tr = [](const std::vector<double>& X, int r, int c) {
+      std::vector<double> Z((size_t)r * c);
+      for (int i = 0; i < r; i++)
+         for (int j = 0; j < c; j++) Z[(size_t)j * r + i] = X[(size_t)i * c + j];
+      return Z;
+   };
+   const std::vector<double> vA(A, A + (size_t)m * n), vdA(dA, dA + (size_t)m * n), vAp(Ap, Ap + (size_t)n * m);
+   const std::vector<double> dAT = tr(vdA, m, n), ApT = tr(vAp, n, m);
+   // -Ap dA Ap
+   std::vector<double> t1 = mul(mul(vAp, vdA, n, m, n), vAp, n, n, m);
+   // Ap Ap^T dA^T (I - A Ap)
+   std::vector<double> R = mul(vA, vAp, m, n, m);
+   for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) R[(size_t)i * m + j] = (i == j ? 1.0 : 0.0) - R[(size_t)i * m + j];
+   std::vector<double> t2 = mul(mul(vAp, ApT, n, m, n), mul(dAT, R, n, m, m), n, n, m);
+   // (I - Ap A) dA^T Ap^T Ap
+   std::vector<double> Pn = mul(vAp, vA, n, m, n);
+   for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) Pn[(size_t)i * n + j] = (i == j ? 1.0 : 0.0) - Pn[(size_t)i * n + j];
+   std::vector<double> t3 = mul(Pn, mul(dAT, mul(ApT, vAp, m, n, m), n, m, m), n, n, m);
+   for (size_t k = 0; k < (size_t)n * m; k++) dAp[k] = -t1[k] + t2[k] + t3[k];
+}
+
+void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
+
+// The kappa-derivative of the compressed operator as it is applied, with
+// the tree, ranks and bases (VT, U) held fixed: the near-field blocks of
+// the differentiated integrals; per level P' = pinv' Q2S + pinv Q2S' and
+// I' = (pinv' Q2Sp + pinv Q2Sp')^T, so PV' = VT P'^T and UTI' = U^T I'
+// (level_pv_uti with pinv' and the Yukawa integrals plus level_pv_uti with
+// pinv and the differentiated ones); the spectra of -e^{-kr}.  pinv' of the
+// sphere -> grid matrix Q follows from Q' by the pseudoinverse derivative.
+void build_dkappa(TreeOp& op, const PanelGeom* d_panels, hipStream_t s) {
+   if (op.kernel != YUKAWA) throw std::runtime_error("kappa derivatives exist for Yukawa operators only");
+   if (op.dk && op.dk->kappa == op.kappa) return;
+   auto dk = std::make_unique<DkData>();
+   dk->kappa = op.kappa;
+   const double kappa = op.kappa;
+   const Tree& t = op.tree;
+   const ApplyPlan& P = op.plan;
+   std::vector<double> ptsT(op.pts_tree.n);
+   FSVD_CHECK(hipMemcpyAsync(ptsT.data(), op.pts_tree.p, sizeof(double) * ptsT.size(), hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   // near field, in the layout of D64
+   dk->D64.alloc(std::max<size_t>(op.D64.n, 1));
+   {
+      std::vector<int> bpan(op.h_dcol.size());
+      for (size_t k = 0; k < bpan.size(); k++) bpan[k] = t.pan_perm[op.h_dcol[k]];
+      std::vector<IntBlock> blocks;
+      for (const LeafDesc& L : op.h_leaves) {
+         if (L.m == 0) continue;
+         IntBlock b;
+         b.npt = L.p;
+         b.npan = L.m;
+         b.pt_off = L.q0;
+         b.pan_off = L.col_off;
+         b.out_off = L.d_off;
+         b.ld_pt = 1;
+         b.ld_pan = 2 * L.p;
+         b.layer_stride = L.p;
+         push_blocks(blocks, b);
+      }
+      run_integrals(blocks, ptsT, bpan, d_panels, YUKAWA_DKAPPA, kappa, dk->D64.p, s);
+   }
+   // far field, in the layouts of the plan's PV64, Tsp64 and UTI64
+   dk->PV64.alloc(std::max<size_t>(P.PV64.n, 1));
+   dk->Tsp64.alloc(std::max<size_t>(P.Tsp64.n, 2));
+   FSVD_CHECK(hipMemsetAsync(dk->Tsp64.p, 0, sizeof(double) * dk->Tsp64.n, s));
+   dk->UTI64.alloc(std::max<size_t>(P.UTI64.n, 1));
+   const double root = t.cubes[0].b1[0] - t.cubes[0].b0[0];
+   long long pv_b = 0, t_b = 0, uti_b = 0;
+   for (const FarLevel& lv : op.levels) {
+      const int l = lv.level, n = lv.n, gp3 = lv.gp3;
+      std::vector<int> cl;
+      for (size_t i = 0; i < t.cubes.size(); i++)
+         if (t.cubes[i].level == l && !t.cubes[i].itr.empty()) cl.push_back((int)i);
+      // Q (sphere -> grid, as in the build) and Q'
+      const double box = root / std::pow(2.0, l), h = box / (n - 1), sc = box * 1.5 * 1, ctr = 0.5 * box;
+      std::vector<double> Q((size_t)NQ * gp3), dQ((size_t)NQ * gp3), dpinv((size_t)gp3 * NQ);
+      for (int q = 0; q < NQ; q++) {
+         double mq[3];
+         for (int a = 0; a < 3; a++) mq[a] = SPHERE49[q][a] * sc + ctr;
+         int g = 0;
+         for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+               for (int k = 0; k < n; k++, g++) {
+                  double p0 = i * h - mq[0], p1 = j * h - mq[1], p2 = k * h - mq[2];
+                  double r = std::sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+                  Q[(size_t)q * gp3 + g] = std::exp(-kappa * r) / r;
+                  dQ[(size_t)q * gp3 + g] = -std::exp(-kappa * r);
+               }
+      }
+      pinv_derivative(Q.data(), dQ.data(), op.h_pinv[l].data(), NQ, gp3, dpinv.data());
+      DBuf<double> d_pinv, d_dpinv, pva, utia, pvb, utib;
+      d_pinv.upload(op.h_pinv[l], s);
+      d_dpinv.upload(dpinv, s);
+      pva.alloc(std::max<size_t>(lv.PV64.n, 1));
+      pvb.alloc(std::max<size_t>(lv.PV64.n, 1));
+      utia.alloc(std::max<size_t>(lv.UTI64.n, 1));
+      utib.alloc(std::max<size_t>(lv.UTI64.n, 1));
+      level_pv_uti(t, cl, lv, ptsT, d_panels, d_dpinv.p, YUKAWA, kappa, lv.VT64.p, lv.UT64.p, pva.p, utia.p, s);
+      level_pv_uti(t, cl, lv, ptsT, d_panels, d_pinv.p, YUKAWA_DKAPPA, kappa, lv.VT64.p, lv.UT64.p, pvb.p, utib.p,
+                   s);
+      if (lv.PV64.n) launch_axpby((long long)lv.PV64.n, 1.0, pva.p, 1.0, pvb.p, dk->PV64.p + pv_b, s);
+      if (lv.UTI64.n) launch_axpby((long long)lv.UTI64.n, 1.0, utia.p, 1.0, utib.p, dk->UTI64.p + uti_b, s);
+      {
+         std::vector<std::complex<double>> T((size_t)343 * lv.pad);
+         translation_spectra(n, h, YUKAWA_DKAPPA, kappa, T.data());
+         DBuf<double> dT;
+         dT.upload((const double*)T.data(), 2 * T.size(), s);
+         FSVD_CHECK(hipMemcpy2DAsync(dk->Tsp64.p + t_b, sizeof(double) * 2 * tr_pst(lv.pad), dT.p,
+                                     sizeof(double) * 2 * lv.pad, sizeof(double) * 2 * lv.pad, 343,
+                                     hipMemcpyDeviceToDevice, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+      }
+      pv_b += (long long)lv.PV64.n;
+      t_b += 2LL * 343 * tr_pst(lv.pad);
+      uti_b += (long long)lv.UTI64.n;
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   op.dk = std::move(dk);
+}
+
+// The far field U UTI T PV VT is a product of three kappa-dependent
+// factors: its derivative is U (UTI' T PV + UTI T' PV + UTI T PV') VT.
+// Each term is one apply with one factor swapped (the near field D' in
+// each), and the near field alone (the leaf stage with the far-field
+// weights zeroed) is taken out twice: y = y1 + y2 + y3 - 2 D'x.
+void tree_apply_dkappa(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s) {
+   if (!op.dk) throw std::runtime_error("build_dkappa first");
+   const long long npt = (long long)op.tree.npt;
+   DBuf<double> yk, y0;
+   yk.alloc(std::max<long long>(3 * npt, 1));
+   y0.alloc(std::max<long long>(npt, 1));
+   for (int term = 0; term < 3; term++) {
+      ApplyArgs a = make_args<double>(op, xs, xd, yk.p + term * npt);
+      a.D = op.dk->D64.p;
+      if (term == 0) a.UTI = op.dk->UTI64.p;
+      if (term == 1) a.Tsp = op.dk->Tsp64.p;
+      if (term == 2) a.PV = op.dk->PV64.p;
+      for (int stage = 0; stage < 5; stage++) launch_apply_stage<double>(stage, a, s);
+   }
+   {
+      ApplyArgs a = make_args<double>(op, xs, xd, y0.p);
+      a.D = op.dk->D64.p;
+      FSVD_CHECK(hipMemsetAsync(op.plan.w.p, 0, sizeof(double) * op.plan.w.n, s));
+      launch_apply_stage<double>(4, a, s);
+   }
+   launch_axpby(npt, 1.0, yk.p, 1.0, yk.p + npt, y, s);
+   launch_axpby(npt, 1.0, y, 1.0, yk.p + 2 * npt, y, s);
+   launch_axpby(npt, 1.0, y, -2.0, y0.p, y, s);
+   FSVD_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace fsvd

@@ -134,6 +134,15 @@ struct ApplyPlan {
    DBuf<float> VTc32, PV32, Tsp32, UTI32, UT32;
 };
 
+// d/dkappa of a Yukawa operator at fixed tree, ranks and bases, in the
+// layouts of the FP64 apply plan: near-field blocks of the kappa-derivative
+// integrals, the derivatives of PV, of the T spectra and of UTI (VT and U
+// unchanged).  Built on demand for the adjoint kappa gradient.
+struct DkData {
+   double kappa = 0.0;
+   DBuf<double> D64, PV64, Tsp64, UTI64;
+};
+
 struct TreeOp {
    Tree tree;
    int kernel = LAPLACE;
@@ -157,6 +166,7 @@ struct TreeOp {
    TreeOpStats stats;
    std::vector<int> cube_src_slot, cube_tgt_slot, cube_leaf;   // per cube: slot in its level / leaf index, -1
    ApplyPlan plan;
+   std::unique_ptr<DkData> dk;
 };
 
 // The device plan of one launch covering the FP32 translations of the given
@@ -185,6 +195,11 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
 // source transforms, 2 translation, 3 target transforms), for per-kernel
 // timing.
 void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s);
+// kappa-derivative of a Yukawa tree's operator, at fixed ranks and bases:
+// build_dkappa fills op.dk (once per kappa); tree_apply_dkappa applies it
+// (FP64), y_t = S' xs + D' xd in tree order.
+void build_dkappa(TreeOp& op, const PanelGeom* d_panels, hipStream_t s);
+void tree_apply_dkappa(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s);
 // y_t = A_layer^T x_t (panels, tree order).
 void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s);
 

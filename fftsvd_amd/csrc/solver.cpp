@@ -1160,14 +1160,15 @@ SolveInfo problem_solve_adjoint(Problem& pr, int mode, const double* rhs, double
    return problem_solve(pr, mode, rhs, sol, tol, maxit, nullptr);
 }
 
-SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
-                               double* grad_host) {
-   if (pr.qual) throw std::runtime_error("charge gradients of the qualocation operator are not provided");
+// The adjoint of the energy E = 1/2 q^T C x at the forward solution x:
+// phi = C x (per charge, host) and lambda = A^-T C^T q (device).
+static SolveInfo energy_adjoint(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
+                                std::vector<double>& phi, DBuf<double>& dlam) {
    hipStream_t s = pr.stream;
    const int nq = (int)pr.q.size();
    const long long nv = pr.nvec;
    // phi = C x
-   std::vector<double> phi(std::max(nq, 1), 0.0);
+   phi.assign(std::max(nq, 1), 0.0);
    problem_potentials(pr, sol_host, phi.data(), false);
    // g = C^T q
    std::vector<double> g(nv, 0.0);
@@ -1210,10 +1211,20 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
       }
    }
    // lambda = A^-T g
-   DBuf<double> dg, dlam;
+   DBuf<double> dg;
    dg.upload(g, s);
    dlam.alloc(nv);
-   SolveInfo si = problem_solve_adjoint(pr, mode, dg.p, dlam.p, tol, maxit);
+   return problem_solve_adjoint(pr, mode, dg.p, dlam.p, tol, maxit);
+}
+
+SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
+                               double* grad_host) {
+   if (pr.qual) throw std::runtime_error("charge gradients of the qualocation operator are not provided");
+   hipStream_t s = pr.stream;
+   const int nq = (int)pr.q.size();
+   std::vector<double> phi;
+   DBuf<double> dlam;
+   SolveInfo si = energy_adjoint(pr, mode, sol_host, tol, maxit, phi, dlam);
    // B^T lambda over the right-hand-side rows of each charged region
    DBuf<double> dgq;
    dgq.alloc(std::max(nq, 1));
@@ -1244,6 +1255,48 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
    if (nq) FSVD_CHECK(hipMemcpyAsync(bt.data(), dgq.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
    FSVD_CHECK(hipStreamSynchronize(s));
    for (int j = 0; j < nq; j++) grad_host[j] = 0.5 * (phi[j] + bt[j]);
+   return si;
+}
+
+// dE/dkappa = -1/2 lambda^T (dA/dkappa) x: the charges sit in Laplace
+// regions, so only the operators of the Yukawa regions depend on kappa;
+// per such region the join of x, the kappa-derivative of its compressed
+// operator at fixed ranks and bases (build_dkappa / tree_apply_dkappa),
+// and the split into the rows the region writes (the -4 pi jump does not
+// depend on kappa).
+SolveInfo problem_grad_kappa(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
+                             double* dedk) {
+   if (pr.qual) throw std::runtime_error("the qualocation operator has no kappa");
+   hipStream_t s = pr.stream;
+   const long long nv = pr.nvec;
+   std::vector<double> phi;
+   DBuf<double> dlam;
+   SolveInfo si = energy_adjoint(pr, mode, sol_host, tol, maxit, phi, dlam);
+   DBuf<double> dx, da;
+   dx.upload(sol_host, nv, s);
+   da.alloc(nv);
+   FSVD_CHECK(hipMemsetAsync(da.p, 0, sizeof(double) * nv, s));
+   for (Region& R : pr.regions) {
+      if (R.kernel != YUKAWA || R.panel_ids.empty()) continue;
+      const int n = (int)R.panel_ids.size();
+      Region& O = R.shares_tree_with >= 0 ? pr.regions[R.shares_tree_with] : R;
+      TreeOp& op = *O.op;
+      DBuf<double> zero;
+      zero.alloc(n);
+      FSVD_CHECK(hipMemsetAsync(zero.p, 0, sizeof(double) * n, s));
+      launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, dx.p, R.xs.p, R.xd.p, s);
+      build_dkappa(op, O.d_panels.p, s);
+      tree_apply_dkappa(op, R.xs.p, R.xd.p, R.yt.p, s);
+      launch_split(n, R.out_idx.p, zero.p, R.yt.p, R.xd.p, da.p, s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+   std::vector<double> lam(nv), a(nv);
+   FSVD_CHECK(hipMemcpyAsync(lam.data(), dlam.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipMemcpyAsync(a.data(), da.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+   FSVD_CHECK(hipStreamSynchronize(s));
+   double acc = 0.0;
+   for (long long k = 0; k < nv; k++) acc += lam[k] * a[k];
+   *dedk = -0.5 * acc;
    return si;
 }
 

@@ -160,6 +160,9 @@ void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs);
 SolveInfo problem_solve_adjoint(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit);
 SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
                                double* grad_host);
+// dE/dkappa (energy units of grad_charges) at the forward solution sol by
+// the adjoint: -1/2 lambda^T (dA/dkappa) x with lambda = A^-T C^T q.
+SolveInfo problem_grad_kappa(Problem& pr, int mode, const double* sol_host, double tol, int maxit, double* dedk);
 void problem_potentials_batch(Problem& pr, int ncol, const double* sol_host, double* phi_host);
 
 }  // namespace fsvd

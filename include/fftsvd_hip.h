@@ -112,6 +112,18 @@ int fftsvd_problem_solve_adjoint(fftsvd_problem* p, int precision, const double*
                                  int maxit, int* iters, double* info);
 int fftsvd_problem_grad_charges(fftsvd_problem* p, int precision, const double* sol, double tol, int maxit,
                                 double* grad, int* iters);
+/* dE/dkappa of the same energy at the forward solution sol (2N, host), in
+ * the units of fftsvd_problem_grad_charges: -1/2 lambda^T (dA/dkappa) x
+ * with A^T lambda = C^T q.  dA/dkappa is the derivative of each Yukawa
+ * region's compressed operator as it is applied, at fixed tree, ranks and
+ * bases: near-field blocks of the kappa-derivative of the reference's
+ * Yukawa scheme (FlatIntegration.c:473-949 differentiated term by term),
+ * and the derivatives of the equivalent densities (through the
+ * pseudoinverse), of the translation spectra and of the target-side
+ * projections.  Built once per operator on the first call.  No counterpart
+ * in the reference (FFTSVDpbe has no kappa derivative). */
+int fftsvd_problem_grad_kappa(fftsvd_problem* p, int precision, const double* sol, double tol, int maxit,
+                              double* dedk, int* iters);
 /* Operator statistics, up to 32 doubles: 0 panels, 1 regions, 2 setup ms,
  * 3 near entries, 4 far entries, 5 spectrum bins, 6 translation MACs,
  * 7 depth, 8 leaves, 9 tree applications per matvec, 10/11 algorithmic
@@ -151,7 +163,8 @@ int fftsvd_tree_apply(fftsvd_problem* p, int which, int precision, const double*
 int fftsvd_tree_apply_transpose(fftsvd_problem* p, int which, int precision, int layer, const double* x, double* y);
 
 /* Panel integrals on the GPU: slp, dlp (npt x npan) for kernel 0 = Laplace,
- * 1 = Yukawa (Integration, Integration.c:32-185). */
+ * 1 = Yukawa (Integration, Integration.c:32-185), 2 = the kappa-derivatives
+ * of the Yukawa pair (the gradient kernel of fftsvd_problem_grad_kappa). */
 int fftsvd_integrate(int kernel, double kappa, const double* verts9, int npan, const double* pts, int npt,
                      double* slp, double* dlp);
 

@@ -30,6 +30,7 @@ def _load():
     sig = {
         "orc_panel_size": (C.c_size_t, []),
         "orc_integrate_batch": (None, [C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+        "orc_yukawa_dk_batch": (None, [C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
         "orc_panel_geom": (None, [_dp, _dp]),
         "orc_gl4": (None, [_dp, _dp]),
         "orc_yukawa_rule_v9": (None, [_dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
@@ -49,6 +50,7 @@ def _load():
                                             C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                             C.c_double]),
         "orc_problem_ntotal": (C.c_int, [C.c_void_p]),
+        "orc_problem_make_dense": (None, [C.c_void_p]),
         "orc_problem_free": (None, [C.c_void_p]),
         "orc_so_matvec": (None, [C.c_void_p, _dp, _dp]),
         "orc_rhs": (None, [C.c_void_p, _dp]),
@@ -124,6 +126,16 @@ def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):
     s = np.zeros((len(p), len(v)))
     d = np.zeros((len(p), len(v)))
     lib().orc_integrate_batch(kernel, kappa, _d(v), len(v), _d(p), len(p), _d(s), _d(d))
+    return s, d
+
+
+def yukawa_dk(verts9, pts, kappa):
+    """d/dkappa of the Yukawa (slp, dlp) of integrate(): arrays (npt, npan)."""
+    v = f64(verts9).reshape(-1, 9)
+    p = f64(pts).reshape(-1, 3)
+    s = np.zeros((len(p), len(v)))
+    d = np.zeros((len(p), len(v)))
+    lib().orc_yukawa_dk_batch(kappa, _d(v), len(v), _d(p), len(p), _d(s), _d(d))
     return s, d
 
 
@@ -352,6 +364,12 @@ class Problem:
                                              _d(self.qpos), _d(self.q), len(self.q), eps_in, eps_out, kappa,
                                              maxpanels, gridpoints, svdeps)
         lib().orc_precond_build(self.h)
+
+    def make_dense(self):
+        """Apply every region's all-pairs integrals instead of the compressed
+        operator from now on (the uncompressed BEM operator)."""
+        lib().orc_problem_make_dense(self.h)
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -55,6 +55,10 @@ void orc_yukawa_line(double z, double x, double y1, double y2, double kappa, int
 void orc_yukawa_rule_v9(const double* x, const double* v9, double kappa, int nq, const double* gx, const double* gw,
                         double* slp, double* dlp);
 void orc_integrate(int kernel, double kappa, const double* x, const orc_panel* p, double* slp, double* dlp);
+/* kappa-derivatives of the Yukawa single and double layer of orc_yukawa */
+void orc_yukawa_dk(const double* x, const orc_panel* p, double kappa, double* dslp, double* ddlp);
+void orc_yukawa_dk_batch(double kappa, const double* v9, int npan, const double* pts, int npt, double* ds,
+                         double* dd);   /* npt x npan */
 double orc_green(int kernel, double kappa, const double* dest, const double* src);
 void orc_gl4(double* x, double* w);
 
@@ -102,6 +106,8 @@ typedef struct {
    double** pinv;             /* per level, gp3 x 49 row-major */
    double** T;                /* per level, 343 offsets x pad complex (NULL for inner offsets) */
    int double_only;           /* DOUBLE_LAYER_INT tree: bases sampled from the double layer only */
+   double* dS;                /* dense mode: all-pairs integrals (npt x npan each), or NULL */
+   double* dD;
 } orc_tree;
 
 orc_tree* orc_tree_build(const orc_panel* panels, int npan, const double* pts, int npt,
@@ -167,6 +173,10 @@ orc_problem* orc_problem_layout(const double* verts9, const int* counts, int nsa
                                 const double* qpos, const double* q, int ncharge, double eps_in, double eps_out,
                                 double kappa, int stern, int maxpanels, int gridpoints, double svdeps);
 void orc_problem_free(orc_problem* pr);
+/* Dense mode: every region's tree applies its all-pairs integrals
+ * (orc_tree_dense) instead of the compressed operator -- the uncompressed
+ * BEM operator, a checker for derivatives of the energy. */
+void orc_problem_make_dense(orc_problem* pr);
 void orc_so_matvec(orc_problem* pr, const double* x, double* y);
 void orc_rhs(orc_problem* pr, double* rhs);
 void orc_precond_build(orc_problem* pr);

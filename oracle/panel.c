@@ -331,6 +331,97 @@ static void yuk_edge(const double* a, const double* b, const double* pnt, double
    }
 }
 
+/* d/dkappa of yuk_line's sums: the same pieces and nodes with the integrands
+ * differentiated, -(e^{-kR} - e^{-k|z|})/k -> (R e^{-kR} - |z| e^{-k|z|})/k
+ * + (e^{-kR} - e^{-k|z|})/k^2 and -(sgn z e^{-k|z|} - (z/R) e^{-kR}) ->
+ * z (e^{-k|z|} - e^{-kR}). */
+static void yuk_line_dk(double z, double x, double y1, double y2, double kappa, double* sl, double* dl) {
+   double gx[4], gw[4];
+   orc_gl4(gx, gw);
+   double az = z > 0.0 ? z : -z;
+   if (x < 0) x = -x;
+   double start = y1, end;
+   if (x < y1)
+      end = (y2 > 10.0 * y1) ? 10.0 * y1 : y2;
+   else
+      end = (x > y2) ? y2 : x;
+   double ekz = exp(-kappa * az);
+   double s = 0.0, d = 0.0;
+   while (start < y2) {
+      double ph1 = atan2(start, x), ph2 = atan2(end, x);
+      double dph = ph2 - ph1;
+      for (int i = 0; i < 4; i++) {
+         double th = ph1 + (ph2 - ph1) * gx[i];
+         double q = x * (1.0 / cos(th));
+         double R = sqrt(q * q + z * z);
+         double ekr = exp(-kappa * R);
+         double wd = gw[i] * dph;
+         s = s + wd * ((R * ekr - az * ekz) / kappa + (ekr - ekz) / (kappa * kappa));
+         d = d + wd * z * (ekz - ekr);
+      }
+      start = end;
+      if (end == y2)
+         end = 10.0 * y2;
+      else if (end * 10.0 > y2)
+         end = y2;
+      else
+         end = 10.0 * end;
+   }
+   *sl = s;
+   *dl = d;
+}
+
+/* d/dkappa of orc_yukawa's 4-point scheme: edges, splits and signs as there;
+ * the on-plane double layer (a constant) has derivative zero. */
+void orc_yukawa_dk(const double* x, const orc_panel* p, double kappa, double* dslp, double* ddlp) {
+   double r0[3], pnt[3];
+   sub3(r0, x, p->c);
+   pnt[0] = dot3(p->axn[0], r0);
+   pnt[1] = dot3(p->axn[1], r0);
+   pnt[2] = dot3(p->axn[2], r0);
+   double S = 0.0, D = 0.0;
+   for (int j = 0; j < 3; j++) {
+      int nx = (j + 1) % 3;
+      double d = dot3(pnt, p->en[j]) - p->erhs[j];
+      double sg = ((d > 0.0 && p->eov[j] > 0.0) || (d < 0.0 && p->eov[j] < 0.0)) ? 1.0 : -1.0;
+      const double* a = p->pvn[j];
+      const double* b = p->pvn[nx];
+      double u[2] = {a[0] - pnt[0], a[1] - pnt[1]};
+      double w[2] = {b[0] - pnt[0], b[1] - pnt[1]};
+      double z = pnt[2] - a[2];
+      double dv[2] = {u[0] - w[0], u[1] - w[1]};
+      double L = sqrt(dv[0] * dv[0] + dv[1] * dv[1]);
+      double c = dv[0] * (1.0 / L), s = dv[1] * (1.0 / L);
+      double ux = s * u[0] - c * u[1], uy = c * u[0] + s * u[1];
+      double wy = c * w[0] + s * w[1];
+      double sl = 0.0, dl = 0.0;
+      if (!(fabs(ux) < 10 * DBL_EPSILON)) {
+         double y1 = wy, y2 = uy;
+         if (y2 < 0 && y1 < 0) {
+            y1 = -y1;
+            y2 = -y2;
+         }
+         if (y2 < y1) {
+            double t = y1;
+            y1 = y2;
+            y2 = t;
+         }
+         yuk_line_dk(z, ux, (y1 >= 0) ? y1 : 0, y2, kappa, &sl, &dl);
+         if (y1 < 0.0) {
+            double s2, d2;
+            yuk_line_dk(z, ux, 0.0, -y1, kappa, &s2, &d2);
+            sl += s2;
+            dl += d2;
+         }
+      }
+      S += sg * sl;
+      D += sg * dl;
+   }
+   if (fabs(pnt[2]) < 10 * FLT_EPSILON) D = 0.0;
+   *dslp = S;
+   *ddlp = D;
+}
+
 /* Yukawa e^{-kr}/r single and double layer (FlatIntegration_ekroverr_numerical
  * FlatIntegration.c:1035-1049 -> calcp, FlatIntegration.c:882-949). */
 void orc_yukawa(const double* x, const orc_panel* p, double kappa, double* slp, double* dlp) {
@@ -480,4 +571,13 @@ void orc_yukawa_rule_v9(const double* x, const double* v9, double kappa, int nq,
    orc_panel p;
    orc_panel_init(&p, v9, v9 + 3, v9 + 6);
    orc_yukawa_rule(x, &p, kappa, nq, gx, gw, slp, dlp);
+}
+
+void orc_yukawa_dk_batch(double kappa, const double* v9, int npan, const double* pts, int npt, double* ds, double* dd) {
+   orc_panel* P = (orc_panel*)malloc(sizeof(orc_panel) * (size_t)(npan > 0 ? npan : 1));
+   orc_panels_from_verts(v9, npan, P);
+   for (int i = 0; i < npt; i++)
+      for (int j = 0; j < npan; j++)
+         orc_yukawa_dk(pts + 3 * i, &P[j], kappa, &ds[(size_t)i * npan + j], &dd[(size_t)i * npan + j]);
+   free(P);
 }

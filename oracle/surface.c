@@ -265,6 +265,18 @@ static void region_free(orc_region* r) {
    free(r);
 }
 
+static void region_make_dense(orc_region* r) {
+   orc_tree* t = r->tree;
+   if (!t->dS) {
+      t->dS = (double*)malloc(sizeof(double) * (size_t)t->npt * t->npan + 1);
+      t->dD = (double*)malloc(sizeof(double) * (size_t)t->npt * t->npan + 1);
+      orc_tree_dense(t, t->dS, t->dD);
+   }
+   for (int c = 0; c < r->nchild; c++) region_make_dense(r->child[c]);
+}
+
+void orc_problem_make_dense(orc_problem* pr) { region_make_dense(pr->root); }
+
 void orc_problem_free(orc_problem* pr) {
    region_free(pr->root);
    free(pr->panels);

@@ -481,6 +481,8 @@ void orc_tree_fill(orc_tree* t) {
 
 void orc_tree_free(orc_tree* t) {
    if (!t) return;
+   free(t->dS);
+   free(t->dD);
    free_cubes(t);
    free(t->cubes);
    if (t->pinv)
@@ -538,6 +540,15 @@ static void cmac(double* acc, const double* T, const double* S, int pad) {
  * 1730-1798).  y is overwritten. */
 void orc_tree_apply_both(orc_tree* t, const double* xs, const double* xd, double* y) {
    memset(y, 0, sizeof(double) * t->npt);
+   if (t->dS) {
+      for (int i = 0; i < t->npt; i++) {
+         double a = 0.0;
+         for (int j = 0; j < t->npan; j++)
+            a += t->dS[(size_t)i * t->npan + j] * xs[j] + t->dD[(size_t)i * t->npan + j] * xd[j];
+         y[i] = a;
+      }
+      return;
+   }
    for (int i = 0; i < t->ncubes; i++) {
       orc_cube* c = &t->cubes[i];
       if (c->nint > 0) {

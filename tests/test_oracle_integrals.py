@@ -22,6 +22,7 @@ import pytest
 
 import oracle
 import quad
+from fftsvd_amd import meshes
 
 
 def random_tri(rng, scale=1.0):
@@ -242,3 +243,23 @@ def test_yukawa_line_schedule(case, z):
     for alt in alts[1 if len(pieces) == 1 else 0:]:
         als, ald = _line_by_pieces(z, x, alt, kappa)
         assert abs(als - s) > 1e-10 * abs(s) or abs(ald - d) > 1e-10 * max(abs(d), 1e-3)
+
+
+def test_yukawa_kappa_derivative_matches_finite_differences():
+    """orc_yukawa_dk, the 4-point scheme differentiated term by term in kappa
+    (the kernel of the kappa gradient), equals the central difference of the
+    scheme's own values (h = 1e-5, error O(h^2)) near, on and far from the
+    panels, for both layers."""
+    V, F = meshes.ellipsoid((3.0, 2.2, 1.6), subdivisions=3, jitter=0.03, seed=11)
+    P = meshes.panels_from_mesh(V, F)
+    rng = np.random.default_rng(0)
+    Pt = P[rng.choice(len(P), 40, replace=False)]
+    cent = P.reshape(-1, 3, 3).mean(1)
+    pts = np.vstack([cent[rng.choice(len(P), 60, replace=False)], Pt.reshape(-1, 3, 3).mean(1)[:5]])
+    for k in (0.124, 0.6):
+        h = 1e-5
+        sp, dp = oracle.integrate(Pt, pts, oracle.YUKAWA, k + h)
+        sm, dm = oracle.integrate(Pt, pts, oracle.YUKAWA, k - h)
+        ds, dd = oracle.yukawa_dk(Pt, pts, k)
+        assert np.abs((sp - sm) / (2 * h) - ds).max() <= 2e-8 * np.abs(ds).max()
+        assert np.abs((dp - dm) / (2 * h) - dd).max() <= 2e-8 * np.abs(dd).max()
