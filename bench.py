@@ -214,10 +214,12 @@ def main():
         t0 = time.time()
         rhs = prob.rhs()
         t_rhs = time.time() - t0
+        sols = {}
         for name, prec in (("fp64", fa.F64), ("mixed", fa.MIXED)):
             t0 = time.time()
             sol, it, hist, info = prob.solve(rhs, 1e-6, precision=prec)
             t_gmres = time.time() - t0
+            sols[name] = sol
             t0 = time.time()
             phi = prob.reaction_potentials(sol)
             t_pot = time.time() - t0
@@ -273,6 +275,56 @@ def main():
         print("# batched energies: mixed vs fp64 max %.2e (median %.2e); vs fp64 at 1e-10: fp64 %.2e, mixed %.2e" %
               (rel.max(), np.median(rel), multi["fp64"]["energy_rel_err_vs_fp64_tol1e-10_max"],
                multi["mixed"]["energy_rel_err_vs_fp64_tol1e-10_max"]), flush=True)
+
+    # gradients of the energy (the adjoint): d/d(charges) and d/d(kappa) at
+    # the mixed solution, timed; against central differences (charges: the
+    # energy is quadratic, 8 seeded charges, batched FP64 solves at 1e-10;
+    # kappa: operators rebuilt at kappa +- h, Richardson over h = 2e-3, 1e-3)
+    grads = None
+    if not args.no_solve:
+        conv = 332.112 / 4.0
+        t0 = time.time()
+        gq = prob.grad_charges(sols["mixed"], 1e-6, precision=fa.MIXED)
+        t_gq = time.time() - t0
+        t0 = time.time()
+        gk = prob.grad_kappa(sols["mixed"], 1e-6, precision=fa.MIXED)
+        t_gk_first = time.time() - t0
+        t0 = time.time()
+        gk = prob.grad_kappa(sols["mixed"], 1e-6, precision=fa.MIXED)
+        t_gk = time.time() - t0
+        x64, _, _, _ = prob.solve(prob.rhs(), 1e-10, precision=fa.F64)
+        gq64 = prob.grad_charges(x64, 1e-10, precision=fa.F64)
+        gk64 = prob.grad_kappa(x64, 1e-10, precision=fa.F64)
+        js = np.random.default_rng(args.seed + 100).choice(len(q), 8, replace=False)
+        hq = 1e-3
+        Qf = np.repeat(q[:, None], 2 * len(js), axis=1)
+        for c, j in enumerate(js):
+            Qf[j, 2 * c] += hq
+            Qf[j, 2 * c + 1] -= hq
+        Xf, _, _ = prob.solve_batch(prob.rhs_batch(Qf), 1e-10, precision=fa.F64)
+        Ef = 0.5 * np.sum(Qf * prob.potentials_batch(Xf), axis=0) * conv
+        fdq = (Ef[0::2] - Ef[1::2]) / (2 * hq)
+        dk = {}
+        for hk in (2e-3, 1e-3):
+            e = []
+            for kk in (0.124 + hk, 0.124 - hk):
+                pk = fa.Problem(P, xyz, q, 4.0, 80.0, kk, want_f32=False)
+                xk, _, _, _ = pk.solve(pk.rhs(), 1e-10, precision=fa.F64)
+                e.append(pk.energy(xk))
+                pk.close()
+                del pk
+            dk[hk] = (e[0] - e[1]) / (2 * hk)
+        fdk = (4 * dk[1e-3] - dk[2e-3]) / 3
+        grads = {"charges_s": t_gq, "kappa_s": t_gk, "kappa_first_call_s": t_gk_first,
+                 "dE_dkappa_kcal_per_A": gk, "charge_fd_max_rel": float(np.max(np.abs(gq64[js] - fdq) / np.abs(fdq))),
+                 "charge_mixed_vs_fp64_rel": float(np.linalg.norm(gq - gq64) / np.linalg.norm(gq64)),
+                 "kappa_fd_rel": abs(gk64 - fdk) / abs(fdk), "kappa_mixed_vs_fp64_rel": abs(gk - gk64) / abs(gk64),
+                 "note": "central differences of the energy with the operator rebuilt at kappa +- h at the "
+                         "default SVD tolerance 1e-5: re-sampled bases move it by a few 1e-5; at 1e-8 the "
+                         "tests find 1e-9"}
+        print("# gradients: charges %.3fs, kappa %.3fs (first call %.3fs); dE/dkappa %.6f, vs central "
+              "differences %.2e (charges %.2e)" % (t_gq, t_gk, t_gk_first, gk, grads["kappa_fd_rel"],
+                                                  grads["charge_fd_max_rel"]), flush=True)
 
     # matvec throughput, device-resident vectors, HIP-event timed on the library stream
     prob.bench_matvec(fa.MIXED, max(1, args.warmup))
@@ -348,6 +400,7 @@ def main():
         "device": {"name": dev_name, "cus": dev_cu},
         "solve": solve,
         "multi_rhs": multi,
+        "gradients": grads,
         "operator": st,
         "roofline": roofline,
         "roofline_stages": rf,
