@@ -1258,26 +1258,28 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
    return si;
 }
 
-// dE/dkappa = -1/2 lambda^T (dA/dkappa) x: the charges sit in Laplace
-// regions, so only the operators of the Yukawa regions depend on kappa;
-// per such region the join of x, the kappa-derivative of its compressed
-// operator at fixed ranks and bases (build_dkappa / tree_apply_dkappa),
-// and the split into the rows the region writes (the -4 pi jump does not
-// depend on kappa).
+// dE/dkappa = -1/2 lambda^T (dA/dkappa) x = -1/2 q^T C A^-1 (dA/dkappa) x:
+// kappa is one parameter, so the tangent form needs one forward solve,
+// A w = (dA/dkappa) x, and the potentials of w, instead of the adjoint
+// solve.  The charges sit in Laplace regions, so only the operators of the
+// Yukawa regions depend on kappa; per such region the join of x, the
+// kappa-derivative of its compressed operator at fixed ranks and bases
+// (build_dkappa / tree_apply_dkappa), and the split into the rows the
+// region writes (the -4 pi jump does not depend on kappa).
 SolveInfo problem_grad_kappa(Problem& pr, int mode, const double* sol_host, double tol, int maxit,
                              double* dedk) {
    if (pr.qual) throw std::runtime_error("the qualocation operator has no kappa");
    hipStream_t s = pr.stream;
    const long long nv = pr.nvec;
-   std::vector<double> phi;
-   DBuf<double> dlam;
-   SolveInfo si = energy_adjoint(pr, mode, sol_host, tol, maxit, phi, dlam);
-   DBuf<double> dx, da;
+   const int nq = (int)pr.q.size();
+   DBuf<double> dx, da, dw;
    dx.upload(sol_host, nv, s);
    da.alloc(nv);
    FSVD_CHECK(hipMemsetAsync(da.p, 0, sizeof(double) * nv, s));
+   bool any = false;
    for (Region& R : pr.regions) {
       if (R.kernel != YUKAWA || R.panel_ids.empty()) continue;
+      any = true;
       const int n = (int)R.panel_ids.size();
       Region& O = R.shares_tree_with >= 0 ? pr.regions[R.shares_tree_with] : R;
       TreeOp& op = *O.op;
@@ -1290,12 +1292,19 @@ SolveInfo problem_grad_kappa(Problem& pr, int mode, const double* sol_host, doub
       launch_split(n, R.out_idx.p, zero.p, R.yt.p, R.xd.p, da.p, s);
       FSVD_CHECK(hipStreamSynchronize(s));
    }
-   std::vector<double> lam(nv), a(nv);
-   FSVD_CHECK(hipMemcpyAsync(lam.data(), dlam.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-   FSVD_CHECK(hipMemcpyAsync(a.data(), da.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+   SolveInfo si;
+   if (!any) {
+      *dedk = 0.0;
+      return si;
+   }
+   dw.alloc(nv);
+   si = problem_solve(pr, mode, da.p, dw.p, tol, maxit, nullptr);
+   std::vector<double> w(nv), phi(std::max(nq, 1), 0.0);
+   FSVD_CHECK(hipMemcpyAsync(w.data(), dw.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
    FSVD_CHECK(hipStreamSynchronize(s));
+   problem_potentials(pr, w.data(), phi.data(), false);
    double acc = 0.0;
-   for (long long k = 0; k < nv; k++) acc += lam[k] * a[k];
+   for (int j = 0; j < nq; j++) acc += pr.q[j] * phi[j];
    *dedk = -0.5 * acc;
    return si;
 }

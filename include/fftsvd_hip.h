@@ -114,7 +114,8 @@ int fftsvd_problem_grad_charges(fftsvd_problem* p, int precision, const double* 
                                 double* grad, int* iters);
 /* dE/dkappa of the same energy at the forward solution sol (2N, host), in
  * the units of fftsvd_problem_grad_charges: -1/2 lambda^T (dA/dkappa) x
- * with A^T lambda = C^T q.  dA/dkappa is the derivative of each Yukawa
+ * with A^T lambda = C^T q, evaluated in tangent form as -1/2 q^T C w with
+ * A w = (dA/dkappa) x (one forward solve; iters reports it).  dA/dkappa is the derivative of each Yukawa
  * region's compressed operator as it is applied, at fixed tree, ranks and
  * bases: near-field blocks of the kappa-derivative of the reference's
  * Yukawa scheme (FlatIntegration.c:473-949 differentiated term by term),
