@@ -1134,7 +1134,8 @@ template <int n, typename T>
 __global__ void __launch_bounds__(256) k_farT_target(const SrcDesc* __restrict__ src, const T* __restrict__ VT,
                                                      const T* __restrict__ PV,
                                                      const typename Cplx<T>::type* __restrict__ acc, int layer,
-                                                     double* __restrict__ yfar, int maxr) {
+                                                     double* __restrict__ yfar, double* __restrict__ yfar2,
+                                                     int maxr) {
    using C2 = typename Cplx<T>::type;
    constexpr int N = FarSizes<n>::N, H = FarSizes<n>::H, PAD = FarSizes<n>::PAD, GP3 = FarSizes<n>::GP3;
    extern __shared__ __align__(16) unsigned char smem[];
@@ -1152,36 +1153,51 @@ __global__ void __launch_bounds__(256) k_farT_target(const SrcDesc* __restrict__
    __syncthreads();
    grid_backward<n, T>(S, C1, Cc, h, cs, sn);
    __syncthreads();
-   const T* pv = PV + d.pv_off + (layer ? (long long)d.r * GP3 : 0);
-   for (int k = wave; k < d.r; k += nw) {
-      const T* row = pv + (long long)k * GP3;
-      T a = 0;
-      for (int q = lane; q < GP3; q += 64) a += row[q] * h[q];
-      a = wave_sum(a);
-      if (lane == 0) w[k] = a;
+   // yfar2 set: both layers from the one inverse transform (single layer
+   // into yfar, double layer into yfar2), each summed as alone
+   const int nl = yfar2 ? 2 : 1;
+   for (int li = 0; li < nl; li++) {
+      const T* pv = PV + d.pv_off + ((yfar2 ? li : layer) ? (long long)d.r * GP3 : 0);
+      for (int k = wave; k < d.r; k += nw) {
+         const T* row = pv + (long long)k * GP3;
+         T a = 0;
+         for (int q = lane; q < GP3; q += 64) a += row[q] * h[q];
+         a = wave_sum(a);
+         if (lane == 0) w[li * d.r + k] = a;
+      }
    }
    __syncthreads();
    const T* vt = VT + d.vt_off;
-   for (int i = tid; i < d.np; i += blockDim.x) {
-      T a = 0;
-      for (int k = 0; k < d.r; k++) a += vt[(long long)k * d.np + i] * w[k];
-      yfar[d.p0 + i] += (double)a;
+   for (int li = 0; li < nl; li++) {
+      double* out = li ? yfar2 : yfar;
+      for (int i = tid; i < d.np; i += blockDim.x) {
+         T a = 0;
+         for (int k = 0; k < d.r; k++) a += vt[(long long)k * d.np + i] * w[li * d.r + k];
+         out[d.p0 + i] += (double)a;
+      }
    }
 }
 
 // Near field transposed: y[cols] += D_layer^T x[rows] (global atomics; columns
 // are shared between neighbouring leaves).
+// With y2: both layers (single into y, double into y2) from one pass over
+// the block.
 template <typename T>
 __global__ void __launch_bounds__(256) k_nearT(const LeafDesc* __restrict__ leaves, const int* __restrict__ dcol,
                                                const T* __restrict__ D, const double* __restrict__ x, int layer,
-                                               double* __restrict__ y) {
+                                               double* __restrict__ y, double* __restrict__ y2) {
    const LeafDesc L = leaves[blockIdx.x];
-   const T* blk = D + L.d_off + (layer ? L.p : 0);
+   const T* blk = D + L.d_off + ((layer && !y2) ? L.p : 0);
    for (int c = threadIdx.x; c < L.m; c += blockDim.x) {
       const T* colp = blk + (long long)c * 2 * L.p;
       double a = 0.0;
       for (int r = 0; r < L.p; r++) a += (double)colp[r] * x[L.q0 + r];
       atomicAdd(&y[dcol[L.col_off + c]], a);
+      if (y2) {
+         double b = 0.0;
+         for (int r = 0; r < L.p; r++) b += (double)colp[L.p + r] * x[L.q0 + r];
+         atomicAdd(&y2[dcol[L.col_off + c]], b);
+      }
    }
 }
 
@@ -1358,20 +1374,20 @@ void launch_farT_source(int n, int nblk, const TgtDesc* tgt, const T* UT, const 
 
 template <typename T>
 void launch_farT_target(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const void* acc, int layer,
-                        double* yfar, int maxr, hipStream_t s) {
-   size_t lds = far_target_lds<T>(n, maxr);
+                        double* yfar, double* yfar2, int maxr, hipStream_t s) {
+   size_t lds = far_target_lds<T>(n, yfar2 ? 2 * maxr : maxr);
    check_lds(lds);
 #define CALL(NN)                                                                                                    \
    hipLaunchKernelGGL((k_farT_target<NN, T>), dim3(nblk), dim3(256), lds, s, src, VT, PV,                          \
-                      (const typename Cplx<T>::type*)acc, layer, yfar, maxr)
+                      (const typename Cplx<T>::type*)acc, layer, yfar, yfar2, maxr)
    FSVD_DISPATCH_N(n, CALL)
 #undef CALL
 }
 
 template <typename T>
 void launch_nearT(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* x, int layer,
-                  double* y, hipStream_t s) {
-   hipLaunchKernelGGL((k_nearT<T>), dim3(nleaf), dim3(256), 0, s, leaves, dcol, D, x, layer, y);
+                  double* y, double* y2, hipStream_t s) {
+   hipLaunchKernelGGL((k_nearT<T>), dim3(nleaf), dim3(256), 0, s, leaves, dcol, D, x, layer, y, y2);
 }
 
 void launch_zero(double* a, long long n, hipStream_t s) {
@@ -1387,9 +1403,11 @@ void launch_zero(double* a, long long n, hipStream_t s) {
    template void launch_transpose_batched<T>(int, const TransDesc*, hipStream_t);                                    \
    template void launch_farT_source<T>(int, int, const TgtDesc*, const T*, const T*, const double*, void*, int,       \
                                        hipStream_t);                                                                  \
-   template void launch_farT_target<T>(int, int, const SrcDesc*, const T*, const T*, const void*, int, double*, int,  \
+   template void launch_farT_target<T>(int, int, const SrcDesc*, const T*, const T*, const void*, int, double*,     \
+                                       double*, int,                                                              \
                                        hipStream_t);                                                                  \
-   template void launch_nearT<T>(int, const LeafDesc*, const int*, const T*, const double*, int, double*, hipStream_t);
+   template void launch_nearT<T>(int, const LeafDesc*, const int*, const T*, const double*, int, double*, double*,   \
+                                 hipStream_t);
 FSVD_INST(float)
 FSVD_INST(double)
 

@@ -1502,8 +1502,9 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
    }
 }
 
+// y2 set: both layers from one pass (single layer into y, double into y2)
 template <typename T>
-static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipStream_t s) {
+static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipStream_t s, double* y2 = nullptr) {
    const bool f32 = std::is_same<T, float>::value;
    for (FarLevel& lv : op.levels)
       if (!lv.specT.p) {
@@ -1511,6 +1512,7 @@ static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipSt
          lv.accT.alloc(2 * (size_t)std::max(1, lv.nsrc) * lv.pad);
       }
    launch_zero(y, op.tree.npan, s);
+   if (y2) launch_zero(y2, op.tree.npan, s);
    for (FarLevel& lv : op.levels)
       if (lv.ntgt)
          launch_farT_source<T>(lv.n, lv.ntgt, lv.tgt.p, f32 ? (const T*)lv.UT32.p : (const T*)lv.UT64.p,
@@ -1523,9 +1525,20 @@ static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipSt
    for (FarLevel& lv : op.levels)
       if (lv.nsrc)
          launch_farT_target<T>(lv.n, lv.nsrc, lv.src.p, f32 ? (const T*)lv.VT32.p : (const T*)lv.VT64.p,
-                               f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, lv.accT.p, layer, y, lv.maxr_src, s);
+                               f32 ? (const T*)lv.PV32.p : (const T*)lv.PV64.p, lv.accT.p, layer, y, y2, lv.maxr_src,
+                               s);
    if (op.nleaf)
-      launch_nearT<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, x, layer, y, s);
+      launch_nearT<T>(op.nleaf, op.leaves.p, op.dcol.p, f32 ? (const T*)op.D32.p : (const T*)op.D64.p, x, layer, y, y2,
+                      s);
+}
+
+void tree_apply_transpose_both(TreeOp& op, int prec, const double* x, double* ys, double* yd, hipStream_t s) {
+   if (prec == PREC_F32) {
+      if (!op.have32) throw std::runtime_error("operator has no FP32 copy");
+      applyT_impl<float>(op, 0, x, ys, s, yd);
+   } else {
+      applyT_impl<double>(op, 0, x, ys, s, yd);
+   }
 }
 
 void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s) {

@@ -202,6 +202,9 @@ void build_dkappa(TreeOp& op, const PanelGeom* d_panels, hipStream_t s);
 void tree_apply_dkappa(TreeOp& op, const double* xs, const double* xd, double* y, hipStream_t s);
 // y_t = A_layer^T x_t (panels, tree order).
 void tree_apply_transpose(TreeOp& op, int prec, int layer, const double* x, double* y, hipStream_t s);
+// Both layers at once, ys = S^T x and yd = D^T x: the points-side
+// transforms and the translation are shared by the two layers.
+void tree_apply_transpose_both(TreeOp& op, int prec, const double* x, double* ys, double* yd, hipStream_t s);
 
 // Block map of a launch over n logical blocks with the given work: block b
 // of the returned grid runs logical block map[b] (-1: idle); XCD x (blocks
@@ -223,10 +226,10 @@ void launch_farT_source(int n, int nblk, const TgtDesc* tgt, const T* UT, const 
                         int maxr, hipStream_t s);
 template <typename T>
 void launch_farT_target(int n, int nblk, const SrcDesc* src, const T* VT, const T* PV, const void* acc, int layer,
-                        double* yfar, int maxr, hipStream_t s);
+                        double* yfar, double* yfar2, int maxr, hipStream_t s);
 template <typename T>
 void launch_nearT(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D, const double* x, int layer,
-                  double* y, hipStream_t s);
+                  double* y, double* y2, hipStream_t s);
 void launch_zero(double* a, long long n, hipStream_t s);
 void launch_integral_blocks(int nblk, const IntBlock* blocks, const double* bpts, const int* bpan,
                             const PanelGeom* panels, int kernel, double kappa, double* out, hipStream_t s);

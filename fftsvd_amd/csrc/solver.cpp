@@ -624,8 +624,7 @@ static void adjoint_matvec(Problem& pr, int prec, const XT* y, double* x) {
       TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
       if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
       launch_gather_out<XT>(n, R.out_idx.p, y, R.yt.p, s);
-      tree_apply_transpose(op, prec, 0, R.yt.p, R.xs.p, s);
-      tree_apply_transpose(op, prec, 1, R.yt.p, R.xd.p, s);
+      tree_apply_transpose_both(op, prec, R.yt.p, R.xs.p, R.xd.p, s);
       launch_scatter_adj(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, R.corr.p, R.yt.p, R.xs.p, R.xd.p, x, s);
    }
 }
