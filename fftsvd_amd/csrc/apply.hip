@@ -239,7 +239,7 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
       const T* pc = colp + (long long)c * E;
       T d[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) d[u] = pc[u * step];
+      for (int u = 0; u < 8; u++) d[u] = __builtin_nontemporal_load(pc + u * step);
       a0 += d[0] * xl[c];
       a1 += d[1] * xl[c + G];
       a2 += d[2] * xl[c + 2 * G];
@@ -254,7 +254,7 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
       const T* pc = colp + (long long)c * E;
       T d[7];
 #pragma unroll
-      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? pc[u * step] : (T)0;
+      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? __builtin_nontemporal_load(pc + u * step) : (T)0;
 #pragma unroll
       for (int u = 0; u < 7; u++)
          if (c + u * G < m) a0 += d[u] * xl[c + u * G];
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
          for (; i + 7 * G < m; i += 8 * G) {
             T v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = colp[(long long)i * E + u * step];
+            for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(colp + (long long)i * E + u * step);
 #pragma unroll
             for (int u = 0; u < 8; u += 2) {
                s0 += v[u] * xl[i + u * G];
@@ -325,7 +325,8 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
             // the last partial batch, all of its loads in flight together
             T v[7];
 #pragma unroll
-            for (int u = 0; u < 7; u++) v[u] = i + u * G < m ? colp[(long long)i * E + u * step] : (T)0;
+            for (int u = 0; u < 7; u++)
+               v[u] = i + u * G < m ? __builtin_nontemporal_load(colp + (long long)i * E + u * step) : (T)0;
 #pragma unroll
             for (int u = 0; u < 7; u++)
                if (i + u * G < m) {
@@ -403,7 +404,7 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
          for (; k + 7 * KG < R2; k += 8 * KG) {
             T v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = pv[(long long)(k + u * KG) * GP3 + q];
+            for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(pv + (long long)(k + u * KG) * GP3 + q);
             a0 += v[0] * t[k];
             a1 += v[1] * t[k + KG];
             a2 += v[2] * t[k + 2 * KG];
@@ -417,7 +418,8 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
             // the last partial batch, all of its loads in flight together
             T v[7];
 #pragma unroll
-            for (int u = 0; u < 7; u++) v[u] = k + u * KG < R2 ? pv[(long long)(k + u * KG) * GP3 + q] : (T)0;
+            for (int u = 0; u < 7; u++)
+               v[u] = k + u * KG < R2 ? __builtin_nontemporal_load(pv + (long long)(k + u * KG) * GP3 + q) : (T)0;
 #pragma unroll
             for (int u = 0; u < 7; u++)
                if (k + u * KG < R2) a0 += v[u] * t[k + u * KG];
