@@ -401,13 +401,87 @@ void assignRadiiCharges(PDBentry* PDBentries, unsigned int numPDBentries, SIZent
    }
 }
 
-// Flat dielectric surface for a molecule with this library's SES mesher
-// (in place of the reference's external meshmaker + MSMS run,
-// FFTSVDpbeAPI.c:1034-1040): atoms from a PDB, CRD or XYZR file (by
+// One connected component of a mesh: vertices, 0-based faces (MSMS
+// orientation) and the signed volume they enclose.
+namespace {
+struct MeshComp {
+   std::vector<double> V;
+   std::vector<int> F;
+   double vol = 0.0;
+};
+
+std::vector<MeshComp> split_components(const double* V, int nv, const int* F, const int* comp_faces, int ncomp) {
+   std::vector<MeshComp> out(ncomp);
+   int f0 = 0;
+   for (int k = 0; k < ncomp; k++) {
+      MeshComp& c = out[k];
+      std::vector<int> remap(nv, -1);
+      for (int q = f0; q < f0 + comp_faces[k]; q++)
+         for (int s = 0; s < 3; s++) {
+            const int v = F[3 * q + s];
+            if (remap[v] < 0) {
+               remap[v] = (int)(c.V.size() / 3);
+               c.V.insert(c.V.end(), V + 3 * (size_t)v, V + 3 * (size_t)v + 3);
+            }
+            c.F.push_back(remap[v]);
+         }
+      f0 += comp_faces[k];
+      for (size_t q = 0; q < c.F.size(); q += 3) {
+         const double* a = &c.V[3 * (size_t)c.F[q]];
+         const double* b = &c.V[3 * (size_t)c.F[q + 1]];
+         const double* d = &c.V[3 * (size_t)c.F[q + 2]];
+         c.vol += (a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                   a[2] * (b[0] * d[1] - b[1] * d[0])) / 6.0;
+      }
+   }
+   return out;
+}
+
+// Winding number of the closed, positively oriented surface `s` about p:
+// the solid angles of its faces (Van Oosterom and Strackee) over 4 pi;
+// 1 inside, 0 outside.
+double winding(const MeshComp& s, const double* p) {
+   double w = 0.0;
+   for (size_t q = 0; q < s.F.size(); q += 3) {
+      double r[3][3], l[3];
+      for (int k = 0; k < 3; k++) {
+         for (int d = 0; d < 3; d++) r[k][d] = s.V[3 * (size_t)s.F[q + k] + d] - p[d];
+         l[k] = std::sqrt(r[k][0] * r[k][0] + r[k][1] * r[k][1] + r[k][2] * r[k][2]);
+      }
+      const double det = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]) -
+                         r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]) +
+                         r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+      auto dot = [&](int i, int j) { return r[i][0] * r[j][0] + r[i][1] * r[j][1] + r[i][2] * r[j][2]; };
+      const double den = l[0] * l[1] * l[2] + dot(0, 1) * l[2] + dot(0, 2) * l[1] + dot(1, 2) * l[0];
+      w += 2.0 * std::atan2(det, den);
+   }
+   return w / (4.0 * M_PI);
+}
+
+// Every vertex of `inner` inside `outer` (meshmaker's isInsideExhaustive);
+// with `all` false, its first vertex only (isInside).
+bool comp_inside(const MeshComp& inner, const MeshComp& outer, bool all) {
+   const size_t nv = inner.V.size() / 3;
+   for (size_t v = 0; v < (all ? nv : std::min<size_t>(nv, 1)); v++)
+      if (!(winding(outer, &inner.V[3 * v]) > 0.5)) return false;
+   return nv > 0;
+}
+}  // namespace
+
+// Flat surfaces for a molecule with this library's SES mesher, in place of
+// the reference's external meshmaker + MSMS run (FFTSVDpbeAPI.c:1034-1040;
+// meshmaker.c:119-262,600-690): atoms from a PDB, CRD or XYZR file (by
 // extension), radii from the SIZ table (XYZR carries its own), probe
 // `proberadius`, about `dielectricvertexdensity` vertices per square
-// angstrom.  Writes <srf>.d0.vert / .face next to the .srf and the .srf.
-// Salt (Stern) surfaces are not generated: that formulation is rejected.
+// angstrom.  As meshmaker does with MSMS's components, every connected
+// component enclosing a positive volume is a dielectric surface and every
+// other one a dielectric cavity; with salt and an ion-exclusion radius the
+// grown-sphere surface is meshed too (salt surfaces and salt cavities).
+// Parents as meshmaker assigns them: a dielectric the first salt surface
+// containing it, a dielectric cavity the first dielectric containing all
+// its vertices, a salt cavity the first dielectric cavity containing it;
+// 999 where none does.  Writes <srf>.d<k>, .c<k>, .s<k>, .l<k> .vert/.face
+// next to the .srf, and the .srf.
 void generateFlatSRF(const char* pdbfilename, const char* sizfilename, const char* srffilename) {
    // with salt and an ion-exclusion radius (the Stern layer) the salt
    // surface is meshed too: the union of the atom spheres grown by
@@ -446,34 +520,64 @@ void generateFlatSRF(const char* pdbfilename, const char* sizfilename, const cha
    const std::string srf(srffilename);
    const size_t slash = srf.rfind('/');
    const std::string stem = slash == std::string::npos ? srf : srf.substr(slash + 1);
-   // meshes one surface into <srf><tag>.vert/.face; the mesher's vertex
-   // density is about 4.5 / h^2 per square angstrom
-   auto mesh_to = [&](const char* tag, double density, bool grown) {
+   // the components of one surface kind, outer surfaces then cavities; the
+   // mesher's vertex density is about 4.5 / h^2 per square angstrom
+   auto mesh = [&](double density, bool grown, std::vector<MeshComp>& surfs, std::vector<MeshComp>& cavs) {
       const double h = std::sqrt(4.5 / (density > 0.0 ? density : 1.0));
       double* V = nullptr;
-      int* F = nullptr;
-      int nv = 0, nf = 0;
-      const int rc = grown ? fftsvd_mesh_union(xyzr.data(), (int)(xyzr.size() / 4), ionexclusionradius, h, &V, &nv,
-                                               &F, &nf)
-                           : fftsvd_mesh_ses(xyzr.data(), (int)(xyzr.size() / 4), probe, h, &V, &nv, &F, &nf);
+      int *F = nullptr, *cf = nullptr;
+      int nv = 0, nf = 0, nc = 0;
+      const int rc = fftsvd_mesh_components(xyzr.data(), (int)(xyzr.size() / 4), grown ? ionexclusionradius : probe, h,
+                                            grown ? 1 : 0, &V, &nv, &F, &nf, &cf, &nc);
       if (rc) error((char*)"surface generation failed for %s", pdbfilename);
-      const std::string base = srf + tag;
-      FILE* fv = open_or_die("vert", (base + ".vert").c_str(), "w");
-      for (int i = 0; i < nv; i++) std::fprintf(fv, "%.9f %.9f %.9f\n", V[3 * i], V[3 * i + 1], V[3 * i + 2]);
-      std::fclose(fv);
-      FILE* ff = open_or_die("face", (base + ".face").c_str(), "w");
-      for (int i = 0; i < nf; i++) std::fprintf(ff, "%d %d %d 1\n", F[3 * i] + 1, F[3 * i + 1] + 1, F[3 * i + 2] + 1);
-      std::fclose(ff);
+      for (MeshComp& c : split_components(V, nv, F, cf, nc)) (c.vol > 0.0 ? surfs : cavs).push_back(std::move(c));
       fftsvd_free(V);
       fftsvd_free(F);
+      fftsvd_free(cf);
+      if (surfs.empty()) error((char*)"No surface with positive volume was created");
+   };
+   auto write = [&](const std::string& tag, const MeshComp& c) {
+      const std::string base = srf + tag;
+      FILE* fv = open_or_die("vert", (base + ".vert").c_str(), "w");
+      for (size_t i = 0; i < c.V.size(); i += 3) std::fprintf(fv, "%.9f %.9f %.9f\n", c.V[i], c.V[i + 1], c.V[i + 2]);
+      std::fclose(fv);
+      FILE* ff = open_or_die("face", (base + ".face").c_str(), "w");
+      for (size_t i = 0; i < c.F.size(); i += 3)
+         std::fprintf(ff, "%d %d %d 1\n", c.F[i] + 1, c.F[i + 1] + 1, c.F[i + 2] + 1);
+      std::fclose(ff);
       return stem + tag;
    };
-   const std::string dname = mesh_to(".d0", dielectricvertexdensity, false);
-   const std::string sname = stern ? mesh_to(".s0", saltvertexdensity, true) : std::string();
+   std::vector<MeshComp> diel, dcav, salt, scav;
+   mesh(dielectricvertexdensity, false, diel, dcav);
+   if (stern) mesh(saltvertexdensity, true, salt, scav);
+   // a cavity, oriented into its void, encloses the void with the opposite
+   // sign: flip it for the containment tests of the salt cavities
+   std::vector<MeshComp> dcav_pos = dcav;
+   for (MeshComp& c : dcav_pos)
+      for (size_t q = 0; q < c.F.size(); q += 3) std::swap(c.F[q + 1], c.F[q + 2]);
+   auto names = [&](const char* kind, const std::vector<MeshComp>& cs) {
+      std::string line;
+      for (size_t k = 0; k < cs.size(); k++) line += (k ? " " : "") + write("." + std::string(kind) + std::to_string(k), cs[k]);
+      return line;
+   };
+   auto parent = [](const MeshComp& c, const std::vector<MeshComp>& outers, bool all) {
+      for (size_t k = 0; k < outers.size(); k++)
+         if (comp_inside(c, outers[k], all)) return std::to_string(k);
+      return std::string("999");
+   };
+   auto parents = [&](const std::vector<MeshComp>& cs, const std::vector<MeshComp>& outers, bool all) {
+      std::string line;
+      for (size_t k = 0; k < cs.size(); k++) line += (k ? " " : "") + parent(cs[k], outers, all);
+      return line;
+   };
+   const std::string sline = names("s", salt), dline = names("d", diel), cline = names("c", dcav),
+                     lline = names("l", scav);
    // .srf: panel types, salt surfaces, dielectrics, dielectric cavities,
    // salt cavities, then the parents of dielectrics and of both cavity kinds
    FILE* fs = open_or_die("surface", srffilename, "w");
-   std::fprintf(fs, "f\nf\n%s\n%s\n\n\n0\n\n\n", sname.c_str(), dname.c_str());
+   std::fprintf(fs, "f\nf\n%s\n%s\n%s\n%s\n%s\n%s\n%s\n", sline.c_str(), dline.c_str(), cline.c_str(), lline.c_str(),
+                parents(diel, salt, false).c_str(), parents(dcav, diel, true).c_str(),
+                parents(scav, dcav_pos, false).c_str());
    std::fclose(fs);
 }
 

@@ -8,8 +8,9 @@
 // grown spheres.  f is sampled on a regular grid and contoured by marching
 // tetrahedra (six Kuhn tetrahedra per cell, so neighbouring cells agree and
 // the surface is closed).  Triangles are oriented counter-clockwise seen
-// from outside (the MSMS convention) and only the largest connected
-// component is kept, which drops internal cavities.
+// from the solute into the solvent (the MSMS convention: outward on an
+// outer surface, into the void on a cavity).  fftsvd_mesh_ses keeps the
+// largest connected component; fftsvd_mesh_components returns all of them.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -105,8 +106,11 @@ struct UF {
 // the surface of the union of the atom spheres grown by `probe` (union_mode
 // true: the ion-exclusion surface, which the reference's meshmaker asks of
 // MSMS with a vanishing probe, meshmaker.c:105-133).
+// comp_faces set: every connected component, largest first (faces and
+// vertices of a component contiguous), its face counts in *comp_faces;
+// otherwise only the largest component.
 int mesh_impl(const double* xyzr, int natoms, double probe, double h, bool union_mode, double** verts_out,
-              int* nverts, int** faces_out, int* nfaces) {
+              int* nverts, int** faces_out, int* nfaces, std::vector<int>* comp_faces = nullptr) {
    {
       if (natoms < 1 || !(h > 0) || !(probe > 0)) throw std::runtime_error("bad mesher arguments");
       std::vector<double> ctr(3 * (size_t)natoms);
@@ -282,27 +286,34 @@ int mesh_impl(const double* xyzr, int natoms, double probe, double h, bool union
          uf.unite(Fc[3 * q], Fc[3 * q + 1]);
          uf.unite(Fc[3 * q], Fc[3 * q + 2]);
       }
-      std::unordered_map<int, int> compsize;
-      for (int q = 0; q < nf; q++) compsize[uf.find(Fc[3 * q])]++;
-      int best = -1, bestn = -1;
-      for (auto& kv : compsize)
-         if (kv.second > bestn) {
-            bestn = kv.second;
-            best = kv.first;
-         }
+      std::unordered_map<int, int> compsize, firstface;
+      for (int q = 0; q < nf; q++) {
+         const int r = uf.find(Fc[3 * q]);
+         if (compsize[r]++ == 0) firstface[r] = q;
+      }
+      // components by face count, largest first (ties by first face)
+      std::vector<std::pair<int, int>> order;
+      for (auto& kv : compsize) order.push_back({kv.first, kv.second});
+      std::sort(order.begin(), order.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+         return a.second != b.second ? a.second > b.second : firstface[a.first] < firstface[b.first];
+      });
+      if (!comp_faces) order.resize(1);
       std::vector<int> remap(nv, -1);
       std::vector<double> Vo;
       std::vector<int> Fo;
-      for (int q = 0; q < nf; q++) {
-         if (uf.find(Fc[3 * q]) != best) continue;
-         for (int s = 0; s < 3; s++) {
-            int v = Fc[3 * q + s];
-            if (remap[v] < 0) {
-               remap[v] = (int)(Vo.size() / 3);
-               Vo.insert(Vo.end(), &V[3 * (size_t)v], &V[3 * (size_t)v] + 3);
+      for (const auto& c : order) {
+         for (int q = 0; q < nf; q++) {
+            if (uf.find(Fc[3 * q]) != c.first) continue;
+            for (int s = 0; s < 3; s++) {
+               int v = Fc[3 * q + s];
+               if (remap[v] < 0) {
+                  remap[v] = (int)(Vo.size() / 3);
+                  Vo.insert(Vo.end(), &V[3 * (size_t)v], &V[3 * (size_t)v] + 3);
+               }
+               Fo.push_back(remap[v]);
             }
-            Fo.push_back(remap[v]);
          }
+         if (comp_faces) comp_faces->push_back(c.second);
       }
       *nverts = (int)(Vo.size() / 3);
       *nfaces = (int)(Fo.size() / 3);
@@ -334,6 +345,21 @@ int fftsvd_mesh_union(const double* xyzr, int natoms, double grow, double h, dou
       return mesh_impl(xyzr, natoms, grow, h, true, verts_out, nverts, faces_out, nfaces);
    } catch (const std::exception& e) {
       std::fprintf(stderr, "fftsvd_mesh_union: %s\n", e.what());
+      return -1;
+   }
+}
+
+int fftsvd_mesh_components(const double* xyzr, int natoms, double probe, double h, int union_mode, double** verts_out,
+                           int* nverts, int** faces_out, int* nfaces, int** comp_faces, int* ncomp) {
+   try {
+      std::vector<int> cf;
+      const int rc = mesh_impl(xyzr, natoms, probe, h, union_mode != 0, verts_out, nverts, faces_out, nfaces, &cf);
+      *ncomp = (int)cf.size();
+      *comp_faces = (int*)std::malloc(sizeof(int) * std::max<size_t>(cf.size(), 1));
+      std::memcpy(*comp_faces, cf.data(), sizeof(int) * cf.size());
+      return rc;
+   } catch (const std::exception& e) {
+      std::fprintf(stderr, "fftsvd_mesh_components: %s\n", e.what());
       return -1;
    }
 }

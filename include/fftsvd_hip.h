@@ -197,6 +197,16 @@ int fftsvd_mesh_ses(const double* xyzr, int natoms, double probe, double h, doub
  * and a vanishing probe (meshmaker.c:105-133). */
 int fftsvd_mesh_union(const double* xyzr, int natoms, double grow, double h, double** verts, int* nverts,
                       int** faces, int* nfaces);
+/* Every connected component of the SES (union_mode 0, probe) or of the
+ * grown-sphere union (union_mode 1, grow), largest first: the faces and the
+ * vertices of a component are contiguous and comp_faces[k] counts the
+ * faces of component k (ncomp of them).  Faces keep the MSMS convention
+ * (counter-clockwise seen from the solvent side), so an outer surface
+ * encloses a positive volume and a cavity a negative one, the split the
+ * reference's meshmaker makes from MSMS's -all_components output
+ * (meshmaker.c:119-262).  Free the three arrays with fftsvd_free. */
+int fftsvd_mesh_components(const double* xyzr, int natoms, double probe, double h, int union_mode, double** verts,
+                           int* nverts, int** faces, int* nfaces, int** comp_faces, int* ncomp);
 void fftsvd_free(void* p);
 
 /* Precision used by solveSurfaceOperator (0 FP64, 1 mixed); default 0. */

@@ -266,3 +266,44 @@ def test_generate_flat_srf_stern(tmp_path):
     vs_pts = Ps.reshape(-1, 3)
     inside = np.all((vs_pts > lo + 1.0) & (vs_pts < hi - 1.0), axis=1)
     assert not inside.any()
+
+
+def hollow_shell(n=400, R=7.0):
+    """Atom centres on a Fibonacci sphere: a closed shell around a void the
+    probe fits in."""
+    i = np.arange(n) + 0.5
+    phi, th = np.arccos(1 - 2 * i / n), np.pi * (1 + 5 ** 0.5) * i
+    return np.c_[R * np.cos(th) * np.sin(phi), R * np.sin(th) * np.sin(phi), R * np.cos(phi)]
+
+
+def test_generate_flat_srf_with_cavity(tmp_path):
+    """A shell of atoms around a void: the mesher's components give one
+    dielectric surface (positive enclosed volume) and one dielectric cavity
+    (negative), written as meshmaker writes MSMS's components
+    (meshmaker.c:119-262,600-690): the dielectric's parent 999 without a
+    salt surface, the cavity's parent 0 (all its vertices inside the
+    dielectric).  readSRF reads the cavity without the face flip, so its
+    panels enclose the void with the normals pointing out of it."""
+    xyz = hollow_shell()
+    (tmp_path / "m.xyzr").write_text("".join("%.3f %.3f %.3f 1.8\n" % tuple(p) for p in xyz))
+    refapi.set_global("proberadius", 1.4)
+    refapi.set_global("dielectricvertexdensity", 2.0)
+    refapi.set_global("ionicstrength", 0.0)
+    srf = tmp_path / "op.srf"
+    L.generateFlatSRF(str(tmp_path / "m.xyzr").encode(), b"", str(srf).encode())
+    lines = srf.read_text().split("\n")
+    assert lines[2:9] == ["", "op.srf.d0", "op.srf.c0", "", "999", "0", ""]
+    s = refapi.read_srf(srf)
+    assert [x.value for x in s.num] == [0, 1, 1, 0]
+    assert s.parents[1][0] == 0
+    Pd, Pc = panels_of(s, 1, 0).reshape(-1, 3, 3), panels_of(s, 2, 0).reshape(-1, 3, 3)
+    assert s.total.value == len(Pd) + len(Pc)
+    closed_d, vol_d = _closed_outward(Pd)
+    closed_c, vol_c = _closed_outward(Pc)
+    assert closed_d < 1e-8 and closed_c < 1e-8
+    # the void: probe centres within 7 - 1.8 - 1.4 = 3.8 of the centre, the
+    # cavity surface about 1.4 beyond them (a sphere of 5.2 would hold 589)
+    assert 250 < vol_c < 650
+    assert vol_d > 4.0 / 3.0 * np.pi * 7.0 ** 3
+    rc = np.linalg.norm(Pc.mean(axis=1), axis=1)
+    assert rc.max() < 5.6 and rc.min() > 3.5

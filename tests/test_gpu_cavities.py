@@ -185,3 +185,51 @@ def test_reference_sequence_with_cavities(tmp_path):
     np.testing.assert_allclose(pot, 561.0 * phi / 4.0, rtol=1e-6, atol=1e-9 * np.abs(pot).max())
     refapi.free(rhs)
     refapi.free(sol)
+
+
+def test_driver_meshes_and_solves_a_cavity(tmp_path):
+    """The drop-in driver on a shell of atoms around a void: generateFlatSRF
+    writes the void as a dielectric cavity (parent 0), the solve runs with
+    the cavity region, and the FRC potentials equal the extended API's on
+    the same two surfaces (and differ from the outer surface alone)."""
+    import subprocess
+
+    from fftsvd_amd import build
+    from test_frontend_host import hollow_shell, pdb_line
+    from test_gpu_pbe_driver import CHARGES, NAMES, RADII
+
+    xyz = np.round(hollow_shell(), 3)
+    lines = [pdb_line("ATOM", i + 1, NAMES[i % 4], "ALA", "A", i // 4 + 1, *p) for i, p in enumerate(xyz)]
+    for f in ("fort.13", "fort.15"):
+        (tmp_path / f).write_text("".join(lines) + "END\n")
+    (tmp_path / "fort.11").write_text("atom__res_radius_\n" +
+                                      "".join("%-6s%-3s%8.3f\n" % (k, "", v) for k, v in RADII.items()))
+    (tmp_path / "fort.12").write_text("atom__resnumbc_charge_\n" +
+                                      "".join("%-6s%-3s%-4s%1s%8.3f\n" % (k, "", "", "", v) for k, v in CHARGES.items()))
+    salt = 0.145
+    (tmp_path / "fort.10").write_text("indi=4.0\nexdi=80.0\nsalt=%g\nionrad=0\nprbrad=1.4\nde=1e-6\ndvert=2.0\n"
+                                      "svert=1.0\nenergy(g)\nout(frc)\n" % salt)
+    r = subprocess.run([build.PBE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    srf = (tmp_path / "operator.srf").read_text().split("\n")
+    assert srf[3:5] == ["operator.srf.d0", "operator.srf.c0"] and srf[7] == "0"
+    frc = (tmp_path / "fort.16").read_text().splitlines()
+    pot = np.array([float(l[35:50]) for l in frc[12:] if not l.startswith("total energy")])
+    q = np.array([CHARGES[NAMES[i % 4]] for i in range(len(xyz))])
+
+    def panels(tag, flip):
+        V = np.loadtxt(tmp_path / ("operator.srf.%s.vert" % tag))
+        F = np.loadtxt(tmp_path / ("operator.srf.%s.face" % tag), dtype=np.int64)[:, :3] - 1
+        return meshes.panels_from_mesh(V, F, flip=flip)
+
+    Pd, Pc = panels("d0", True), panels("c0", False)
+    kappa = np.sqrt(salt) / 3.047
+    prob = fa.Problem(Pd, xyz, q, 4.0, 80.0, kappa, cavities=[Pc])
+    x, _, _, _ = prob.solve(prob.rhs(), 1e-6, precision=fa.F64)
+    ref = 561.0 * prob.reaction_potentials(x) / 4.0
+    np.testing.assert_allclose(pot, ref, rtol=0, atol=2e-6 * np.abs(ref).max() + 1e-6)
+    outer = fa.Problem(Pd, xyz, q, 4.0, 80.0, kappa)
+    x0, _, _, _ = outer.solve(outer.rhs(), 1e-6, precision=fa.F64)
+    ref0 = 561.0 * outer.reaction_potentials(x0) / 4.0
+    assert np.abs(ref - ref0).max() > 1e-3 * np.abs(ref).max()
