@@ -45,6 +45,8 @@ _SIGS = {
     "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_grad_kappa": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
+    "fftsvd_set_autotune": (C.c_int, [C.c_int]),
+    "fftsvd_translation_params": (C.c_int, [_ip, _ip, _ip, _ip, _dp]),
     "fftsvd_problem_grad_charges": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
     "fftsvd_qual_create": (C.c_int, [_dp, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                      C.c_double, C.c_int, C.POINTER(_vp)]),
@@ -116,6 +118,21 @@ def copy_bandwidth(nbytes=1 << 30, nrep=20):
     g = C.c_double()
     _check(lib().fftsvd_copy_bandwidth(int(nbytes), int(nrep), C.byref(g)))
     return g.value
+
+
+def set_autotune(on):
+    """Autotune the FP32 translation of the next operator built (see
+    fftsvd_set_autotune); off restores the defaults."""
+    _check(lib().fftsvd_set_autotune(int(bool(on))))
+
+
+def translation_params():
+    """The FP32 translation parameters in use: dict(mode, budget, tile,
+    tuned, tuned_ms)."""
+    v = [C.c_int() for _ in range(4)]
+    ms = C.c_double()
+    _check(lib().fftsvd_translation_params(*[C.byref(x) for x in v], C.byref(ms)))
+    return dict(mode=v[0].value, budget=v[1].value, tile=v[2].value, tuned=bool(v[3].value), tuned_ms=ms.value)
 
 
 def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):

@@ -484,6 +484,24 @@ static TreeOp& region_op(Problem& pr, int which) {
  * 8 leaves, 9 unique trees, 10 algorithmic bytes per FP64 matvec,
  * 11 algorithmic bytes per FP32 matvec, 12 near bytes per FP64 matvec,
  * 13 near bytes per FP32 matvec, 14 near-field column indices (all trees) */
+int fftsvd_set_autotune(int on) {
+   API_TRY
+   set_autotune(on != 0);
+   API_CATCH
+}
+
+int fftsvd_translation_params(int* mode, int* budget, int* tile, int* tuned, double* tuned_ms) {
+   API_TRY
+   bool t = false;
+   const PipeParams pp = current_pipe_params(&t);
+   if (mode) *mode = pp.mode;
+   if (budget) *budget = pp.budget;
+   if (tile) *tile = pp.tile;
+   if (tuned) *tuned = t ? 1 : 0;
+   if (tuned_ms) *tuned_ms = pp.tuned_ms;
+   API_CATCH
+}
+
 int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout) {
    API_TRY
    Problem& pr = *p->pr;

@@ -148,13 +148,6 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
    return map;
 }
 
-// Per-level data the translation plans need.
-struct TrLevel {
-   int t0, nt;         // the level's targets: tgt[t0 .. t0 + nt)
-   int pad, pst, nsrc;
-   long long t_off;    // its T spectra (complex index; code c at t_off + c * pst)
-   long long sbase;    // its forward spectra: source slot k at sbase + k * pst
-};
 
 // Tiles of consecutive targets of one level: a tile grows while it has
 // fewer than max_tile targets and the union of their sources stays within
@@ -383,7 +376,7 @@ static void pipe_tile_job(PipeTileJob& J, const TrLevel& L, const std::vector<Tg
 }
 
 static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
-                                 const std::vector<int>& isrc, const std::vector<int>& itoff) {
+                                 const std::vector<int>& isrc, const std::vector<int>& itoff, const PipeParams& pp) {
    PipeHost& H = P.pipe_h;
    H = PipeHost();
    std::vector<PipeTileJob> jobs;
@@ -391,7 +384,7 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
    for (int li = 0; li < (int)lvls.size(); li++) {
       maxsrc = std::max(maxsrc, lvls[li].nsrc);
       make_tiles(
-          lvls[li], tgt, isrc, TP_TILE, TP_SRC_BUDGET, TP_MAX_SRC,
+          lvls[li], tgt, isrc, pp.tile, pp.budget, TP_MAX_SRC,
           [&](int k0, int k1, const std::vector<int>& so) {
              PipeTileJob J;
              J.level = li;
@@ -516,7 +509,7 @@ static std::vector<int4> assign_pipe_blocks(const std::vector<int4>& tasks, cons
    return out;
 }
 
-void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s) {
+void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s, int mode) {
    std::vector<TpTile> tiles;
    std::vector<TpOct> octs;
    std::vector<long long> oacc;
@@ -582,11 +575,7 @@ void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s)
                    (double)targets / std::max<size_t>(8 * octs.size(), 1));
    }
    d.nblk = nblk;
-   {
-      // staging by direct global->LDS transfers unless FFTSVD_TP_MODE=0
-      const char* m = std::getenv("FFTSVD_TP_MODE");
-      d.mode = (m && m[0] == '0') ? 0 : 1;
-   }
+   d.mode = mode;
    if (btasks.empty()) btasks.push_back(make_int4(0, 0, 0, 0));
    d.range.upload(range, s);
    d.tasks.upload(btasks, s);
@@ -596,6 +585,73 @@ void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s)
    d.gsrc.upload(gsrc, s);
    d.gint.upload(gint, s);
    FSVD_CHECK(hipStreamSynchronize(s));
+}
+
+// ---- autotuner of the pipelined translation ----
+// Staging mode, tile source budget and tile size of k_translate_pipe.  By
+// default the hand-tuned values; with autotuning on (fftsvd_set_autotune,
+// or FFTSVD_AUTOTUNE=1), the first FP32 operator of the process times each
+// candidate on its own translation (one warm-up and three timed launches,
+// HIP events on the setup stream) and the fastest is used from then on.
+namespace {
+std::mutex g_tune_mu;
+bool g_autotune = [] {
+   const char* e = std::getenv("FFTSVD_AUTOTUNE");
+   return e && e[0] == '1';
+}();
+bool g_tuned = false;
+PipeParams g_params;
+}  // namespace
+
+void set_autotune(bool on) {
+   std::lock_guard<std::mutex> g(g_tune_mu);
+   g_autotune = on;
+   g_tuned = false;
+   g_params = PipeParams();
+}
+
+PipeParams current_pipe_params(bool* tuned) {
+   std::lock_guard<std::mutex> g(g_tune_mu);
+   if (tuned) *tuned = g_tuned;
+   return g_params;
+}
+
+static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls, const std::vector<TgtPlan>& tgt,
+                                 const std::vector<int>& isrc, const std::vector<int>& itoff, const PipeParams& pp);
+
+static PipeParams pipe_params(TreeOp& op, hipStream_t s) {
+   std::lock_guard<std::mutex> g(g_tune_mu);
+   if (!g_autotune || g_tuned) return g_params;
+   ApplyPlan& P = op.plan;
+   const PipeParams cand[] = {{1, TP_SRC_BUDGET, TP_TILE}, {0, TP_SRC_BUDGET, TP_TILE}, {1, 720, TP_TILE},
+                              {1, TP_SRC_BUDGET, 256}};
+   hipEvent_t e0, e1;
+   FSVD_CHECK(hipEventCreate(&e0));
+   FSVD_CHECK(hipEventCreate(&e1));
+   PipeParams best = g_params;
+   float best_ms = 1e30f;
+   for (const PipeParams& c : cand) {
+      build_translate_pipe(P, P.tr_lv, P.tr_tgt, P.tr_isrc, P.tr_itoff, c);
+      PipeDev d;
+      make_pipe(d, {&op}, s, c.mode);
+      launch_pipe(d, s);
+      FSVD_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; r++) launch_pipe(d, s);
+      FSVD_CHECK(hipEventRecord(e1, s));
+      FSVD_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FSVD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+         best_ms = ms;
+         best = c;
+      }
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   best.tuned_ms = best_ms / 3.0f;
+   g_params = best;
+   g_tuned = true;
+   return best;
 }
 
 // Concatenates every level's far-field blocks into single buffers with
@@ -758,7 +814,12 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          sbase += (long long)lv.nsrc * tr_pst(lv.pad);
       }
       build_translate_f64(P, tl, tgt, isrc, itoff, s);
-      if (want32) build_translate_pipe(P, tl, tgt, isrc, itoff);
+      if (want32) {
+         P.tr_lv = tl;
+         P.tr_tgt = tgt;
+         P.tr_isrc = isrc;
+         P.tr_itoff = itoff;
+      }
       timing_mark("plan: translation", tm);
    }
    {
@@ -800,7 +861,11 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       to_f32(P.Tsp64, P.Tsp32, s);
       to_f32(P.UTI64, P.UTI32, s);
       to_f32(P.UT64, P.UT32, s);
-      make_pipe(P.pipe, {&op}, s);
+      const PipeParams pp = pipe_params(op, s);
+      build_translate_pipe(P, P.tr_lv, P.tr_tgt, P.tr_isrc, P.tr_itoff, pp);
+      make_pipe(P.pipe, {&op}, s, pp.mode);
+      op.stats.pipe = pp;
+      timing_mark("plan: pipelined translation", tm);
    }
 }
 

@@ -71,6 +71,16 @@ struct FarLevel {
    DBuf<float> VT32, PV32, UT32, UTI32;
 };
 
+// Parameters of the pipelined FP32 translation (the autotuner's choices):
+// staging mode (0 through registers, 1 direct global->LDS), the source
+// union at which tile growth stops, targets per tile at most.
+struct PipeParams {
+   int mode = 1, budget = TP_SRC_BUDGET, tile = TP_TILE;
+   float tuned_ms = 0.f;   // the chosen candidate's time per launch, when tuned
+};
+void set_autotune(bool on);
+PipeParams current_pipe_params(bool* tuned);
+
 struct TreeOpStats {
    long long near_entries = 0;     // stored near-field entries (both layers)
    long long far_entries = 0;      // VT + PV + U^T + UTI entries
@@ -82,6 +92,7 @@ struct TreeOpStats {
    // 2 pinv + T spectra, 3 sampling integrals, 4 pivoted MGS, 5 bases,
    // PV and UTI, 6 lists, FP32 copies and apply plan
    double phase_ms[8] = {0};
+   PipeParams pipe;   // the translation parameters used
 };
 
 // Pipelined FP32 translation plan of one tree, host form: tiles whose
@@ -110,6 +121,14 @@ struct PipeDev {
    DBuf<int> gsrc, gint;
 };
 
+// Per-level data the translation plans need.
+struct TrLevel {
+   int t0, nt;         // the level's targets: tgt[t0 .. t0 + nt)
+   int pad, pst, nsrc;
+   long long t_off;    // its T spectra (complex index; code c at t_off + c * pst)
+   long long sbase;    // its forward spectra: source slot k at sbase + k * pst
+};
+
 // Forward-apply plan of a tree: every level's blocks concatenated, with
 // absolute offsets, so each stage is one launch.
 struct ApplyPlan {
@@ -126,6 +145,11 @@ struct ApplyPlan {
    DBuf<int> gsrc, gint;
    PipeHost pipe_h;   // FP32 translation (k_translate_pipe) of this tree, host form
    PipeDev pipe;      // ... on the device
+   // the translation lists the pipelined plan is built from (kept for the
+   // autotuner's re-planning)
+   std::vector<TrLevel> tr_lv;
+   std::vector<TgtPlan> tr_tgt;
+   std::vector<int> tr_isrc, tr_itoff;
    DBuf<int> map_leaf;   // XCD block map of the leaf launch
    int nmap_leaf = 0;
    DBuf<LeafU> ublk;
@@ -171,7 +195,7 @@ struct TreeOp {
 
 // The device plan of one launch covering the FP32 translations of the given
 // trees (each with its own spectra), tasks dealt to the persistent blocks.
-void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s);
+void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s, int mode = 1);
 void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
                            const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s);
 inline void launch_pipe(const PipeDev& d, hipStream_t s) {

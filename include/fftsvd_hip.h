@@ -209,6 +209,15 @@ int fftsvd_mesh_components(const double* xyzr, int natoms, double probe, double 
                            int* nverts, int** faces, int* nfaces, int** comp_faces, int* ncomp);
 void fftsvd_free(void* p);
 
+/* Autotuning of the FP32 translation (staging mode, tile source budget,
+ * targets per tile): with on != 0 (or FFTSVD_AUTOTUNE=1 in the
+ * environment) the next FP32 operator built times each candidate on its own
+ * translation and the fastest is kept for the rest of the process; off
+ * restores the hand-tuned defaults.  fftsvd_translation_params reports the
+ * parameters in use, whether they were tuned and the chosen candidate's
+ * time per launch.  No counterpart in the reference. */
+int fftsvd_set_autotune(int on);
+int fftsvd_translation_params(int* mode, int* budget, int* tile, int* tuned, double* tuned_ms);
 /* Precision used by solveSurfaceOperator (0 FP64, 1 mixed); default 0. */
 int fftsvd_set_precision(int precision);
 

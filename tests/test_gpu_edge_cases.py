@@ -60,3 +60,30 @@ def test_charge_at_centroid():
     sol = np.random.default_rng(2).normal(size=2 * len(P))
     pg, po = g.reaction_potentials(sol), o.reaction_potentials(sol)
     assert np.abs(pg - po).max() <= 1e-12 * np.abs(po).max()
+
+
+def test_translation_autotuner():
+    """With autotuning on, the first FP32 operator times the translation
+    candidates and keeps the fastest; the mixed matvec stays within the
+    FP32 rounding of the hand-tuned plan's (the candidates tile the same
+    sums differently), the FP64 matvec is untouched, and turning autotuning
+    off restores the defaults."""
+    V, F = meshes.ellipsoid((3.0, 2.2, 1.6), subdivisions=4, jitter=0.04, seed=7)
+    P = meshes.panels_from_mesh(V, F)
+    q = [[0.2, -0.3, 0.1], [-0.5, 0.4, 0.2]], [0.7, -0.4]
+    x = np.random.default_rng(8).uniform(-1, 1, 2 * len(P))
+    base = fa.Problem(P, *q, 4.0, 80.0, 0.124)
+    y32, y64 = base.matvec(x, fa.MIXED), base.matvec(x, fa.F64)
+    assert fa.translation_params()["tuned"] is False
+    try:
+        fa.set_autotune(True)
+        tuned = fa.Problem(P, *q, 4.0, 80.0, 0.124)
+        tp = fa.translation_params()
+        print("autotuned translation:", tp, flush=True)
+        assert tp["tuned"] and tp["mode"] in (0, 1) and tp["budget"] in (880, 720) and tp["tile"] in (512, 256)
+        assert tp["tuned_ms"] > 0
+        assert np.linalg.norm(tuned.matvec(x, fa.MIXED) - y32) <= 1e-6 * np.linalg.norm(y32)
+        assert np.array_equal(tuned.matvec(x, fa.F64), y64)
+    finally:
+        fa.set_autotune(False)
+    assert fa.translation_params() == dict(mode=1, budget=880, tile=512, tuned=False, tuned_ms=0.0)
