@@ -42,6 +42,7 @@ _SIGS = {
     "fftsvd_problem_solve_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "fftsvd_problem_matvec_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_grad_kappa": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
@@ -58,6 +59,7 @@ _SIGS = {
     "fftsvd_problem_stats": (C.c_int, [_vp, _dp, C.c_int]),
     "fftsvd_copy_bandwidth": (C.c_int, [C.c_longlong, C.c_int, _dp]),
     "fftsvd_problem_bench_matvec": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "fftsvd_problem_bench_matvec_batch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "fftsvd_tree_info": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
     "fftsvd_tree_cube_info": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_tree_cube_panels": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip]),
@@ -274,6 +276,13 @@ class Problem:
         _check(lib().fftsvd_problem_matvec(self.h, precision, _d(_f64(x)), _d(y)))
         return y
 
+    def matvec_batch(self, X, precision=F64):
+        """A applied to the columns of X (nvec x ncol) in one batched call."""
+        X = np.asfortranarray(_f64(X).reshape(self.nvec, -1))
+        Y = np.zeros(X.shape, order="F")
+        _check(lib().fftsvd_problem_matvec_batch(self.h, precision, X.shape[1], _d(X), _d(Y)))
+        return Y
+
     def rhs(self):
         r = np.zeros(self.nvec)
         _check(lib().fftsvd_problem_rhs(self.h, _d(r)))
@@ -375,6 +384,13 @@ class Problem:
                 "points_applied", "src_bins", "tgt_bins", "ms_tree", "ms_near", "ms_pinv_T", "ms_sampling",
                 "ms_mgs", "ms_bases", "ms_plan")
         return dict(zip(keys, out.tolist()))
+
+    def bench_matvec_batch(self, ncol, precision=F64, nrep=5):
+        """ms per batched matvec of ncol columns, and per stage."""
+        tot = C.c_double()
+        parts = np.zeros(4)
+        _check(lib().fftsvd_problem_bench_matvec_batch(self.h, precision, ncol, nrep, C.byref(tot), _d(parts)))
+        return tot.value / nrep, dict(zip(("leaf", "project_source", "translate", "target"), parts.tolist()))
 
     def bench_matvec(self, precision=F64, nrep=20):
         tot = C.c_double()

@@ -226,10 +226,22 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
 //                containing the leaf, writes y
 // Every sum runs in a fixed order: the result does not depend on scheduling.
 
+// Operator entries read once per launch are loaded nontemporal (NT), so
+// they do not evict reusable lines from L2.  The column-batched launches
+// read each block once per column copy and load it normally, so that the
+// copies after the first hit L2.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_op(const T* p) {
+   if constexpr (NT)
+      return __builtin_nontemporal_load(p);
+   else
+      return *p;
+}
+
 // Rows e of a column-major E x m block (column c at blk + c*E), streamed
 // with unit stride by G = 256/Ec column groups per row chunk; returns the
 // thread's partial sum for (row e, layer inputs x0/x1 selected by e >= split).
-template <typename T>
+template <typename T, bool NT>
 __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int m, int g, int G, const T* xl) {
    // eight columns in flight per thread
    const long long step = (long long)G * E;
@@ -239,7 +251,7 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
       const T* pc = colp + (long long)c * E;
       T d[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) d[u] = __builtin_nontemporal_load(pc + u * step);
+      for (int u = 0; u < 8; u++) d[u] = ld_op<NT>(pc + u * step);
       a0 += d[0] * xl[c];
       a1 += d[1] * xl[c + G];
       a2 += d[2] * xl[c + 2 * G];
@@ -254,7 +266,7 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
       const T* pc = colp + (long long)c * E;
       T d[7];
 #pragma unroll
-      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? __builtin_nontemporal_load(pc + u * step) : (T)0;
+      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? ld_op<NT>(pc + u * step) : (T)0;
 #pragma unroll
       for (int u = 0; u < 7; u++)
          if (c + u * G < m) a0 += d[u] * xl[c + u * G];
@@ -267,13 +279,18 @@ __device__ __forceinline__ T stream_rows(const T* __restrict__ colp, int E, int 
 // tpart[out_off + r + e] = sum_i VT[e, i] xd[i] over the chunk's columns.
 // Lane (g, e) owns row e and the columns g, g + G, ... (G = 64 / rows of the
 // row chunk); the G partial sums of a row are added in group order.
-template <typename T>
+template <typename T, bool NT>
 __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ chunks, int nchunk,
                                                  const T* __restrict__ VTc, const double* __restrict__ xs,
-                                                 const double* __restrict__ xd, T* __restrict__ tpart, int wave_lds) {
+                                                 const double* __restrict__ xd, T* __restrict__ tpart, int wave_lds,
+                                                 int ncol, long long cs_x, long long cs_tpart) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ci = blockIdx.x * 4 + wave;
+   const ColBlk cb = col_block(ncol);
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ci = cb.b * 4 + wave;
    if (ci >= nchunk) return;
+   xs += cb.c * cs_x;
+   xd += cb.c * cs_x;
+   tpart += cb.c * cs_tpart;
    const WaveGroup gr{lane};
    const ProjChunk c = chunks[ci];
    const int m = c.ncol, E = c.r;
@@ -312,7 +329,7 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
          for (; i + 7 * G < m; i += 8 * G) {
             T v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(colp + (long long)i * E + u * step);
+            for (int u = 0; u < 8; u++) v[u] = ld_op<NT>(colp + (long long)i * E + u * step);
 #pragma unroll
             for (int u = 0; u < 8; u += 2) {
                s0 += v[u] * xl[i + u * G];
@@ -326,7 +343,7 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
             T v[7];
 #pragma unroll
             for (int u = 0; u < 7; u++)
-               v[u] = i + u * G < m ? __builtin_nontemporal_load(colp + (long long)i * E + u * step) : (T)0;
+               v[u] = i + u * G < m ? ld_op<NT>(colp + (long long)i * E + u * step) : (T)0;
 #pragma unroll
             for (int u = 0; u < 7; u++)
                if (i + u * G < m) {
@@ -353,7 +370,7 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
    }
 }
 
-template <int n, typename T, typename G>
+template <int n, typename T, bool NT, typename G>
 __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const T* __restrict__ PV,
                             typename Cplx<T>::type* __restrict__ spec, unsigned char* smem, const G& gr) {
    using C2 = typename Cplx<T>::type;
@@ -404,7 +421,7 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
          for (; k + 7 * KG < R2; k += 8 * KG) {
             T v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = __builtin_nontemporal_load(pv + (long long)(k + u * KG) * GP3 + q);
+            for (int u = 0; u < 8; u++) v[u] = ld_op<NT>(pv + (long long)(k + u * KG) * GP3 + q);
             a0 += v[0] * t[k];
             a1 += v[1] * t[k + KG];
             a2 += v[2] * t[k + 2 * KG];
@@ -419,7 +436,7 @@ __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const
             T v[7];
 #pragma unroll
             for (int u = 0; u < 7; u++)
-               v[u] = k + u * KG < R2 ? __builtin_nontemporal_load(pv + (long long)(k + u * KG) * GP3 + q) : (T)0;
+               v[u] = k + u * KG < R2 ? ld_op<NT>(pv + (long long)(k + u * KG) * GP3 + q) : (T)0;
 #pragma unroll
             for (int u = 0; u < 7; u++)
                if (k + u * KG < R2) a0 += v[u] * t[k + u * KG];
@@ -457,29 +474,34 @@ __host__ __device__ constexpr size_t source_cube_lds(int n, int maxr, int nt) {
 // block, each wave in its own LDS slice of wave_lds bytes.  The small cubes
 // run while the large ones finish.
 constexpr int CUBE_NT = 512;
-template <typename T>
+template <typename T, bool NT>
 __global__ void __launch_bounds__(CUBE_NT) k_source(const SrcPlan* __restrict__ src, int nbig, int nsrc,
                                                     const T* __restrict__ tpart, const T* __restrict__ PV,
-                                                    typename Cplx<T>::type* __restrict__ spec, int wave_lds) {
+                                                    typename Cplx<T>::type* __restrict__ spec, int wave_lds, int nb,
+                                                    int ncol, long long cs_tpart, long long cs_spec) {
    extern __shared__ __align__(16) unsigned char smem[];
-   if ((int)blockIdx.x < nbig) {
-      const SrcPlan d = src[blockIdx.x];
+   const ColBlk cb = col_block(ncol);
+   if (cb.b >= nb) return;
+   tpart += cb.c * cs_tpart;
+   spec += cb.c * cs_spec;
+   if (cb.b < nbig) {
+      const SrcPlan d = src[cb.b];
       const BlockGroup gr = block_group();
       switch (d.n) {
-         case 3: source_body<3, T>(d, tpart, PV, spec, smem, gr); break;
-         case 5: source_body<5, T>(d, tpart, PV, spec, smem, gr); break;
-         case 6: source_body<6, T>(d, tpart, PV, spec, smem, gr); break;
-         case 7: source_body<7, T>(d, tpart, PV, spec, smem, gr); break;
-         case 8: source_body<8, T>(d, tpart, PV, spec, smem, gr); break;
-         case 9: source_body<9, T>(d, tpart, PV, spec, smem, gr); break;
+         case 3: source_body<3, T, NT>(d, tpart, PV, spec, smem, gr); break;
+         case 5: source_body<5, T, NT>(d, tpart, PV, spec, smem, gr); break;
+         case 6: source_body<6, T, NT>(d, tpart, PV, spec, smem, gr); break;
+         case 7: source_body<7, T, NT>(d, tpart, PV, spec, smem, gr); break;
+         case 8: source_body<8, T, NT>(d, tpart, PV, spec, smem, gr); break;
+         case 9: source_body<9, T, NT>(d, tpart, PV, spec, smem, gr); break;
          default: break;
       }
       return;
    }
-   const int wave = threadIdx.x >> 6, i = nbig + ((int)blockIdx.x - nbig) * (CUBE_NT / 64) + wave;
+   const int wave = threadIdx.x >> 6, i = nbig + (cb.b - nbig) * (CUBE_NT / 64) + wave;
    if (i >= nsrc) return;
    const SrcPlan d = src[i];
-   source_body<4, T>(d, tpart, PV, spec, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
+   source_body<4, T, NT>(d, tpart, PV, spec, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
 }
 
 // FP64 translation by tiles: a block owns BC frequency bins (one TR_ROW-byte
@@ -520,13 +542,18 @@ __global__ void __launch_bounds__(TR_TILE * LPT) k_translate(const int4* __restr
                                                            const int* __restrict__ gsrc, const int* __restrict__ gint,
                                                            const typename Cplx<T>::type* __restrict__ Tspec,
                                                            const typename Cplx<T>::type* __restrict__ spec,
-                                                           typename Cplx<T>::type* __restrict__ acc) {
+                                                           typename Cplx<T>::type* __restrict__ acc, int nb, int ncol,
+                                                           long long cs_spec, long long cs_acc) {
    using C2 = typename Cplx<T>::type;
    using V = typename TrVec<T, LPT>::type;
    constexpr int NB = TrVec<T, LPT>::NB;
    static_assert(BC * sizeof(C2) == TR_ROW && NB * sizeof(C2) * LPT == TR_ROW, "translation row layout");
    extern __shared__ __align__(16) unsigned char smem[];
-   const int4 tk = tasks[blockIdx.x];   // first tile, tile count, first bin
+   const ColBlk cb = col_block(ncol);
+   if (cb.b >= nb) return;   // uniform over the block: no barrier is reached
+   spec += cb.c * cs_spec;
+   acc += cb.c * cs_acc;
+   const int4 tk = tasks[cb.b];   // first tile, tile count, first bin
    const int b0 = tk.z, tid = threadIdx.x, nthr = TR_TILE * LPT;
    const int pad = tiles[tk.x].pad, pst = tiles[tk.x].pst;
    const long long t_off = tiles[tk.x].t_off;
@@ -777,11 +804,12 @@ __device__ __forceinline__ void tp_sum(const TpOct& o, int t, TpPair A, TpPair B
 template <int MODE>
 __device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
                                             const int* __restrict__ gsrc, unsigned char* smem, int* ctr, int pw,
-                                            int lane) {
+                                            int lane, long long sdel) {
    long long key0, key1 = -1;   // the T rows (level and chunk) each buffer holds
    {
       const int4 t0 = tasks[0];
-      const TpTile tl = tiles[t0.x];
+      TpTile tl = tiles[t0.x];
+      tl.spec = (const float2*)((const char*)tl.spec + sdel);
       key0 = (long long)(tl.T + t0.y);
       if constexpr (MODE == 1)
          tp_stage_dma(t0, tl, true, smem, pw, lane, gsrc);
@@ -795,7 +823,8 @@ __device__ __forceinline__ void tp_producer(const int4* __restrict__ tasks, int 
       if (pw == 0 && lane == 0) ctr[(k + 2) % 3] = 0;
       if (k + 1 < n) {
          const int4 tn = tasks[k + 1];
-         const TpTile tl = tiles[tn.x];
+         TpTile tl = tiles[tn.x];
+         tl.spec = (const float2*)((const char*)tl.spec + sdel);
          const long long key = (long long)(tl.T + tn.y);
          const int nb = (k + 1) & 1;
          const bool st = (nb ? key1 : key0) != key;
@@ -830,7 +859,7 @@ __device__ __forceinline__ int tp_claim(int* ctr, int k, int lane) {
 __device__ __forceinline__ void tp_consumer(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
                                             const TpOct* __restrict__ octs, const long long* __restrict__ oacc,
                                             const int* __restrict__ gint, const unsigned char* smem, int* ctr,
-                                            int lane) {
+                                            int lane, long long adel) {
    const int t = lane >> 3, b = lane & 7;
    int kc = -1, ic = 0;   // the claimed octet (task kc, index ic), none when kc < k
    TpOct o = {0, 0, 0, 0};
@@ -840,7 +869,7 @@ __device__ __forceinline__ void tp_consumer(const int4* __restrict__ tasks, int 
    for (int k = 0; k < n; k++) {
       const int4 tk = tasks[k];
       const int pad = tiles[tk.x].pad;
-      float2* __restrict__ acc = tiles[tk.x].acc;
+      float2* __restrict__ acc = (float2*)((char*)tiles[tk.x].acc + adel);
       const unsigned char* Tl = smem + (k & 1) * TP_BUF + b * 8;
       const unsigned char* Sl = Tl + TP_TROWS * 64;
       if (kc < k) {
@@ -894,9 +923,13 @@ __global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict
                                                           const TpTile* __restrict__ tiles,
                                                           const TpOct* __restrict__ octs,
                                                           const long long* __restrict__ oacc,
-                                                          const int* __restrict__ gsrc, const int* __restrict__ gint) {
+                                                          const int* __restrict__ gsrc, const int* __restrict__ gint,
+                                                          int nb, int ncol, long long sdel, long long adel,
+                                                          long long scs, long long acs) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const int2 rg = ranges[blockIdx.x];
+   const ColBlk cb = col_block(ncol);
+   if (cb.b >= nb) return;   // uniform over the block: no barrier is reached
+   const int2 rg = ranges[cb.b];
    const int tid = threadIdx.x, lane = tid & 63;
    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    // the zero T row (padding words) of both buffers, which no transfer
@@ -907,9 +940,9 @@ __global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict
    if (rg.y == 0) return;   // uniform over the block: no barrier is reached
    // both roles pass rg.y + 1 barriers
    if (wave >= TP_CW)
-      tp_producer<MODE>(tasks_all + rg.x, rg.y, tiles, gsrc, smem, ctr, wave - TP_CW, lane);
+      tp_producer<MODE>(tasks_all + rg.x, rg.y, tiles, gsrc, smem, ctr, wave - TP_CW, lane, sdel + cb.c * scs);
    else
-      tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, smem, ctr, lane);
+      tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, smem, ctr, lane, adel + cb.c * acs);
 }
 
 template <int n, typename T, typename G>
@@ -991,10 +1024,15 @@ template <typename T>
 __global__ void __launch_bounds__(CUBE_NT) k_target(const TgtPlan* __restrict__ tgt, int nbig, int ntgt,
                                                     const T* __restrict__ UTI,
                                                     const typename Cplx<T>::type* __restrict__ acc,
-                                                    T* __restrict__ w, int wave_lds) {
+                                                    T* __restrict__ w, int wave_lds, int nb, int ncol,
+                                                    long long cs_acc, long long cs_w) {
    extern __shared__ __align__(16) unsigned char smem[];
-   if ((int)blockIdx.x < nbig) {
-      const TgtPlan d = tgt[blockIdx.x];
+   const ColBlk cb = col_block(ncol);
+   if (cb.b >= nb) return;
+   acc += cb.c * cs_acc;
+   w += cb.c * cs_w;
+   if (cb.b < nbig) {
+      const TgtPlan d = tgt[cb.b];
       const BlockGroup gr = block_group();
       switch (d.n) {
          case 3: target_body<3, T>(d, UTI, acc, w, smem, gr); break;
@@ -1007,7 +1045,7 @@ __global__ void __launch_bounds__(CUBE_NT) k_target(const TgtPlan* __restrict__ 
       }
       return;
    }
-   const int wave = threadIdx.x >> 6, i = nbig + ((int)blockIdx.x - nbig) * (CUBE_NT / 64) + wave;
+   const int wave = threadIdx.x >> 6, i = nbig + (cb.b - nbig) * (CUBE_NT / 64) + wave;
    if (i >= ntgt) return;
    const TgtPlan d = tgt[i];
    target_body<4, T>(d, UTI, acc, w, smem + (size_t)wave * wave_lds, WaveGroup{(int)(threadIdx.x & 63)});
@@ -1017,15 +1055,22 @@ __global__ void __launch_bounds__(CUBE_NT) k_target(const TgtPlan* __restrict__ 
 // leaf's far-field blocks of U w.  Thread (g, e) owns block row e and the
 // columns c = g, g+G, ...; the G partial sums of each row are added in group
 // order, then the two layers and the far field.
-template <typename T>
+template <typename T, bool NT>
 __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leaves, const int* __restrict__ dcol,
                                               const T* __restrict__ D, const double* __restrict__ xs,
                                               const double* __restrict__ xd, const LeafU* __restrict__ ublk,
                                               const T* __restrict__ UT, const T* __restrict__ w,
-                                              double* __restrict__ y, const int* __restrict__ bmap) {
+                                              double* __restrict__ y, const int* __restrict__ bmap, int nb, int ncol,
+                                              long long cs_x, long long cs_w, long long cs_y) {
    extern __shared__ __align__(16) unsigned char smem[];
-   const int li = bmap[blockIdx.x];
+   const ColBlk cb = col_block(ncol);
+   if (cb.b >= nb) return;
+   const int li = bmap[cb.b];
    if (li < 0) return;
+   xs += cb.c * cs_x;
+   xd += cb.c * cs_x;
+   w += cb.c * cs_w;
+   y += cb.c * cs_y;
    const LeafDesc L = leaves[li];
    const int E = 2 * L.p, m = L.m, p = L.p;
    T* xg = (T*)smem;       // 2*m
@@ -1043,7 +1088,7 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
       const int Ec = min(256, E - e0), G = 256 / Ec;
       const int g = tid / Ec, e = e0 + tid - g * Ec;
       T acc = 0;
-      if (g < G) acc = stream_rows<T>(blk + e, E, m, g, G, xg + (e >= p ? m : 0));
+      if (g < G) acc = stream_rows<T, NT>(blk + e, E, m, g, G, xg + (e >= p ? m : 0));
       part[tid] = acc;
       __syncthreads();
       if (tid < Ec) {
@@ -1243,7 +1288,8 @@ void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const i
 }
 
 void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
-                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s) {
+                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s,
+                           int ncol, long long sdel, long long adel, long long scs, long long acs) {
    // two LDS buffers per persistent block, three counters
    constexpr size_t lds = 2 * (size_t)TP_BUF + 16;
    check_lds(lds);
@@ -1255,12 +1301,13 @@ void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* ta
                                      (int)lds));
    });
    if (nblk <= 0) return;
+   const int g = col_grid(nblk, ncol);
    if (mode == 1)
-      hipLaunchKernelGGL(k_translate_pipe<1>, dim3(nblk), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
-                         gint);
+      hipLaunchKernelGGL(k_translate_pipe<1>, dim3(g), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
+                         gint, nblk, ncol, sdel, adel, scs, acs);
    else
-      hipLaunchKernelGGL(k_translate_pipe<0>, dim3(nblk), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
-                         gint);
+      hipLaunchKernelGGL(k_translate_pipe<0>, dim3(g), dim3(TP_NT), lds, s, range, tasks, tiles, octs, oacc, gsrc,
+                         gint, nblk, ncol, sdel, adel, scs, acs);
 }
 
 template <typename T>
@@ -1271,9 +1318,12 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       case 0: {
          const int wl = (int)((sizeof(T) * (size_t)(2 * a.proj_maxcols + 128) + 15) / 16 * 16);
          check_lds(4 * (size_t)wl);
-         if (a.nchunk)
-            hipLaunchKernelGGL((k_project<T>), dim3((a.nchunk + 3) / 4), dim3(256), 4 * (size_t)wl, s, a.chunks,
-                               a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart, wl);
+         if (a.nchunk) {
+            auto kern = a.ncol == 1 ? k_project<T, true> : k_project<T, false>;
+            hipLaunchKernelGGL(kern, dim3(col_grid((a.nchunk + 3) / 4, a.ncol)), dim3(256), 4 * (size_t)wl, s,
+                               a.chunks, a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart, wl, a.ncol, a.cs_x,
+                               a.cs_tpart);
+         }
          break;
       }
       case 1: {
@@ -1284,19 +1334,25 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          check_lds(lds);
          static std::once_flag attr;   // large n needs more than the default dynamic LDS
          std::call_once(attr, [] {
-            FSVD_CHECK(hipFuncSetAttribute((const void*)k_source<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           163840));
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_source<T, true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+            FSVD_CHECK(hipFuncSetAttribute((const void*)k_source<T, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
          });
          const int nb = a.nsrc_big + (a.nsrc - a.nsrc_big + CUBE_NT / 64 - 1) / (CUBE_NT / 64);
-         if (nb)
-            hipLaunchKernelGGL((k_source<T>), dim3(nb), dim3(CUBE_NT), lds, s, a.src, a.nsrc_big, a.nsrc,
-                               (const T*)a.tpart, (const T*)a.PV, (C2*)a.spec, wl);
+         if (nb) {
+            auto kern = a.ncol == 1 ? k_source<T, true> : k_source<T, false>;
+            hipLaunchKernelGGL(kern, dim3(col_grid(nb, a.ncol)), dim3(CUBE_NT), lds, s, a.src, a.nsrc_big, a.nsrc,
+                               (const T*)a.tpart, (const T*)a.PV, (C2*)a.spec, wl, nb, a.ncol, a.cs_tpart,
+                               a.cs_spec);
+         }
          break;
       }
       case 2: {
          if constexpr (std::is_same<T, float>::value) {
             launch_translate_pipe(a.nblk_p, a.mode_p, a.range_p, a.task_p, a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p,
-                                  a.gint_p, s);
+                                  a.gint_p, s, a.ncol, a.spec_delta, a.acc_delta, a.cs_spec * (long long)sizeof(C2),
+                                  a.cs_acc * (long long)sizeof(C2));
          } else {
             // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
             constexpr int BC = TR_BINS_F64, LPT = TR_LPT_F64;
@@ -1310,8 +1366,9 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
             });
             if (a.ntask_d)
-               hipLaunchKernelGGL((k_translate<T, BC, LPT>), dim3(a.ntask_d), dim3(TR_TILE * LPT), lds, s, a.task_d,
-                                  a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp, (const C2*)a.spec, (C2*)a.acc);
+               hipLaunchKernelGGL((k_translate<T, BC, LPT>), dim3(col_grid(a.ntask_d, a.ncol)), dim3(TR_TILE * LPT),
+                                  lds, s, a.task_d, a.tiles, a.gtgt, a.gsrc, a.gint, (const C2*)a.Tsp,
+                                  (const C2*)a.spec, (C2*)a.acc, a.ntask_d, a.ncol, a.cs_spec, a.cs_acc);
          }
          break;
       }
@@ -1328,16 +1385,19 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          });
          const int nb = a.ntgt_big + (a.ntgt - a.ntgt_big + CUBE_NT / 64 - 1) / (CUBE_NT / 64);
          if (nb)
-            hipLaunchKernelGGL((k_target<T>), dim3(nb), dim3(CUBE_NT), lds, s, a.tgt, a.ntgt_big, a.ntgt,
-                               (const T*)a.UTI, (const C2*)a.acc, (T*)a.w, wl);
+            hipLaunchKernelGGL((k_target<T>), dim3(col_grid(nb, a.ncol)), dim3(CUBE_NT), lds, s, a.tgt, a.ntgt_big,
+                               a.ntgt, (const T*)a.UTI, (const C2*)a.acc, (T*)a.w, wl, nb, a.ncol, a.cs_acc, a.cs_w);
          break;
       }
       case 4: {
          size_t lds = sizeof(T) * (size_t)(2 * a.maxm + 256 + 3 * a.maxp) + 64;
          check_lds(lds);
-         if (a.nleaf)
-            hipLaunchKernelGGL((k_leaf<T>), dim3(a.nmap_leaf), dim3(256), lds, s, a.leaves, a.dcol,
-                               (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y, a.map_leaf);
+         if (a.nleaf) {
+            auto kern = a.ncol == 1 ? k_leaf<T, true> : k_leaf<T, false>;
+            hipLaunchKernelGGL(kern, dim3(col_grid(a.nmap_leaf, a.ncol)), dim3(256), lds, s, a.leaves, a.dcol,
+                               (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y, a.map_leaf,
+                               a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y);
+         }
          break;
       }
       default: break;

@@ -372,8 +372,16 @@ int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const
    DBuf<double> db, dx;
    db.upload(rhs, nv * ncol, pr.stream);
    dx.alloc(nv * ncol);
+   // surface operators: all columns in lockstep over the column-batched
+   // matvec; the qualocation operator: column after column
+   std::vector<SolveInfo> all;
+   if (!pr.qual)
+      problem_solve_batch(pr, precision, ncol, db.p, dx.p, tol, maxit, all);
+   else
+      for (int c = 0; c < ncol; c++) all.push_back(problem_solve(pr, precision, db.p + nv * c, dx.p + nv * c, tol,
+                                                                 maxit, nullptr));
    for (int c = 0; c < ncol; c++) {
-      SolveInfo si = problem_solve(pr, precision, db.p + nv * c, dx.p + nv * c, tol, maxit, nullptr);
+      const SolveInfo& si = all[c];
       if (iters) iters[c] = si.iterations;
       if (info) {
          double* o = info + 6 * (size_t)c;
@@ -386,6 +394,21 @@ int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const
       }
    }
    FSVD_CHECK(hipMemcpyAsync(sol, dx.p, sizeof(double) * nv * ncol, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   API_CATCH
+}
+
+int fftsvd_problem_matvec_batch(fftsvd_problem* p, int precision, int ncol, const double* x, double* y) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (pr.qual) throw std::runtime_error("the batched matvec takes surface operators");
+   if (ncol < 1) throw std::runtime_error("ncol must be positive");
+   const size_t nv = (size_t)pr.nvec;
+   DBuf<double> dx, dy;
+   dx.upload(x, nv * ncol, pr.stream);
+   dy.alloc(nv * ncol);
+   problem_matvec_batch(pr, precision, ncol, dx.p, dy.p);
+   FSVD_CHECK(hipMemcpyAsync(y, dy.p, sizeof(double) * nv * ncol, hipMemcpyDeviceToHost, pr.stream));
    FSVD_CHECK(hipStreamSynchronize(pr.stream));
    API_CATCH
 }
@@ -584,6 +607,54 @@ int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, doub
                if ((int)R.panel_ids.size() == 0) continue;
                TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
                tree_apply_part(op, precision, part, R.xs.p, R.xd.p, R.yt.p, s);
+            }
+         FSVD_CHECK(hipEventRecord(e1, s));
+         FSVD_CHECK(hipEventSynchronize(e1));
+         float m = 0;
+         FSVD_CHECK(hipEventElapsedTime(&m, e0, e1));
+         ms_parts[part] = m / nrep;
+      }
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   API_CATCH
+}
+
+int fftsvd_problem_bench_matvec_batch(fftsvd_problem* p, int precision, int ncol, int nrep, double* ms_total,
+                                      double* ms_parts) {
+   API_TRY
+   Problem& pr = *p->pr;
+   if (pr.qual) throw std::runtime_error("the batched matvec takes surface operators");
+   if (ncol < 1 || nrep < 1) throw std::runtime_error("ncol and nrep must be positive");
+   hipStream_t s = pr.stream;
+   const size_t n = (size_t)pr.nvec;
+   std::vector<double> hx(n * ncol);
+   std::mt19937_64 rng(12345);
+   std::uniform_real_distribution<double> U(-1.0, 1.0);
+   for (auto& v : hx) v = U(rng);
+   DBuf<double> dx, dy;
+   dx.upload(hx, s);
+   dy.alloc(n * ncol);
+   for (int w = 0; w < 2; w++) problem_matvec_batch(pr, precision, ncol, dx.p, dy.p);
+   hipEvent_t e0, e1;
+   FSVD_CHECK(hipEventCreate(&e0));
+   FSVD_CHECK(hipEventCreate(&e1));
+   FSVD_CHECK(hipEventRecord(e0, s));
+   for (int r = 0; r < nrep; r++) problem_matvec_batch(pr, precision, ncol, dx.p, dy.p);
+   FSVD_CHECK(hipEventRecord(e1, s));
+   FSVD_CHECK(hipEventSynchronize(e1));
+   float ms = 0;
+   FSVD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+   if (ms_total) *ms_total = ms;
+   if (ms_parts) {
+      for (int part = 0; part < 4; part++) {
+         FSVD_CHECK(hipEventRecord(e0, s));
+         for (int r = 0; r < nrep; r++)
+            for (Region& R : pr.regions) {
+               const int np = (int)R.panel_ids.size();
+               if (np == 0) continue;
+               TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+               tree_apply_part_batch(op, precision, part, ncol, R.xsB.p, R.xdB.p, np, R.ytB.p, np, s);
             }
          FSVD_CHECK(hipEventRecord(e1, s));
          FSVD_CHECK(hipEventSynchronize(e1));

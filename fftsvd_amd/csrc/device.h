@@ -239,6 +239,30 @@ struct ApplyArgs {
    void *tpart, *spec, *acc, *w;
    const double *xs, *xd;
    double* y;
+   // ncol columns at once: column c of every vector at c times its column
+   // stride (in elements of the vector's own type); the pipelined
+   // translation's tiles point at the single-column spectra, so its
+   // column-c buffers are at byte offsets spec_delta / acc_delta + c times
+   // the stride from them
+   int ncol = 1;
+   long long cs_x = 0, cs_y = 0, cs_tpart = 0, cs_spec = 0, cs_acc = 0, cs_w = 0;
+   long long spec_delta = 0, acc_delta = 0;
 };
+
+// Column-batched launches: the ncol copies of virtual block b (one per
+// column) run on b's XCD one after another (workgroups are dealt to the
+// eight XCDs round-robin by index), so the copies share that XCD's L2 for
+// the operator rows they all read.  A grid of col_grid(nb, ncol) blocks;
+// copies of virtual blocks b >= nb have no work.
+struct ColBlk {
+   int b, c;
+};
+__device__ __forceinline__ ColBlk col_block(int ncol) {
+   const int lin = (int)blockIdx.x;
+   if (ncol == 1) return {lin, 0};
+   const int k = lin >> 3;
+   return {(k / ncol) * 8 + (lin & 7), k % ncol};
+}
+inline int col_grid(int nb, int ncol) { return ncol == 1 ? nb : (nb + 7) / 8 * 8 * ncol; }
 
 }  // namespace fsvd

@@ -1558,6 +1558,66 @@ void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const dou
       apply_part_impl<double>(op, part, xs, xd, y, s);
 }
 
+// Column copies of the work vectors.  The spectra buffers start zeroed (the
+// bins between a spectrum's pad and its padded stride stay zero), as the
+// single-column ones.
+void tree_batch_reserve(TreeOp& op, int ncol, hipStream_t s) {
+   ApplyPlan& P = op.plan;
+   if (P.ncolB >= ncol) return;
+   FSVD_CHECK(hipStreamSynchronize(s));   // the old buffers may be in use
+   P.tpartB.alloc((size_t)ncol * P.tpart.n);
+   P.specB.alloc((size_t)ncol * P.spec.n);
+   P.accB.alloc((size_t)ncol * P.acc.n);
+   P.wB.alloc((size_t)ncol * P.w.n);
+   FSVD_CHECK(hipMemsetAsync(P.specB.p, 0, sizeof(double) * P.specB.n, s));
+   FSVD_CHECK(hipMemsetAsync(P.accB.p, 0, sizeof(double) * P.accB.n, s));
+   P.ncolB = ncol;
+}
+
+template <typename T>
+static void apply_part_batch_impl(TreeOp& op, int part, int ncol, const double* xs, const double* xd, long long ldx,
+                                  double* y, long long ldy, hipStream_t s) {
+   ApplyPlan& P = op.plan;
+   tree_batch_reserve(op, ncol, s);
+   ApplyArgs a = make_args<T>(op, xs, xd, y);
+   // strides in elements of each buffer's type; every single-column buffer
+   // is sized for FP64, so these strides hold a column of either precision
+   a.ncol = ncol;
+   a.cs_x = ldx;
+   a.cs_y = ldy;
+   a.tpart = P.tpartB.p;
+   a.cs_tpart = (long long)P.tpart.n;
+   a.spec = P.specB.p;
+   a.cs_spec = (long long)P.spec.n / 2;
+   a.acc = P.accB.p;
+   a.cs_acc = (long long)P.acc.n / 2;
+   a.w = P.wB.p;
+   a.cs_w = (long long)P.w.n;
+   a.spec_delta = (long long)((const char*)P.specB.p - (const char*)P.spec.p);
+   a.acc_delta = (long long)((const char*)P.accB.p - (const char*)P.acc.p);
+   switch (part) {
+      case 0: launch_apply_stage<T>(4, a, s); break;
+      case 1:
+         launch_apply_stage<T>(0, a, s);
+         launch_apply_stage<T>(1, a, s);
+         break;
+      case 2: launch_apply_stage<T>(2, a, s); break;
+      case 3: launch_apply_stage<T>(3, a, s); break;
+      default: break;
+   }
+}
+
+void tree_apply_part_batch(TreeOp& op, int prec, int part, int ncol, const double* xs, const double* xd,
+                           long long ldx, double* y, long long ldy, hipStream_t s) {
+   if (ncol < 1) throw std::runtime_error("column-batched apply: ncol must be positive");
+   if (prec == PREC_F32) {
+      if (!op.have32) throw std::runtime_error("operator has no FP32 copy");
+      apply_part_batch_impl<float>(op, part, ncol, xs, xd, ldx, y, ldy, s);
+   } else {
+      apply_part_batch_impl<double>(op, part, ncol, xs, xd, ldx, y, ldy, s);
+   }
+}
+
 void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double* y, hipStream_t s) {
    if (prec == PREC_F32) {
       if (!op.have32) throw std::runtime_error("operator has no FP32 copy");

@@ -154,6 +154,10 @@ struct ApplyPlan {
    int nmap_leaf = 0;
    DBuf<LeafU> ublk;
    DBuf<double> tpart, spec, acc, w;   // work vectors, sized for FP64
+   // the same for ncolB columns of a column-batched apply, column c at c
+   // times the single-column size
+   int ncolB = 0;
+   DBuf<double> tpartB, specB, accB, wB;
    DBuf<double> VTc64, PV64, Tsp64, UTI64, UT64;
    DBuf<float> VTc32, PV32, Tsp32, UTI32, UT32;
 };
@@ -196,8 +200,12 @@ struct TreeOp {
 // The device plan of one launch covering the FP32 translations of the given
 // trees (each with its own spectra), tasks dealt to the persistent blocks.
 void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s, int mode = 1);
+// ncol > 1: every column's spectra and accumulators at byte offsets sdel +
+// c * scs and adel + c * acs from the buffers the tiles point at.
 void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
-                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s);
+                           const TpOct* octs, const long long* oacc, const int* gsrc, const int* gint, hipStream_t s,
+                           int ncol = 1, long long sdel = 0, long long adel = 0, long long scs = 0,
+                           long long acs = 0);
 inline void launch_pipe(const PipeDev& d, hipStream_t s) {
    launch_translate_pipe(d.nblk, d.mode, d.range.p, d.tasks.p, d.tiles.p, d.octs.p, d.oacc.p, d.gsrc.p, d.gint.p, s);
 }
@@ -219,6 +227,15 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
 // source transforms, 2 translation, 3 target transforms), for per-kernel
 // timing.
 void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const double* xd, double* y, hipStream_t s);
+// The same part for ncol columns in one launch per stage: column c of xs and
+// xd at c * ldx, of y at c * ldy.  Every column is computed as the
+// single-column apply computes it, so its result is the same bit for bit;
+// the columns' copies of a block read the operator together (one XCD's L2).
+void tree_apply_part_batch(TreeOp& op, int prec, int part, int ncol, const double* xs, const double* xd,
+                           long long ldx, double* y, long long ldy, hipStream_t s);
+// Sizes the operator's column work vectors for ncol columns (before a
+// batched apply is enqueued on several streams).
+void tree_batch_reserve(TreeOp& op, int ncol, hipStream_t s);
 // kappa-derivative of a Yukawa tree's operator, at fixed ranks and bases:
 // build_dkappa fills op.dk (once per kappa); tree_apply_dkappa applies it
 // (FP64), y_t = S' xs + D' xd in tree order.

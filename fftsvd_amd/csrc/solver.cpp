@@ -88,6 +88,8 @@ Problem::~Problem() {
    basis.clear();
    basis32.clear();
    if (h_pinned) (void)hipHostFree(h_pinned);
+   gm_cols.clear();
+   if (h_pinned_b) (void)hipHostFree(h_pinned_b);
    for (int k = 0; k < MAX_AUX; k++) {
       if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
       if (aux[k]) (void)hipStreamDestroy(aux[k]);
@@ -994,6 +996,368 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
    FSVD_CHECK(hipStreamSynchronize(s));
    tot.ms = now_ms() - t0;
    return tot;
+}
+
+// ---- many right-hand sides in lockstep ----
+
+// y_c = A x_c for the ncol columns (x[c], y[c] device vectors): matvec_impl's
+// steps per region with the tree applies column-batched; the joins and
+// splits stay per column.
+template <typename XT>
+static void matvec_batch(Problem& pr, int prec, int ncol, const XT* const* x, double* const* y) {
+   if (pr.adjoint || pr.qual) throw std::runtime_error("batched matvec: surface operators only");
+   hipStream_t s = pr.stream;
+   for (Region& R : pr.regions) {
+      if (R.panel_ids.empty()) continue;
+      const size_t need = (size_t)ncol * R.panel_ids.size();
+      if (R.xsB.n < need) {
+         FSVD_CHECK(hipStreamSynchronize(s));
+         R.xsB.alloc(need);
+         R.xdB.alloc(need);
+         R.ytB.alloc(need);
+      }
+      TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+      if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
+      tree_batch_reserve(op, ncol, s);
+   }
+   if (pr.naux > 0) {
+      FSVD_CHECK(hipEventRecord(pr.ev_fork, s));
+      for (int k = 0; k < pr.naux; k++) FSVD_CHECK(hipStreamWaitEvent(pr.aux[k], pr.ev_fork, 0));
+   }
+   std::vector<std::vector<int>> lane_regions(pr.naux + 1);
+   for (size_t r = 0; r < pr.regions.size(); r++)
+      if (!pr.regions[r].panel_ids.empty()) lane_regions[pr.regions[r].lane].push_back((int)r);
+   constexpr int NSTEP = 6;
+   static const int parts[4] = {1, 2, 3, 0};
+   size_t maxsteps = 0;
+   for (auto& lr : lane_regions) maxsteps = std::max(maxsteps, lr.size() * NSTEP);
+   for (size_t k = 0; k < maxsteps; k++) {
+      for (size_t l = 0; l < lane_regions.size(); l++) {
+         if (k >= lane_regions[l].size() * NSTEP) continue;
+         Region& R = pr.regions[lane_regions[l][k / NSTEP]];
+         const int step = (int)(k % NSTEP), n = (int)R.panel_ids.size();
+         hipStream_t rs = R.lane == 0 ? s : pr.aux[R.lane - 1];
+         TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
+         if (step == 0) {
+            for (int c = 0; c < ncol; c++)
+               launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x[c], R.xsB.p + (size_t)c * n,
+                           R.xdB.p + (size_t)c * n, rs);
+         } else if (step < 5) {
+            tree_apply_part_batch(op, prec, parts[step - 1], ncol, R.xsB.p, R.xdB.p, n, R.ytB.p, n, rs);
+         } else {
+            for (int c = 0; c < ncol; c++)
+               launch_split(n, R.out_idx.p, R.corr.p, R.ytB.p + (size_t)c * n, R.xdB.p + (size_t)c * n, y[c], rs);
+         }
+      }
+   }
+   for (int k = 0; k < pr.naux; k++) {
+      FSVD_CHECK(hipEventRecord(pr.ev_join[k], pr.aux[k]));
+      FSVD_CHECK(hipStreamWaitEvent(s, pr.ev_join[k], 0));
+   }
+}
+
+void problem_matvec_batch(Problem& pr, int prec, int ncol, const double* x, double* y) {
+   std::vector<const double*> xs(ncol);
+   std::vector<double*> ys(ncol);
+   for (int c = 0; c < ncol; c++) {
+      xs[c] = x + (size_t)c * pr.nvec;
+      ys[c] = y + (size_t)c * pr.nvec;
+   }
+   matvec_batch(pr, prec, ncol, xs.data(), ys.data());
+}
+
+static std::vector<std::unique_ptr<DBuf<double>>>& slabs_of(GmCol& g, double*) { return g.basis64; }
+static std::vector<std::unique_ptr<DBuf<float>>>& slabs_of(GmCol& g, float*) { return g.basis32; }
+
+template <typename VT>
+static VT* col_basis(Problem& pr, GmCol& g, int j) {
+   constexpr int S = Problem::BASIS_SLAB;
+   const size_t off = (size_t)(j % S) * (size_t)pr.nvec;
+   auto& slabs = slabs_of(g, (VT*)nullptr);
+   while ((int)slabs.size() <= j / S) {
+      slabs.emplace_back(new DBuf<VT>());
+      slabs.back()->alloc((size_t)S * pr.nvec);
+   }
+   return slabs[j / S]->p + off;
+}
+
+template <typename VT>
+static const VT* const* col_basis_ptrs(Problem& pr, GmCol& g, int m) {
+   constexpr bool f64 = std::is_same<VT, double>::value;
+   int& have = f64 ? g.nptr64 : g.nptr32;
+   if (have < m) {
+      const int cap = ((m + 63) / 64) * 64;
+      std::vector<const VT*> p(cap);
+      for (int j = 0; j < cap; j++) p[j] = col_basis<VT>(pr, g, j);
+      if constexpr (f64)
+         g.ptr64.upload(p, pr.stream);
+      else
+         g.ptr32.upload(p, pr.stream);
+      FSVD_CHECK(hipStreamSynchronize(pr.stream));
+      have = cap;
+   }
+   if constexpr (f64)
+      return g.ptr64.p;
+   else
+      return g.ptr32.p;
+}
+
+static void grow(DBuf<double>& b, size_t n) {
+   if (b.n < n) b.alloc(n);
+}
+
+// gmres_impl for the columns cols[0..ncol) at once: column c solves
+// rhs[c] -> sol[c] to cols[c]->tol with gmres_impl's kernels, inputs and
+// stopping rule (so with its iterates); the matvec of every column still
+// iterating is one batched call.  Adds the column's counts to cols[c]->info.
+template <typename VT>
+static void gmres_batch_impl(Problem& pr, int prec, int ncol, GmCol* const* cols, const double* const* rhs,
+                             double* const* sol, int maxit) {
+   hipStream_t s = pr.stream;
+   const long long n = pr.nvec;
+   const int np = partial_blocks(n);
+   if (maxit < 1 || maxit > 2040) throw std::runtime_error("GMRES: maxit must be in 1..2040");
+   const int W = maxit + 1;
+   const int nch = (int)std::max<long long>(1, std::min<long long>(256, (n + 2047) / 2048));
+   const int nax = multiaxpy_blocks(n);
+   constexpr int CHECK = 4;
+   if (pr.h_pinned_bcols < ncol) {
+      if (pr.h_pinned_b) FSVD_CHECK(hipHostFree(pr.h_pinned_b));
+      pr.h_pinned_b = nullptr;
+      FSVD_CHECK(hipHostMalloc((void**)&pr.h_pinned_b, sizeof(double) * Problem::PIN_COL * ncol));
+      pr.h_pinned_bcols = ncol;
+   }
+   double* pin = pr.h_pinned_b;
+   for (int c = 0; c < ncol; c++) {
+      GmCol& g = *cols[c];
+      grow(g.w, n);
+      grow(g.r, n);
+      grow(g.partial, np);
+      grow(g.partial_m, (size_t)(maxit + 1) * nch);
+      grow(g.htmp, maxit + 2);
+      grow(g.hbuf, maxit + 2);
+      grow(g.pnorm, nax);
+      grow(g.H, (size_t)W * W);
+      grow(g.v, 4 * (size_t)(W + 1));
+      col_basis<VT>(pr, g, 0);
+   }
+   for (int c = 0; c < ncol; c++) {
+      GmCol& g = *cols[c];
+      problem_precond(pr, rhs[c], g.r.p);
+      launch_dot_partial(n, np, g.r.p, g.r.p, g.partial.p, s);
+      FSVD_CHECK(hipMemcpyAsync(pin + (size_t)c * Problem::PIN_COL, g.partial.p, sizeof(double) * np,
+                                hipMemcpyDeviceToHost, s));
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   std::vector<double> g0(W + 1, 0.0);
+   for (int c = 0; c < ncol; c++) {
+      GmCol& g = *cols[c];
+      double nr2 = 0.0;
+      for (int k = 0; k < np; k++) nr2 += pin[(size_t)c * Problem::PIN_COL + k];
+      g.normr = std::sqrt(nr2);
+      g.done = -1;
+      g.run = !(pr.gmres_zero_test ? g.normr < g.tol : !(g.normr > 0.0));
+      if (!g.run) {
+         FSVD_CHECK(hipMemsetAsync(sol[c], 0, sizeof(double) * n, s));
+         continue;
+      }
+      double* dg = g.v.p + 2 * (W + 1);
+      g0[0] = g.normr;
+      FSVD_CHECK(hipMemcpyAsync(dg, g0.data(), sizeof(double) * (W + 1), hipMemcpyHostToDevice, s));
+      // the upload reads g0 before it is changed for the next column
+      FSVD_CHECK(hipStreamSynchronize(s));
+      launch_normalize(n, np, g.partial.p, g.hbuf.p, g.r.p, col_basis<VT>(pr, g, 0), s);
+   }
+   std::vector<int> act;
+   std::vector<const VT*> xin;
+   std::vector<double*> yout;
+   std::vector<int> started;
+   for (int c = 0; c < ncol; c++)
+      if (cols[c]->run) started.push_back(c);
+   int i, last = -1;
+   for (i = 0; i < maxit; i++) {
+      act.clear();
+      xin.clear();
+      yout.clear();
+      for (int c = 0; c < ncol; c++)
+         if (cols[c]->run) {
+            act.push_back(c);
+            xin.push_back(col_basis<VT>(pr, *cols[c], i));
+            yout.push_back(cols[c]->w.p);
+         }
+      if (act.empty()) break;
+      for (int c : act) col_basis<VT>(pr, *cols[c], i + 1);   // allocations before the batched launches
+      matvec_batch(pr, prec, (int)act.size(), xin.data(), yout.data());
+      for (int c : act) {
+         GmCol& g = *cols[c];
+         if (prec == PREC_F32)
+            g.info.matvecs32++;
+         else
+            g.info.matvecs64++;
+         problem_precond(pr, g.w.p, g.w.p);
+         const VT* const* V = col_basis_ptrs<VT>(pr, g, i + 2);
+         for (int pass = 0; pass < 2; pass++) {
+            launch_multidot<VT>(n, i + 1, nch, V, g.w.p, g.partial_m.p, s);
+            launch_multidot_finish(i + 1, nch, g.partial_m.p, g.htmp.p, g.hbuf.p, pass, s);
+            launch_multiaxpy<VT>(n, i + 1, g.htmp.p, V, g.w.p, pass ? g.pnorm.p : nullptr, s);
+         }
+         double* dcs = g.v.p;
+         launch_normalize_givens(n, nax, g.pnorm.p, g.w.p, col_basis<VT>(pr, g, i + 1), i, W, g.hbuf.p, g.H.p, dcs,
+                                 dcs + (W + 1), dcs + 2 * (W + 1), dcs + 3 * (W + 1), g.normr, s);
+      }
+      if ((i + 1) % CHECK == 0 || i == maxit - 1) {
+         const int a0 = last + 1;
+         for (int c : act)
+            FSVD_CHECK(hipMemcpyAsync(pin + (size_t)c * Problem::PIN_COL, cols[c]->v.p + 3 * (W + 1) + a0,
+                                      sizeof(double) * (i + 1 - a0), hipMemcpyDeviceToHost, s));
+         FSVD_CHECK(hipStreamSynchronize(s));
+         for (int c : act) {
+            GmCol& g = *cols[c];
+            for (int k = a0; k <= i; k++) {
+               const double res = pin[(size_t)c * Problem::PIN_COL + (k - a0)];
+               g.info.final_residual = res;
+               if (res < g.tol) {
+                  g.done = k;
+                  break;
+               }
+            }
+            if (g.done >= 0) g.run = false;
+         }
+         last = i;
+      }
+   }
+   // back substitution and the combination, column by column
+   for (int c : started) {
+      GmCol& g = *cols[c];
+      int e = g.done >= 0 ? g.done : i;
+      if (e >= maxit) e = maxit - 1;
+      std::vector<double> H((size_t)W * (e + 1)), gg(e + 1), y(e + 1);
+      FSVD_CHECK(hipMemcpyAsync(H.data(), g.H.p, sizeof(double) * (size_t)W * (e + 1), hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipMemcpyAsync(gg.data(), g.v.p + 2 * (W + 1), sizeof(double) * (e + 1), hipMemcpyDeviceToHost, s));
+      FSVD_CHECK(hipStreamSynchronize(s));
+      for (int k = 0; k <= e; k++) y[k] = gg[k];
+      for (int k = e; k >= 0; k--) {
+         y[k] /= H[(size_t)k * W + k];
+         for (int j = k - 1; j >= 0; j--) y[j] -= H[(size_t)k * W + j] * y[k];
+      }
+      g.info.iterations += e + 1;
+      g.run = false;
+      DBuf<double> dy;
+      dy.upload(y.data(), e + 1, s);
+      launch_combine<VT>(n, e + 1, dy.p, col_basis_ptrs<VT>(pr, g, e + 1), sol[c], s);
+      FSVD_CHECK(hipStreamSynchronize(s));
+   }
+}
+
+// precond_norm with the column's own scratch vector
+static double col_precond_norm(Problem& pr, GmCol& g, const double* v) {
+   const long long n = pr.nvec;
+   const int np = partial_blocks(n);
+   problem_precond(pr, v, g.tmp.p);
+   launch_dot_partial(n, np, g.tmp.p, g.tmp.p, g.partial.p, pr.stream);
+   FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, g.partial.p, sizeof(double) * np, hipMemcpyDeviceToHost, pr.stream));
+   FSVD_CHECK(hipStreamSynchronize(pr.stream));
+   double a = 0.0;
+   for (int k = 0; k < np; k++) a += pr.h_pinned[k];
+   return std::sqrt(a);
+}
+
+void problem_solve_batch(Problem& pr, int mode, int ncol, const double* rhs, double* sol, double tol, int maxit,
+                         std::vector<SolveInfo>& info) {
+   if (pr.adjoint || pr.qual) throw std::runtime_error("batched solve: surface operators only");
+   const double t0 = now_ms();
+   hipStream_t s = pr.stream;
+   const long long n = pr.nvec;
+   const int np = partial_blocks(n);
+   info.assign(ncol, SolveInfo());
+   // columns per lockstep group (work vectors and bases per column)
+   constexpr int GROUP = 64;
+   for (int c0 = 0; c0 < ncol; c0 += GROUP) {
+      const int nc = std::min(GROUP, ncol - c0);
+      while ((int)pr.gm_cols.size() < nc) pr.gm_cols.emplace_back(new GmCol());
+      std::vector<GmCol*> cols(nc);
+      std::vector<const double*> b(nc);
+      std::vector<double*> x(nc);
+      for (int c = 0; c < nc; c++) {
+         cols[c] = pr.gm_cols[c].get();
+         cols[c]->info = SolveInfo();
+         cols[c]->tol = tol;
+         grow(cols[c]->partial, np);
+         b[c] = rhs + (size_t)(c0 + c) * n;
+         x[c] = sol + (size_t)(c0 + c) * n;
+      }
+      if (mode == PREC_F64) {
+         gmres_batch_impl<double>(pr, PREC_F64, nc, cols.data(), b.data(), x.data(), maxit);
+      } else {
+         gmres_batch_impl<float>(pr, PREC_F32, nc, cols.data(), b.data(), x.data(), maxit);
+         for (int c = 0; c < nc; c++) {
+            GmCol& g = *cols[c];
+            grow(g.res, n);
+            grow(g.e, n);
+            grow(g.tmp, n);
+            g.bn = col_precond_norm(pr, g, b[c]);
+            g.refine = true;
+         }
+         constexpr int MAX_REFINE = 4;   // as problem_solve
+         for (int it = 0;; it++) {
+            std::vector<int> act;
+            std::vector<const double*> xin;
+            std::vector<double*> yout;
+            for (int c = 0; c < nc; c++)
+               if (cols[c]->refine) {
+                  act.push_back(c);
+                  xin.push_back(x[c]);
+                  yout.push_back(cols[c]->tmp.p);
+               }
+            if (act.empty()) break;
+            matvec_batch(pr, PREC_F64, (int)act.size(), xin.data(), yout.data());
+            std::vector<GmCol*> rc;
+            std::vector<const double*> rb;
+            std::vector<double*> re;
+            for (int c : act) {
+               GmCol& g = *cols[c];
+               g.info.matvecs64++;
+               launch_axpby(n, 1.0, b[c], -1.0, g.tmp.p, g.res.p, s);
+               g.rn = col_precond_norm(pr, g, g.res.p);
+               g.info.final_residual = g.bn > 0 ? g.rn / g.bn : g.rn;
+               if (!(g.info.final_residual >= tol) || !(g.rn > 0.0) || it == MAX_REFINE) {
+                  g.refine = false;
+                  continue;
+               }
+               launch_axpby(n, 1.0 / g.rn, g.res.p, 0.0, g.res.p, g.res.p, s);
+               g.tol = std::min(0.5, 0.5 * tol / g.info.final_residual);
+               rc.push_back(&g);
+               rb.push_back(g.res.p);
+               re.push_back(g.e.p);
+            }
+            if (rc.empty()) break;
+            // the correction solves' own counts (the residual of the
+            // refined solution is set by the next pass)
+            std::vector<SolveInfo> keep(rc.size());
+            for (size_t k = 0; k < rc.size(); k++) {
+               keep[k] = rc[k]->info;
+               rc[k]->info = SolveInfo();
+            }
+            gmres_batch_impl<float>(pr, PREC_F32, (int)rc.size(), rc.data(), rb.data(), re.data(), maxit);
+            for (size_t k = 0; k < rc.size(); k++) {
+               GmCol& g = *rc[k];
+               SolveInfo tot = keep[k];
+               tot.iterations += g.info.iterations;
+               tot.matvecs32 += g.info.matvecs32;
+               tot.refinements++;
+               g.info = tot;
+            }
+            for (int c : act) {
+               GmCol& g = *cols[c];
+               if (g.refine) launch_axpby(n, 1.0, x[c], g.rn, g.e.p, x[c], s);
+            }
+         }
+      }
+      FSVD_CHECK(hipStreamSynchronize(s));
+      for (int c = 0; c < nc; c++) info[c0 + c] = cols[c]->info;
+   }
+   const double ms = now_ms() - t0;
+   for (SolveInfo& si : info) si.ms = ms;
 }
 
 void problem_potentials(Problem& pr, const double* sol, double* phi_out, bool use_m3) {

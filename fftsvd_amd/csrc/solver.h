@@ -60,6 +60,7 @@ struct Region {
    DBuf<int> phi_idx, dphi_idx, out_idx;
    DBuf<double> s_phi, s_dphi, corr;
    DBuf<double> xs, xd, yt;
+   DBuf<double> xsB, xdB, ytB;       // the same for the columns of a batched matvec
    bool has_charges = false;
    int shares_tree_with = -1;        // reuse another region's operator (same panels and kernel)
    // M3 tree (region panels -> charges) and the region join in its panel order
@@ -76,6 +77,24 @@ struct SolveInfo {
    double final_residual = 0.0;
    double ms = 0.0;
    int matvecs64 = 0, matvecs32 = 0;
+};
+
+// One column of a lockstep batched solve: its GMRES work vectors, Givens
+// state and Krylov basis (slabs of Problem::BASIS_SLAB vectors, with device
+// copies of their pointers), the refinement vectors of the mixed mode, and
+// the column's host-side state.
+struct GmCol {
+   DBuf<double> w, r, partial, partial_m, htmp, hbuf, pnorm, H, v;
+   DBuf<double> res, e, tmp;
+   std::vector<std::unique_ptr<DBuf<double>>> basis64;
+   std::vector<std::unique_ptr<DBuf<float>>> basis32;
+   DBuf<const double*> ptr64;
+   DBuf<const float*> ptr32;
+   int nptr64 = 0, nptr32 = 0;
+   double normr = 0.0, bn = 0.0, rn = 0.0, tol = 0.0;
+   int done = -1;
+   bool run = false, refine = false;
+   SolveInfo info;
 };
 
 struct Problem {
@@ -127,6 +146,12 @@ struct Problem {
    DBuf<double> gm_H, gm_v;             // GMRES Givens state (H; rotations, g, history)
    DBuf<double> pnorm;                  // block partials of |w|^2 from the last multi-axpy
    double* h_pinned = nullptr;
+   // lockstep batched solves: per-column state and a pinned staging area of
+   // PIN_COL doubles per column
+   static constexpr int PIN_COL = 2048;
+   std::vector<std::unique_ptr<GmCol>> gm_cols;
+   double* h_pinned_b = nullptr;
+   int h_pinned_bcols = 0;
    double setup_ms = 0.0;
    ~Problem();
 };
@@ -144,6 +169,17 @@ void problem_rhs(Problem& pr, double* rhs_dev);
 SolveInfo problem_gmres(Problem& pr, int prec, const double* rhs, double* sol, double tol, int maxit,
                         std::vector<double>* hist);
 // Mixed precision: FP32-operator GMRES with FP64 residual refinement.
+// ncol right-hand sides (device, column c at rhs + c * nvec) solved in
+// lockstep: every column runs problem_solve's GMRES (and, in mixed mode, its
+// FP64 refinement) on its own, and the columns still iterating share one
+// column-batched matvec per step, so each column's solution, iteration
+// count and residual equal problem_solve's on that column bit for bit.
+// Surface operators only.  info[c]: that column's counts; ms is the batch's.
+void problem_solve_batch(Problem& pr, int mode, int ncol, const double* rhs, double* sol, double tol, int maxit,
+                         std::vector<SolveInfo>& info);
+// y_c = A x_c for ncol vectors (device, column c at c * nvec) by the
+// column-batched matvec.
+void problem_matvec_batch(Problem& pr, int prec, int ncol, const double* x, double* y);
 SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, double tol, int maxit,
                         std::vector<double>* hist);
 // Reaction potentials at the charges: direct sums, or through the M3 trees

@@ -93,10 +93,21 @@ int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi)
  * ncol, info 6 per column as fftsvd_problem_solve.  Column c equals the
  * single-vector call with the problem's charges replaced by q_c; the
  * integrals of the right-hand sides and potentials are formed once for all
- * columns. */
+ * columns.  fftsvd_problem_solve_batch runs one GMRES per column (and, in
+ * mixed mode, its FP64 refinement) in lockstep, the columns still
+ * iterating sharing one column-batched matvec per step, so every column's
+ * solution and counts equal fftsvd_problem_solve's bit for bit; info[3]
+ * (ms) is the whole batch's time.  Up to 64 columns share a step; more
+ * are taken 64 at a time. */
 int fftsvd_problem_rhs_batch(fftsvd_problem* p, int ncol, const double* q, double* rhs);
 int fftsvd_problem_solve_batch(fftsvd_problem* p, int precision, int ncol, const double* rhs, double* sol,
                                double tol, int maxit, int* iters, double* info);
+/* y_c = A x_c for ncol column-major vectors (nvec x ncol), the operator
+ * applied to all columns in one launch per stage (each column's result is
+ * the single-vector fftsvd_problem_matvec's, bit for bit).  Surface
+ * operators.  No counterpart in the reference, whose hessian() applies the
+ * operator once per charge vector. */
+int fftsvd_problem_matvec_batch(fftsvd_problem* p, int precision, int ncol, const double* x, double* y);
 int fftsvd_problem_potentials_batch(fftsvd_problem* p, int ncol, const double* sol, double* phi);
 /* Adjoint (surface operators): x = A^T y, the transposes of the region
  * join / tree apply / split (Tree_multiply_transpose, Tree.c:769-832, per
@@ -141,6 +152,10 @@ int fftsvd_problem_stats(fftsvd_problem* p, double* out, int nout);
  * (leaf: near field + far-field output; projection + source transforms;
  * translation; target transforms) from separately timed passes. */
 int fftsvd_problem_bench_matvec(fftsvd_problem* p, int precision, int nrep, double* ms_total, double* ms_parts);
+/* The same for nrep column-batched matvecs of ncol columns each
+ * (fftsvd_problem_matvec_batch on device-resident vectors). */
+int fftsvd_problem_bench_matvec_batch(fftsvd_problem* p, int precision, int ncol, int nrep, double* ms_total,
+                                      double* ms_parts);
 
 /* Per-tree views for parity tests.  which = region index (0 = exterior). */
 int fftsvd_tree_info(fftsvd_problem* p, int which, int* depth, int* ncubes, int* npanels);
