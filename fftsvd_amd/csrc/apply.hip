@@ -1133,6 +1133,160 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
    for (int r = tid; r < p; r += 256) y[L.q0 + r] = (double)((rows[r] + rows[p + r]) + rows[E + r]);
 }
 
+// columns per block of the multi-column leaf kernel
+constexpr int LEAF_NC = 4;
+
+// stream_rows for NC columns at once: every entry of the block is loaded
+// once and multiplied into each column's accumulators; xl holds the
+// columns' inputs interleaved (input i of column j at xl[i * NC + j]).  Each
+// column's accumulators see the single-column sequence of operations, so
+// out[j] is stream_rows's result for column j bit for bit.
+template <typename T, int NC>
+__device__ __forceinline__ void stream_rows_mc(const T* __restrict__ colp, int E, int m, int g, int G,
+                                               const T* xl, T* out) {
+   const long long step = (long long)G * E;
+   T a[4][NC];
+#pragma unroll
+   for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int j = 0; j < NC; j++) a[q][j] = 0;
+   int c = g;
+   for (; c + 7 * G < m; c += 8 * G) {
+      const T* pc = colp + (long long)c * E;
+      T d[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) d[u] = pc[u * step];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         const T* xv = xl + (size_t)(c + u * G) * NC;
+#pragma unroll
+         for (int j = 0; j < NC; j++) a[u & 3][j] += d[u] * xv[j];
+      }
+   }
+   if (c < m) {
+      const T* pc = colp + (long long)c * E;
+      T d[7];
+#pragma unroll
+      for (int u = 0; u < 7; u++) d[u] = c + u * G < m ? pc[u * step] : (T)0;
+#pragma unroll
+      for (int u = 0; u < 7; u++)
+         if (c + u * G < m) {
+            const T* xv = xl + (size_t)(c + u * G) * NC;
+#pragma unroll
+            for (int j = 0; j < NC; j++) a[0][j] += d[u] * xv[j];
+         }
+   }
+#pragma unroll
+   for (int j = 0; j < NC; j++) out[j] = (a[0][j] + a[1][j]) + (a[2][j] + a[3][j]);
+}
+
+// k_leaf for NC columns per block: the near-field block and the U rows are
+// read once for the NC columns of the block's group (columns c0 .. c0+NC-1
+// of ncol; a partial last group repeats its last column and stores only its
+// own).  Per column the arithmetic is k_leaf's, so y equals the
+// single-column result bit for bit.  Grid: col_grid(nb, number of groups).
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_leaf_mc(const LeafDesc* __restrict__ leaves, const int* __restrict__ dcol,
+                                                 const T* __restrict__ D, const double* __restrict__ xs,
+                                                 const double* __restrict__ xd, const LeafU* __restrict__ ublk,
+                                                 const T* __restrict__ UT, const T* __restrict__ w,
+                                                 double* __restrict__ y, const int* __restrict__ bmap, int nb,
+                                                 int ncol, long long cs_x, long long cs_w, long long cs_y) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int ngrp = (ncol + NC - 1) / NC;
+   const ColBlk cb = col_block(ngrp);
+   if (cb.b >= nb) return;
+   const int li = bmap[cb.b];
+   if (li < 0) return;
+   const int c0 = cb.c * NC, nc = min(NC, ncol - c0);
+   const LeafDesc L = leaves[li];
+   const int E = 2 * L.p, m = L.m, p = L.p;
+   T* xg = (T*)smem;                 // 2m x NC
+   T* part = xg + 2 * m * NC;        // NC x 256
+   T* rows = part + 256 * NC;        // NC x (2p near rows, then p far rows)
+   const int tid = threadIdx.x, R3 = 3 * p;
+   for (int c = tid; c < m; c += 256) {
+      const int col = dcol[L.col_off + c];
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+         const long long o = (long long)(c0 + min(j, nc - 1)) * cs_x + col;
+         xg[(size_t)c * NC + j] = (T)xs[o];
+         xg[(size_t)(m + c) * NC + j] = (T)xd[o];
+      }
+   }
+   __syncthreads();
+   const T* blk = D + L.d_off;
+   for (int e0 = 0; e0 < E; e0 += 256) {
+      const int Ec = min(256, E - e0), G = 256 / Ec;
+      const int g = tid / Ec, e = e0 + tid - g * Ec;
+      T acc[NC];
+#pragma unroll
+      for (int j = 0; j < NC; j++) acc[j] = 0;
+      if (g < G) stream_rows_mc<T, NC>(blk + e, E, m, g, G, xg + (e >= p ? (size_t)m * NC : 0), acc);
+#pragma unroll
+      for (int j = 0; j < NC; j++) part[j * 256 + tid] = acc[j];
+      __syncthreads();
+      if (tid < Ec) {
+#pragma unroll
+         for (int j = 0; j < NC; j++) {
+            T sum = 0;
+            for (int gg = 0; gg < G; gg++) sum += part[j * 256 + gg * Ec + tid];
+            rows[j * R3 + e0 + tid] = sum;
+         }
+      }
+      __syncthreads();
+   }
+   for (int e0 = 0; e0 < p; e0 += 256) {
+      const int Ec = min(256, p - e0), G = 256 / Ec;
+      const int g = tid / Ec, e = e0 + tid - g * Ec;
+      T acc[NC];
+#pragma unroll
+      for (int j = 0; j < NC; j++) acc[j] = 0;
+      if (g < G)
+         for (int u = 0; u < L.nu; u++) {
+            const LeafU b = ublk[L.u0 + u];
+            const T* col = UT + b.u_off + e;
+            for (int k = g; k < b.r; k += 8 * G) {
+               T cv[8];
+#pragma unroll
+               for (int v = 0; v < 8; v++) {
+                  const int kk = k + v * G;
+                  cv[v] = kk < b.r ? col[(long long)kk * b.ld] : (T)0;
+               }
+#pragma unroll
+               for (int j = 0; j < NC; j++) {
+                  const T* wb = w + (long long)(c0 + min(j, nc - 1)) * cs_w + b.w_off;
+                  T wv[8];
+#pragma unroll
+                  for (int v = 0; v < 8; v++) {
+                     const int kk = k + v * G;
+                     wv[v] = kk < b.r ? wb[kk] : (T)0;
+                  }
+#pragma unroll
+                  for (int v = 0; v < 8; v++) acc[j] += cv[v] * wv[v];
+               }
+            }
+         }
+#pragma unroll
+      for (int j = 0; j < NC; j++) part[j * 256 + tid] = acc[j];
+      __syncthreads();
+      if (tid < Ec) {
+#pragma unroll
+         for (int j = 0; j < NC; j++) {
+            T sum = 0;
+            for (int gg = 0; gg < G; gg++) sum += part[j * 256 + gg * Ec + tid];
+            rows[j * R3 + E + e0 + tid] = sum;
+         }
+      }
+      __syncthreads();
+   }
+   for (int j = 0; j < nc; j++) {
+      const T* rj = rows + j * R3;
+      double* yj = y + (long long)(c0 + j) * cs_y;
+      for (int r = tid; r < p; r += 256) yj[L.q0 + r] = (double)((rj[r] + rj[p + r]) + rj[E + r]);
+   }
+}
+
 __global__ void k_zero(double* __restrict__ a, long long n) {
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
       a[i] = 0.0;
@@ -1392,11 +1546,28 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       case 4: {
          size_t lds = sizeof(T) * (size_t)(2 * a.maxm + 256 + 3 * a.maxp) + 64;
          check_lds(lds);
-         if (a.nleaf) {
-            auto kern = a.ncol == 1 ? k_leaf<T, true> : k_leaf<T, false>;
-            hipLaunchKernelGGL(kern, dim3(col_grid(a.nmap_leaf, a.ncol)), dim3(256), lds, s, a.leaves, a.dcol,
+         if (a.nleaf && a.ncol == 1) {
+            hipLaunchKernelGGL((k_leaf<T, true>), dim3(a.nmap_leaf), dim3(256), lds, s, a.leaves, a.dcol,
                                (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y, a.map_leaf,
-                               a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y);
+                               a.nmap_leaf, 1, a.cs_x, a.cs_w, a.cs_y);
+         } else if (a.nleaf && sizeof(T) * (size_t)LEAF_NC * (2 * a.maxm + 256 + 3 * a.maxp) + 64 > 163840) {
+            // leaves too wide for LEAF_NC columns in LDS: one block copy per column
+            hipLaunchKernelGGL((k_leaf<T, false>), dim3(col_grid(a.nmap_leaf, a.ncol)), dim3(256), lds, s, a.leaves,
+                               a.dcol, (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y,
+                               a.map_leaf, a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y);
+         } else if (a.nleaf) {
+            // LEAF_NC columns per block
+            const size_t lmc = sizeof(T) * (size_t)LEAF_NC * (2 * a.maxm + 256 + 3 * a.maxp) + 64;
+            check_lds(lmc);
+            static std::once_flag attr;
+            std::call_once(attr, [] {
+               FSVD_CHECK(hipFuncSetAttribute((const void*)k_leaf_mc<T, LEAF_NC>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+            });
+            const int ngrp = (a.ncol + LEAF_NC - 1) / LEAF_NC;
+            hipLaunchKernelGGL((k_leaf_mc<T, LEAF_NC>), dim3(col_grid(a.nmap_leaf, ngrp)), dim3(256), lmc, s,
+                               a.leaves, a.dcol, (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w,
+                               a.y, a.map_leaf, a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y);
          }
          break;
       }
