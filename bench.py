@@ -235,6 +235,23 @@ def main():
         solve["rhs_s"] = t_rhs
         solve["gmres_speedup_mixed"] = solve["fp64"]["gmres_s"] / solve["mixed"]["gmres_s"]
 
+    # the column-batched matvec alone: the batched solve's columns per call,
+    # device resident, HIP-event timed
+    block = None
+    if args.multi_rhs > 0:
+        ncb = args.multi_rhs
+        block = {"columns": ncb}
+        for name, prec, nrep in (("mixed", fa.MIXED, 5), ("fp64", fa.F64, 3)):
+            prob.bench_matvec_batch(ncb, prec, 1)
+            msb, partsb = prob.bench_matvec_batch(ncb, prec, nrep)
+            block[name] = {"ms_per_call": msb, "ms_per_column": msb / ncb, "matvecs_per_s": 1000.0 * ncb / msb,
+                           "parts_ms_per_column": {k: v / ncb for k, v in partsb.items()}}
+        print("# block matvec, %d columns per call: mixed %.4f ms per column (%.0f matvecs/s), fp64 %.4f ms; "
+              "parts mixed %s" % (ncb, block["mixed"]["ms_per_column"], block["mixed"]["matvecs_per_s"],
+                                  block["fp64"]["ms_per_column"],
+                                  {k: round(v, 4) for k, v in block["mixed"]["parts_ms_per_column"].items()}),
+              flush=True)
+
     # batched reaction-potential solve: seeded i.i.d. U[-0.8, 0.8] charge
     # vectors on the same atoms, one operator; per-column energy error of
     # the mixed mode against FP64 and the aggregate matvec rate of the solves
@@ -247,20 +264,6 @@ def main():
         R = prob.rhs_batch(Q)
         t_rhs_b = time.time() - t0
         multi = {"columns": ncol, "rhs_s": t_rhs_b, "gmres_tol": MULTI_TOL}
-        # the column-batched matvec alone: all columns per call, device
-        # resident, HIP-event timed
-        blk = {}
-        for name, prec, nrep in (("mixed", fa.MIXED, 5), ("fp64", fa.F64, 3)):
-            prob.bench_matvec_batch(ncol, prec, 1)
-            msb, partsb = prob.bench_matvec_batch(ncol, prec, nrep)
-            blk[name] = {"ms_per_call": msb, "ms_per_column": msb / ncol, "matvecs_per_s": 1000.0 * ncol / msb,
-                         "parts_ms_per_column": {k: v / ncol for k, v in partsb.items()}}
-        multi["block_matvec"] = blk
-        print("# block matvec, %d columns per call: mixed %.4f ms per column (%.0f matvecs/s), fp64 %.4f ms; "
-              "parts mixed %s" % (ncol, blk["mixed"]["ms_per_column"], blk["mixed"]["matvecs_per_s"],
-                                  blk["fp64"]["ms_per_column"],
-                                  {k: round(v, 4) for k, v in blk["mixed"]["parts_ms_per_column"].items()}),
-              flush=True)
         E = {}
         for name, prec in (("fp64", fa.F64), ("mixed", fa.MIXED)):
             t0 = time.time()
@@ -414,6 +417,7 @@ def main():
         "device": {"name": dev_name, "cus": dev_cu},
         "solve": solve,
         "multi_rhs": multi,
+        "block_matvec": block,
         "gradients": grads,
         "operator": st,
         "roofline": roofline,

@@ -249,6 +249,16 @@ struct ApplyArgs {
    long long spec_delta = 0, acc_delta = 0;
 };
 
+// One column of a lockstep GMRES step (device copy, one per active
+// column): its basis pointer array (const VT* const*), the next basis
+// vector, the work vectors and Givens state, and its initial residual norm.
+struct GmDev {
+   const void* V;
+   void* vnext;
+   double *w, *partial_m, *htmp, *hbuf, *pnorm, *H, *cs, *sn, *g, *hist;
+   double normr;
+};
+
 // Column-batched launches: the ncol copies of virtual block b (one per
 // column) run on b's XCD one after another (workgroups are dealt to the
 // eight XCDs round-robin by index), so the copies share that XCD's L2 for

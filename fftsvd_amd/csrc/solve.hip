@@ -21,6 +21,20 @@ __global__ void k_join(int n, const int* __restrict__ phi_idx, const int* __rest
       xs[k] = s_dphi[k] * (double)x[dphi_idx[k]];
    }
 }
+// k_join for the columns blockIdx.y: input x[c], outputs at c * ld
+template <typename XT>
+__global__ void k_join_b(int n, const int* __restrict__ phi_idx, const int* __restrict__ dphi_idx,
+                         const double* __restrict__ s_phi, const double* __restrict__ s_dphi,
+                         const XT* const* __restrict__ xc, double* __restrict__ xs, double* __restrict__ xd,
+                         long long ld) {
+   const int c = blockIdx.y;
+   const XT* x = xc[c];
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k < n) {
+      xd[c * ld + k] = s_phi[k] * (double)x[phi_idx[k]];
+      xs[c * ld + k] = s_dphi[k] * (double)x[dphi_idx[k]];
+   }
+}
 
 // Split back to the global result with the -4 pi jump on child panels
 // (SurfaceOperator.c:141-147 and splitpotentials/joinglobal, :78-107).
@@ -28,6 +42,14 @@ __global__ void k_split(int n, const int* __restrict__ out_idx, const double* __
                         const double* __restrict__ yt, const double* __restrict__ xd, double* __restrict__ y) {
    int k = blockIdx.x * blockDim.x + threadIdx.x;
    if (k < n) y[out_idx[k]] = yt[k] + corr[k] * xd[k];
+}
+// k_split for the columns blockIdx.y: inputs at c * ld, output yc[c]
+__global__ void k_split_b(int n, const int* __restrict__ out_idx, const double* __restrict__ corr,
+                          const double* __restrict__ yt, const double* __restrict__ xd, long long ld,
+                          double* const* __restrict__ yc) {
+   const int c = blockIdx.y;
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k < n) yc[c][out_idx[k]] = yt[c * ld + k] + corr[k] * xd[c * ld + k];
 }
 
 // Transposed region operator, per region on one stream: gather
@@ -66,8 +88,8 @@ __global__ void k_precond_t(int nb, const int* __restrict__ ia, const int* __res
 }
 
 // x = P^{-1} b for disjoint 2x2 blocks (factored on the host); in place safe.
-__global__ void k_precond(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
-                          const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x) {
+__device__ __forceinline__ void precond_body(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
+                                             const double* __restrict__ M, const double* b, double* x) {
    int k = blockIdx.x * blockDim.x + threadIdx.x;
    if (k < nb) {
       int i = ia[k], j = ib[k];
@@ -76,6 +98,16 @@ __global__ void k_precond(int nb, const int* __restrict__ ia, const int* __restr
       x[i] = m[0] * bi + m[1] * bj;
       x[j] = m[2] * bi + m[3] * bj;
    }
+}
+__global__ void k_precond(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
+                          const double* __restrict__ M, const double* __restrict__ b, double* __restrict__ x) {
+   precond_body(nb, ia, ib, M, b, x);
+}
+// in place on the w of column blockIdx.y
+__global__ void k_precond_b(int nb, const int* __restrict__ ia, const int* __restrict__ ib,
+                            const double* __restrict__ M, const GmDev* __restrict__ d) {
+   double* w = d[blockIdx.y].w;
+   precond_body(nb, ia, ib, M, w, w);
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -147,8 +179,8 @@ __global__ void __launch_bounds__(256) k_normalize(long long n, const double* __
 // than the separate finish launch.)
 constexpr int MD_COLS = 8;
 template <typename VT>
-__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const VT* const* __restrict__ V,
-                                                  const double* __restrict__ w, double* __restrict__ partial) {
+__device__ __forceinline__ void multidot_body(long long n, int m, int nch, const VT* const* __restrict__ V,
+                                              const double* __restrict__ w, double* __restrict__ partial) {
    __shared__ double red[4][MD_COLS];
    const long long chunk = (n + nch - 1) / nch;
    const long long i0 = blockIdx.x * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
@@ -179,12 +211,23 @@ __global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, c
       partial[(long long)(j0 + u) * nch + blockIdx.x] = ((red[0][u] + red[1][u]) + red[2][u]) + red[3][u];
    }
 }
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multidot(long long n, int m, int nch, const VT* const* __restrict__ V,
+                                                  const double* __restrict__ w, double* __restrict__ partial) {
+   multidot_body<VT>(n, m, nch, V, w, partial);
+}
+// the column blockIdx.z
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multidot_b(long long n, int m, int nch, const GmDev* __restrict__ d) {
+   const GmDev& a = d[blockIdx.z];
+   multidot_body<VT>(n, m, nch, (const VT* const*)a.V, a.w, a.partial_m);
+}
 
 // h[j] = sum of column j's nch partials (one wave per column, lanes over
 // the partials, then a fixed-order wave reduction); hacc[j] = h[j] (first
 // pass) or hacc[j] + h[j] (second pass).
-__global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const double* __restrict__ partial,
-                                                        double* __restrict__ h, double* __restrict__ hacc, int add) {
+__device__ __forceinline__ void multidot_finish_body(int m, int nch, const double* __restrict__ partial,
+                                                     double* __restrict__ h, double* __restrict__ hacc, int add) {
    const int j = blockIdx.x;
    if (j >= m) return;
    double sum = 0.0;
@@ -195,6 +238,14 @@ __global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const do
       h[j] = sum;
       hacc[j] = add ? hacc[j] + sum : sum;
    }
+}
+__global__ void __launch_bounds__(64) k_multidot_finish(int m, int nch, const double* __restrict__ partial,
+                                                        double* __restrict__ h, double* __restrict__ hacc, int add) {
+   multidot_finish_body(m, nch, partial, h, hacc, add);
+}
+__global__ void __launch_bounds__(64) k_multidot_finish_b(int m, int nch, const GmDev* __restrict__ d, int add) {
+   const GmDev& a = d[blockIdx.y];
+   multidot_finish_body(m, nch, a.partial_m, a.htmp, a.hbuf, add);
 }
 
 // One Arnoldi column's Givens step (reference GMRES.c:3-22, 495-510): the
@@ -255,12 +306,11 @@ __device__ void arnoldi_givens(int i, int W, const double* __restrict__ hcol, do
 // v = w / nrm, H[i+1, i] = nrm (hcol[i+1]), and block 0's first wave does
 // the Givens step of column i.
 template <typename VT>
-__global__ void __launch_bounds__(256) k_normalize_givens(long long n, const double* __restrict__ pin, int np,
-                                                          const double* __restrict__ w, VT* __restrict__ v, int i,
-                                                          int W, double* __restrict__ hcol, double* __restrict__ H,
-                                                          double* __restrict__ cs, double* __restrict__ sn,
-                                                          double* __restrict__ g, double* __restrict__ hist,
-                                                          double normr) {
+__device__ __forceinline__ void normalize_givens_body(long long n, const double* __restrict__ pin, int np,
+                                                      const double* __restrict__ w, VT* __restrict__ v, int i, int W,
+                                                      double* __restrict__ hcol, double* __restrict__ H,
+                                                      double* __restrict__ cs, double* __restrict__ sn,
+                                                      double* __restrict__ g, double* __restrict__ hist, double normr) {
    extern __shared__ double gsm[];
    __shared__ double hs;
    if (threadIdx.x < 64) {
@@ -277,13 +327,29 @@ __global__ void __launch_bounds__(256) k_normalize_givens(long long n, const dou
       arnoldi_givens(i, W, hcol, nrm, H, cs, sn, g, hist, normr, gsm);
    }
 }
+template <typename VT>
+__global__ void __launch_bounds__(256) k_normalize_givens(long long n, const double* __restrict__ pin, int np,
+                                                          const double* __restrict__ w, VT* __restrict__ v, int i,
+                                                          int W, double* __restrict__ hcol, double* __restrict__ H,
+                                                          double* __restrict__ cs, double* __restrict__ sn,
+                                                          double* __restrict__ g, double* __restrict__ hist,
+                                                          double normr) {
+   normalize_givens_body<VT>(n, pin, np, w, v, i, W, hcol, H, cs, sn, g, hist, normr);
+}
+template <typename VT>
+__global__ void __launch_bounds__(256) k_normalize_givens_b(long long n, int np, int i, int W,
+                                                            const GmDev* __restrict__ d) {
+   const GmDev& a = d[blockIdx.y];
+   normalize_givens_body<VT>(n, a.pnorm, np, a.w, (VT*)a.vnext, i, W, a.hbuf, a.H, a.cs, a.sn, a.g, a.hist,
+                             a.normr);
+}
 
 // w -= sum_j h[j] V_j, summed over j in order.
 // With pnorm: pnorm[block] = this block's sum of the updated w^2.
 template <typename VT>
-__global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const double* __restrict__ h,
-                                                   const VT* const* __restrict__ V, double* __restrict__ w,
-                                                   double* __restrict__ pnorm) {
+__device__ __forceinline__ void multiaxpy_body(long long n, int m, const double* __restrict__ h,
+                                               const VT* const* __restrict__ V, double* __restrict__ w,
+                                               double* __restrict__ pnorm) {
    __shared__ double red[8];
    double nn = 0.0;
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -317,6 +383,17 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const dou
       nn = block_sum(nn, red);
       if (threadIdx.x == 0) pnorm[blockIdx.x] = nn;
    }
+}
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multiaxpy(long long n, int m, const double* __restrict__ h,
+                                                   const VT* const* __restrict__ V, double* __restrict__ w,
+                                                   double* __restrict__ pnorm) {
+   multiaxpy_body<VT>(n, m, h, V, w, pnorm);
+}
+template <typename VT>
+__global__ void __launch_bounds__(256) k_multiaxpy_b(long long n, int m, const GmDev* __restrict__ d, int norm) {
+   const GmDev& a = d[blockIdx.y];
+   multiaxpy_body<VT>(n, m, a.htmp, (const VT* const*)a.V, a.w, norm ? a.pnorm : nullptr);
 }
 
 // x = sum_j y[j] * V[j], in order of j.
@@ -670,6 +747,49 @@ void launch_normalize_givens(long long n, int np, const double* pin, const doubl
    hipLaunchKernelGGL((k_normalize_givens<VT>), dim3(nblocks(n, 256, 512)), dim3(256), lds, s, n, pin, np, w, v, i,
                       W, hcol, H, cs, sn, g, hist, normr);
 }
+// The lockstep GMRES step for ncol columns (d: their device descriptors),
+// each launch the single-column launch with one grid slice per column.
+template <typename XT>
+void launch_join_b(int n, int ncol, const int* pi, const int* di, const double* sp, const double* sd,
+                   const XT* const* x, double* xs, double* xd, long long ld, hipStream_t s) {
+   if (n > 0 && ncol > 0)
+      hipLaunchKernelGGL((k_join_b<XT>), dim3(nblocks(n), ncol), dim3(256), 0, s, n, pi, di, sp, sd, x, xs, xd, ld);
+}
+template void launch_join_b<double>(int, int, const int*, const int*, const double*, const double*,
+                                    const double* const*, double*, double*, long long, hipStream_t);
+template void launch_join_b<float>(int, int, const int*, const int*, const double*, const double*,
+                                   const float* const*, double*, double*, long long, hipStream_t);
+void launch_split_b(int n, int ncol, const int* oi, const double* corr, const double* yt, const double* xd,
+                    long long ld, double* const* y, hipStream_t s) {
+   if (n > 0 && ncol > 0)
+      hipLaunchKernelGGL(k_split_b, dim3(nblocks(n), ncol), dim3(256), 0, s, n, oi, corr, yt, xd, ld, y);
+}
+void launch_precond_b(int nb, int ncol, const int* ia, const int* ib, const double* M, const GmDev* d,
+                      hipStream_t s) {
+   if (nb > 0 && ncol > 0) hipLaunchKernelGGL(k_precond_b, dim3(nblocks(nb), ncol), dim3(256), 0, s, nb, ia, ib, M, d);
+}
+template <typename VT>
+void launch_cgs2_b(long long n, int m, int nch, int ncol, const GmDev* d, hipStream_t s) {
+   if (m <= 0 || ncol <= 0) return;
+   for (int pass = 0; pass < 2; pass++) {
+      hipLaunchKernelGGL((k_multidot_b<VT>), dim3(nch, (m + MD_COLS - 1) / MD_COLS, ncol), dim3(256), 0, s, n, m, nch,
+                         d);
+      hipLaunchKernelGGL(k_multidot_finish_b, dim3(m, ncol), dim3(64), 0, s, m, nch, d, pass);
+      hipLaunchKernelGGL((k_multiaxpy_b<VT>), dim3(multiaxpy_blocks(n), ncol), dim3(256), 0, s, n, m, d, pass);
+   }
+}
+template <typename VT>
+void launch_normalize_givens_b(long long n, int np, int i, int W, int ncol, const GmDev* d, hipStream_t s) {
+   const size_t lds = sizeof(double) * (size_t)(3 * i + 2);
+   if (ncol > 0)
+      hipLaunchKernelGGL((k_normalize_givens_b<VT>), dim3(nblocks(n, 256, 512), ncol), dim3(256), lds, s, n, np, i, W,
+                         d);
+}
+template void launch_cgs2_b<double>(long long, int, int, int, const GmDev*, hipStream_t);
+template void launch_cgs2_b<float>(long long, int, int, int, const GmDev*, hipStream_t);
+template void launch_normalize_givens_b<double>(long long, int, int, int, int, const GmDev*, hipStream_t);
+template void launch_normalize_givens_b<float>(long long, int, int, int, int, const GmDev*, hipStream_t);
+
 template <typename VT>
 void launch_combine(long long n, int m, const double* y, const VT* const* V, double* x, hipStream_t s) {
    hipLaunchKernelGGL((k_combine<VT>), dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, n, m, y, V, x);

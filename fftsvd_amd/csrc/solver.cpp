@@ -38,6 +38,17 @@ void launch_normalize_givens(long long n, int np, const double* pin, const doubl
                              double* hcol, double* H, double* cs, double* sn, double* g, double* hist, double normr,
                              hipStream_t s);
 void launch_axpby(long long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
+template <typename XT>
+void launch_join_b(int n, int ncol, const int* pi, const int* di, const double* sp, const double* sd,
+                   const XT* const* x, double* xs, double* xd, long long ld, hipStream_t s);
+void launch_split_b(int n, int ncol, const int* oi, const double* corr, const double* yt, const double* xd,
+                    long long ld, double* const* y, hipStream_t s);
+void launch_precond_b(int nb, int ncol, const int* ia, const int* ib, const double* M, const GmDev* d,
+                      hipStream_t s);
+template <typename VT>
+void launch_cgs2_b(long long n, int m, int nch, int ncol, const GmDev* d, hipStream_t s);
+template <typename VT>
+void launch_normalize_givens_b(long long n, int np, int i, int W, int ncol, const GmDev* d, hipStream_t s);
 void launch_rhs(int ne, const double* cent, const int* out, int nq, const double* qpos, const double* q, double* rhs,
                 hipStream_t s);
 void launch_potentials(int nq, int ne, const PanelGeom* panels, const int* pidx, const double* coef,
@@ -90,6 +101,9 @@ Problem::~Problem() {
    if (h_pinned) (void)hipHostFree(h_pinned);
    gm_cols.clear();
    if (h_pinned_b) (void)hipHostFree(h_pinned_b);
+   for (hipEvent_t e : ring_ev)
+      if (e) (void)hipEventDestroy(e);
+   if (ring_h) (void)hipHostFree(ring_h);
    for (int k = 0; k < MAX_AUX; k++) {
       if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
       if (aux[k]) (void)hipStreamDestroy(aux[k]);
@@ -1000,9 +1014,31 @@ SolveInfo problem_solve(Problem& pr, int mode, const double* rhs, double* sol, d
 
 // ---- many right-hand sides in lockstep ----
 
+// Copies `bytes` from host memory to a device slot of the upload ring, on
+// the library stream; returns the device address.  A slot's host side is
+// rewritten only after its previous copy has completed; its device side is
+// rewritten by a copy on the library stream, which runs after everything
+// enqueued before it there (and every aux stream is joined back into it).
+static const void* ring_upload(Problem& pr, const void* data, size_t bytes) {
+   if (bytes > Problem::RING_BYTES) throw std::runtime_error("batched step descriptors above the upload slot size");
+   if (!pr.ring_h) {
+      FSVD_CHECK(hipHostMalloc((void**)&pr.ring_h, Problem::RING * Problem::RING_BYTES));
+      pr.ring_d.alloc(Problem::RING * Problem::RING_BYTES);
+      for (hipEvent_t& e : pr.ring_ev) FSVD_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+   }
+   const int k = pr.ring_next;
+   pr.ring_next = (k + 1) % Problem::RING;
+   FSVD_CHECK(hipEventSynchronize(pr.ring_ev[k]));
+   unsigned char* h = pr.ring_h + (size_t)k * Problem::RING_BYTES;
+   unsigned char* d = pr.ring_d.p + (size_t)k * Problem::RING_BYTES;
+   std::memcpy(h, data, bytes);
+   FSVD_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, pr.stream));
+   FSVD_CHECK(hipEventRecord(pr.ring_ev[k], pr.stream));
+   return d;
+}
+
 // y_c = A x_c for the ncol columns (x[c], y[c] device vectors): matvec_impl's
-// steps per region with the tree applies column-batched; the joins and
-// splits stay per column.
+// steps per region, every launch covering all columns.
 template <typename XT>
 static void matvec_batch(Problem& pr, int prec, int ncol, const XT* const* x, double* const* y) {
    if (pr.adjoint || pr.qual) throw std::runtime_error("batched matvec: surface operators only");
@@ -1020,6 +1056,8 @@ static void matvec_batch(Problem& pr, int prec, int ncol, const XT* const* x, do
       if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
       tree_batch_reserve(op, ncol, s);
    }
+   const XT* const* dx = (const XT* const*)ring_upload(pr, x, sizeof(XT*) * ncol);
+   double* const* dy = (double* const*)ring_upload(pr, y, sizeof(double*) * ncol);
    if (pr.naux > 0) {
       FSVD_CHECK(hipEventRecord(pr.ev_fork, s));
       for (int k = 0; k < pr.naux; k++) FSVD_CHECK(hipStreamWaitEvent(pr.aux[k], pr.ev_fork, 0));
@@ -1039,14 +1077,11 @@ static void matvec_batch(Problem& pr, int prec, int ncol, const XT* const* x, do
          hipStream_t rs = R.lane == 0 ? s : pr.aux[R.lane - 1];
          TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
          if (step == 0) {
-            for (int c = 0; c < ncol; c++)
-               launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x[c], R.xsB.p + (size_t)c * n,
-                           R.xdB.p + (size_t)c * n, rs);
+            launch_join_b<XT>(n, ncol, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, dx, R.xsB.p, R.xdB.p, n, rs);
          } else if (step < 5) {
             tree_apply_part_batch(op, prec, parts[step - 1], ncol, R.xsB.p, R.xdB.p, n, R.ytB.p, n, rs);
          } else {
-            for (int c = 0; c < ncol; c++)
-               launch_split(n, R.out_idx.p, R.corr.p, R.ytB.p + (size_t)c * n, R.xdB.p + (size_t)c * n, y[c], rs);
+            launch_split_b(n, ncol, R.out_idx.p, R.corr.p, R.ytB.p, R.xdB.p, n, dy, rs);
          }
       }
    }
@@ -1186,25 +1221,39 @@ static void gmres_batch_impl(Problem& pr, int prec, int ncol, GmCol* const* cols
             yout.push_back(cols[c]->w.p);
          }
       if (act.empty()) break;
-      for (int c : act) col_basis<VT>(pr, *cols[c], i + 1);   // allocations before the batched launches
-      matvec_batch(pr, prec, (int)act.size(), xin.data(), yout.data());
-      for (int c : act) {
-         GmCol& g = *cols[c];
-         if (prec == PREC_F32)
-            g.info.matvecs32++;
-         else
-            g.info.matvecs64++;
-         problem_precond(pr, g.w.p, g.w.p);
-         const VT* const* V = col_basis_ptrs<VT>(pr, g, i + 2);
-         for (int pass = 0; pass < 2; pass++) {
-            launch_multidot<VT>(n, i + 1, nch, V, g.w.p, g.partial_m.p, s);
-            launch_multidot_finish(i + 1, nch, g.partial_m.p, g.htmp.p, g.hbuf.p, pass, s);
-            launch_multiaxpy<VT>(n, i + 1, g.htmp.p, V, g.w.p, pass ? g.pnorm.p : nullptr, s);
-         }
-         double* dcs = g.v.p;
-         launch_normalize_givens(n, nax, g.pnorm.p, g.w.p, col_basis<VT>(pr, g, i + 1), i, W, g.hbuf.p, g.H.p, dcs,
-                                 dcs + (W + 1), dcs + 2 * (W + 1), dcs + 3 * (W + 1), g.normr, s);
+      // allocations and pointer uploads before the batched launches
+      std::vector<GmDev> desc(act.size());
+      for (size_t k = 0; k < act.size(); k++) {
+         GmCol& g = *cols[act[k]];
+         GmDev& d = desc[k];
+         d.vnext = col_basis<VT>(pr, g, i + 1);
+         d.V = col_basis_ptrs<VT>(pr, g, i + 2);
+         d.w = g.w.p;
+         d.partial_m = g.partial_m.p;
+         d.htmp = g.htmp.p;
+         d.hbuf = g.hbuf.p;
+         d.pnorm = g.pnorm.p;
+         d.H = g.H.p;
+         d.cs = g.v.p;
+         d.sn = d.cs + (W + 1);
+         d.g = d.sn + (W + 1);
+         d.hist = d.g + (W + 1);
+         d.normr = g.normr;
       }
+      const int na = (int)act.size();
+      matvec_batch(pr, prec, na, xin.data(), yout.data());
+      const GmDev* dd = (const GmDev*)ring_upload(pr, desc.data(), sizeof(GmDev) * na);
+      for (int c : act) {
+         if (prec == PREC_F32)
+            cols[c]->info.matvecs32++;
+         else
+            cols[c]->info.matvecs64++;
+      }
+      // the per-column steps of gmres_impl, one launch for all columns each:
+      // P^-1 w, CGS2 over v_0..v_i, |w|, v_{i+1} and the Givens step
+      launch_precond_b(pr.nblocks, na, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, dd, s);
+      launch_cgs2_b<VT>(n, i + 1, nch, na, dd, s);
+      launch_normalize_givens_b<VT>(n, nax, i, W, na, dd, s);
       if ((i + 1) % CHECK == 0 || i == maxit - 1) {
          const int a0 = last + 1;
          for (int c : act)

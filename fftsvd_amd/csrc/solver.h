@@ -152,6 +152,15 @@ struct Problem {
    std::vector<std::unique_ptr<GmCol>> gm_cols;
    double* h_pinned_b = nullptr;
    int h_pinned_bcols = 0;
+   // small uploads of the batched paths (pointer arrays, column
+   // descriptors): RING slots of RING_BYTES, pinned and on the device, each
+   // with the event of its last copy
+   static constexpr int RING = 8;
+   static constexpr size_t RING_BYTES = 65536;
+   unsigned char* ring_h = nullptr;
+   DBuf<unsigned char> ring_d;
+   hipEvent_t ring_ev[RING] = {};
+   int ring_next = 0;
    double setup_ms = 0.0;
    ~Problem();
 };
