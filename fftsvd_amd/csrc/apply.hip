@@ -945,6 +945,234 @@ __global__ void __launch_bounds__(TP_NT) k_translate_pipe(const int2* __restrict
       tp_consumer(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, smem, ctr, lane, adel + cb.c * acs);
 }
 
+// ---------------------------------------------------------------------------
+// The pipelined translation for two columns at once (column-batched
+// applies): the same tiles, octets, interaction words and per-target
+// summation order as k_translate_pipe, with 4-bin chunks.  A source row
+// holds both columns' bins interleaved (bin j: column c0 then c1, 16
+// bytes), a T row 4 bins (32 bytes); lane (target, bin j) reads its T bin
+// once and both columns' source bins with one 16-byte read, so every T read
+// serves two complex multiply-adds.  Each column's accumulators see the
+// single-column sequence of operations, so both outputs equal
+// k_translate_pipe's bit for bit.  A wave sums two octets (16 targets, 4
+// lanes each); staging goes through registers (the interleave rules out
+// direct transfers).
+constexpr int TP2_BUF = TP_TROWS * 32 + TP_MAX_SRC * 64;
+static_assert(2 * TP2_BUF + 16 <= 163840, "two pair buffers must fit the LDS of a CU");
+constexpr int TP2_SPIECES = (TP_MAX_SRC * 2 + TP_PL - 1) / TP_PL;   // source half-rows per producer lane
+constexpr int TP2_TPIECES = (343 * 2 + TP_PL - 1) / TP_PL;          // T half-rows per producer lane
+
+__device__ __forceinline__ void tp_stage2(const int4& tk, const TpTile& tl, const float2* spec1, bool stage_t,
+                                          unsigned char* buf, int pl, const int* __restrict__ gsrc) {
+   const int b0 = tk.y;
+   const int npiece = tl.nsrc * 2;
+   unsigned so[TP2_SPIECES];
+#pragma unroll
+   for (int u = 0; u < TP2_SPIECES; u++) {
+      const int p = pl + TP_PL * u;
+      so[u] = 8u * (unsigned)(gsrc[tl.src_off + min(p >> 1, tl.nsrc - 1)] + b0 + 2 * (p & 1));
+   }
+   unsigned char* sbuf = buf + TP_TROWS * 32;
+   constexpr int B = 6;
+#pragma unroll
+   for (int u0 = 0; u0 < TP2_SPIECES; u0 += B) {
+      float4 a[B], b[B];
+#pragma unroll
+      for (int u = 0; u < B; u++)
+         if (u0 + u < TP2_SPIECES) {
+            a[u] = *(const float4*)((const char*)tl.spec + so[u0 + u]);
+            b[u] = *(const float4*)((const char*)spec1 + so[u0 + u]);
+         }
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+         const int p = pl + TP_PL * (u0 + u);
+         if (u0 + u < TP2_SPIECES && p < npiece) {
+            unsigned char* row = sbuf + 64 * (p >> 1) + 32 * (p & 1);
+            *(float4*)row = make_float4(a[u].x, a[u].y, b[u].x, b[u].y);
+            *(float4*)(row + 16) = make_float4(a[u].z, a[u].w, b[u].z, b[u].w);
+         }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+   }
+   if (stage_t) {
+      float4 v[TP2_TPIECES];
+#pragma unroll
+      for (int u = 0; u < TP2_TPIECES; u++) {
+         const int p = min(pl + TP_PL * u, 343 * 2 - 1);
+         v[u] = *(const float4*)(tl.T + (long long)(p >> 1) * tl.pst + b0 + 2 * (p & 1));
+      }
+#pragma unroll
+      for (int u = 0; u < TP2_TPIECES; u++)
+         if (pl + TP_PL * u < 343 * 2) *(float4*)(buf + 16 * (pl + TP_PL * u)) = v[u];
+   }
+}
+
+__device__ __forceinline__ void tp_producer2(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
+                                             const int* __restrict__ gsrc, unsigned char* smem, int* ctr, int pl,
+                                             long long sdel0, long long sdel1) {
+   long long key0, key1 = -1;
+   {
+      const int4 t0 = tasks[0];
+      TpTile tl = tiles[t0.x];
+      const float2* s1 = (const float2*)((const char*)tl.spec + sdel1);
+      tl.spec = (const float2*)((const char*)tl.spec + sdel0);
+      key0 = (long long)(tl.T + t0.y);
+      tp_stage2(t0, tl, s1, true, smem, pl, gsrc);
+   }
+   __syncthreads();
+   for (int k = 0; k < n; k++) {
+      if (pl == 0) ctr[(k + 2) % 3] = 0;
+      if (k + 1 < n) {
+         const int4 tn = tasks[k + 1];
+         TpTile tl = tiles[tn.x];
+         const float2* s1 = (const float2*)((const char*)tl.spec + sdel1);
+         tl.spec = (const float2*)((const char*)tl.spec + sdel0);
+         const long long key = (long long)(tl.T + tn.y);
+         const int nb = (k + 1) & 1;
+         const bool st = (nb ? key1 : key0) != key;
+         if (nb)
+            key1 = key;
+         else
+            key0 = key;
+         tp_stage2(tn, tl, s1, st, smem + nb * TP2_BUF, pl, gsrc);
+      }
+      __syncthreads();
+   }
+}
+
+// Words of group i (steps 4i..4i+3) of target t of octet o; null words past
+// the octet's steps (the other octet of the wave may run longer).
+__device__ __forceinline__ int4 tp_group2(const int* __restrict__ gint, const TpOct& o, int i, int t) {
+   if (4 * i >= o.nk) return make_int4((int)TP_NULL_WORD, (int)TP_NULL_WORD, (int)TP_NULL_WORD, (int)TP_NULL_WORD);
+   return *((const int4*)(gint + o.int_off) + 8 * i + t);
+}
+
+// Two word groups (eight steps) for both columns: all sixteen LDS reads
+// issued before the first multiply-add, then the steps in list order, step
+// q on accumulator q & 3, as the single-column sum.
+__device__ __forceinline__ void tp_steps2(const int4& x, const int4& y, const unsigned char* Tl,
+                                          const unsigned char* Sl, float* re0, float* im0, float* re1, float* im1) {
+   const unsigned u[8] = {(unsigned)x.x, (unsigned)x.y, (unsigned)x.z, (unsigned)x.w,
+                          (unsigned)y.x, (unsigned)y.y, (unsigned)y.z, (unsigned)y.w};
+   float2 t[8];
+   float4 v[8];
+#pragma unroll
+   for (int q = 0; q < 8; q++) {
+      t[q] = *(const float2*)(Tl + ((u[q] & 0xffffu) >> 1));
+      v[q] = *(const float4*)(Sl + (u[q] >> 16));
+   }
+   __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+   for (int q = 0; q < 8; q++) {
+      float& r0 = re0[q & 3];
+      float& m0 = im0[q & 3];
+      float& r1 = re1[q & 3];
+      float& m1 = im1[q & 3];
+      r0 = fmaf(t[q].x, v[q].x, r0);
+      r0 = fmaf(-t[q].y, v[q].y, r0);
+      m0 = fmaf(t[q].x, v[q].y, m0);
+      m0 = fmaf(t[q].y, v[q].x, m0);
+      r1 = fmaf(t[q].x, v[q].z, r1);
+      r1 = fmaf(-t[q].y, v[q].w, r1);
+      m1 = fmaf(t[q].x, v[q].w, m1);
+      m1 = fmaf(t[q].y, v[q].z, m1);
+   }
+}
+
+__device__ __forceinline__ void tp_consumer2(const int4* __restrict__ tasks, int n, const TpTile* __restrict__ tiles,
+                                             const TpOct* __restrict__ octs, const long long* __restrict__ oacc,
+                                             const int* __restrict__ gint, const unsigned char* smem, int* ctr,
+                                             int lane, long long adel0, long long adel1) {
+   const int h = lane >> 5, t = (lane >> 2) & 7, j = lane & 3;
+   __syncthreads();
+   for (int k = 0; k < n; k++) {
+      const int4 tk = tasks[k];
+      const int pad = tiles[tk.x].pad;
+      float2* acc0 = (float2*)((char*)tiles[tk.x].acc + adel0);
+      float2* acc1 = (float2*)((char*)tiles[tk.x].acc + adel1);
+      const unsigned char* Tl = smem + (k & 1) * TP2_BUF + j * 8;
+      const unsigned char* Sl = smem + (k & 1) * TP2_BUF + TP_TROWS * 32 + j * 16;
+      for (;;) {
+         int ic = 0;
+         if (lane == 0) ic = atomicAdd(ctr + k % 3, 2);
+         ic = __builtin_amdgcn_readfirstlane(ic);
+         if (ic >= tk.w) break;
+         // octets ic (lanes 0-31) and ic + 1 (lanes 32-63, if any); the wave
+         // runs the longer one's steps, the other half reads null words
+         const TpOct oa = octs[tk.z + ic];
+         const bool hb = ic + 1 < tk.w;
+         const TpOct ob = hb ? octs[tk.z + ic + 1] : TpOct{0, 0, 0, 0};
+         const TpOct o = h ? ob : oa;
+         const int nkw = max(oa.nk, ob.nk);
+         const long long ao = (h == 0 || hb) ? oacc[o.acc_idx + t] : -1;
+         float re0[4] = {0.f, 0.f, 0.f, 0.f}, im0[4] = {0.f, 0.f, 0.f, 0.f};
+         float re1[4] = {0.f, 0.f, 0.f, 0.f}, im1[4] = {0.f, 0.f, 0.f, 0.f};
+         // word-group pairs, loaded two pairs ahead of their use in three
+         // rotating register sets
+         const int np = nkw / 8;
+         TpPair A = {tp_group2(gint, o, 0, t), tp_group2(gint, o, 1, t)};
+         TpPair B = {tp_group2(gint, o, 2, t), tp_group2(gint, o, 3, t)};
+         for (int q = 0; q < np; q += 3) {
+            const TpPair C = {tp_group2(gint, o, 2 * q + 4, t), tp_group2(gint, o, 2 * q + 5, t)};
+            tp_steps2(A.x, A.y, Tl, Sl, re0, im0, re1, im1);
+            if (q + 1 >= np) break;
+            A = {tp_group2(gint, o, 2 * q + 6, t), tp_group2(gint, o, 2 * q + 7, t)};
+            tp_steps2(B.x, B.y, Tl, Sl, re0, im0, re1, im1);
+            if (q + 2 >= np) break;
+            B = {tp_group2(gint, o, 2 * q + 8, t), tp_group2(gint, o, 2 * q + 9, t)};
+            tp_steps2(C.x, C.y, Tl, Sl, re0, im0, re1, im1);
+         }
+         if (ao >= 0 && tk.y + j < pad) {
+            acc0[ao + tk.y + j] = make_float2((re0[0] + re0[1]) + (re0[2] + re0[3]), (im0[0] + im0[1]) + (im0[2] + im0[3]));
+            acc1[ao + tk.y + j] = make_float2((re1[0] + re1[1]) + (re1[2] + re1[3]), (im1[0] + im1[1]) + (im1[2] + im1[3]));
+         }
+      }
+      __syncthreads();
+   }
+}
+
+__global__ void __launch_bounds__(TP_NT) k_translate_pipe2(const int2* __restrict__ ranges,
+                                                           const int4* __restrict__ tasks_all,
+                                                           const TpTile* __restrict__ tiles,
+                                                           const TpOct* __restrict__ octs,
+                                                           const long long* __restrict__ oacc,
+                                                           const int* __restrict__ gsrc, const int* __restrict__ gint,
+                                                           int nb, int ncol, long long sdel, long long adel,
+                                                           long long scs, long long acs) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const ColBlk cb = col_block_major(nb, (ncol + 1) / 2);
+   if (cb.b >= nb) return;   // uniform over the block: no barrier is reached
+   const int c0 = 2 * cb.c, c1 = min(c0 + 1, ncol - 1);
+   const int2 rg = ranges[cb.b];
+   const int tid = threadIdx.x, lane = tid & 63;
+   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   // the zero T row of both buffers (4 bins) and the three octet counters
+   int* ctr = (int*)(smem + 2 * TP2_BUF);
+   if (tid < 8) ((float2*)(smem + (tid >> 2) * TP2_BUF + TP_ZERO_ROW * 32))[tid & 3] = make_float2(0.f, 0.f);
+   if (tid < 3) ctr[tid] = 0;
+   if (rg.y == 0) return;
+   if (wave >= TP_CW)
+      tp_producer2(tasks_all + rg.x, rg.y, tiles, gsrc, smem, ctr, 64 * (wave - TP_CW) + lane, sdel + c0 * scs,
+                   sdel + c1 * scs);
+   else
+      tp_consumer2(tasks_all + rg.x, rg.y, tiles, octs, oacc, gint, smem, ctr, lane, adel + c0 * acs,
+                   adel + c1 * acs);
+}
+
+void launch_translate_pipe2(int nblk, const int2* range, const int4* tasks, const TpTile* tiles, const TpOct* octs,
+                            const long long* oacc, const int* gsrc, const int* gint, hipStream_t s, int ncol,
+                            long long sdel, long long adel, long long scs, long long acs) {
+   constexpr size_t lds = 2 * (size_t)TP2_BUF + 16;
+   static std::once_flag attr;
+   std::call_once(attr, [] {
+      FSVD_CHECK(hipFuncSetAttribute((const void*)k_translate_pipe2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+   });
+   if (nblk <= 0) return;
+   hipLaunchKernelGGL(k_translate_pipe2, dim3(col_grid(nblk, (ncol + 1) / 2)), dim3(TP_NT), lds, s, range, tasks,
+                      tiles, octs, oacc, gsrc, gint, nblk, ncol, sdel, adel, scs, acs);
+}
+
 template <int n, typename T, typename G>
 __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
                             const typename Cplx<T>::type* __restrict__ acc, T* __restrict__ w, unsigned char* smem,
@@ -1504,9 +1732,14 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       }
       case 2: {
          if constexpr (std::is_same<T, float>::value) {
-            launch_translate_pipe(a.nblk_p, a.mode_p, a.range_p, a.task_p, a.tiles_p, a.oct_p, a.oacc_p, a.gsrc_p,
-                                  a.gint_p, s, a.ncol, a.spec_delta, a.acc_delta, a.cs_spec * (long long)sizeof(C2),
-                                  a.cs_acc * (long long)sizeof(C2));
+            if (a.ncol >= 2 && a.nblk_p2 > 0)
+               launch_translate_pipe2(a.nblk_p2, a.range_p2, a.task_p2, a.tiles_p2, a.oct_p2, a.oacc_p2, a.gsrc_p2,
+                                      a.gint_p2, s, a.ncol, a.spec_delta, a.acc_delta,
+                                      a.cs_spec * (long long)sizeof(C2), a.cs_acc * (long long)sizeof(C2));
+            else
+               launch_translate_pipe(a.nblk_p, a.mode_p, a.range_p, a.task_p, a.tiles_p, a.oct_p, a.oacc_p,
+                                     a.gsrc_p, a.gint_p, s, a.ncol, a.spec_delta, a.acc_delta,
+                                     a.cs_spec * (long long)sizeof(C2), a.cs_acc * (long long)sizeof(C2));
          } else {
             // LDS: T spectra and at most TR_MAX_SRC source spectra of BC bins
             constexpr int BC = TR_BINS_F64, LPT = TR_LPT_F64;

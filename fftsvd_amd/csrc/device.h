@@ -230,6 +230,14 @@ struct ApplyArgs {
    const TpOct* oct_p;
    const long long* oacc_p;
    const int *gsrc_p, *gint_p;
+   // its two-column form (column-batched applies)
+   int nblk_p2 = 0;
+   const int2* range_p2 = nullptr;
+   const int4* task_p2 = nullptr;
+   const TpTile* tiles_p2 = nullptr;
+   const TpOct* oct_p2 = nullptr;
+   const long long* oacc_p2 = nullptr;
+   const int *gsrc_p2 = nullptr, *gint_p2 = nullptr;
    int nmap_leaf;                    // XCD block map of the leaf launch
    const int* map_leaf;
    const LeafDesc* leaves;
@@ -274,5 +282,14 @@ __device__ __forceinline__ ColBlk col_block(int ncol) {
    return {(k / ncol) * 8 + (lin & 7), k % ncol};
 }
 inline int col_grid(int nb, int ncol) { return ncol == 1 ? nb : (nb + 7) / 8 * 8 * ncol; }
+// The same grid taken column by column (all blocks of column 0, then of
+// column 1, ...), block b still on XCD b % 8: for the persistent
+// translation, whose copies share little but the T rows.
+__device__ __forceinline__ ColBlk col_block_major(int nb, int ncol) {
+   const int lin = (int)blockIdx.x;
+   if (ncol == 1) return {lin, 0};
+   const int nbp = (nb + 7) / 8 * 8;
+   return {lin % nbp, lin / nbp};
+}
 
 }  // namespace fsvd

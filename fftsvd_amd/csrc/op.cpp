@@ -509,7 +509,7 @@ static std::vector<int4> assign_pipe_blocks(const std::vector<int4>& tasks, cons
    return out;
 }
 
-void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s, int mode) {
+void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s, int mode, bool pair) {
    std::vector<TpTile> tiles;
    std::vector<TpOct> octs;
    std::vector<long long> oacc;
@@ -545,6 +545,29 @@ void make_pipe(PipeDev& d, const std::vector<const TreeOp*>& ops, hipStream_t s,
          tasks.push_back(t);
          cost.push_back(H.cost[i]);
       }
+   }
+   if (pair) {
+      // each run of tasks of one level and 8-bin chunk (consecutive, over
+      // the level's tiles) becomes the run of its first 4-bin halves, then
+      // the run of its second halves, so consecutive tasks keep sharing
+      // their T rows
+      std::vector<int4> t2;
+      std::vector<double> c2;
+      for (size_t a = 0; a < tasks.size();) {
+         size_t b = a + 1;
+         while (b < tasks.size() && tasks[b].y == tasks[a].y && tiles[tasks[b].x].T == tiles[tasks[a].x].T) b++;
+         for (int half = 0; half < 2; half++)
+            for (size_t i = a; i < b; i++) {
+               int4 t = tasks[i];
+               t.y += 4 * half;
+               if (t.y >= tiles[t.x].pad) continue;
+               t2.push_back(t);
+               c2.push_back(0.5 * cost[i]);
+            }
+         a = b;
+      }
+      tasks.swap(t2);
+      cost.swap(c2);
    }
    if (gint.empty()) gint.push_back((int)TP_NULL_WORD);
    if (oacc.empty()) oacc.push_back(-1);
@@ -1595,6 +1618,17 @@ static void apply_part_batch_impl(TreeOp& op, int part, int ncol, const double* 
    a.cs_w = (long long)P.w.n;
    a.spec_delta = (long long)((const char*)P.specB.p - (const char*)P.spec.p);
    a.acc_delta = (long long)((const char*)P.accB.p - (const char*)P.acc.p);
+   if (std::is_same<T, float>::value && ncol >= 2 && part == 2) {
+      if (!P.pipe2.nblk && P.pipe.nblk) make_pipe(P.pipe2, {&op}, s, 0, true);
+      a.nblk_p2 = P.pipe2.nblk;
+      a.range_p2 = P.pipe2.range.p;
+      a.task_p2 = P.pipe2.tasks.p;
+      a.tiles_p2 = P.pipe2.tiles.p;
+      a.oct_p2 = P.pipe2.octs.p;
+      a.oacc_p2 = P.pipe2.oacc.p;
+      a.gsrc_p2 = P.pipe2.gsrc.p;
+      a.gint_p2 = P.pipe2.gint.p;
+   }
    switch (part) {
       case 0: launch_apply_stage<T>(4, a, s); break;
       case 1:

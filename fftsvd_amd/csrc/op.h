@@ -145,6 +145,7 @@ struct ApplyPlan {
    DBuf<int> gsrc, gint;
    PipeHost pipe_h;   // FP32 translation (k_translate_pipe) of this tree, host form
    PipeDev pipe;      // ... on the device
+   PipeDev pipe2;     // the two-column form (4-bin tasks), made at the first column-batched apply
    // the translation lists the pipelined plan is built from (kept for the
    // autotuner's re-planning)
    std::vector<TrLevel> tr_lv;
@@ -199,7 +200,13 @@ struct TreeOp {
 
 // The device plan of one launch covering the FP32 translations of the given
 // trees (each with its own spectra), tasks dealt to the persistent blocks.
-void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s, int mode = 1);
+// pair: the tasks of the two-column kernel (each 8-bin task split into its
+// two 4-bin halves, every level's first halves before its second halves).
+void make_pipe(PipeDev& d, const std::vector<const struct TreeOp*>& ops, hipStream_t s, int mode = 1,
+               bool pair = false);
+void launch_translate_pipe2(int nblk, const int2* range, const int4* tasks, const TpTile* tiles, const TpOct* octs,
+                            const long long* oacc, const int* gsrc, const int* gint, hipStream_t s, int ncol,
+                            long long sdel, long long adel, long long scs, long long acs);
 // ncol > 1: every column's spectra and accumulators at byte offsets sdel +
 // c * scs and adel + c * acs from the buffers the tiles point at.
 void launch_translate_pipe(int nblk, int mode, const int2* range, const int4* tasks, const TpTile* tiles,
