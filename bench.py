@@ -176,6 +176,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--multi-rhs", type=int, default=64, help="charge vectors of the batched solve (0: skip)")
+    ap.add_argument("--uneven", type=int, default=8, help="molecules of the uneven batch (0: skip)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -292,6 +293,44 @@ def main():
         print("# batched energies: mixed vs fp64 max %.2e (median %.2e); vs fp64 at 1e-10: fp64 %.2e, mixed %.2e" %
               (rel.max(), np.median(rel), multi["fp64"]["energy_rel_err_vs_fp64_tol1e-10_max"],
                multi["mixed"]["energy_rel_err_vs_fp64_tol1e-10_max"]), flush=True)
+
+    # uneven batch: seeded random-walk molecules with log-uniform atom counts
+    # (the globule's recipe and grid, confinement scaled with size), each on
+    # its own operator; all mixed solves one after another, then at once
+    uneven = None
+    if not args.no_solve and args.uneven > 0:
+        from fftsvd_amd import meshes
+        rng = np.random.default_rng(args.seed + 500)
+        counts = np.exp(rng.uniform(np.log(300), np.log(3000), size=args.uneven)).astype(int)
+        probs, rhs_u, npan_u = [], [], []
+        t0 = time.time()
+        for k, na in enumerate(counts):
+            xu, ru, qu = meshes.synthetic_globule(natoms=int(na), seed=args.seed + 600 + k,
+                                                  confine=26.0 * (na / 2000.0) ** (1 / 3))
+            Vu, Fu = meshes.ses_mesh(xu, ru, probe=1.4, spacing=h)
+            pk = fa.Problem(meshes.panels_from_mesh(Vu, Fu), xu, qu, 4.0, 80.0, 0.124, want_f32=True)
+            probs.append(pk)
+            rhs_u.append(pk.rhs())
+            npan_u.append(pk.nvec // 2)
+        t_build = time.time() - t0
+        t0 = time.time()
+        seq = [pk.solve(r, 1e-6, precision=fa.MIXED) for pk, r in zip(probs, rhs_u)]
+        t_seq = time.time() - t0
+        t0 = time.time()
+        Xu, itu, infu = fa.solve_many(probs, rhs_u, 1e-6, precision=fa.MIXED)
+        t_conc = time.time() - t0
+        same = all(np.array_equal(Xu[k], seq[k][0]) for k in range(len(probs)))
+        nmv = sum(d["matvecs64"] + d["matvecs32"] for d in infu)
+        uneven = {"molecules": len(probs), "atoms": [int(c) for c in counts], "panels": npan_u,
+                  "build_s": t_build, "solve_sequential_s": t_seq, "solve_concurrent_s": t_conc,
+                  "speedup": t_seq / t_conc, "matvecs": int(nmv), "matvecs_per_s_concurrent": nmv / t_conc,
+                  "concurrent_equals_sequential": bool(same)}
+        print("# uneven batch: %d molecules (%d..%d panels), mixed solves %.3fs one after another, %.3fs at once "
+              "(%.2fx, %.0f matvecs/s), identical %s; meshing + setup %.1fs" %
+              (len(probs), min(npan_u), max(npan_u), t_seq, t_conc, t_seq / t_conc, nmv / t_conc, same, t_build),
+              flush=True)
+        for pk in probs:
+            pk.close()
 
     # gradients of the energy (the adjoint): d/d(charges) and d/d(kappa) at
     # the mixed solution, timed; against central differences (charges: the
@@ -418,6 +457,7 @@ def main():
         "solve": solve,
         "multi_rhs": multi,
         "block_matvec": block,
+        "uneven_batch": uneven,
         "gradients": grads,
         "operator": st,
         "roofline": roofline,

@@ -43,6 +43,8 @@ _SIGS = {
     "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "fftsvd_problems_solve": (C.c_int, [C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_double,
+                                        C.c_int, C.POINTER(C.c_int), _dp]),
     "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
     "fftsvd_problem_solve_adjoint": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, _ip, _dp]),
     "fftsvd_problem_grad_kappa": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_int, _dp, _ip]),
@@ -135,6 +137,26 @@ def translation_params():
     ms = C.c_double()
     _check(lib().fftsvd_translation_params(*[C.byref(x) for x in v], C.byref(ms)))
     return dict(mode=v[0].value, budget=v[1].value, tile=v[2].value, tuned=bool(v[3].value), tuned_ms=ms.value)
+
+
+def solve_many(problems, rhs, tol=1e-6, maxit=1000, precision=F64):
+    """Solves problems[k] with right-hand side rhs[k] for every k at once
+    (fftsvd_problems_solve: concurrent solves, each as Problem.solve
+    computes it).  Returns (solutions, iterations, info dicts)."""
+    n = len(problems)
+    b = [_f64(r).reshape(-1) for r in rhs]
+    x = [np.zeros(p.nvec) for p in problems]
+    for p, r in zip(problems, b):
+        if r.size != p.nvec:
+            raise ValueError("right-hand side of the wrong length")
+    hs = (_vp * n)(*[p.h for p in problems])
+    bp = (_dp * n)(*[_d(r) for r in b])
+    xp = (_dp * n)(*[_d(v) for v in x])
+    its = np.zeros(n, dtype=np.int32)
+    info = np.zeros(6 * n)
+    _check(lib().fftsvd_problems_solve(n, hs, precision, bp, xp, tol, maxit, _i(its), _d(info)))
+    keys = ("iterations", "refinements", "final_residual", "ms", "matvecs64", "matvecs32")
+    return x, its, [dict(zip(keys, info[6 * k:6 * k + 6].tolist())) for k in range(n)]
 
 
 def integrate(verts9, pts, kernel=LAPLACE, kappa=0.0):

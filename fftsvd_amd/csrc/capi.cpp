@@ -1,5 +1,7 @@
 // C-ABI of libfftsvd_hip.so: the reference's FFTSVDpbeAPI.h solve path and
 // the extended array-based entry points of fftsvd_hip.h.
+#include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -10,6 +12,8 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/FFTSVDpbeAPI.h"
 #include "../../include/fftsvd_hip.h"
@@ -345,6 +349,61 @@ int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, do
       info[4] = si.matvecs64;
       info[5] = si.matvecs32;
    }
+   API_CATCH
+}
+
+int fftsvd_problems_solve(int nprob, fftsvd_problem* const* probs, int precision, const double* const* rhs,
+                          double* const* sol, double tol, int maxit, int* iters, double* info) {
+   API_TRY
+   if (nprob < 1) throw std::runtime_error("nprob must be positive");
+   if (maxit < 1 || maxit > 1000) throw std::runtime_error("maxit must be in [1, 1000]");
+   for (int k = 0; k < nprob; k++)
+      if (!probs[k] || !probs[k]->pr) throw std::runtime_error("null problem in the batch");
+   // largest first, taken by MULTI_THREADS workers, each problem's solve on
+   // its own streams, so the problems' kernels share the GPU
+   constexpr int MULTI_THREADS = 8;
+   std::vector<int> order(nprob);
+   for (int k = 0; k < nprob; k++) order[k] = k;
+   std::stable_sort(order.begin(), order.end(),
+                    [&](int a, int b) { return probs[a]->pr->nvec > probs[b]->pr->nvec; });
+   int dev = 0;
+   FSVD_CHECK(hipGetDevice(&dev));
+   std::atomic<int> next{0};
+   std::vector<std::exception_ptr> err(nprob);
+   auto work = [&] {
+      for (int j; (j = next.fetch_add(1)) < nprob;) {
+         const int k = order[j];
+         try {
+            FSVD_CHECK(hipSetDevice(dev));
+            Problem& pr = *probs[k]->pr;
+            const size_t n = (size_t)pr.nvec;
+            DBuf<double> db, dx;
+            db.upload(rhs[k], n, pr.stream);
+            dx.alloc(n);
+            const SolveInfo si = problem_solve(pr, precision, db.p, dx.p, tol, maxit, nullptr);
+            FSVD_CHECK(hipMemcpyAsync(sol[k], dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, pr.stream));
+            FSVD_CHECK(hipStreamSynchronize(pr.stream));
+            if (iters) iters[k] = si.iterations;
+            if (info) {
+               double* o = info + 6 * (size_t)k;
+               o[0] = si.iterations;
+               o[1] = si.refinements;
+               o[2] = si.final_residual;
+               o[3] = si.ms;
+               o[4] = si.matvecs64;
+               o[5] = si.matvecs32;
+            }
+         } catch (...) {
+            err[k] = std::current_exception();
+         }
+      }
+   };
+   std::vector<std::thread> th;
+   for (int t = 1; t < std::min(nprob, MULTI_THREADS); t++) th.emplace_back(work);
+   work();
+   for (auto& t : th) t.join();
+   for (auto& e : err)
+      if (e) std::rethrow_exception(e);
    API_CATCH
 }
 

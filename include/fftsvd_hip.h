@@ -84,6 +84,15 @@ int fftsvd_problem_rhs(fftsvd_problem* p, double* rhs);
  * FP64 matvecs, FP32 matvecs. */
 int fftsvd_problem_solve(fftsvd_problem* p, int precision, const double* rhs, double* sol, double tol, int maxit,
                          int* iters, double* resid_hist, double* info);
+/* Several problems (an uneven batch of molecules, each with its own
+ * operator) solved at once: problem k solves rhs[k] -> sol[k] (host arrays
+ * of its nvec) exactly as fftsvd_problem_solve would, iters[k] and info[6k
+ * ..] likewise; the solves run concurrently, each on its own problem's
+ * streams, largest first, so small molecules share the GPU instead of
+ * taking it in turn.  The reference solves one molecule per program run
+ * (FFTSVDpbe.c:101-268); there is no counterpart. */
+int fftsvd_problems_solve(int nprob, fftsvd_problem* const* probs, int precision, const double* const* rhs,
+                          double* const* sol, double tol, int maxit, int* iters, double* info);
 /* reaction potentials at the charges (SurfaceOperator.c:323-401 direct branch) */
 int fftsvd_problem_potentials(fftsvd_problem* p, const double* sol, double* phi);
 /* Many charge vectors at once (the multi-right-hand-side pattern of the
