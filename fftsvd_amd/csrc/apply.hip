@@ -370,6 +370,117 @@ __global__ void __launch_bounds__(256) k_project(const ProjChunk* __restrict__ c
    }
 }
 
+// k_project for NC columns per wave (column-batched applies): each VT entry
+// is loaded once and multiplied into the NC columns' accumulators, each in
+// k_project's order (inputs interleaved, input i of column j at
+// xl[i * NC + j]), so every column's partials are k_project's bit for bit.
+// A partial last group repeats its last column and stores only its own.
+// columns per wave: four FP32, two FP64 (whose four would take the whole
+// LDS of a CU for the widest chunk)
+template <typename T>
+constexpr int proj_nc() { return sizeof(T) == 4 ? 4 : 2; }
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_project_mc(const ProjChunk* __restrict__ chunks, int nchunk,
+                                                    const T* __restrict__ VTc, const double* __restrict__ xs,
+                                                    const double* __restrict__ xd, T* __restrict__ tpart,
+                                                    int wave_lds, int ncol, long long cs_x, long long cs_tpart) {
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int ngrp = (ncol + NC - 1) / NC;
+   const ColBlk cb = col_block(ngrp);
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ci = cb.b * 4 + wave;
+   if (ci >= nchunk) return;
+   const int c0 = cb.c * NC, nc = min(NC, ncol - c0);
+   const WaveGroup gr{lane};
+   const ProjChunk c = chunks[ci];
+   const int m = c.ncol, E = c.r;
+   T* part = (T*)(smem + (size_t)wave * wave_lds);   // 2 * 64 * NC
+   T* xl = part + 128 * NC;                           // 2 * m * NC
+   for (int i = lane; i < m; i += 64) {
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+         const long long o = (long long)(c0 + min(j, nc - 1)) * cs_x + c.p0 + i;
+         xl[(size_t)i * NC + j] = (T)xs[o];
+         xl[(size_t)(m + i) * NC + j] = (T)xd[o];
+      }
+   }
+   gr.sync();
+   const T* blk = VTc + c.vt_off;
+   for (int e0 = 0; e0 < E; e0 += 64) {
+      const int Ec = min(64, E - e0), G = 64 / Ec;
+      const int g = lane / Ec, e = e0 + lane - g * Ec;
+      T as[NC], ad[NC];
+#pragma unroll
+      for (int j = 0; j < NC; j++) as[j] = ad[j] = 0;
+      if (g < G) {
+         const T* colp = blk + e;
+         const long long step = (long long)G * E;
+         T s0[NC], s1[NC], d0[NC], d1[NC];
+#pragma unroll
+         for (int j = 0; j < NC; j++) s0[j] = s1[j] = d0[j] = d1[j] = 0;
+         int i = g;
+         for (; i + 7 * G < m; i += 8 * G) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = colp[(long long)i * E + u * step];
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) {
+               const T* xa = xl + (size_t)(i + u * G) * NC;
+               const T* xb = xl + (size_t)(i + (u + 1) * G) * NC;
+               const T* ya = xl + (size_t)(m + i + u * G) * NC;
+               const T* yb = xl + (size_t)(m + i + (u + 1) * G) * NC;
+#pragma unroll
+               for (int j = 0; j < NC; j++) {
+                  s0[j] += v[u] * xa[j];
+                  d0[j] += v[u] * ya[j];
+                  s1[j] += v[u + 1] * xb[j];
+                  d1[j] += v[u + 1] * yb[j];
+               }
+            }
+         }
+         if (i < m) {
+            T v[7];
+#pragma unroll
+            for (int u = 0; u < 7; u++) v[u] = i + u * G < m ? colp[(long long)i * E + u * step] : (T)0;
+#pragma unroll
+            for (int u = 0; u < 7; u++)
+               if (i + u * G < m) {
+                  const T* xa = xl + (size_t)(i + u * G) * NC;
+                  const T* ya = xl + (size_t)(m + i + u * G) * NC;
+#pragma unroll
+                  for (int j = 0; j < NC; j++) {
+                     s0[j] += v[u] * xa[j];
+                     d0[j] += v[u] * ya[j];
+                  }
+               }
+         }
+#pragma unroll
+         for (int j = 0; j < NC; j++) {
+            as[j] = s0[j] + s1[j];
+            ad[j] = d0[j] + d1[j];
+         }
+      }
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+         part[(2 * j) * 64 + lane] = as[j];
+         part[(2 * j + 1) * 64 + lane] = ad[j];
+      }
+      gr.sync();
+      if (lane < Ec) {
+         for (int j = 0; j < nc; j++) {
+            T ss = 0, sd = 0;
+            for (int gg = 0; gg < G; gg++) {
+               ss += part[(2 * j) * 64 + gg * Ec + lane];
+               sd += part[(2 * j + 1) * 64 + gg * Ec + lane];
+            }
+            T* tp = tpart + (long long)(c0 + j) * cs_tpart;
+            tp[c.out_off + e0 + lane] = ss;
+            tp[c.out_off + E + e0 + lane] = sd;
+         }
+      }
+      gr.sync();
+   }
+}
+
 template <int n, typename T, bool NT, typename G>
 __device__ void source_body(const SrcPlan& d, const T* __restrict__ tpart, const T* __restrict__ PV,
                             typename Cplx<T>::type* __restrict__ spec, unsigned char* smem, const G& gr) {
@@ -1700,11 +1811,23 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
       case 0: {
          const int wl = (int)((sizeof(T) * (size_t)(2 * a.proj_maxcols + 128) + 15) / 16 * 16);
          check_lds(4 * (size_t)wl);
-         if (a.nchunk) {
-            auto kern = a.ncol == 1 ? k_project<T, true> : k_project<T, false>;
-            hipLaunchKernelGGL(kern, dim3(col_grid((a.nchunk + 3) / 4, a.ncol)), dim3(256), 4 * (size_t)wl, s,
-                               a.chunks, a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart, wl, a.ncol, a.cs_x,
+         if (a.nchunk && a.ncol == 1) {
+            hipLaunchKernelGGL((k_project<T, true>), dim3((a.nchunk + 3) / 4), dim3(256), 4 * (size_t)wl, s,
+                               a.chunks, a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart, wl, 1, a.cs_x,
                                a.cs_tpart);
+         } else if (a.nchunk) {
+            constexpr int NC = proj_nc<T>();
+            const int wlm = (int)((sizeof(T) * (size_t)NC * (2 * a.proj_maxcols + 128) + 15) / 16 * 16);
+            check_lds(4 * (size_t)wlm);
+            static std::once_flag attr;
+            std::call_once(attr, [] {
+               FSVD_CHECK(hipFuncSetAttribute((const void*)k_project_mc<T, NC>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+            });
+            const int ngrp = (a.ncol + NC - 1) / NC;
+            hipLaunchKernelGGL((k_project_mc<T, NC>), dim3(col_grid((a.nchunk + 3) / 4, ngrp)), dim3(256),
+                               4 * (size_t)wlm, s, a.chunks, a.nchunk, (const T*)a.VTc, a.xs, a.xd, (T*)a.tpart,
+                               wlm, a.ncol, a.cs_x, a.cs_tpart);
          }
          break;
       }
