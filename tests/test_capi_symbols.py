@@ -61,3 +61,17 @@ def test_argument_errors_before_any_device_call():
         fa.Qual(v, q, c, gridpoints=2)
     with pytest.raises(fa.FFTSVDError, match="sizes"):
         fa.Qual(np.zeros((0, 9)), q, c)
+
+
+def test_batch_entry_points_reject_bad_sizes():
+    """The batched entry points check their sizes (and null handles) before
+    any device call."""
+    if not os.path.exists(fa.LIB_PATH):
+        pytest.skip("library not built")
+    import ctypes as C
+    L = fa.lib()
+    assert L.fftsvd_problems_solve(0, None, 0, None, None, 1e-6, 10, None, None) < 0
+    assert "nprob" in L.fftsvd_last_error().decode()
+    hs = (C.c_void_p * 1)(None)
+    assert L.fftsvd_problems_solve(1, hs, 0, None, None, 1e-6, 0, None, None) < 0
+    assert "maxit" in L.fftsvd_last_error().decode()
