@@ -1512,7 +1512,9 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
                                               const double* __restrict__ xd, const LeafU* __restrict__ ublk,
                                               const T* __restrict__ UT, const T* __restrict__ w,
                                               double* __restrict__ y, const int* __restrict__ bmap, int nb, int ncol,
-                                              long long cs_x, long long cs_w, long long cs_y) {
+                                              long long cs_x, long long cs_w, long long cs_y,
+                                              const int* __restrict__ sp_out, const double* __restrict__ sp_corr,
+                                              double* __restrict__ sp_y) {
    extern __shared__ __align__(16) unsigned char smem[];
    const ColBlk cb = col_block(ncol);
    if (cb.b >= nb) return;
@@ -1581,7 +1583,14 @@ __global__ void __launch_bounds__(256) k_leaf(const LeafDesc* __restrict__ leave
       }
       __syncthreads();
    }
-   for (int r = tid; r < p; r += 256) y[L.q0 + r] = (double)((rows[r] + rows[p + r]) + rows[E + r]);
+   // with sp_y: the region split fused in (k_split's y[out] = yt + corr xd)
+   for (int r = tid; r < p; r += 256) {
+      const double v = (double)((rows[r] + rows[p + r]) + rows[E + r]);
+      if (sp_y)
+         sp_y[sp_out[L.q0 + r]] = v + sp_corr[L.q0 + r] * xd[L.q0 + r];
+      else
+         y[L.q0 + r] = v;
+   }
 }
 
 // columns per block of the multi-column leaf kernel
@@ -2041,12 +2050,12 @@ void launch_apply_stage(int stage, const ApplyArgs& a, hipStream_t s) {
          if (a.nleaf && a.ncol == 1) {
             hipLaunchKernelGGL((k_leaf<T, true>), dim3(a.nmap_leaf), dim3(256), lds, s, a.leaves, a.dcol,
                                (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y, a.map_leaf,
-                               a.nmap_leaf, 1, a.cs_x, a.cs_w, a.cs_y);
+                               a.nmap_leaf, 1, a.cs_x, a.cs_w, a.cs_y, a.sp_out, a.sp_corr, a.sp_y);
          } else if (a.nleaf && sizeof(T) * (size_t)LEAF_NC * (2 * a.maxm + 256 + 3 * a.maxp) + 64 > 163840) {
             // leaves too wide for LEAF_NC columns in LDS: one block copy per column
             hipLaunchKernelGGL((k_leaf<T, false>), dim3(col_grid(a.nmap_leaf, a.ncol)), dim3(256), lds, s, a.leaves,
                                a.dcol, (const T*)a.D, a.xs, a.xd, a.ublk, (const T*)a.UT, (const T*)a.w, a.y,
-                               a.map_leaf, a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y);
+                               a.map_leaf, a.nmap_leaf, a.ncol, a.cs_x, a.cs_w, a.cs_y, nullptr, nullptr, nullptr);
          } else if (a.nleaf) {
             // LEAF_NC columns per block
             const size_t lmc = sizeof(T) * (size_t)LEAF_NC * (2 * a.maxm + 256 + 3 * a.maxp) + 64;

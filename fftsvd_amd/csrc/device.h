@@ -255,6 +255,11 @@ struct ApplyArgs {
    int ncol = 1;
    long long cs_x = 0, cs_y = 0, cs_tpart = 0, cs_spec = 0, cs_acc = 0, cs_w = 0;
    long long spec_delta = 0, acc_delta = 0;
+   // single column: the region split fused into the leaf stage (output
+   // sp_y[sp_out[k]] = y_k + sp_corr[k] xd[k] instead of y)
+   const int* sp_out = nullptr;
+   const double* sp_corr = nullptr;
+   double* sp_y = nullptr;
 };
 
 // One column of a lockstep GMRES step (device copy, one per active

@@ -1581,6 +1581,24 @@ void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const dou
       apply_part_impl<double>(op, part, xs, xd, y, s);
 }
 
+template <typename T>
+static void leaf_split_impl(TreeOp& op, const double* xs, const double* xd, const int* out_idx, const double* corr,
+                            double* yout, hipStream_t s) {
+   ApplyArgs a = make_args<T>(op, xs, xd, nullptr);
+   a.sp_out = out_idx;
+   a.sp_corr = corr;
+   a.sp_y = yout;
+   launch_apply_stage<T>(4, a, s);
+}
+
+void tree_apply_leaf_split(TreeOp& op, int prec, const double* xs, const double* xd, const int* out_idx,
+                           const double* corr, double* yout, hipStream_t s) {
+   if (prec == PREC_F32)
+      leaf_split_impl<float>(op, xs, xd, out_idx, corr, yout, s);
+   else
+      leaf_split_impl<double>(op, xs, xd, out_idx, corr, yout, s);
+}
+
 // Column copies of the work vectors.  The spectra buffers start zeroed (the
 // bins between a spectrum's pad and its padded stride stay zero), as the
 // single-column ones.

@@ -240,6 +240,11 @@ void tree_apply_part(TreeOp& op, int prec, int part, const double* xs, const dou
 // the columns' copies of a block read the operator together (one XCD's L2).
 void tree_apply_part_batch(TreeOp& op, int prec, int part, int ncol, const double* xs, const double* xd,
                            long long ldx, double* y, long long ldy, hipStream_t s);
+// The leaf stage (part 0) with the region split fused in: instead of y,
+// writes yout[out_idx[k]] = y_k + corr[k] xd[k] (k_split's result, bit for
+// bit).
+void tree_apply_leaf_split(TreeOp& op, int prec, const double* xs, const double* xd, const int* out_idx,
+                           const double* corr, double* yout, hipStream_t s);
 // Sizes the operator's column work vectors for ncol columns (before a
 // batched apply is enqueued on several streams).
 void tree_batch_reserve(TreeOp& op, int ncol, hipStream_t s);

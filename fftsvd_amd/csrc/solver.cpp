@@ -665,8 +665,9 @@ static void matvec_impl(Problem& pr, int prec, const XT* x, double* y) {
    std::vector<std::vector<int>> lane_regions(pr.naux + 1);
    for (size_t r = 0; r < pr.regions.size(); r++)
       if (!pr.regions[r].panel_ids.empty()) lane_regions[pr.regions[r].lane].push_back((int)r);
-   constexpr int NSTEP = 6;
-   static const int parts[4] = {1, 2, 3, 0};
+   // steps: join, projection+source, translation, target, leaf with the
+   // split fused in
+   constexpr int NSTEP = 5;
    size_t maxsteps = 0;
    for (auto& lr : lane_regions) maxsteps = std::max(maxsteps, lr.size() * NSTEP);
    for (size_t k = 0; k < maxsteps; k++) {
@@ -679,10 +680,10 @@ static void matvec_impl(Problem& pr, int prec, const XT* x, double* y) {
          if (step == 0) {
             if (prec == PREC_F32 && !op.have32) throw std::runtime_error("operator has no FP32 copy");
             launch_join(n, R.phi_idx.p, R.dphi_idx.p, R.s_phi.p, R.s_dphi.p, x, R.xs.p, R.xd.p, rs);
-         } else if (step < 5) {
-            tree_apply_part(op, prec, parts[step - 1], R.xs.p, R.xd.p, R.yt.p, rs);
+         } else if (step < 4) {
+            tree_apply_part(op, prec, step, R.xs.p, R.xd.p, R.yt.p, rs);
          } else {
-            launch_split(n, R.out_idx.p, R.corr.p, R.yt.p, R.xd.p, y, rs);
+            tree_apply_leaf_split(op, prec, R.xs.p, R.xd.p, R.out_idx.p, R.corr.p, y, rs);
          }
       }
    }
