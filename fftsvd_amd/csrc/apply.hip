@@ -1318,32 +1318,38 @@ __device__ void target_body(const TgtPlan& d, const T* __restrict__ UTI,
    grid_backward<n, T>(S, C1, Cc, h, cs, sn, gr);
    gr.sync();
    const T* uti = UTI + d.uti_off;
-   // w = UTI h: 16 lanes per row, nt/16 rows per pass, two rows per lane in
-   // flight
+   // w = UTI h: 16 lanes per row, nt/16 rows per pass, four rows per lane in
+   // flight (each row summed over q in order, then a fixed butterfly)
    const int kr = tid >> 4, ql = tid & 15, RP = nt >> 4;
-   for (int k = kr; k < d.r; k += 2 * RP) {
-      const T* row0 = uti + (long long)k * GP3;
-      const bool two = k + RP < d.r;
-      const T* row1 = two ? row0 + (long long)RP * GP3 : row0;   // a valid row either way
-      T a = 0, b = 0;
+   for (int k = kr; k < d.r; k += 4 * RP) {
+      const T* row[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         ok[u] = k + u * RP < d.r;
+         row[u] = uti + (long long)(ok[u] ? k + u * RP : k) * GP3;   // a valid row either way
+      }
+      T a[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int q0 = 0; q0 < GP3; q0 += 16) {
          // clamped column and selects: no branch around the loads
          const int q = q0 + ql < GP3 ? q0 + ql : 0;
          const bool okq = q0 + ql < GP3;
-         const T x0 = row0[q], x1 = row1[q], hq = h[q];
-         a += okq ? x0 * hq : (T)0;
-         b += okq && two ? x1 * hq : (T)0;
+         T x[4];
+#pragma unroll
+         for (int u = 0; u < 4; u++) x[u] = row[u][q];
+         const T hq = h[q];
+#pragma unroll
+         for (int u = 0; u < 4; u++) a[u] += okq && ok[u] ? x[u] * hq : (T)0;
       }
 #pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-         a += __shfl_xor(a, o, 64);
-         b += __shfl_xor(b, o, 64);
-      }
-      if (ql == 0) {
-         w[d.w_off + k] = a;
-         if (two) w[d.w_off + k + RP] = b;
-      }
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+         for (int o = 8; o > 0; o >>= 1) a[u] += __shfl_xor(a[u], o, 64);
+      if (ql == 0)
+#pragma unroll
+         for (int u = 0; u < 4; u++)
+            if (ok[u]) w[d.w_off + k + u * RP] = a[u];
    }
 }
 
