@@ -13,7 +13,10 @@ throughput workload).  The oracle's FP64 energy on that mesh is a
 committed fixture (tests/golden/make_globule100k.py: five minutes of single-core
 CPU); the mesh checksum in the fixture guards that the mesher produced the
 same panels.  Gates: FP64 GPU energy within 1e-8 of the oracle's, GMRES
-iterations within one, mixed within 1e-6 of FP64.
+iterations within one, mixed within 1e-6 of FP64; on the same operator, 8
+of the batched workload's seeded charge vectors solved in lockstep to 1e-8:
+every column's mixed energy within 1e-6 of FP64, and a column equal to its
+single solve bit for bit.
 """
 import hashlib
 import json
@@ -79,4 +82,18 @@ def test_globule_100k_against_oracle_fixture():
     assert abs(E64 - ref["energy_fp64"]) <= 1e-8 * abs(ref["energy_fp64"])
     assert abs(E32 - E64) <= 1e-6 * abs(E64)
     assert info["final_residual"] < 1e-6
+    # the batched charge-vector workload on this mesh (8 of its 64 seeded
+    # columns, GMRES to 1e-8): every column's mixed energy within 1e-6 of
+    # FP64, and the lockstep solve equal to the single solve bit for bit
+    Q = np.column_stack([np.random.default_rng(ref["seed"] + 1 + c).uniform(-0.8, 0.8, len(q)) for c in range(8)])
+    R = g.rhs_batch(Q)
+    X64, _, _ = g.solve_batch(R, 1e-8, precision=fa.F64)
+    X32, its, _ = g.solve_batch(R, 1e-8, precision=fa.MIXED)
+    Eb64 = 0.5 * np.sum(Q * g.potentials_batch(X64), axis=0)
+    Eb32 = 0.5 * np.sum(Q * g.potentials_batch(X32), axis=0)
+    print("batched columns: mixed vs fp64 max rel %.2e" % np.max(np.abs(Eb32 - Eb64) / np.abs(Eb64)), flush=True)
+    assert np.all(np.abs(Eb32 - Eb64) <= 1e-6 * np.abs(Eb64))
+    x5, it5, _, _ = g.solve(R[:, 5], 1e-8, precision=fa.MIXED)
+    np.testing.assert_array_equal(X32[:, 5], x5)
+    assert its[5] == it5
     g.close()
