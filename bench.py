@@ -405,11 +405,20 @@ def main():
                              "algorithmic_bytes_per_launch": roofline["bytes_per_matvec"] / st["trees"]})
     # the translation is bound by LDS, not HBM: every complex multiply-add
     # reads one T bin and one source bin (2 x 8 bytes) from LDS
+    # reads one T bin and one source bin (2 x 8 bytes) from LDS.  The peak is
+    # the nominal 128 B/clk/CU at 2.4 GHz; the measured LDS read rate of this
+    # GPU with the kernel's 8-byte reads (fftsvd_lds_bandwidth) is beside it
     lds_bytes = 16.0 * st["translate_macs"]
     lds_rate = lds_bytes / (parts32["translate"] * 1e-3) / 1e9
+    lds_meas8 = max(fa.lds_bandwidth(8, 5) for _ in range(2))
+    lds_meas16 = max(fa.lds_bandwidth(16, 5) for _ in range(2))
     roofline_lds = {"kernel": STAGE_KERNELS["translate"], "bound": "lds", "achieved": lds_rate,
                     "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds_rate / LDS_PEAK_GBS,
+                    "measured_peak_8B_reads": lds_meas8, "frac_of_measured": lds_rate / lds_meas8,
+                    "measured_peak_16B_reads": lds_meas16,
                     "lds_bytes_per_matvec": lds_bytes, "complex_macs_per_matvec": st["translate_macs"]}
+    print("# LDS read rate measured: %.0f GB/s with 8-byte reads, %.0f with 16-byte reads; translation at %.0f "
+          "(%.2f of the measured 8-byte rate)" % (lds_meas8, lds_meas16, lds_rate, lds_rate / lds_meas8), flush=True)
 
     # measured copy bandwidth of this GPU (1 GiB copied 20 times)
     copy_gbs = fa.copy_bandwidth(1 << 30, 20)

@@ -43,6 +43,7 @@ _SIGS = {
     "fftsvd_problem_potentials_batch": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_transpose": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "fftsvd_problem_matvec_batch": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
+    "fftsvd_lds_bandwidth": (C.c_int, [C.c_int, C.c_int, _dp]),
     "fftsvd_problems_solve": (C.c_int, [C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.c_double,
                                         C.c_int, C.POINTER(C.c_int), _dp]),
     "fftsvd_tree_cube_points": (C.c_int, [_vp, C.c_int, C.c_int, _ip]),
@@ -121,6 +122,14 @@ def copy_bandwidth(nbytes=1 << 30, nrep=20):
     """Device-to-device copy rate in GB/s (bytes read + written per second)."""
     g = C.c_double()
     _check(lib().fftsvd_copy_bandwidth(int(nbytes), int(nrep), C.byref(g)))
+    return g.value
+
+
+def lds_bandwidth(width=8, nrep=5):
+    """LDS read rate of the device in GB/s with width-byte reads
+    (fftsvd_lds_bandwidth)."""
+    g = C.c_double()
+    _check(lib().fftsvd_lds_bandwidth(int(width), int(nrep), C.byref(g)))
     return g.value
 
 

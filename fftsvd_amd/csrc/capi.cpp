@@ -48,6 +48,7 @@ static int g_precision = 0;
 
 namespace fsvd {
 void launch_copy(long long n4, const void* a, void* b, hipStream_t s);
+void launch_lds_probe(int width, int nblk, int niter, float* out, hipStream_t s);
 // the message fftsvd_last_error() returns, for entry points in other files
 void set_last_error(const char* msg) { g_err = msg; }
 }
@@ -110,6 +111,37 @@ int fftsvd_copy_bandwidth(long long bytes, int nrep, double* gbs) {
    (void)hipStreamDestroy(s);
    // bytes read + bytes written per copy
    if (gbs) *gbs = 2.0 * 32.0 * (double)n4 * nrep / (ms * 1e-3) / 1e9;
+   API_CATCH
+}
+
+int fftsvd_lds_bandwidth(int width, int nrep, double* gbs) {
+   API_TRY
+   if (nrep < 1 || (width != 8 && width != 16)) throw std::runtime_error("LDS probe: width 8 or 16, nrep >= 1");
+   int dev = 0, ncu = 0;
+   FSVD_CHECK(hipGetDevice(&dev));
+   FSVD_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+   // eight 256-thread blocks per CU (32 KiB of LDS each: five resident per
+   // CU at a time), enough rounds for about a millisecond
+   const int nblk = 8 * std::max(ncu, 1), niter = 2048;
+   DBuf<float> out;
+   out.alloc(nblk);
+   hipStream_t s;
+   FSVD_CHECK(hipStreamCreate(&s));
+   launch_lds_probe(width, nblk, niter, out.p, s);
+   hipEvent_t e0, e1;
+   FSVD_CHECK(hipEventCreate(&e0));
+   FSVD_CHECK(hipEventCreate(&e1));
+   FSVD_CHECK(hipEventRecord(e0, s));
+   for (int r = 0; r < nrep; r++) launch_lds_probe(width, nblk, niter, out.p, s);
+   FSVD_CHECK(hipEventRecord(e1, s));
+   FSVD_CHECK(hipEventSynchronize(e1));
+   float ms = 0;
+   FSVD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   FSVD_CHECK(hipStreamSynchronize(s));
+   (void)hipStreamDestroy(s);
+   if (gbs) *gbs = (double)nblk * 256.0 * 8.0 * niter * width * nrep / (ms * 1e-3) / 1e9;
    API_CATCH
 }
 

@@ -3,6 +3,8 @@
 // FP64.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.h"
 #include "integrals.h"
 
@@ -680,6 +682,44 @@ __global__ void k_integrate_all(int kernel, double kappa, const PanelGeom* __res
    dlp[e] = d;
 }
 
+// LDS read probe: every lane reads W bytes (8 or 16) per load from a 32 KiB
+// LDS array, the 64 lanes of a wave one contiguous 64 W bytes (no bank
+// conflicts), eight loads in flight, niter rounds; integer three-input adds
+// (two or one vector ALU operations per load) keep the ALUs ahead of the
+// LDS; the sum is written only if it takes an impossible value, so the
+// loads stay.  Bytes read = blocks * 256 * 8 * niter * W.
+template <int W>
+__global__ void __launch_bounds__(256) k_lds_probe(int niter, float* out) {
+   using V = typename std::conditional<W == 16, uint4, uint2>::type;
+   constexpr int NV = 32768 / W;
+   __shared__ V buf[NV];
+   for (int i = threadIdx.x; i < NV; i += 256) {
+      V v;
+      v.x = (unsigned)i;
+      v.y = 1u;
+      if constexpr (W == 16) {
+         v.z = 2u;
+         v.w = 3u;
+      }
+      buf[i] = v;
+   }
+   __syncthreads();
+   unsigned a0 = 0u, a1 = 0u;
+   int idx = threadIdx.x;
+   for (int k = 0; k < niter; k++) {
+      V v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = buf[(idx + 256 * u) & (NV - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         a0 = a0 + v[u].x + v[u].y;
+         if constexpr (W == 16) a1 = a1 + v[u].z + v[u].w;
+      }
+      idx = (idx + 64) & (NV - 1);
+   }
+   if (a0 + a1 == 1u) out[blockIdx.x] = (float)a0;
+}
+
 // Plain copy (bandwidth probe and counter calibration).
 __global__ void k_copy(long long n4, const double4* __restrict__ a, double4* __restrict__ b) {
    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
@@ -712,6 +752,12 @@ void launch_precond(int nb, const int* ia, const int* ib, const double* M, const
 }
 void launch_dot_partial(long long n, int np, const double* a, const double* b, double* partial, hipStream_t s) {
    hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(256), 0, s, n, a, b, partial);
+}
+void launch_lds_probe(int width, int nblk, int niter, float* out, hipStream_t s) {
+   if (width == 16)
+      hipLaunchKernelGGL(k_lds_probe<16>, dim3(nblk), dim3(256), 0, s, niter, out);
+   else
+      hipLaunchKernelGGL(k_lds_probe<8>, dim3(nblk), dim3(256), 0, s, niter, out);
 }
 void launch_copy(long long n4, const void* a, void* b, hipStream_t s) {
    if (n4 > 0)

@@ -26,6 +26,12 @@ int fftsvd_device_info(char* name, int len, int* num_cu);
  * stream; gbs = (read + written bytes) / time.  A bandwidth probe and the
  * reference kernel for calibrating memory counters. */
 int fftsvd_copy_bandwidth(long long bytes, int nrep, double* gbs);
+/* LDS read rate of the whole device in GB/s: every lane of eight
+ * 256-thread blocks per CU reads width-byte words (8 or 16; a wave one
+ * contiguous 64 * width bytes, no bank conflicts) from a 32 KiB LDS array,
+ * repeated nrep times.  The measured ceiling for the LDS-bound
+ * translation, whose lanes read 8 bytes at a time. */
+int fftsvd_lds_bandwidth(int width, int nrep, double* gbs);
 
 /* One dielectric surface (panels as v0,v1,v2 triples, the vertex order the
  * reference builds from a flipped MSMS face), all charges inside; salt if
