@@ -5,15 +5,39 @@
 // flat array (parents first) and panels/points renumbered into "tree order"
 // so that every cube owns contiguous index ranges.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 #include "internal.h"
 
 namespace fsvd {
 
 namespace {
+
+// fn(a, b) over [0, n) in contiguous chunks on up to 16 host threads (one
+// below 16384 items); each item is handled by exactly one call, so results
+// do not depend on the split.
+template <typename F>
+void tree_parallel_for(long long n, F&& fn) {
+   const long long grain = 16384;
+   int nth = (int)std::min<long long>({16LL, (long long)std::max(1u, std::thread::hardware_concurrency()),
+                                        (n + grain - 1) / grain});
+   if (nth <= 1) {
+      if (n > 0) fn(0LL, n);
+      return;
+   }
+   std::vector<std::thread> th;
+   const long long chunk = (n + nth - 1) / nth;
+   for (int t = 1; t < nth; t++) {
+      const long long a = t * chunk, b = std::min(n, a + chunk);
+      if (a < b) th.emplace_back([&fn, a, b] { fn(a, b); });
+   }
+   fn(0LL, std::min(n, chunk));
+   for (auto& x : th) x.join();
+}
 
 struct Builder {
    Tree& t;
@@ -95,8 +119,17 @@ void make_lists(Tree& t) {
       for (int d = 0; d < 3; d++) idx[3 * (size_t)i + d] = t.cubes[i].idx[d];
       for (int k = 0; k < 8; k++) child[8 * (size_t)i + k] = t.cubes[i].child[k];
    }
-   int near[8 * 27], far[8 * 27];
-   for (int id = 0; id < nc; id++) {
+   // a cube's lists read its parent's local list: level by level, the
+   // cubes of one level in parallel
+   int maxlev = 0;
+   for (const Cube& c : t.cubes) maxlev = std::max(maxlev, c.level);
+   std::vector<std::vector<int>> bylev(maxlev + 1);
+   for (int id = 0; id < nc; id++) bylev[t.cubes[id].level].push_back(id);
+   for (const std::vector<int>& ids : bylev)
+      tree_parallel_for((long long)ids.size(), [&](long long a, long long b) {
+         int near[8 * 27], far[8 * 27];
+         for (long long r = a; r < b; r++) {
+      const int id = ids[r];
       Cube& c = t.cubes[id];
       c.loc.clear();
       c.itr.clear();
@@ -122,7 +155,8 @@ void make_lists(Tree& t) {
       }
       c.loc.assign(near, near + nn);
       c.itr.assign(far, far + nf);
-   }
+         }
+      });
 }
 
 // Level-synchronous construction of the same tree on flat arrays.  Every
@@ -189,27 +223,32 @@ class LevelTree {
    // one level deeper for every entity, regroup, list the cells
    bool deepen() {
       const int ne = npan_ + npt_;
-      for (int e = 0; e < ne; e++) {
-         int k = 0;
-         for (int d = 0; d < 3; d++) {
-            double& lo = lo_[3 * (size_t)e + d];
-            double& hi = hi_[3 * (size_t)e + d];
-            const double s = lo + hi;
-            const double ctr = s * 0.5;
-            const double g = x_[3 * (size_t)e + d];
-            if (g >= ctr) {
-               const double diff = ctr - lo;
-               lo += diff;
-               hi = ctr + diff;
-               k |= 4 >> d;
-            } else if (g < ctr) {
-               hi = ctr;
-            } else {
-               return false;
+      std::atomic<bool> ok{true};
+      tree_parallel_for(ne, [&](long long a, long long b) {
+         for (long long e = a; e < b; e++) {
+            int k = 0;
+            for (int d = 0; d < 3; d++) {
+               double& lo = lo_[3 * (size_t)e + d];
+               double& hi = hi_[3 * (size_t)e + d];
+               const double s = lo + hi;
+               const double ctr = s * 0.5;
+               const double g = x_[3 * (size_t)e + d];
+               if (g >= ctr) {
+                  const double diff = ctr - lo;
+                  lo += diff;
+                  hi = ctr + diff;
+                  k |= 4 >> d;
+               } else if (g < ctr) {
+                  hi = ctr;
+               } else {
+                  ok = false;
+                  return;
+               }
             }
+            code_[e] = code_[e] << 3 | (uint64_t)k;
          }
-         code_[e] = code_[e] << 3 | (uint64_t)k;
-      }
+      });
+      if (!ok) return false;
       const std::vector<int>& prev = ord_.back();
       std::vector<int> next(ne);
       // panels then points, each regrouped inside the previous cells
