@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "device.h"
+#include "geometry.h"
 #include "integrals.h"
 #include "op.h"
 
@@ -549,5 +550,32 @@ __global__ void k_batched_copy(const CopyDesc* __restrict__ d) {
 }
 void launch_batched_copy(int n, const void* descs, hipStream_t s) {
    if (n > 0) hipLaunchKernelGGL(k_batched_copy, dim3(n), dim3(256), 0, s, (const CopyDesc*)descs);
+}
+}  // namespace fsvd
+
+namespace fsvd {
+// Panel geometry records on the device (FlatPanel_allocate +
+// FlatPanel_moments, FlatPanel.c:5-179,210-298) with the host's arithmetic,
+// so every field equals the host record's bit for bit: record k is that of
+// panel ids[k] (panel k without ids) of the 9-double vertex records verts9.
+__global__ void __launch_bounds__(64) k_panel_records(const double* __restrict__ verts9, const int* __restrict__ ids,
+                                                      int n, PanelGeom* __restrict__ out) {
+   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k >= n) return;
+   const double* v = verts9 + 9 * (long long)(ids ? ids[k] : k);
+   double v0[3], v1[3], v2[3];
+   for (int d = 0; d < 3; d++) {
+      v0[d] = v[d];
+      v1[d] = v[3 + d];
+      v2[d] = v[6 + d];
+   }
+   PanelGeom g;
+   geom::panel_geometry_hd(v0, v1, v2, &g);
+   const double* src = reinterpret_cast<const double*>(&g);
+   double* dst = reinterpret_cast<double*>(out + k);
+   for (int q = 0; q < (int)(sizeof(PanelGeom) / sizeof(double)); q++) dst[q] = src[q];
+}
+void launch_panel_records(const double* verts9, const int* ids, int n, PanelGeom* out, hipStream_t s) {
+   if (n > 0) hipLaunchKernelGGL(k_panel_records, dim3((n + 63) / 64), dim3(64), 0, s, verts9, ids, n, out);
 }
 }  // namespace fsvd

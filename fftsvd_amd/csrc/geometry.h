@@ -98,15 +98,20 @@ __host__ __device__ inline void moments(PanelGeom* p) {
          p->mom[k] = I[M][N] * CS[k];
       }
 }
+// The panel centroid (the collocation point), exactly as the record holds it.
+__host__ __device__ inline void panel_centroid_hd(const double* v0, const double* v1, const double* v2, double* c) {
+#pragma clang fp contract(off)
+   for (int d = 0; d < 3; d++) {
+      double s = v0[d] + v1[d];
+      s = s + v2[d];
+      c[d] = s * (1.0 / 3.0);
+   }
+}
 __host__ __device__ inline void panel_geometry_hd(const double* v0, const double* v1, const double* v2, PanelGeom* p) {
 #pragma clang fp contract(off)
    double* raw = reinterpret_cast<double*>(p);
    for (int i = 0; i < (int)(sizeof(PanelGeom) / sizeof(double)); i++) raw[i] = 0.0;
-   for (int d = 0; d < 3; d++) {
-      double s = v0[d] + v1[d];
-      s = s + v2[d];
-      p->c[d] = s * (1.0 / 3.0);
-   }
+   panel_centroid_hd(v0, v1, v2, p->c);
    p->el[0] = dist3(v1, v0);
    p->el[1] = dist3(v2, v1);
    p->el[2] = dist3(v0, v2);

@@ -87,8 +87,9 @@ struct Tree {
 };
 
 // Builds the octree exactly as the reference decides it (cubic root box,
-// centroid split, depth by the 90%-of-leaves rule) and the lists.
-void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, int npt, int maxpanels,
+// centroid split, depth by the 90%-of-leaves rule) and the lists, from the
+// panel centroids (3 per panel) and the points.
+void build_tree(Tree& t, const double* pan_cent, int npan, const double* pts, int npt, int maxpanels,
                 int gridpoints);
 
 // Device-resident compressed operator of one tree in one storage precision.

@@ -215,9 +215,11 @@ static void build_translate_f64(ApplyPlan& P, const std::vector<TrLevel>& lvls, 
    std::vector<int4> td;
    for (const TrLevel& L : lvls) {
       const int first = (int)tiles.size();
+      // a tile's local index of each of its sources, by the source's slot
+      std::vector<int> loc(std::max(L.nsrc, 1), -1);
+      auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
       make_tiles(L, tgt, isrc, TR_TILE, TR_SRC_BUDGET, TR_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
-         std::unordered_map<int, int> loc;
-         for (size_t i = 0; i < so.size(); i++) loc[so[i]] = (int)i;
+         for (size_t i = 0; i < so.size(); i++) loc[slot_of(so[i])] = (int)i;
          TrTile t;
          t.nt = k1 - k0;
          t.tgt0 = (int)gtgt.size();
@@ -237,7 +239,7 @@ static void build_translate_f64(ApplyPlan& P, const std::vector<TrLevel>& lvls, 
             g.nint = tp.nint;
             for (int e = 0; e < tp.nint; e++)
                gint.push_back((int)((unsigned)(itoff[tp.int_off + e] * TR_ROW) |
-                                    ((unsigned)(loc.at(isrc[tp.int_off + e]) * TR_ROW) << 16)));
+                                    ((unsigned)(loc[slot_of(isrc[tp.int_off + e])] * TR_ROW) << 16)));
             gtgt.push_back(g);
          }
          tiles.push_back(t);
@@ -825,8 +827,11 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
    // translation plans over the targets of each level (tree order, so
    // consecutive targets are spatially compact): FP64 tiles for k_translate,
    // FP32 tiles and persistent-block task ranges for k_translate_pipe
+   // The FP64 plan is worked out on a host thread of its own, beside the
+   // rest of the plan (it reads tl, tgt, isrc and itoff, which stay
+   // unchanged, and writes only its own fields of P); joined before return.
+   std::vector<TrLevel> tl;
    {
-      std::vector<TrLevel> tl;
       size_t tbase = 0;
       long long tb = 0, sbase = 0;
       for (size_t li = 0; li < L; li++) {
@@ -836,15 +841,30 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
          tb += 2LL * 343 * tr_pst(lv.pad);
          sbase += (long long)lv.nsrc * tr_pst(lv.pad);
       }
-      build_translate_f64(P, tl, tgt, isrc, itoff, s);
       if (want32) {
          P.tr_lv = tl;
          P.tr_tgt = tgt;
          P.tr_isrc = isrc;
          P.tr_itoff = itoff;
       }
-      timing_mark("plan: translation", tm);
    }
+   int dev = 0;
+   FSVD_CHECK(hipGetDevice(&dev));
+   std::exception_ptr f64_err;
+   struct Joiner {
+      std::thread t;
+      ~Joiner() {
+         if (t.joinable()) t.join();
+      }
+   } f64_plan;
+   f64_plan.t = std::thread([&] {
+      try {
+         FSVD_CHECK(hipSetDevice(dev));
+         build_translate_f64(P, tl, tgt, isrc, itoff, s);
+      } catch (...) {
+         f64_err = std::current_exception();
+      }
+   });
    {
       std::vector<double> wl(op.h_leaves.size());
       for (size_t k = 0; k < wl.size(); k++) {
@@ -890,6 +910,9 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       op.stats.pipe = pp;
       timing_mark("plan: pipelined translation", tm);
    }
+   f64_plan.t.join();
+   if (f64_err) std::rethrow_exception(f64_err);
+   timing_mark("plan: FP64 translation joined", tm);
 }
 
 // Equivalent densities of one level (Cube_fill_P_I + Cube_fill_PV_UTI,
@@ -1045,7 +1068,7 @@ static void level_pv_uti(const Tree& t, const std::vector<int>& cl, const FarLev
    run_gemms(g2, s);
 }
 
-std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
+std::unique_ptr<TreeOp> build_tree_op(const double* pan_cent, int npan, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s,
                                       const Tree* pre, int layers) {
@@ -1059,12 +1082,11 @@ std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, cons
    t.kernel = kernel;
    t.kappa = kappa;
    t.svdeps = svdeps;
-   const int npan = (int)panels.size();
    double tp = now_ms();
    if (pre) {
       t = *pre;
    } else {
-      build_tree(t, panels.data(), npan, pts, npt, maxpanels, gridpoints);
+      build_tree(t, pan_cent, npan, pts, npt, maxpanels, gridpoints);
    }
    t.kernel = kernel;
    t.kappa = kappa;

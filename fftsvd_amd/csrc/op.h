@@ -223,7 +223,9 @@ inline void launch_pipe(const PipeDev& d, hipStream_t s) {
 // depends on neither the kernel nor kappa) and is copied instead of rebuilt.
 // `layers` = 2 samples the bases from both layers (SINGLE_AND_DOUBLE_LAYER_INT),
 // 1 from the double layer only (DOUBLE_LAYER_INT, Cube.c:1171-1412).
-std::unique_ptr<TreeOp> build_tree_op(const std::vector<PanelGeom>& panels, const PanelGeom* d_panels,
+// pan_cent: the npan panel centroids (3 each, host), d_panels their device
+// geometry records, in the same order.
+std::unique_ptr<TreeOp> build_tree_op(const double* pan_cent, int npan, const PanelGeom* d_panels,
                                       const double* pts, int npt, int kernel, double kappa, int maxpanels,
                                       int gridpoints, double svdeps, bool want32, hipStream_t s,
                                       const Tree* pre = nullptr, int layers = 2);
@@ -286,6 +288,9 @@ void launch_nearT(int nleaf, const LeafDesc* leaves, const int* dcol, const T* D
 void launch_zero(double* a, long long n, hipStream_t s);
 void launch_integral_blocks(int nblk, const IntBlock* blocks, const double* bpts, const int* bpan,
                             const PanelGeom* panels, int kernel, double kappa, double* out, hipStream_t s);
+// Device panel geometry records: out[k] = record of panel ids[k] (panel k if
+// ids is null) of the 9-double vertex records verts9, equal to the host's.
+void launch_panel_records(const double* verts9, const int* ids, int n, PanelGeom* out, hipStream_t s);
 void launch_green_blocks(int nblk, const GreenBlock* blocks, const double* dest, const double* src, int kernel,
                          double kappa, double* out, hipStream_t s);
 void launch_gemm_batched(int nbatch, int maxtiles, const GemmDesc* descs, hipStream_t s);

@@ -2,6 +2,8 @@
 // potentials on the GPU.
 #include "solver.h"
 
+#include "geometry.h"
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -156,18 +158,24 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    pr.qpos = spec.qpos;
    pr.q = spec.q;
    const bool stern = !spec.salt_verts.empty();
-   // global panel order: [salt surfaces,] dielectrics, dielectric cavities[, salt cavities]
+   // global panel order: [salt surfaces,] dielectrics, dielectric cavities[,
+   // salt cavities].  The host keeps the vertex records and the centroids
+   // (the octree's input); the geometry records are built on the device.
+   std::vector<double> verts;
+   int npan_all = 0;
    auto add_surfaces = [&](const std::vector<std::vector<double>>& surf) {
-      std::vector<int> start(surf.size() + 1, (int)pr.panels.size());
+      std::vector<int> start(surf.size() + 1, npan_all);
       for (size_t i = 0; i < surf.size(); i++) {
          const int n = (int)(surf[i].size() / 9);
          start[i + 1] = start[i] + n;
-         const size_t base = pr.panels.size();
-         pr.panels.resize(base + n);
+         const size_t base = (size_t)npan_all;
+         verts.insert(verts.end(), surf[i].begin(), surf[i].begin() + 9 * (size_t)n);
+         pr.cent.resize(3 * (base + n));
          host_parallel_for(n, [&](long long k) {
             const double* v = &surf[i][9 * (size_t)k];
-            panel_geometry(v, v + 3, v + 6, &pr.panels[base + k]);
+            geom::panel_centroid_hd(v, v + 3, v + 6, &pr.cent[3 * (base + k)]);
          });
+         npan_all += n;
       }
       return start;
    };
@@ -175,8 +183,13 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
    const std::vector<int> dstart = add_surfaces(spec.dielectric_verts);
    const std::vector<int> cstart = add_surfaces(spec.cavity_verts);
    const std::vector<int> scstart = add_surfaces(spec.salt_cavity_verts);
-   pr.ntotal = (int)pr.panels.size();
+   pr.ntotal = npan_all;
    pr.nvec = 2LL * pr.ntotal;
+   DBuf<double> d_verts;
+   d_verts.upload(verts, s);
+   pr.d_allpanels.alloc(std::max(pr.ntotal, 1));
+   launch_panel_records(d_verts.p, nullptr, pr.ntotal, pr.d_allpanels.p, s);
+   FSVD_CHECK(hipGetLastError());
    setup_event("panel geometry");
    const int nsalt = (int)spec.salt_verts.size(), nd = (int)spec.dielectric_verts.size();
    const int nc = (int)spec.cavity_verts.size(), nsc = (int)spec.salt_cavity_verts.size();
@@ -275,16 +288,12 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
          if (R.shares_tree_with < 0) todo.push_back((int)r);
       }
       const size_t nt = todo.size();
-      // operators over the same panel set share its host and device
-      // geometry and its octree (tree_src: the first such operator)
-      std::vector<std::vector<PanelGeom>> tp(nt);
+      // operators over the same panel set share its device geometry and its
+      // octree (tree_src: the first such operator)
       std::vector<std::vector<double>> cent(nt);
       std::vector<int> tree_src(nt, -1);
       std::vector<Tree> trees(nt);
-      std::vector<std::thread> uploads;
-      std::vector<std::exception_ptr> up_err(nt);
-      int dev = 0;
-      FSVD_CHECK(hipGetDevice(&dev));
+      std::vector<DBuf<int>> ids(nt);
       for (size_t k = 0; k < nt; k++) {
          Region& R = pr.regions[todo[k]];
          for (size_t o = 0; o < k; o++)
@@ -294,32 +303,19 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
             }
          if (tree_src[k] >= 0) continue;
          const size_t n = R.panel_ids.size();
-         tp[k].resize(n);
+         // the records in the region's panel order, built on the device while
+         // the host builds the octree
+         ids[k].upload(R.panel_ids, s);
+         R.d_panels.alloc(std::max<size_t>(n, 1));
+         launch_panel_records(d_verts.p, ids[k].p, (int)n, R.d_panels.p, s);
+         FSVD_CHECK(hipGetLastError());
          cent[k].resize(3 * n);
          host_parallel_for((long long)n, [&](long long i) {
-            tp[k][i] = pr.panels[R.panel_ids[i]];
-            for (int d = 0; d < 3; d++) cent[k][3 * i + d] = tp[k][i].c[d];
+            for (int d = 0; d < 3; d++) cent[k][3 * i + d] = pr.cent[3 * (size_t)R.panel_ids[i] + d];
          });
-         // the pageable copy blocks its thread: overlap it with the host tree
-         uploads.emplace_back([&, k] {
-            try {
-               FSVD_CHECK(hipSetDevice(dev));
-               pr.regions[todo[k]].d_panels.upload(tp[k], s);
-            } catch (...) {
-               up_err[k] = std::current_exception();
-            }
-         });
-         try {
-            build_tree(trees[k], tp[k].data(), (int)n, cent[k].data(), (int)n, spec.maxpanels, spec.gridpoints);
-         } catch (...) {
-            for (auto& t : uploads) t.join();
-            throw;
-         }
+         build_tree(trees[k], cent[k].data(), (int)n, cent[k].data(), (int)n, spec.maxpanels, spec.gridpoints);
          setup_event("panels and tree");
       }
-      for (auto& t : uploads) t.join();
-      for (auto& e : up_err)
-         if (e) std::rethrow_exception(e);
       FSVD_CHECK(hipStreamSynchronize(s));
       setup_event("trees ready");
       std::vector<hipStream_t> streams(nt);
@@ -332,7 +328,8 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
                Region& R = pr.regions[todo[k]];
                const size_t src = tree_src[k] < 0 ? k : (size_t)tree_src[k];
                const Tree& pre = trees[src];
-               R.op = build_tree_op(tp[src], pr.regions[todo[src]].d_panels.p, cent[src].data(), (int)tp[src].size(),
+               const int n = (int)(cent[src].size() / 3);
+               R.op = build_tree_op(cent[src].data(), n, pr.regions[todo[src]].d_panels.p, cent[src].data(), n,
                                     R.kernel, R.kappa, spec.maxpanels, spec.gridpoints, spec.svdeps, spec.want32,
                                     streams[k], &pre);
                FSVD_CHECK(hipStreamSynchronize(streams[k]));
@@ -421,11 +418,16 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
       for (Region& R : pr.regions) {
          if (!R.has_charges) continue;
          const int n = (int)R.panel_ids.size();
-         std::vector<PanelGeom> tp(n);
-         for (int i = 0; i < n; i++) tp[i] = pr.panels[R.panel_ids[i]];
-         R.m3_panels.upload(tp, s);
-         R.m3 = build_tree_op(tp, R.m3_panels.p, pr.qpos.data(), (int)pr.q.size(), LAPLACE, 0.0, spec.maxpanels,
-                              spec.gridpoints, spec.svdeps, false, s);
+         std::vector<double> mc(3 * (size_t)n);
+         for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) mc[3 * (size_t)i + d] = pr.cent[3 * (size_t)R.panel_ids[i] + d];
+         DBuf<int> mids;
+         mids.upload(R.panel_ids, s);
+         R.m3_panels.alloc(std::max(n, 1));
+         launch_panel_records(d_verts.p, mids.p, n, R.m3_panels.p, s);
+         FSVD_CHECK(hipGetLastError());
+         R.m3 = build_tree_op(mc.data(), n, R.m3_panels.p, pr.qpos.data(), (int)pr.q.size(), LAPLACE, 0.0,
+                              spec.maxpanels, spec.gridpoints, spec.svdeps, false, s);
          std::vector<int> lpi(n), ldi(n);
          std::vector<double> lsp(n), lsd(n);
          for (int l = 0; l < R.mynpan; l++) {
@@ -555,12 +557,12 @@ std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec) {
    R.has_charges = !spec.q.empty();
    R.panel_ids.resize(n);
    for (int i = 0; i < n; i++) R.panel_ids[i] = i;
-   std::vector<double> cent(3 * (size_t)n);
+   pr.cent.resize(3 * (size_t)n);
    for (int i = 0; i < n; i++)
-      for (int d = 0; d < 3; d++) cent[3 * (size_t)i + d] = pr.panels[i].c[d];
+      for (int d = 0; d < 3; d++) pr.cent[3 * (size_t)i + d] = pr.panels[i].c[d];
    R.d_panels.upload(pr.panels, s);
-   R.op = build_tree_op(pr.panels, R.d_panels.p, cent.data(), n, LAPLACE, 0.0, spec.maxpanels, spec.gridpoints,
-                        spec.svdeps, spec.want32, s, nullptr, 1);
+   R.op = build_tree_op(pr.cent.data(), n, R.d_panels.p, pr.cent.data(), n, LAPLACE, 0.0, spec.maxpanels,
+                        spec.gridpoints, spec.svdeps, spec.want32, s, nullptr, 1);
    TreeOp& op = *R.op;
    if (op.tree.pt_perm != op.tree.pan_perm) throw std::runtime_error("point/panel partitions differ");
    // Tree_extractdiagonal(DOUBLE_LAYER_INT) (Tree.c:861-875): leaf D_double[i][i]
@@ -589,7 +591,7 @@ std::unique_ptr<Problem> build_qual_problem(const QualSpec& spec) {
    // Laplace, both layers, panels -> charges
    if (spec.m3 && !spec.q.empty()) {
       const int nq = (int)spec.q.size();
-      R.m3 = build_tree_op(pr.panels, R.d_panels.p, spec.qpos.data(), nq, LAPLACE, 0.0, spec.maxpanels,
+      R.m3 = build_tree_op(pr.cent.data(), n, R.d_panels.p, spec.qpos.data(), nq, LAPLACE, 0.0, spec.maxpanels,
                            spec.gridpoints, spec.svdeps, false, s);
       R.m3_xs.alloc(std::max(n, nq) + 1);
       R.m3_xd.alloc(std::max(n, nq) + 1);
@@ -733,13 +735,13 @@ void problem_rhs(Problem& pr, double* rhs) {
       std::vector<double> cent;
       std::vector<int> out;
       for (int j = 0; j < R.mynpan; j++) {
-         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)R.panel_ids[j] + d]);
          out.push_back(R.startphi + j);
       }
       for (int c : R.child) {
          const Region& C = pr.regions[c];
          for (int j = 0; j < C.mynpan; j++) {
-            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)C.panel_ids[j] + d]);
             out.push_back(C.startdphi + j);
          }
       }
@@ -770,13 +772,13 @@ void problem_rhs_batch(Problem& pr, int ncol, const double* q, double* rhs) {
       std::vector<double> cent;
       std::vector<int> out;
       for (int j = 0; j < R.mynpan; j++) {
-         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)R.panel_ids[j] + d]);
          out.push_back(R.startphi + j);
       }
       for (int c : R.child) {
          const Region& C = pr.regions[c];
          for (int j = 0; j < C.mynpan; j++) {
-            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)C.panel_ids[j] + d]);
             out.push_back(C.startdphi + j);
          }
       }
@@ -809,12 +811,11 @@ static VT* basis_vec(Problem& pr, int j) {
    }
 }
 
-// The first basis slab of each precision the problem can solve in, and the
-// device copy of the panels, made while the problem is built.
+// The first basis slab of each precision the problem can solve in, made
+// while the problem is built.
 static void reserve_solver_memory(Problem& pr, bool want32) {
    basis_vec<double>(pr, 0);
    if (want32) basis_vec<float>(pr, 0);
-   if (!pr.qual) pr.d_allpanels.upload(pr.panels, pr.stream);
 }
 
 // Device array of the basis pointers v_0..v_{m-1} (grown in steps of 64).
@@ -1647,13 +1648,13 @@ SolveInfo problem_grad_charges(Problem& pr, int mode, const double* sol_host, do
       std::vector<double> cent;
       std::vector<int> out;
       for (int j = 0; j < R.mynpan; j++) {
-         for (int d = 0; d < 3; d++) cent.push_back(pr.panels[R.panel_ids[j]].c[d]);
+         for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)R.panel_ids[j] + d]);
          out.push_back(R.startphi + j);
       }
       for (int c : R.child) {
          const Region& C = pr.regions[c];
          for (int j = 0; j < C.mynpan; j++) {
-            for (int d = 0; d < 3; d++) cent.push_back(pr.panels[C.panel_ids[j]].c[d]);
+            for (int d = 0; d < 3; d++) cent.push_back(pr.cent[3 * (size_t)C.panel_ids[j] + d]);
             out.push_back(C.startdphi + j);
          }
       }

@@ -119,8 +119,9 @@ struct Problem {
    // while set, problem_matvec applies A^T and problem_precond P^-T (the
    // adjoint solves of problem_solve_adjoint)
    bool adjoint = false;
-   std::vector<PanelGeom> panels;        // global panel geometry
-   DBuf<PanelGeom> d_allpanels;          // its device copy (direct potentials, charge gradient)
+   std::vector<double> cent;             // panel centroids (3 per panel, global order)
+   std::vector<PanelGeom> panels;        // host panel geometry (qualocation problems only)
+   DBuf<PanelGeom> d_allpanels;          // device panel geometry, global order (direct potentials, gradients)
    std::vector<Region> regions;          // regions[0] = exterior (root)
    std::vector<double> qpos, q;
    double eps_in = 4.0;

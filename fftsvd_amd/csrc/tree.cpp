@@ -5,6 +5,7 @@
 // flat array (parents first) and panels/points renumbered into "tree order"
 // so that every cube owns contiguous index ranges.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -17,12 +18,11 @@ namespace fsvd {
 
 namespace {
 
-// fn(a, b) over [0, n) in contiguous chunks on up to 16 host threads (one
-// below 16384 items); each item is handled by exactly one call, so results
-// do not depend on the split.
+// fn(a, b) over [0, n) in contiguous chunks on up to 16 host threads, at
+// least `grain` items per thread; each item is handled by exactly one call,
+// so results do not depend on the split.
 template <typename F>
-void tree_parallel_for(long long n, F&& fn) {
-   const long long grain = 16384;
+void tree_parallel_for(long long n, F&& fn, long long grain = 16384) {
    int nth = (int)std::min<long long>({16LL, (long long)std::max(1u, std::thread::hardware_concurrency()),
                                         (n + grain - 1) / grain});
    if (nth <= 1) {
@@ -41,12 +41,12 @@ void tree_parallel_for(long long n, F&& fn) {
 
 struct Builder {
    Tree& t;
-   const PanelGeom* P;
+   const double* P;   // panel centroids, 3 per panel
    const double* pts;
    // temporary per-cube sorted index lists
    std::vector<std::vector<int>> pan, pt;
 
-   explicit Builder(Tree& tt, const PanelGeom* p, const double* q) : t(tt), P(p), pts(q) {}
+   explicit Builder(Tree& tt, const double* p, const double* q) : t(tt), P(p), pts(q) {}
 
    int build(std::vector<int>&& mypan, std::vector<int>&& mypt, int level, const int idx[3], const double b0[3],
              const double b1[3], int parent) {
@@ -82,7 +82,7 @@ struct Builder {
          return k;
       };
       for (int i : pan[id]) {
-         int k = child_of(P[i].c);
+         int k = child_of(P + 3 * (size_t)i);
          if (k >= 0) bp[k].push_back(i);
       }
       for (int i : pt[id]) {
@@ -172,29 +172,32 @@ class LevelTree {
  public:
    static constexpr int MAXLEV = 20;
 
-   LevelTree(const PanelGeom* panels, int npan, const double* pts, int npt) : npan_(npan), npt_(npt) {
+   LevelTree(const double* pan_cent, int npan, const double* pts, int npt) : npan_(npan), npt_(npt) {
       const int ne = npan + npt;
       x_.resize(3 * (size_t)ne);
-      for (int i = 0; i < npan; i++)
-         for (int d = 0; d < 3; d++) x_[3 * (size_t)i + d] = panels[i].c[d];
+      if (npan) std::memcpy(x_.data(), pan_cent, sizeof(double) * 3 * (size_t)npan);
       if (npt) std::memcpy(&x_[3 * (size_t)npan], pts, sizeof(double) * 3 * (size_t)npt);
    }
 
    bool build(Tree& t, const double b0[3], const double b1[3], int maxpanels) {
       const int ne = npan_ + npt_;
-      for (size_t i = 0; i < x_.size(); i++)
-         if (!std::isfinite(x_[i])) return false;
       lo_.resize(3 * (size_t)ne);
       hi_.resize(3 * (size_t)ne);
-      for (int e = 0; e < ne; e++)
-         for (int d = 0; d < 3; d++) {
-            lo_[3 * (size_t)e + d] = b0[d];
-            hi_[3 * (size_t)e + d] = b1[d];
-         }
-      code_.assign(ne, 0);
+      code_.resize(ne);
       ord_.assign(1, std::vector<int>(ne));
-      for (int i = 0; i < npan_; i++) ord_[0][i] = i;
-      for (int i = 0; i < npt_; i++) ord_[0][npan_ + i] = npan_ + i;
+      std::atomic<bool> finite{true};
+      tree_parallel_for(ne, [&](long long a, long long b) {
+         for (long long e = a; e < b; e++) {
+            for (int d = 0; d < 3; d++) {
+               if (!std::isfinite(x_[3 * (size_t)e + d])) finite = false;
+               lo_[3 * (size_t)e + d] = b0[d];
+               hi_[3 * (size_t)e + d] = b1[d];
+            }
+            code_[e] = 0;
+            ord_[0][e] = (int)e;
+         }
+      });
+      if (!finite) return false;
       cells_.assign(1, {});
       cells_[0].push_back({0, 0, npan_, npan_, ne});
       int depth = 2;
@@ -250,38 +253,47 @@ class LevelTree {
       });
       if (!ok) return false;
       const std::vector<int>& prev = ord_.back();
+      const std::vector<Cell>& pcells = cells_.back();
+      const size_t npc = pcells.size();
       std::vector<int> next(ne);
-      // panels then points, each regrouped inside the previous cells
+      // the panels, then the points, of each previous cell (a run of one
+      // parent code) regrouped by child octant, stably
       auto regroup = [&](int a, int b) {
-         int i = a;
-         while (i < b) {
-            const uint64_t parent = code_[prev[i]] >> 3;
-            int j = i + 1;
-            while (j < b && code_[prev[j]] >> 3 == parent) j++;
-            int cnt[9] = {0};
-            for (int r = i; r < j; r++) cnt[(code_[prev[r]] & 7) + 1]++;
-            for (int k = 0; k < 8; k++) cnt[k + 1] += cnt[k];
-            for (int r = i; r < j; r++) next[i + cnt[code_[prev[r]] & 7]++] = prev[r];
-            i = j;
-         }
+         int cnt[9] = {0};
+         for (int r = a; r < b; r++) cnt[(code_[prev[r]] & 7) + 1]++;
+         for (int k = 0; k < 8; k++) cnt[k + 1] += cnt[k];
+         for (int r = a; r < b; r++) next[a + cnt[code_[prev[r]] & 7]++] = prev[r];
       };
-      regroup(0, npan_);
-      regroup(npan_, ne);
-      // cells: merge of the distinct panel and point codes
-      std::vector<Cell> cells;
-      int i = 0, j = npan_;
-      while (i < npan_ || j < ne) {
-         uint64_t c;
-         if (j >= ne) c = code_[next[i]];
-         else if (i >= npan_) c = code_[next[j]];
-         else c = std::min(code_[next[i]], code_[next[j]]);
-         Cell cell{c, i, i, j, j};
-         while (i < npan_ && code_[next[i]] == c) i++;
-         while (j < ne && code_[next[j]] == c) j++;
-         cell.pb = i;
-         cell.qb = j;
-         cells.push_back(cell);
-      }
+      // each previous cell's child cells: merge of its distinct panel and
+      // point codes (the previous cells are in code order, so the
+      // concatenation is in code order)
+      std::vector<std::array<Cell, 8>> kids(npc);
+      std::vector<int> nkid(npc + 1, 0);
+      tree_parallel_for((long long)npc, [&](long long a, long long b) {
+         for (long long pc = a; pc < b; pc++) {
+            const Cell& P = pcells[pc];
+            regroup(P.pa, P.pb);
+            regroup(P.qa, P.qb);
+            int i = P.pa, j = P.qa, m = 0;
+            while (i < P.pb || j < P.qb) {
+               uint64_t c;
+               if (j >= P.qb) c = code_[next[i]];
+               else if (i >= P.pb) c = code_[next[j]];
+               else c = std::min(code_[next[i]], code_[next[j]]);
+               Cell cell{c, i, i, j, j};
+               while (i < P.pb && code_[next[i]] == c) i++;
+               while (j < P.qb && code_[next[j]] == c) j++;
+               cell.pb = i;
+               cell.qb = j;
+               kids[pc][m++] = cell;
+            }
+            nkid[pc + 1] = m;
+         }
+      }, 1);
+      for (size_t pc = 0; pc < npc; pc++) nkid[pc + 1] += nkid[pc];
+      std::vector<Cell> cells(nkid[npc]);
+      for (size_t pc = 0; pc < npc; pc++)
+         std::copy(kids[pc].begin(), kids[pc].begin() + (nkid[pc + 1] - nkid[pc]), cells.begin() + nkid[pc]);
       ord_.push_back(std::move(next));
       cells_.push_back(std::move(cells));
       return true;
@@ -377,7 +389,7 @@ class LevelTree {
 
 }  // namespace
 
-void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, int npt, int maxpanels,
+void build_tree(Tree& t, const double* pan_cent, int npan, const double* pts, int npt, int maxpanels,
                 int gridpoints) {
    t.npan = npan;
    t.npt = npt;
@@ -386,8 +398,8 @@ void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, i
    double mn[3] = {1000000.0, 1000000.0, 1000000.0}, mx[3] = {-1000000.0, -1000000.0, -1000000.0};
    for (int i = 0; i < npan; i++)
       for (int d = 0; d < 3; d++) {
-         if (panels[i].c[d] < mn[d]) mn[d] = panels[i].c[d];
-         if (panels[i].c[d] > mx[d]) mx[d] = panels[i].c[d];
+         if (pan_cent[3 * (size_t)i + d] < mn[d]) mn[d] = pan_cent[3 * (size_t)i + d];
+         if (pan_cent[3 * (size_t)i + d] > mx[d]) mx[d] = pan_cent[3 * (size_t)i + d];
       }
    for (int i = 0; i < npt; i++)
       for (int d = 0; d < 3; d++) {
@@ -403,7 +415,7 @@ void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, i
       b0[d] = ctr[d] - ext * 0.5;
       b1[d] = ctr[d] + ext * 0.5;
    }
-   if (LevelTree(panels, npan, pts, npt).build(t, b0, b1, maxpanels)) {
+   if (LevelTree(pan_cent, npan, pts, npt).build(t, b0, b1, maxpanels)) {
       t.gpl.assign(t.depth, 0);
       t.gpl[t.depth - 1] = gridpoints;
       if (t.depth >= 2) t.gpl[t.depth - 2] = gridpoints;
@@ -432,7 +444,7 @@ void build_tree(Tree& t, const PanelGeom* panels, int npan, const double* pts, i
    std::vector<std::vector<int>> pan_lists, pt_lists;
    for (t.depth = 2;; t.depth++) {
       t.cubes.clear();
-      Builder b(t, panels, pts);
+      Builder b(t, pan_cent, pts);
       b.build(std::vector<int>(all_pan), std::vector<int>(all_pt), 0, zero, b0, b1, -1);
       pan_lists.swap(b.pan);
       pt_lists.swap(b.pt);
