@@ -7,6 +7,7 @@
 // pattern and the per-level pseudoinverses and translation spectra.
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <mutex>
 #include <exception>
 #include <thread>
@@ -149,6 +150,28 @@ std::vector<int> xcd_block_map(const std::vector<double>& work) {
 }
 
 
+// The slot of an absolute spectrum offset of level L: (so - sbase) / pst,
+// an exact division, done as a shift by pst's trailing zeros and a multiply
+// by the inverse of its odd part modulo 2^32 (the planners take it for every
+// interaction-list entry; a hardware divide per entry dominated them).
+struct SlotOf {
+   long long sbase;
+   int shift;
+   uint32_t inv;
+   explicit SlotOf(const TrLevel& L) : sbase(L.sbase), shift(0), inv(1) {
+      uint32_t odd = (uint32_t)L.pst;
+      if (odd == 0) throw std::runtime_error("spectrum stride is zero");
+      while (!(odd & 1u)) {
+         odd >>= 1;
+         shift++;
+      }
+      uint32_t x = odd;   // Newton's iteration doubles the correct low bits: 3, 6, 12, 24, 48
+      for (int k = 0; k < 4; k++) x *= 2u - odd * x;
+      inv = x;
+   }
+   int operator()(int so) const { return (int)(((uint32_t)((long long)so - sbase) >> shift) * inv); }
+};
+
 // Tiles of consecutive targets of one level: a tile grows while it has
 // fewer than max_tile targets and the union of their sources stays within
 // budget; a tile whose union exceeds max_src is halved until it fits.
@@ -159,7 +182,7 @@ template <typename F>
 static void make_tiles(const TrLevel& L, const std::vector<TgtPlan>& tgt, const std::vector<int>& isrc, int max_tile,
                        int budget, int max_src, F&& emit, int quantum = 1) {
    std::vector<int> stamp(std::max(L.nsrc, 1), -1);
-   auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
+   const SlotOf slot_of(L);
    int epoch = 0;
    std::function<void(int, int)> make = [&](int k0, int k1) {
       const int mark = -2 - (epoch++);
@@ -217,7 +240,7 @@ static void build_translate_f64(ApplyPlan& P, const std::vector<TrLevel>& lvls, 
       const int first = (int)tiles.size();
       // a tile's local index of each of its sources, by the source's slot
       std::vector<int> loc(std::max(L.nsrc, 1), -1);
-      auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
+      const SlotOf slot_of(L);
       make_tiles(L, tgt, isrc, TR_TILE, TR_SRC_BUDGET, TR_MAX_SRC, [&](int k0, int k1, const std::vector<int>& so) {
          for (size_t i = 0; i < so.size(); i++) loc[slot_of(so[i])] = (int)i;
          TrTile t;
@@ -273,39 +296,66 @@ static void build_translate_f64(ApplyPlan& P, const std::vector<TrLevel>& lvls, 
 // quarter boundary).
 static void octet_words(const std::vector<std::vector<unsigned>>& lists, int nk4, std::vector<unsigned>& out) {
    out.assign(32 * (size_t)(nk4 / 4), TP_NULL_WORD);
+   auto cls_of = [](unsigned w) { return (int)(((w & 0xffffu) / 64 & 3) * 4 + ((w >> 16) / 64 & 3)); };
+   std::vector<unsigned> flat;   // a half's words: per target, grouped by class, list order within a class
    for (int h = 0; h < 2; h++) {
-      std::vector<unsigned> cls[4][16];
-      int head[4][16] = {}, rem[4] = {0, 0, 0, 0};
+      int start[4][17] = {}, left[4][16] = {}, rem[4] = {0, 0, 0, 0};
+      size_t base[4] = {0, 0, 0, 0}, tot = 0;
+      for (int j = 0; j < 4; j++)
+         if (4 * h + j < (int)lists.size()) tot += lists[4 * h + j].size();
+      flat.resize(tot);
+      size_t b = 0;
       for (int j = 0; j < 4; j++) {
          const int t = 4 * h + j;
+         base[j] = b;
          if (t >= (int)lists.size()) continue;
-         for (unsigned w : lists[t]) cls[j][((w & 0xffffu) / 64 & 3) * 4 + ((w >> 16) / 64 & 3)].push_back(w);
+         int* st = start[j];
+         for (unsigned w : lists[t]) st[cls_of(w) + 1]++;
+         for (int c = 0; c < 16; c++) st[c + 1] += st[c];
+         int fill[16];
+         for (int c = 0; c < 16; c++) fill[c] = st[c];
+         for (unsigned w : lists[t]) flat[b + fill[cls_of(w)]++] = w;
+         for (int c = 0; c < 16; c++) left[j][c] = st[c + 1] - st[c];
          rem[j] = (int)lists[t].size();
+         b += lists[t].size();
       }
       for (int k = 0; k < nk4; k++) {
+         // targets by words left, most first, ties in target order
          int order[4] = {0, 1, 2, 3};
-         std::stable_sort(order, order + 4, [&](int x, int y) { return rem[x] > rem[y]; });
-         int uT[4] = {0, 0, 0, 0}, uS[4] = {0, 0, 0, 0};
+         for (int a = 1; a < 4; a++) {
+            const int v = order[a];
+            int q = a - 1;
+            while (q >= 0 && rem[order[q]] < rem[v]) {
+               order[q + 1] = order[q];
+               q--;
+            }
+            order[q + 1] = v;
+         }
+         // collisions so far in this step, per class: T quarter c >> 2 and
+         // source quarter c & 3
+         int used[16] = {};
          for (int oi = 0; oi < 4; oi++) {
             const int j = order[oi];
             unsigned w = TP_NULL_WORD;
             if (rem[j] > 0) {
-               int best = -1, best_cost = 1 << 30, best_n = 0;
+               // least collisions, then most words left, then lowest class:
+               // the smallest key among the non-empty classes
+               int best_key = INT_MAX;
                for (int c = 0; c < 16; c++) {
-                  const int n = (int)cls[j][c].size() - head[j][c];
-                  if (n <= 0) continue;
-                  const int cst = uT[c >> 2] + uS[c & 3];
-                  if (cst < best_cost || (cst == best_cost && n > best_n)) {
-                     best = c;
-                     best_cost = cst;
-                     best_n = n;
-                  }
+                  const int n = left[j][c];
+                  const int key = n > 0 ? (used[c] << 20) - (n << 4) + c : INT_MAX;
+                  best_key = std::min(best_key, key);
                }
-               w = cls[j][best][head[j][best]++];
+               const int best = best_key & 15;
+               w = flat[base[j] + start[j][best] + (start[j][best + 1] - start[j][best] - left[j][best])];
+               left[j][best]--;
                rem[j]--;
             }
-            uT[(w & 0xffffu) / 64 & 3]++;
-            uS[(w >> 16) / 64 & 3]++;
+            const int tq = (int)((w & 0xffffu) / 64 & 3), sq = (int)((w >> 16) / 64 & 3);
+            for (int c = 0; c < 4; c++) {
+               used[4 * tq + c]++;
+               used[4 * c + sq]++;
+            }
             out[(size_t)32 * (k / 4) + 4 * (4 * h + j) + (k & 3)] = w;
          }
       }
@@ -317,38 +367,70 @@ static void octet_words(const std::vector<std::vector<unsigned>>& lists, int nk4
 // length and cut into octets of eight, longest first (the consumer waves
 // claim them in this order); tasks (tile, 8-bin chunk, first octet, octets)
 // listed by level, then chunk, then tile, and dealt to the persistent
-// blocks as described below.  The tiles are cut level by level, then
-// worked out on up to 16 host threads (each tile's octets and words are
-// independent) and concatenated in tile order, so the plan does not
-// depend on the thread count.
+// blocks as described below.  The tiles are cut level by level; then, on up
+// to 16 host threads, each tile's targets are ordered and cut into octets
+// (lists per octet), every octet's words are scheduled (octets are
+// independent), and the words are concatenated in tile and octet order, so
+// the plan does not depend on the thread count.
 namespace {
 struct PipeTileJob {
    int level, k0, k1;
    std::vector<int> so;        // the tile's sources (absolute spectrum offsets)
+   // per octet: the targets' words in list order, then the scheduled words
+   std::vector<std::vector<std::vector<unsigned>>> lists;
+   std::vector<std::vector<unsigned>> ow;
    // results
    std::vector<TpOct> octs;    // int_off relative to words, acc_idx relative to oacc
    std::vector<long long> oacc;
-   std::vector<unsigned> words;
+   size_t nwords = 0;          // the tile's words, alignment included
+   size_t wbase = 0;           // its first word in the plan
    int cost = 0;
 };
 }  // namespace
 
-static void pipe_tile_job(PipeTileJob& J, const TrLevel& L, const std::vector<TgtPlan>& tgt,
-                          const std::vector<int>& isrc, const std::vector<int>& itoff, std::vector<int>& loc) {
-   auto slot_of = [&](int so) { return (int)(((long long)so - L.sbase) / L.pst); };
+// Runs fn(i) for i in [0, n) on up to 16 host threads (dynamic claiming);
+// the first exception is rethrown.
+template <typename F>
+static void plan_parallel_for(int n, F&& fn) {
+   const int nth = (int)std::max<size_t>(1, std::min<size_t>({16u, std::thread::hardware_concurrency(),
+                                                              (size_t)std::max(n, 1)}));
+   std::atomic<int> next{0};
+   std::exception_ptr err;
+   std::mutex em;
+   auto work = [&] {
+      for (int j; (j = next.fetch_add(1)) < n;) {
+         try {
+            fn(j);
+         } catch (...) {
+            std::lock_guard<std::mutex> g(em);
+            if (!err) err = std::current_exception();
+         }
+      }
+   };
+   std::vector<std::thread> th;
+   for (int t = 1; t < nth; t++) th.emplace_back(work);
+   work();
+   for (auto& x : th) x.join();
+   if (err) std::rethrow_exception(err);
+}
+
+// A tile's targets sorted by list length (longest first) and cut into
+// octets: each octet's accumulator offsets, step count and word lists.
+static void pipe_tile_lists(PipeTileJob& J, const TrLevel& L, const std::vector<TgtPlan>& tgt,
+                            const std::vector<int>& isrc, const std::vector<int>& itoff, std::vector<int>& loc) {
+   const SlotOf slot_of(L);
    for (size_t i = 0; i < J.so.size(); i++) loc[slot_of(J.so[i])] = (int)i;
    const int n = J.k1 - J.k0;
    std::vector<int> order(n);
    for (int k = 0; k < n; k++) order[k] = k;
    std::stable_sort(order.begin(), order.end(),
                     [&](int x, int y) { return tgt[L.t0 + J.k0 + x].nint > tgt[L.t0 + J.k0 + y].nint; });
-   std::vector<unsigned> ow;
-   std::vector<std::vector<unsigned>> lists;
    for (int o0 = 0; o0 < n; o0 += 8) {
-      lists.clear();
+      std::vector<std::vector<unsigned>> lists;
       TpOct o;
       o.acc_idx = (int)J.oacc.size();
       o.pad_ = 0;
+      o.int_off = 0;
       int longest = 0;
       for (int t8 = 0; t8 < 8; t8++) {
          if (o0 + t8 >= n) {
@@ -365,13 +447,24 @@ static void pipe_tile_job(PipeTileJob& J, const TrLevel& L, const std::vector<Tg
          J.oacc.push_back(tp.acc_off);
       }
       o.nk = std::max(8, (longest + 7) / 8 * 8);
-      octet_words(lists, o.nk, ow);
-      while (J.words.size() % 4) J.words.push_back(TP_NULL_WORD);
-      o.int_off = (int)J.words.size();
-      J.words.insert(J.words.end(), ow.begin(), ow.end());
       J.octs.push_back(o);
+      J.lists.push_back(std::move(lists));
    }
-   // modelled cost of a task of this tile in steps per consumer wave
+   J.ow.resize(J.octs.size());
+}
+
+// The offsets of the tile's octet words (each octet from a 16-byte aligned
+// start, relative to the tile's) and the modelled cost of a task of this
+// tile in steps per consumer wave.
+static void pipe_tile_words(PipeTileJob& J) {
+   size_t nw = 0;
+   for (size_t k = 0; k < J.octs.size(); k++) {
+      nw = (nw + 3) / 4 * 4;
+      J.octs[k].int_off = (int)nw;
+      nw += J.ow[k].size();
+   }
+   J.nwords = nw;
+   std::vector<std::vector<std::vector<unsigned>>>().swap(J.lists);
    double steps = 0;
    for (const TpOct& o : J.octs) steps += o.nk;
    J.cost = (int)(steps / TP_CW + (J.octs.empty() ? 0 : J.octs[0].nk));
@@ -383,45 +476,44 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
    H = PipeHost();
    std::vector<PipeTileJob> jobs;
    int maxsrc = 1;
-   for (int li = 0; li < (int)lvls.size(); li++) {
-      maxsrc = std::max(maxsrc, lvls[li].nsrc);
-      make_tiles(
-          lvls[li], tgt, isrc, pp.tile, pp.budget, TP_MAX_SRC,
-          [&](int k0, int k1, const std::vector<int>& so) {
-             PipeTileJob J;
-             J.level = li;
-             J.k0 = k0;
-             J.k1 = k1;
-             J.so = so;
-             jobs.push_back(std::move(J));
-          },
-          TP_QUANTUM);
+   for (int li = 0; li < (int)lvls.size(); li++) maxsrc = std::max(maxsrc, lvls[li].nsrc);
+   {
+      // the levels' tiles cut concurrently, listed level by level
+      std::vector<std::vector<PipeTileJob>> lj(lvls.size());
+      plan_parallel_for((int)lvls.size(), [&](int li) {
+         make_tiles(
+             lvls[li], tgt, isrc, pp.tile, pp.budget, TP_MAX_SRC,
+             [&](int k0, int k1, const std::vector<int>& so) {
+                PipeTileJob J;
+                J.level = li;
+                J.k0 = k0;
+                J.k1 = k1;
+                J.so = so;
+                lj[li].push_back(std::move(J));
+             },
+             TP_QUANTUM);
+      });
+      for (auto& v : lj)
+         for (PipeTileJob& J : v) jobs.push_back(std::move(J));
    }
    {
-      const int nth = (int)std::max<size_t>(1, std::min<size_t>({16u, std::thread::hardware_concurrency(),
-                                                                   (jobs.size() + 1) / 2}));
-      std::atomic<int> next{0};
-      std::exception_ptr err;
-      std::mutex em;
-      auto work = [&] {
-         std::vector<int> loc(maxsrc, 0);
-         for (int j; (j = next.fetch_add(1)) < (int)jobs.size();) {
-            try {
-               pipe_tile_job(jobs[j], lvls[jobs[j].level], tgt, isrc, itoff, loc);
-            } catch (...) {
-               std::lock_guard<std::mutex> g(em);
-               if (!err) err = std::current_exception();
-            }
-         }
-      };
-      std::vector<std::thread> th;
-      for (int t = 1; t < nth; t++) th.emplace_back(work);
-      work();
-      for (auto& x : th) x.join();
-      if (err) std::rethrow_exception(err);
+      plan_parallel_for((int)jobs.size(), [&](int j) {
+         thread_local std::vector<int> loc;
+         if ((int)loc.size() < maxsrc) loc.assign(maxsrc, 0);
+         pipe_tile_lists(jobs[j], lvls[jobs[j].level], tgt, isrc, itoff, loc);
+      });
+      std::vector<std::pair<int, int>> octets;   // (job, octet)
+      for (size_t j = 0; j < jobs.size(); j++)
+         for (size_t k = 0; k < jobs[j].octs.size(); k++) octets.push_back({(int)j, (int)k});
+      plan_parallel_for((int)octets.size(), [&](int i) {
+         PipeTileJob& J = jobs[octets[i].first];
+         const int k = octets[i].second;
+         octet_words(J.lists[k], J.octs[k].nk, J.ow[k]);
+      });
+      plan_parallel_for((int)jobs.size(), [&](int j) { pipe_tile_words(jobs[j]); });
    }
    // concatenation in tile order; tasks by level, then chunk, then tile
-   size_t j0 = 0;
+   size_t j0 = 0, gint_n = 0;
    for (int li = 0; li < (int)lvls.size(); li++) {
       const TrLevel& L = lvls[li];
       std::vector<int4> lt;   // per tile of the level: index, first octet, octets, modelled cost
@@ -439,9 +531,10 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
          else
             H.gsrc.insert(H.gsrc.end(), J.so.begin(), J.so.end());
          t.nsrc = std::max(1, (int)J.so.size());
-         while (H.gint.size() % 4) H.gint.push_back((int)TP_NULL_WORD);
-         const int wbase = (int)H.gint.size(), abase = (int)H.oacc.size(), oct0 = (int)H.octs.size();
-         H.gint.insert(H.gint.end(), J.words.begin(), J.words.end());
+         gint_n = (gint_n + 3) / 4 * 4;
+         J.wbase = gint_n;
+         const int wbase = (int)gint_n, abase = (int)H.oacc.size(), oct0 = (int)H.octs.size();
+         gint_n += J.nwords;
          H.oacc.insert(H.oacc.end(), J.oacc.begin(), J.oacc.end());
          for (TpOct o : J.octs) {
             o.int_off += wbase;
@@ -451,13 +544,24 @@ static void build_translate_pipe(ApplyPlan& P, const std::vector<TrLevel>& lvls,
          lt.push_back(make_int4((int)H.tiles.size(), oct0, (int)J.octs.size(), J.cost));
          H.tiles.push_back(t);
          H.t_off.push_back(L.t_off);
-         std::vector<unsigned>().swap(J.words);
       }
       for (int b0 = 0; b0 < L.pad; b0 += 8)
          for (const int4& e : lt) {
             H.tasks.push_back(make_int4(e.x, b0, e.y, e.z));
             H.cost.push_back(e.w + 16.0);   // summing steps plus a fixed share per task
          }
+   }
+   // every octet's words in place
+   H.gint.assign(gint_n, (int)TP_NULL_WORD);
+   {
+      std::vector<std::pair<int, int>> octets;   // (job, octet)
+      for (size_t j = 0; j < jobs.size(); j++)
+         for (size_t k = 0; k < jobs[j].octs.size(); k++) octets.push_back({(int)j, (int)k});
+      plan_parallel_for((int)octets.size(), [&](int i) {
+         const PipeTileJob& J = jobs[octets[i].first];
+         const int k = octets[i].second;
+         std::copy(J.ow[k].begin(), J.ow[k].end(), (unsigned*)H.gint.data() + J.wbase + J.octs[k].int_off);
+      });
    }
    // the producers address the FP32 spectra by 32-bit byte offsets
    for (size_t i = 0; i < H.tiles.size(); i++)
@@ -905,7 +1009,9 @@ static void build_apply_plan(TreeOp& op, bool want32, hipStream_t s) {
       to_f32(P.UTI64, P.UTI32, s);
       to_f32(P.UT64, P.UT32, s);
       const PipeParams pp = pipe_params(op, s);
+      timing_mark("plan: FP32 copies, tuning", tm);
       build_translate_pipe(P, P.tr_lv, P.tr_tgt, P.tr_isrc, P.tr_itoff, pp);
+      timing_mark("plan: pipe tiles", tm);
       make_pipe(P.pipe, {&op}, s, pp.mode);
       op.stats.pipe = pp;
       timing_mark("plan: pipelined translation", tm);
@@ -1469,33 +1575,9 @@ std::unique_ptr<TreeOp> build_tree_op(const double* pan_cent, int npan, const Pa
          td.nint = (int)lv.h_isrc.size() - td.int_off;
          op.stats.translate_macs += (long long)td.nint * lv.pad;
       }
-      size_t si = 0;
-      lv.h_tgtT.resize(lv.h_src.size());
-      for (size_t i = 0; i < t.cubes.size(); i++) {
-         const Cube& c = t.cubes[i];
-         if (c.level != lv.level || src_slot[i] < 0) continue;
-         TgtDesc& td = lv.h_tgtT[si];
-         std::memset(&td, 0, sizeof(td));
-         td.spec_slot = src_slot[i];
-         td.int_off = (int)lv.h_isrcT.size();
-         for (int o : c.itr)
-            if (tgt_slot[o] >= 0) {
-               lv.h_isrcT.push_back(tgt_slot[o]);
-               lv.h_itoffT.push_back(offset_code(c.idx, t.cubes[o].idx));
-            }
-         td.nint = (int)lv.h_isrcT.size() - td.int_off;
-         si++;
-      }
       lv.nsrc = (int)lv.h_src.size();
       lv.ntgt = (int)lv.h_tgt.size();
       op.stats.spectra_bins += (long long)(lv.nsrc + lv.ntgt) * lv.pad;
-      lv.src.upload(lv.h_src, s);
-      lv.tgt.upload(lv.h_tgt, s);
-      lv.tgtT.upload(lv.h_tgtT, s);
-      lv.isrc.upload(lv.h_isrc, s);
-      lv.itoff.upload(lv.h_itoff, s);
-      lv.isrcT.upload(lv.h_isrcT, s);
-      lv.itoffT.upload(lv.h_itoffT, s);
    }
    timing_mark("interaction lists", tm);
    if (want32) {
@@ -1701,10 +1783,47 @@ void tree_apply(TreeOp& op, int prec, const double* xs, const double* xd, double
    }
 }
 
+// The transposed apply's per-level descriptors and interaction lists
+// (each source cube receives from the target cubes of its list), built and
+// uploaded on the transposed apply's first use.
+static void transpose_lists(TreeOp& op, hipStream_t s) {
+   if (op.haveT) return;
+   const Tree& t = op.tree;
+   for (FarLevel& lv : op.levels) {
+      lv.h_tgtT.assign(lv.h_src.size(), TgtDesc{});
+      lv.h_isrcT.clear();
+      lv.h_itoffT.clear();
+      size_t si = 0;
+      for (size_t i = 0; i < t.cubes.size(); i++) {
+         const Cube& c = t.cubes[i];
+         if (c.level != lv.level || op.cube_src_slot[i] < 0) continue;
+         TgtDesc& td = lv.h_tgtT[si];
+         std::memset(&td, 0, sizeof(td));
+         td.spec_slot = op.cube_src_slot[i];
+         td.int_off = (int)lv.h_isrcT.size();
+         for (int o : c.itr)
+            if (op.cube_tgt_slot[o] >= 0) {
+               lv.h_isrcT.push_back(op.cube_tgt_slot[o]);
+               lv.h_itoffT.push_back(offset_code(c.idx, t.cubes[o].idx));
+            }
+         td.nint = (int)lv.h_isrcT.size() - td.int_off;
+         si++;
+      }
+      lv.src.upload(lv.h_src, s);
+      lv.tgt.upload(lv.h_tgt, s);
+      lv.tgtT.upload(lv.h_tgtT, s);
+      lv.isrcT.upload(lv.h_isrcT, s);
+      lv.itoffT.upload(lv.h_itoffT, s);
+   }
+   FSVD_CHECK(hipStreamSynchronize(s));
+   op.haveT = true;
+}
+
 // y2 set: both layers from one pass (single layer into y, double into y2)
 template <typename T>
 static void applyT_impl(TreeOp& op, int layer, const double* x, double* y, hipStream_t s, double* y2 = nullptr) {
    const bool f32 = std::is_same<T, float>::value;
+   transpose_lists(op, s);
    for (FarLevel& lv : op.levels)
       if (!lv.specT.p) {
          lv.specT.alloc(2 * (size_t)std::max(1, lv.ntgt) * lv.pad);

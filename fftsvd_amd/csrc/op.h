@@ -58,11 +58,13 @@ struct FarLevel {
    std::vector<SrcDesc> h_src;
    std::vector<TgtDesc> h_tgt;
    std::vector<int> h_isrc, h_itoff;     // target-side interaction lists
-   std::vector<TgtDesc> h_tgtT;          // transposed: panel-side receivers (one per source cube)
+   // the transposed apply's descriptors and lists (transposed: panel-side
+   // receivers, one per source cube), made on its first use
+   std::vector<TgtDesc> h_tgtT;
    std::vector<int> h_isrcT, h_itoffT;
    DBuf<SrcDesc> src;
    DBuf<TgtDesc> tgt, tgtT;
-   DBuf<int> isrc, itoff, isrcT, itoffT;
+   DBuf<int> isrcT, itoffT;
    DBuf<double> T64;     // 343 * pad complex
    DBuf<float> T32;
    DBuf<double> specT, accT;     // ntgt*pad, nsrc*pad complex: transposed apply only, allocated on first use
@@ -177,6 +179,7 @@ struct TreeOp {
    int kernel = LAPLACE;
    double kappa = 0.0;
    bool have32 = false;
+   bool haveT = false;   // the transposed apply's lists are on the device
    // geometry
    DBuf<double> pts_tree;     // tree-order point coordinates
    DBuf<int> pan_perm;        // tree order -> caller panel index

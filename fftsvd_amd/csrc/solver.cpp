@@ -270,6 +270,53 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
       walk(0);
       if (index != 2 * pr.ntotal) throw std::runtime_error("SurfaceOperator indexing is WRONG!");
    }
+   // join / split maps in tree order (the octree's panel order, known
+   // before the operators are built)
+   auto join_split_maps = [&](Region& R, const std::vector<int>& pan_perm) {
+      int n = (int)R.panel_ids.size();
+      std::vector<int> pi(n), di(n), oi(n);
+      std::vector<double> sp(n), sd(n), co(n);
+      // tree-local index -> (child region, offset)
+      std::vector<int> owner(n, -1), local(n, 0);
+      for (int l = 0; l < R.mynpan; l++) local[l] = l;
+      int cur = R.mynpan;
+      for (int c : R.child) {
+         for (int j = 0; j < pr.regions[c].mynpan; j++) {
+            owner[cur + j] = c;
+            local[cur + j] = j;
+         }
+         cur += pr.regions[c].mynpan;
+      }
+      for (int k = 0; k < n; k++) {
+         int l = pan_perm[k];
+         if (owner[l] < 0) {
+            pi[k] = R.startphi + l;
+            di[k] = R.startdphi + l;
+            sp[k] = 1.0;
+            sd[k] = -1.0;
+            oi[k] = R.startphi + l;
+            co[k] = 0.0;
+         } else {
+            const Region& C = pr.regions[owner[l]];
+            int j = local[l];
+            pi[k] = C.startphi + j;
+            di[k] = C.startdphi + j;
+            sp[k] = -1.0;
+            sd[k] = C.eps / R.eps;
+            oi[k] = C.startdphi + j;
+            co[k] = -4.0 * M_PI;
+         }
+      }
+      R.phi_idx.upload(pi, s);
+      R.dphi_idx.upload(di, s);
+      R.out_idx.upload(oi, s);
+      R.s_phi.upload(sp, s);
+      R.s_dphi.upload(sd, s);
+      R.corr.upload(co, s);
+      R.xs.alloc(n + 1);
+      R.xd.alloc(n + 1);
+      R.yt.alloc(n + 1);
+   };
    // operators (share identical trees: same kernel, same panels).  The
    // octree depends only on the panels, so it is built once per distinct
    // panel set; the distinct operators are then built concurrently, one host
@@ -337,8 +384,22 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
                err[k] = std::current_exception();
             }
          });
+      // meanwhile the regions' join / split maps, from the prebuilt trees
+      std::exception_ptr map_err;
+      try {
+         for (size_t r = 0; r < pr.regions.size(); r++) {
+            Region& R = pr.regions[r];
+            const int owner = R.shares_tree_with >= 0 ? R.shares_tree_with : (int)r;
+            const size_t k = (size_t)(std::find(todo.begin(), todo.end(), owner) - todo.begin());
+            const size_t src = tree_src[k] < 0 ? k : (size_t)tree_src[k];
+            join_split_maps(R, trees[src].pan_perm);
+         }
+      } catch (...) {
+         map_err = std::current_exception();
+      }
       for (auto& x : th) x.join();
       for (size_t k = 0; k < nt; k++) (void)hipStreamDestroy(streams[k]);
+      if (map_err) std::rethrow_exception(map_err);
       for (size_t k = 0; k < nt; k++)
          if (err[k]) std::rethrow_exception(err[k]);
       for (int r : todo)
@@ -362,54 +423,6 @@ std::unique_ptr<Problem> build_problem(const SurfaceSpec& spec) {
          FSVD_CHECK(hipStreamCreate(&pr.aux[k]));
          FSVD_CHECK(hipEventCreateWithFlags(&pr.ev_join[k], hipEventDisableTiming));
       }
-   }
-   // join / split maps in tree order
-   for (size_t r = 0; r < pr.regions.size(); r++) {
-      Region& R = pr.regions[r];
-      TreeOp& op = R.shares_tree_with >= 0 ? *pr.regions[R.shares_tree_with].op : *R.op;
-      int n = (int)R.panel_ids.size();
-      std::vector<int> pi(n), di(n), oi(n);
-      std::vector<double> sp(n), sd(n), co(n);
-      // tree-local index -> (child region, offset)
-      std::vector<int> owner(n, -1), local(n, 0);
-      for (int l = 0; l < R.mynpan; l++) local[l] = l;
-      int cur = R.mynpan;
-      for (int c : R.child) {
-         for (int j = 0; j < pr.regions[c].mynpan; j++) {
-            owner[cur + j] = c;
-            local[cur + j] = j;
-         }
-         cur += pr.regions[c].mynpan;
-      }
-      for (int k = 0; k < n; k++) {
-         int l = op.tree.pan_perm[k];
-         if (owner[l] < 0) {
-            pi[k] = R.startphi + l;
-            di[k] = R.startdphi + l;
-            sp[k] = 1.0;
-            sd[k] = -1.0;
-            oi[k] = R.startphi + l;
-            co[k] = 0.0;
-         } else {
-            const Region& C = pr.regions[owner[l]];
-            int j = local[l];
-            pi[k] = C.startphi + j;
-            di[k] = C.startdphi + j;
-            sp[k] = -1.0;
-            sd[k] = C.eps / R.eps;
-            oi[k] = C.startdphi + j;
-            co[k] = -4.0 * M_PI;
-         }
-      }
-      R.phi_idx.upload(pi, s);
-      R.dphi_idx.upload(di, s);
-      R.out_idx.upload(oi, s);
-      R.s_phi.upload(sp, s);
-      R.s_dphi.upload(sd, s);
-      R.corr.upload(co, s);
-      R.xs.alloc(n + 1);
-      R.xd.alloc(n + 1);
-      R.yt.alloc(n + 1);
    }
    // M3 trees (accelerateM3; FFTSVDpbeAPI.c:1256-1266,1503-1513): Laplace,
    // both layers, from the region's tree panels to the charges; the region
