@@ -1210,59 +1210,81 @@ std::unique_ptr<TreeOp> build_tree_op(const double* pan_cent, int npan, const Pa
    op.yfar.alloc(std::max(npt, npan) + 1);
 
    // ---------------- near field (Cube_fill_D, Cube.c:715-793) ----------------
+   // on a host thread and stream of its own, beside the far-field levels
+   // (it writes only the near-field members of op; its stream is
+   // synchronized and the thread joined before the interaction lists)
+   int dev = 0;
+   FSVD_CHECK(hipGetDevice(&dev));
+   std::exception_ptr near_err;
+   auto near_field = [&]() {
+      hipStream_t s = nullptr;
+      try {
+         FSVD_CHECK(hipSetDevice(dev));
+         FSVD_CHECK(hipStreamCreate(&s));
+         double t0 = now_ms();
+         long long dsize = 0;
+         for (size_t i = 0; i < t.cubes.size(); i++) {
+            const Cube& c = t.cubes[i];
+            if (!c.leaf || c.nq == 0) continue;
+            LeafDesc L;
+            L.q0 = c.q0;
+            L.p = c.nq;
+            L.col_off = (int)op.h_dcol.size();
+            for (int k = 0; k < c.np; k++) op.h_dcol.push_back(c.p0 + k);
+            for (int l : c.loc) {
+               const Cube& o = t.cubes[l];
+               for (int k = 0; k < o.np; k++) op.h_dcol.push_back(o.p0 + k);
+            }
+            L.m = (int)op.h_dcol.size() - L.col_off;
+            L.d_off = dsize;
+            L.u0 = 0;
+            L.nu = 0;
+            dsize += 2LL * L.p * L.m;
+            op.h_leaves.push_back(L);
+            op.maxm = std::max(op.maxm, L.m);
+            op.maxp = std::max(op.maxp, L.p);
+         }
+         op.nleaf = (int)op.h_leaves.size();
+         op.stats.nleaf = op.nleaf;
+         op.stats.near_entries = dsize;
+         op.leaves.upload(op.h_leaves, s);
+         op.dcol.upload(op.h_dcol, s);
+         op.D64.alloc(std::max<long long>(dsize, 1));
+         {
+            std::vector<int> bpan(op.h_dcol.size());
+            for (size_t k = 0; k < bpan.size(); k++) bpan[k] = t.pan_perm[op.h_dcol[k]];
+            std::vector<IntBlock> blocks;
+            for (const LeafDesc& L : op.h_leaves) {
+               if (L.m == 0) continue;
+               IntBlock b;
+               b.npt = L.p;
+               b.npan = L.m;
+               b.pt_off = L.q0;
+               b.pan_off = L.col_off;
+               b.out_off = L.d_off;
+               b.ld_pt = 1;
+               b.ld_pan = 2 * L.p;
+               b.layer_stride = L.p;
+               push_blocks(blocks, b);
+            }
+            run_integrals(blocks, ptsT, bpan, d_panels, kernel, kappa, op.D64.p, s);
+         }
+         FSVD_CHECK(hipStreamSynchronize(s));
+         op.stats.setup_ms_near = now_ms() - t0;
+         timing_mark("near field", t0);
+      } catch (...) {
+         near_err = std::current_exception();
+      }
+      if (s) (void)hipStreamDestroy(s);
+   };
+   struct Joiner {
+      std::thread t;
+      ~Joiner() {
+         if (t.joinable()) t.join();
+      }
+   } near_thread;
+   near_thread.t = std::thread(near_field);
    double t0 = now_ms();
-   long long dsize = 0;
-   for (size_t i = 0; i < t.cubes.size(); i++) {
-      const Cube& c = t.cubes[i];
-      if (!c.leaf || c.nq == 0) continue;
-      LeafDesc L;
-      L.q0 = c.q0;
-      L.p = c.nq;
-      L.col_off = (int)op.h_dcol.size();
-      for (int k = 0; k < c.np; k++) op.h_dcol.push_back(c.p0 + k);
-      for (int l : c.loc) {
-         const Cube& o = t.cubes[l];
-         for (int k = 0; k < o.np; k++) op.h_dcol.push_back(o.p0 + k);
-      }
-      L.m = (int)op.h_dcol.size() - L.col_off;
-      L.d_off = dsize;
-      L.u0 = 0;
-      L.nu = 0;
-      dsize += 2LL * L.p * L.m;
-      op.h_leaves.push_back(L);
-      op.maxm = std::max(op.maxm, L.m);
-      op.maxp = std::max(op.maxp, L.p);
-   }
-   op.nleaf = (int)op.h_leaves.size();
-   op.stats.nleaf = op.nleaf;
-   op.stats.near_entries = dsize;
-   op.leaves.upload(op.h_leaves, s);
-   op.dcol.upload(op.h_dcol, s);
-   op.D64.alloc(std::max<long long>(dsize, 1));
-   {
-      std::vector<int> bpan(op.h_dcol.size());
-      for (size_t k = 0; k < bpan.size(); k++) bpan[k] = t.pan_perm[op.h_dcol[k]];
-      std::vector<IntBlock> blocks;
-      for (const LeafDesc& L : op.h_leaves) {
-         if (L.m == 0) continue;
-         IntBlock b;
-         b.npt = L.p;
-         b.npan = L.m;
-         b.pt_off = L.q0;
-         b.pan_off = L.col_off;
-         b.out_off = L.d_off;
-         b.ld_pt = 1;
-         b.ld_pan = 2 * L.p;
-         b.layer_stride = L.p;
-         push_blocks(blocks, b);
-      }
-      run_integrals(blocks, ptsT, bpan, d_panels, kernel, kappa, op.D64.p, s);
-   }
-   FSVD_CHECK(hipStreamSynchronize(s));
-   op.stats.setup_ms_near = now_ms() - t0;
-   op.stats.phase_ms[1] += op.stats.setup_ms_near;
-   timing_mark("near field", t0);
-   t0 = now_ms();
 
    // ---------------- far field ----------------
    const double root = t.cubes[0].b1[0] - t.cubes[0].b0[0];
@@ -1548,6 +1570,9 @@ std::unique_ptr<TreeOp> build_tree_op(const double* pan_cent, int npan, const Pa
       }
    }
 
+   near_thread.t.join();
+   if (near_err) std::rethrow_exception(near_err);
+   op.stats.phase_ms[1] += op.stats.setup_ms_near;
    op.cube_src_slot = src_slot;
    op.cube_tgt_slot = tgt_slot;
    op.cube_leaf.assign(t.cubes.size(), -1);
