@@ -853,6 +853,20 @@ static const VT* const* basis_ptrs(Problem& pr, int m) {
       return pr.d_basis_ptr32.p;
 }
 
+// Iterations until the next convergence check of a GMRES run: `check`, or
+// fewer when the residual history (r_prev at iteration i_prev, r_last at
+// i_last) predicts tol sooner at its recent geometric rate.  The cadence
+// only decides how many iterations run past convergence before it is seen
+// (they are discarded), never the iterate returned.
+static int next_check_gap(double r_prev, int i_prev, double r_last, int i_last, double tol, int check) {
+   if (!(r_prev > 0.0) || !(r_last > 0.0) || i_last <= i_prev) return check;
+   const double rate = std::pow(r_last / r_prev, 1.0 / (double)(i_last - i_prev));
+   if (!(rate < 1.0) || !(rate > 0.0)) return check;
+   const double k = std::ceil(std::log(tol / r_last) / std::log(rate));
+   if (!(k >= 1.0)) return 1;
+   return (int)std::min<double>((double)check, k);
+}
+
 // GMRES over the basis type VT (the reference keeps FP64 vectors; the
 // FP32-operator solves of the mixed mode keep FP32 basis vectors, with the
 // residual refined in FP64 by problem_solve).
@@ -891,8 +905,9 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
    if ((long long)pr.pnorm.n < nax) pr.pnorm.alloc(nax);
    // Givens state on the device: H (W x W, column-major), rotations, g, and
    // the residual history; the host reads the history every CHECK iterations
-   // and stops at the first iteration below tol, so the iteration count and
-   // the solution are those of a per-iteration check.
+   // (sooner when its rate predicts tol sooner) and stops at the first
+   // iteration below tol, so the iteration count and the solution are those
+   // of a per-iteration check.
    constexpr int CHECK = 4;
    if ((long long)pr.gm_H.n < (long long)W * W) pr.gm_H.alloc((size_t)W * W);
    if ((long long)pr.gm_v.n < 4LL * (W + 1)) pr.gm_v.alloc(4 * (size_t)(W + 1));
@@ -907,7 +922,8 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
       FSVD_CHECK(hipMemcpyAsync(dg, g0.data(), sizeof(double) * (W + 1), hipMemcpyHostToDevice, s));
    }
    launch_normalize(n, np, P0, pr.hbuf.p, r, basis_vec<VT>(pr, 0), s);
-   int i, last = -1, done = -1;
+   int i, last = -1, done = -1, next = CHECK - 1, i_prev = -1;
+   double r_prev = 1.0;   // the history is |g_{k+1}| / |r_0|
    for (i = 0; i < maxit; i++) {
       VT* vi = basis_vec<VT>(pr, i);
       VT* vnext_store = basis_vec<VT>(pr, i + 1);
@@ -933,7 +949,7 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
       // |w|, v_{i+1} = w/|w|, H[i+1,i] and the Givens step of column i
       launch_normalize_givens(n, nax, pr.pnorm.p, w, vnext_store, i, W, pr.hbuf.p, dH, dcs, dsn, dg, dhist, normr,
                               s);
-      if ((i + 1) % CHECK == 0 || i == maxit - 1) {
+      if (i == next || i == maxit - 1) {
          const int a0 = last + 1;
          FSVD_CHECK(hipMemcpyAsync(pr.h_pinned, dhist + a0, sizeof(double) * (i + 1 - a0), hipMemcpyDeviceToHost, s));
          FSVD_CHECK(hipStreamSynchronize(s));
@@ -948,6 +964,10 @@ static SolveInfo gmres_impl(Problem& pr, int prec, const double* rhs, double* so
          }
          last = i;
          if (done >= 0) break;
+         const double r_last = pr.h_pinned[i - a0];
+         next = i + next_check_gap(r_prev, i_prev, r_last, i, tol, CHECK);
+         r_prev = r_last;
+         i_prev = i;
       }
    }
    if (done >= 0) i = done;
@@ -1224,7 +1244,9 @@ static void gmres_batch_impl(Problem& pr, int prec, int ncol, GmCol* const* cols
    std::vector<int> started;
    for (int c = 0; c < ncol; c++)
       if (cols[c]->run) started.push_back(c);
-   int i, last = -1;
+   int i, last = -1, next = CHECK - 1;
+   std::vector<double> r_prev(ncol, 1.0);   // per column, as gmres_impl
+   std::vector<int> i_prev(ncol, -1);
    for (i = 0; i < maxit; i++) {
       act.clear();
       xin.clear();
@@ -1269,7 +1291,7 @@ static void gmres_batch_impl(Problem& pr, int prec, int ncol, GmCol* const* cols
       launch_precond_b(pr.nblocks, na, pr.pc_a.p, pr.pc_b.p, pr.pc_m.p, dd, s);
       launch_cgs2_b<VT>(n, i + 1, nch, na, dd, s);
       launch_normalize_givens_b<VT>(n, nax, i, W, na, dd, s);
-      if ((i + 1) % CHECK == 0 || i == maxit - 1) {
+      if (i == next || i == maxit - 1) {
          const int a0 = last + 1;
          for (int c : act)
             FSVD_CHECK(hipMemcpyAsync(pin + (size_t)c * Problem::PIN_COL, cols[c]->v.p + 3 * (W + 1) + a0,
@@ -1287,7 +1309,19 @@ static void gmres_batch_impl(Problem& pr, int prec, int ncol, GmCol* const* cols
             }
             if (g.done >= 0) g.run = false;
          }
+         // the next check at the soonest predicted convergence of the
+         // columns still running
+         int gap = CHECK;
+         for (int c : act) {
+            GmCol& g = *cols[c];
+            if (!g.run) continue;
+            const double r_last = pin[(size_t)c * Problem::PIN_COL + (i - a0)];
+            gap = std::min(gap, next_check_gap(r_prev[c], i_prev[c], r_last, i, g.tol, CHECK));
+            r_prev[c] = r_last;
+            i_prev[c] = i;
+         }
          last = i;
+         next = i + gap;
       }
    }
    // back substitution and the combination, column by column
