@@ -1797,6 +1797,10 @@ __global__ void __launch_bounds__(256) k_farT_source(const TgtDesc* __restrict__
 }
 
 // Panels side backward: h = IDFT, w = PV_layer^T h, y[panels] += VT^T w.
+// The cube's panels are cut into gridDim.y contiguous chunks, one block
+// each (every block forms the same h and w; each output is written by one
+// block with the arithmetic of the unsplit kernel), so that the coarse
+// levels' few large cubes occupy more than one CU each.
 template <int n, typename T>
 __global__ void __launch_bounds__(256) k_farT_target(const SrcDesc* __restrict__ src, const T* __restrict__ VT,
                                                      const T* __restrict__ PV,
@@ -1814,6 +1818,9 @@ __global__ void __launch_bounds__(256) k_farT_target(const SrcDesc* __restrict__
    T* h = sn + N;
    T* w = h + GP3;
    const SrcDesc d = src[blockIdx.x];
+   const int chunk = (d.np + (int)gridDim.y - 1) / (int)gridDim.y;
+   const int i0 = (int)blockIdx.y * chunk, i1 = i0 + chunk < d.np ? i0 + chunk : d.np;
+   if (i0 >= i1) return;   // uniform over the block
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
    twiddle_table<T, N>(cs, sn);
    for (int b = tid; b < PAD; b += blockDim.x) S[b] = acc[(long long)d.spec_slot * PAD + b];
@@ -1837,7 +1844,7 @@ __global__ void __launch_bounds__(256) k_farT_target(const SrcDesc* __restrict__
    const T* vt = VT + d.vt_off;
    for (int li = 0; li < nl; li++) {
       double* out = li ? yfar2 : yfar;
-      for (int i = tid; i < d.np; i += blockDim.x) {
+      for (int i = i0 + tid; i < i1; i += blockDim.x) {
          T a = 0;
          for (int k = 0; k < d.r; k++) a += vt[(long long)k * d.np + i] * w[li * d.r + k];
          out[d.p0 + i] += (double)a;
@@ -2117,8 +2124,11 @@ void launch_farT_target(int n, int nblk, const SrcDesc* src, const T* VT, const 
                         double* yfar, double* yfar2, int maxr, hipStream_t s) {
    size_t lds = far_target_lds<T>(n, yfar2 ? 2 * maxr : maxr);
    check_lds(lds);
+   // panel chunks per cube: the coarse levels (more grid points) have few,
+   // large cubes
+   const int split = n >= 6 ? 8 : (n == 5 ? 4 : 1);
 #define CALL(NN)                                                                                                    \
-   hipLaunchKernelGGL((k_farT_target<NN, T>), dim3(nblk), dim3(256), lds, s, src, VT, PV,                          \
+   hipLaunchKernelGGL((k_farT_target<NN, T>), dim3(nblk, split), dim3(256), lds, s, src, VT, PV,                   \
                       (const typename Cplx<T>::type*)acc, layer, yfar, yfar2, maxr)
    FSVD_DISPATCH_N(n, CALL)
 #undef CALL
