@@ -1760,9 +1760,12 @@ __global__ void k_zero(double* __restrict__ a, long long n) {
 
 // ---------------------------------------------------------------------------
 // Transposed operator of one layer (reference Tree_multiply_transpose,
-// Tree.c:769-832).  Points side forward: u = U^T x, h = UTI^T u, DFT.
+// Tree.c:769-832).  Points side forward: u = U^T x, h = UTI^T u, DFT.  Any
+// block size gives the same results (one wave per row of U^T, one thread
+// per grid value); the coarse levels, with few large cubes, run 16 waves
+// per cube.
 template <int n, typename T>
-__global__ void __launch_bounds__(256) k_farT_source(const TgtDesc* __restrict__ tgt, const T* __restrict__ UT,
+__global__ void __launch_bounds__(1024) k_farT_source(const TgtDesc* __restrict__ tgt, const T* __restrict__ UT,
                                                      const T* __restrict__ UTI, const double* __restrict__ x,
                                                      typename Cplx<T>::type* __restrict__ spec, int maxr) {
    using C2 = typename Cplx<T>::type;
@@ -2112,8 +2115,9 @@ void launch_farT_source(int n, int nblk, const TgtDesc* tgt, const T* UT, const 
                         int maxr, hipStream_t s) {
    size_t lds = far_source_lds<T>(n, maxr);
    check_lds(lds);
+   const int threads = n >= 5 ? 1024 : 256;
 #define CALL(NN)                                                                                                    \
-   hipLaunchKernelGGL((k_farT_source<NN, T>), dim3(nblk), dim3(256), lds, s, tgt, UT, UTI, x,                      \
+   hipLaunchKernelGGL((k_farT_source<NN, T>), dim3(nblk), dim3(threads), lds, s, tgt, UT, UTI, x,                  \
                       (typename Cplx<T>::type*)spec, maxr)
    FSVD_DISPATCH_N(n, CALL)
 #undef CALL
