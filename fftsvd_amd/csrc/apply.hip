@@ -192,8 +192,9 @@ struct FarSizes {
    static constexpr int N = 2 * n - 1, H = N / 2 + 1, PAD = N * N * H, GP3 = n * n * n;
 };
 
-// Translation: S_d[b] = sum_e T[toff_e][b] * G[src_e][b].  One block per
-// target cube, threads over frequency bins.
+// Translation: S_d[b] = sum_e T[toff_e][b] * G[src_e][b].  Blocks
+// (target cube, slice of gridDim.y over the bins), threads over bins; each
+// bin's sum is the same whatever the slicing.
 template <typename T>
 __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict__ tgt,
                                                        const int* __restrict__ int_src,
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256) k_far_translate(const TgtDesc* __restrict
                                                        const typename Cplx<T>::type* __restrict__ spec,
                                                        typename Cplx<T>::type* __restrict__ acc, int pad) {
    const TgtDesc d = tgt[blockIdx.x];
-   for (int b = threadIdx.x; b < pad; b += blockDim.x) {
+   for (int b = threadIdx.x + (int)(blockIdx.y * blockDim.x); b < pad; b += (int)(blockDim.x * gridDim.y)) {
       T re = 0, im = 0;
       for (int e = 0; e < d.nint; e++) {
          auto t = Tspec[(long long)int_toff[d.int_off + e] * pad + b];
@@ -1912,7 +1913,9 @@ static void check_lds(size_t lds) {
 template <typename T>
 void launch_far_translate(int nblk, const TgtDesc* tgt, const int* isrc, const int* itoff, const void* Tspec,
                           const void* spec, void* acc, int pad, hipStream_t s) {
-   hipLaunchKernelGGL((k_far_translate<T>), dim3(nblk), dim3(256), 0, s, tgt, isrc, itoff,
+   // one block per 256 bins of each cube (the coarse levels have few cubes)
+   const int slices = std::max(1, std::min(8, (pad + 255) / 256));
+   hipLaunchKernelGGL((k_far_translate<T>), dim3(nblk, slices), dim3(256), 0, s, tgt, isrc, itoff,
                       (const typename Cplx<T>::type*)Tspec, (const typename Cplx<T>::type*)spec,
                       (typename Cplx<T>::type*)acc, pad);
 }
